@@ -1,0 +1,169 @@
+#!/usr/bin/env python
+"""Benchmark: certified PPR queries/s of the batched FP32 local push on one
+MI355X, beside the plain FP64 single-source GPU path and the CPU baseline.
+
+Workload (one "step"): 1024 seeds (70% uniform over non-isolated nodes, 30%
+degree-proportional) on the synthetic Chung-Lu power-law graph with
+n = 2,394,385 nodes (~9.3M CSR entries, max degree ~1e5, ~10% isolated),
+alpha = 0.1, eps = 1e-7, in one persistent launch: FP32 push with the FP64
+threshold test, FP64 residual recompute and FP64 correction push, so every
+query ends with |r_v| < eps*alpha*d_v in FP64.  The graph is resident in HBM
+before the timed region starts.
+
+Prints ONE JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "faster-local-solver-for-gdes_amd")
+for _p in (PKG, REPO):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+# algorithmic bytes (FP32 push / certificate), per the documented model:
+#   edge = 4 col-idx + 8 r_v RMW + 4 d_v;  node = 8 indptr + 4 r_u R + 4 r_u W
+#          + 8 x_u RMW + 4 d_u + 4 frontier id R + 4 next-frontier W
+B_EDGE_FP32, B_NODE_FP32 = 16, 36
+#   certificate: per support edge 4 col-idx + 4 x gather; per support node
+#   8 indptr + 4 x + 4 d + 8 r W
+B_CERT_EDGE, B_CERT_NODE = 8, 24
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--eps", type=float, default=1e-7)
+    ap.add_argument("--alpha", type=float, default=0.1)
+    ap.add_argument("--nq", type=int, default=1024)
+    ap.add_argument("--plain-queries", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="seeds timed on the CPU (0 = 2 per core)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--slots", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (queries would shard with no "
+              "exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from flsg import _abi, engine, graph
+
+    t0 = time.time()
+    g = graph.powerlaw_graph_large()
+    log(f"graph n={g.n} nnz={g.m} dmax={g.d_max} isolated={(g.degree == 0).mean():.3f} ({time.time() - t0:.1f}s)")
+    dg = engine.DeviceGraph(g)
+    seeds = graph.mixed_seeds(g, args.nq, seed=0)
+    p32 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=args.alpha, eps=args.eps)
+    p64 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=args.alpha, eps=args.eps)
+
+    # ------------------------------------------------------------ batched FP32 (the measured step)
+    for w in range(args.warmup):
+        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots)
+        log(f"warmup {w}: {bo.ms:.1f} ms")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    runs = []
+    for k in range(args.steps):
+        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots)
+        runs.append(bo)
+        log(f"step {k}: {bo.ms:.1f} ms")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    res = runs[-1].results
+    certified = all(r["status"] == 0 and r["cert"] < 1.0 for r in res)
+    ms_step = wall * 1e3 / args.steps
+    qps = args.nq * args.steps / wall
+
+    oper = sum(r["oper"] for r in res)
+    nodes = sum(r["nodes"] for r in res)
+    cert_edges = sum(r["oper_cert"] for r in res)
+    supp = sum(r["supp"] for r in res)
+    bytes_step = B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * cert_edges + B_CERT_NODE * supp
+    kern_ms = float(np.mean([b.ms for b in runs]))
+    achieved = bytes_step / (kern_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "k_batch_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    # ------------------------------------------------------------ plain FP64 single-source
+    nplain = min(args.plain_queries, args.nq)
+    engine.solve(dg, p64, int(seeds[0]))   # warm
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    plain_oper = 0
+    for q in range(nplain):
+        out = engine.solve(dg, p64, int(seeds[q]))
+        plain_oper += out.result["oper"]
+    torch.cuda.synchronize()
+    plain_qps = nplain / (time.perf_counter() - tp)
+    log(f"plain FP64: {plain_qps:.1f} q/s over {nplain} seeds")
+
+    # ------------------------------------------------------------ CPU baseline (oracle, OpenMP over seeds)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = min(os.cpu_count() or 1, 16)
+        sample = args.cpu_sample or 2 * cores
+        sample = min(sample, args.nq)
+        tc = time.perf_counter()
+        opers, times = O.ppr_batch(g, seeds[:sample], args.alpha, args.eps, kind=0, nthreads=cores)
+        wall_c = time.perf_counter() - tc
+        cpu = {"value": sample / wall_c, "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": f"first {sample} of the 1024 seeds, PPR local GD FP64 (C restatement, OpenMP over "
+                         f"seeds), median {float(np.median(times)):.3f} s/query single-thread"}
+        log(f"cpu: {cpu['value']:.2f} q/s on {cores} threads")
+
+    line = {
+        "metric": "certified PPR queries/s, batched FP32 local GD (FP64 threshold + FP64 certificate), eps=1e-7",
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 push, f64 threshold/certificate",
+        "data": "synthetic",
+        "config": {
+            "workload": "1024-seed batched PPR local GD on the 2,394,385-node Chung-Lu power-law graph",
+            "n": g.n, "nnz": g.m, "d_max": g.d_max, "seeds": args.nq, "alpha": args.alpha, "eps": args.eps,
+            "seed_mix": "70% uniform non-isolated, 30% degree-proportional",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "k_batch", "kernel_ms": kern_ms, "algorithmic_bytes": bytes_step,
+        },
+        "cpu_baseline": cpu,
+        "certified_all": certified,
+        "max_cert": max(r["cert"] for r in res),
+        "plain_fp64_qps": plain_qps,
+        "speedup_vs_plain_fp64": qps / plain_qps,
+        "mean_oper": oper / args.nq,
+        "mean_corrections": sum(r["n_corr"] for r in res) / args.nq,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

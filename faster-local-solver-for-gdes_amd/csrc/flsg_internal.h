@@ -1,0 +1,166 @@
+// flsg_internal.h -- shared device helpers and the graph-handle layout.
+//
+// Target: gfx950 (MI355X), wave64.  All kernels are hand-written HIP; the FP64
+// parity path is compiled with -ffp-contract=off so every multiply/add rounds
+// exactly like the reference's numba code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/flsg.h"
+
+namespace flsg {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define FLSG_HIP(call)                                                               \
+    do {                                                                             \
+        hipError_t _e = (call);                                                      \
+        if (_e != hipSuccess)                                                        \
+            return ::flsg::fail(FLSG_EHIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---------------------------------------------------------------- device counters
+// One block of per-solve counters living in device memory.
+struct SolveCounters {
+    unsigned int cnt[2];        // frontier size by parity
+    unsigned int npieces[2];    // hub pieces by parity
+    unsigned int work[2];       // dynamic work-item counter by parity
+    unsigned int act[2];        // global sweeps: active count by parity
+    unsigned long long oper;    // reference operation count
+    unsigned long long iters;   // non-empty iterations / sweeps
+    unsigned long long trace_off;
+    unsigned int overflow;      // trace / piece overflow flags
+    unsigned int pad;
+};
+
+// ---------------------------------------------------------------- graph handle
+struct BatchArena;   // flsg_batch.hip
+
+}  // namespace flsg
+
+struct flsg_graph_s {
+    int64_t n = 0;
+    int64_t nnz_cap = 0;
+    const int32_t *indptr = nullptr;
+    int32_t *indices = nullptr;
+    int32_t *degree = nullptr;
+    int32_t slack = 0;
+    int64_t m_live = 0;             // sum of degree (the reference's m)
+    int device = 0;
+    int num_cus = 0;
+    // single-query workspace (lazily allocated)
+    int32_t *front[2] = {nullptr, nullptr};
+    double *val = nullptr;          // per frontier entry (float or double)
+    int64_t *pieces = nullptr;      // hub pieces: (frontier index << 32) | piece no.
+    int64_t pieces_cap = 0;
+    double *work_a = nullptr;       // n doubles: scratch vectors for global sweeps
+    double *work_b = nullptr;
+    double *work_c = nullptr;
+    double *coef = nullptr;         // per-sweep coefficients (Chebyshev)
+    int64_t coef_cap = 0;
+    flsg::SolveCounters *ctr = nullptr;
+    flsg::SolveCounters *h_ctr = nullptr;   // pinned mirror
+    int64_t *trace_dev_off = nullptr;       // device per-iteration offsets
+    int64_t trace_dev_off_cap = 0;
+    // batched workspace
+    flsg::BatchArena *arena = nullptr;
+    int32_t batch_slots = 0;
+};
+
+namespace flsg {
+
+int ensure_single_workspace(flsg_graph g);
+int recompute_m(flsg_graph g, hipStream_t st);
+int check_graph(flsg_graph g);
+void batch_free(flsg_graph g);
+
+// ---------------------------------------------------------------- device helpers
+__device__ __forceinline__ unsigned int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    unsigned int l = __lane_id();
+    return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
+}
+
+// Wave-aggregated append: every lane of the wave must call it (pred may be
+// false).  Returns the slot for lanes with pred, -1 otherwise.
+__device__ __forceinline__ long long wave_append(bool pred, unsigned int *counter) {
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return -1;
+    int leader = __ffsll((long long)mask) - 1;
+    unsigned int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (!pred) return -1;
+    return (long long)base + __popcll(mask & lanemask_lt());
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ double atomic_add_wg(double *p, double v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ float atomic_add_wg(float *p, float v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ double atomic_add_dev(double *p, double v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float atomic_add_dev(float *p, float v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-wide inclusive scan of one int per thread (blockDim.x == NT).
+// `sh` needs NT/64 ints of LDS.  Returns inclusive prefix; *total = block sum.
+template <int NT>
+__device__ __forceinline__ int block_incl_scan(int v, int *sh, int *total) {
+    const int lane = __lane_id();
+    const int wid = threadIdx.x / kWave;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) sh[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int w = (lane < NT / kWave) ? sh[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < NT / kWave; off <<= 1) {
+            int y = __shfl_up(w, off);
+            if (lane >= off) w += y;
+        }
+        if (lane < NT / kWave) sh[lane] = w;
+    }
+    __syncthreads();
+    int res = x + (wid > 0 ? sh[wid - 1] : 0);
+    *total = sh[NT / kWave - 1];
+    __syncthreads();
+    return res;
+}
+
+// First index k in [lo, hi) with incl[k] > e (incl is non-decreasing).
+__device__ __forceinline__ int upper_bound_lds(const int *incl, int lo, int hi, int e) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (incl[mid] > e) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+}  // namespace flsg

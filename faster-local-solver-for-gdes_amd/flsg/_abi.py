@@ -1,0 +1,69 @@
+"""ctypes mirror of include/flsg.h (structs, constants, function signatures)."""
+from __future__ import annotations
+
+import ctypes as C
+
+FLSG_OK, FLSG_EARG, FLSG_EASSERT, FLSG_EHIP, FLSG_ENOMEM, FLSG_ECAP, FLSG_ENOCONV = range(7)
+FLSG_PPR, FLSG_KATZ, FLSG_HK = 0, 1, 2
+FLSG_GD, FLSG_CHEB = 0, 1
+FLSG_FP64, FLSG_FP32 = 0, 1
+
+ERROR_NAMES = {
+    FLSG_EARG: "invalid argument", FLSG_EASSERT: "assertion", FLSG_EHIP: "HIP runtime error",
+    FLSG_ENOMEM: "out of device memory", FLSG_ECAP: "capacity exceeded",
+    FLSG_ENOCONV: "no convergence within the iteration cap",
+}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("method", C.c_int32), ("local", C.c_int32), ("precision", C.c_int32),
+        ("alpha", C.c_double), ("eps", C.c_double), ("l_max", C.c_double), ("mu_min", C.c_double),
+        ("tau", C.c_double), ("n_terms", C.c_int64), ("max_iters", C.c_int64),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("oper", C.c_int64), ("iters", C.c_int64), ("oper_cert", C.c_int64), ("n_corr", C.c_int64),
+        ("supp", C.c_int64), ("nodes", C.c_int64), ("cert", C.c_double), ("ms", C.c_double), ("status", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class Trace(C.Structure):
+    _fields_ = [
+        ("d_nodes", C.c_void_p), ("cap_nodes", C.c_int64), ("h_off", C.c_void_p), ("cap_iters", C.c_int64),
+        ("n_iters", C.c_int64), ("overflow", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+# name -> (restype, argtypes); the set of symbols include/flsg.h declares
+SIGNATURES = {
+    "flsg_version": (C.c_int, []),
+    "flsg_last_error": (C.c_char_p, []),
+    "flsg_device_info": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "flsg_graph_create": (C.c_int, [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                    C.POINTER(C.c_void_p)]),
+    "flsg_graph_destroy": (C.c_int, [C.c_void_p]),
+    "flsg_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.c_void_p, C.c_void_p,
+                             C.POINTER(Result), C.POINTER(Trace), C.c_void_p]),
+    "flsg_hk_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.c_void_p, C.c_void_p,
+                                C.POINTER(Result), C.c_void_p, C.POINTER(C.c_int64), C.POINTER(Trace),
+                                C.c_void_p]),
+    "flsg_batch_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_void_p, C.c_int64, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Result), C.c_void_p]),
+    "flsg_batch_slot_bytes": (C.c_int64, [C.c_int64]),
+    "flsg_batch_set_slots": (C.c_int, [C.c_void_p, C.c_int32]),
+}
+
+
+def bind(lib):
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib

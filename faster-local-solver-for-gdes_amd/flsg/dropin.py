@@ -1,0 +1,83 @@
+"""Shared plumbing of the drop-in modules (ppr_solver, katz_solver, hk_solver,
+InstantGNN): the reference passes host numpy CSR arrays to every call; here
+the graph is uploaded once per distinct CSR and kept resident in HBM."""
+from __future__ import annotations
+
+import time
+from collections import OrderedDict
+
+import numpy as np
+
+from . import _abi, engine
+from .graph import CSRGraph
+
+_CACHE: "OrderedDict[tuple, tuple]" = OrderedDict()
+_CACHE_MAX = 4
+
+
+def _as_int_degree(degree) -> np.ndarray:
+    d = np.asarray(degree)
+    if d.dtype.kind == "f":
+        di = d.astype(np.int64)
+        if not np.array_equal(di.astype(d.dtype), d):
+            raise ValueError("degree must hold integer values")
+        d = di
+    return np.ascontiguousarray(d, dtype=np.int32)
+
+
+def device_graph(n, indptr, indices, degree) -> engine.DeviceGraph:
+    """Resident device graph for the caller's CSR arrays (cached by identity;
+    the arrays are held so their identities stay valid)."""
+    key = (int(n), id(indptr), id(indices), id(degree))
+    hit = _CACHE.get(key)
+    if hit is not None and hit[1] is indptr and hit[2] is indices and hit[3] is degree:
+        _CACHE.move_to_end(key)
+        return hit[0]
+    ip = np.asarray(indptr, dtype=np.int64)
+    g = CSRGraph(n=int(n), indptr=ip, indices=np.asarray(indices, dtype=np.int32),
+                 degree=_as_int_degree(degree))
+    if ip.size != n + 1:
+        raise ValueError("indptr must have n+1 entries")
+    dg = engine.DeviceGraph(g)
+    _CACHE[key] = (dg, indptr, indices, degree)
+    while len(_CACHE) > _CACHE_MAX:
+        _, old = _CACHE.popitem(last=False)
+        old[0].close()
+    return dg
+
+
+def single(n, indptr, indices, degree, s_node, params) -> tuple:
+    """(xt, rt, oper, algo_time) as numpy FP64 / int / seconds."""
+    dg = device_graph(n, indptr, indices, degree)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = engine.solve(dg, params, int(s_node))
+    torch.cuda.synchronize()
+    algo_time = time.perf_counter() - t0
+    return out.x.cpu().numpy(), out.r.cpu().numpy(), int(out.result["oper"]), algo_time
+
+
+def batch(n, indptr, indices, degree, s_nodes, params) -> engine.BatchOut:
+    dg = device_graph(n, indptr, indices, degree)
+    return engine.batch_solve(dg, params, np.asarray(s_nodes, dtype=np.int64))
+
+
+def per_seed(n, indptr, indices, degree, s_nodes, params):
+    """Run one FP64 solve per seed; returns (opers int64[k], algo_times f64[k])."""
+    k = len(s_nodes)
+    opers = np.zeros(k, dtype=np.int64)
+    times = np.zeros(k)
+    for i in range(k):
+        _, _, opers[i], times[i] = single(n, indptr, indices, degree, s_nodes[i], params)
+    return opers, times
+
+
+def sor_not_built(name: str):
+    raise NotImplementedError(
+        f"{name}: SOR / Gauss-Southwell stay CPU reference baselines in this build (they are "
+        "inherently sequential; a graph-coloured GPU variant would have to meet the same residual "
+        "certificate before replacing them).")
+
+
+P = _abi

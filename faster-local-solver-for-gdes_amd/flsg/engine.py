@@ -1,0 +1,190 @@
+"""Host side of the push engine: device-resident graphs and solve calls.
+
+PyTorch-ROCm is used for device memory and the current stream only; every
+computation runs in libflsg.so's HIP kernels.  Nothing here has a CPU
+fallback: without a GPU or without the built library the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._build import load_lib
+from .graph import CSRGraph, SlackGraph
+
+
+class FlsgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"flsg error {code} ({_abi.ERROR_NAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("flsg needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        lib = load_lib()
+        msg = lib.flsg_last_error()
+        msg = msg.decode() if msg else ""
+        if rc == _abi.FLSG_EASSERT:
+            raise AssertionError(msg)
+        raise FlsgError(rc, msg)
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(torch) -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceGraph:
+    """A CSR (or slack CSR) resident in HBM with a libflsg graph handle.
+
+    The device copy is int32 indptr/indices/degree; the live adjacency of u is
+    ``indices[indptr[u] : indptr[u] + degree[u]]``.
+    """
+
+    def __init__(self, g: CSRGraph | SlackGraph, device: str = "cuda:0"):
+        torch = _torch()
+        self.torch = torch
+        self.device = torch.device(device)
+        self.n = int(g.n)
+        if int(g.indptr[-1]) >= 2 ** 31:
+            raise ValueError("graph too large for int32 CSR")
+        self.slack = isinstance(g, SlackGraph)
+        self.indptr = torch.from_numpy(np.ascontiguousarray(g.indptr, dtype=np.int32)).to(self.device)
+        self.indices = torch.from_numpy(np.ascontiguousarray(g.indices, dtype=np.int32)).to(self.device)
+        self.degree = torch.from_numpy(np.ascontiguousarray(g.degree, dtype=np.int32)).to(self.device)
+        self.nnz_cap = int(self.indices.numel())
+        self.lib = load_lib()
+        h = C.c_void_p()
+        torch.cuda.synchronize(self.device)
+        _check(self.lib.flsg_graph_create(self.n, self.nnz_cap, _ptr(self.indptr), _ptr(self.indices),
+                                          _ptr(self.degree), 1 if self.slack else 0, C.byref(h)))
+        self.handle = h
+        self.d_max = int(g.degree.max()) if self.n else 0
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.flsg_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_params(kind: int, method: int = _abi.FLSG_GD, local: bool = True, precision: int = _abi.FLSG_FP64,
+                alpha: float = 0.1, eps: float = 1e-7, l_max: float = 0.0, mu_min: float = 0.0,
+                tau: float = 0.0, n_terms: int = 0, max_iters: int = 0) -> _abi.Params:
+    return _abi.Params(kind=kind, method=method, local=1 if local else 0, precision=precision, alpha=alpha,
+                       eps=eps, l_max=l_max, mu_min=mu_min, tau=tau, n_terms=n_terms, max_iters=max_iters)
+
+
+@dataclass
+class SolveOut:
+    x: object            # torch float64 [n] (device)
+    r: object            # torch float64 [n] (HK: [2, n]) (device)
+    result: dict
+    trace: list | None   # per-iteration frontier node arrays (numpy), if requested
+    extra: dict
+
+
+def solve(dg: DeviceGraph, params: _abi.Params, seed: int, trace: bool = False,
+          trace_cap_nodes: int | None = None, trace_cap_iters: int = 100000) -> SolveOut:
+    """One single-source solve on the whole GPU (FP64 path)."""
+    torch = dg.torch
+    n = dg.n
+    x = torch.empty(n, dtype=torch.float64, device=dg.device)
+    r = torch.empty((2, n) if params.kind == _abi.FLSG_HK else n, dtype=torch.float64, device=dg.device)
+    res = _abi.Result()
+    tr_ptr = None
+    tr = None
+    if trace:
+        cap = trace_cap_nodes if trace_cap_nodes is not None else max(4 * n, 1 << 20)
+        d_nodes = torch.empty(cap, dtype=torch.int32, device=dg.device)
+        h_off = np.zeros(trace_cap_iters + 1, dtype=np.int64)
+        tr = _abi.Trace(d_nodes=d_nodes.data_ptr(), cap_nodes=cap, h_off=h_off.ctypes.data,
+                        cap_iters=trace_cap_iters, n_iters=0, overflow=0)
+        tr_ptr = C.byref(tr)
+    extra = {}
+    if params.kind == _abi.FLSG_HK:
+        est_n = C.c_int64(0)
+        lvl = np.zeros(2 * 100000, dtype=np.float64)
+        rc = dg.lib.flsg_hk_solve(dg.handle, C.byref(params), int(seed), _ptr(x), _ptr(r), C.byref(res),
+                                  C.c_void_p(lvl.ctypes.data), C.byref(est_n), tr_ptr, _stream_ptr(torch))
+        _check(rc)
+        extra["est_n"] = int(est_n.value)
+        extra["lvl_sums"] = lvl[:2 * int(est_n.value)].copy()
+    else:
+        rc = dg.lib.flsg_solve(dg.handle, C.byref(params), int(seed), _ptr(x), _ptr(r), C.byref(res),
+                               tr_ptr, _stream_ptr(torch))
+        _check(rc)
+    frontiers = None
+    if trace:
+        if tr.overflow:
+            raise FlsgError(_abi.FLSG_ECAP, "trace capacity exceeded")
+        nodes = d_nodes[: int(h_off[tr.n_iters])].cpu().numpy()
+        frontiers = [nodes[h_off[t]:h_off[t + 1]] for t in range(tr.n_iters)]
+    return SolveOut(x=x, r=r, result=res.as_dict(), trace=frontiers, extra=extra)
+
+
+@dataclass
+class BatchOut:
+    ids: object          # torch int32 [total] (device)
+    vals: object         # torch float64 [total] (device)
+    offsets: np.ndarray  # int64 [nq] start of query q in ids/vals
+    results: list        # per-query result dicts
+    ms: float            # wall-clock of the launch (events), milliseconds
+
+    def query(self, q: int):
+        """(ids, values) of query q as numpy arrays."""
+        o = int(self.offsets[q])
+        k = int(self.results[q]["supp"])
+        return self.ids[o:o + k].cpu().numpy(), self.vals[o:o + k].cpu().numpy()
+
+    def dense(self, q: int, n: int) -> np.ndarray:
+        ids, vals = self.query(q)
+        out = np.zeros(n, dtype=np.float64)
+        out[ids] = vals
+        return out
+
+
+def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None = None,
+                slots: int = 0) -> BatchOut:
+    """Many single-source local solves in one persistent launch (FP32 push,
+    FP64 threshold and FP64 certificate)."""
+    torch = dg.torch
+    seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
+    nq = int(seeds.size)
+    if out_cap is None:
+        out_cap = min(nq * dg.n, max(1 << 22, nq * 200000))
+    d_seeds = torch.from_numpy(seeds).to(dg.device)
+    ids = torch.empty(out_cap, dtype=torch.int32, device=dg.device)
+    vals = torch.empty(out_cap, dtype=torch.float64, device=dg.device)
+    offs = torch.empty(max(nq, 1), dtype=torch.int64, device=dg.device)
+    res = (_abi.Result * max(nq, 1))()
+    if slots:
+        _check(dg.lib.flsg_batch_set_slots(dg.handle, int(slots)))
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    rc = dg.lib.flsg_batch_solve(dg.handle, C.byref(params), _ptr(d_seeds), nq, _ptr(ids), _ptr(vals),
+                                 _ptr(offs), int(out_cap), res, _stream_ptr(torch))
+    e1.record()
+    _check(rc)
+    torch.cuda.synchronize()
+    return BatchOut(ids=ids, vals=vals, offsets=offs.cpu().numpy()[:nq], results=[res[i].as_dict() for i in range(nq)],
+                    ms=e0.elapsed_time(e1))

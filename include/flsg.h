@@ -1,0 +1,160 @@
+/*
+ * flsg.h -- C ABI of libflsg.so, the MI355X-native local graph-diffusion push
+ * solvers.
+ *
+ * The reference (JiaheBai/Faster-Local-Solver-for-GDEs) has no C/FFI layer:
+ * its boundary is a set of module-level Python functions (numba @njit /
+ * CuPy).  Each entry point below names the reference function whose
+ * computation it replaces; the Python drop-in modules (ppr_solver.py,
+ * katz_solver.py, hk_solver.py, InstantGNN.py) keep the reference's Python
+ * signatures and call these through ctypes.
+ *
+ * Conventions
+ *   - every function returns int status: FLSG_OK (0) or an FLSG_E* code;
+ *     flsg_last_error() gives a thread-local message for the last failure;
+ *   - buffers named d_* are device pointers (e.g. torch ROCm tensors); h_*
+ *     are host pointers;  no torch or HIP types appear in the signatures --
+ *     streams are passed as void* (a hipStream_t, NULL = default stream);
+ *   - device CSR: int32 indptr[n+1], int32 indices[nnz_cap], int32 degree[n];
+ *     the live adjacency of u is indices[indptr[u] : indptr[u]+degree[u]]
+ *     (for a plain CSR degree[u] == indptr[u+1]-indptr[u]; for the slack CSR
+ *     of the dynamic workloads it is the live row length);
+ *   - a graph handle borrows the caller's CSR buffers (they must outlive it)
+ *     and owns the solver workspaces it allocates; calls on one handle are
+ *     not thread-safe and are serialised on the stream passed in.
+ */
+#ifndef FLSG_H
+#define FLSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLSG_OK 0
+#define FLSG_EARG 1      /* invalid argument (message says which)            */
+#define FLSG_EASSERT 2   /* the reference's `assert eps*degree[s] <= 1.`     */
+#define FLSG_EHIP 3      /* a HIP runtime call failed                         */
+#define FLSG_ENOMEM 4    /* device allocation failed / admission refused      */
+#define FLSG_ECAP 5      /* an output or trace capacity was too small         */
+#define FLSG_ENOCONV 6   /* iteration cap reached without meeting eps         */
+
+/* problem kinds */
+#define FLSG_PPR 0       /* Q = I - (1-alpha) A D^-1, b = alpha e_s           */
+#define FLSG_KATZ 1      /* (I - alpha A) x = e_s, returns x - e_s for GD     */
+#define FLSG_HK 2        /* heat kernel, level-synchronous Taylor push        */
+
+/* methods */
+#define FLSG_GD 0        /* (local) gradient descent = Jacobi push            */
+#define FLSG_CHEB 1      /* (local) Chebyshev                                 */
+
+/* precision */
+#define FLSG_FP64 0      /* plain FP64 path (parity path)                     */
+#define FLSG_FP32 1      /* FP32 r/x, FP64 threshold test + FP64 certificate  */
+
+typedef struct flsg_graph_s *flsg_graph;
+
+/* Solver parameters (one query or one batch). */
+typedef struct {
+    int32_t kind;        /* FLSG_PPR / FLSG_KATZ / FLSG_HK                   */
+    int32_t method;      /* FLSG_GD / FLSG_CHEB                               */
+    int32_t local;       /* 1 = local (frontier) solver, 0 = global sweeps    */
+    int32_t precision;   /* FLSG_FP64 / FLSG_FP32                             */
+    double alpha;        /* PPR teleport / Katz beta                          */
+    double eps;          /* threshold                                         */
+    double l_max;        /* Katz Chebyshev eigen-bounds of I - alpha A        */
+    double mu_min;
+    double tau;          /* heat-kernel t                                     */
+    int64_t n_terms;     /* heat-kernel Taylor order N (<=0: estimate_n)      */
+    int64_t max_iters;   /* iteration cap (<=0: 1e6)                          */
+} flsg_params;
+
+/* Per-query result record. */
+typedef struct {
+    int64_t oper;        /* the reference's operation count                   */
+    int64_t iters;       /* frontier iterations / global sweeps / HK levels   */
+    int64_t oper_cert;   /* edge visits of the FP64 certificate + correction  */
+    int64_t n_corr;      /* nodes pushed by the FP64 correction               */
+    int64_t supp;        /* |supp x|                                          */
+    int64_t nodes;       /* sum over iterations of the frontier size          */
+    double cert;         /* max_v |r_v| / threshold_v after the solve (< 1)   */
+    double ms;           /* device time of the solve, milliseconds            */
+    int32_t status;      /* FLSG_OK or error code for this query              */
+    int32_t pad;
+} flsg_result;
+
+/* Optional per-iteration frontier trace (device buffers): iteration t
+ * processed d_nodes[h_off[t] : h_off[t+1]] (the order within an iteration is
+ * the GPU's, sets are what parity compares). */
+typedef struct {
+    int32_t *d_nodes;    /* device, capacity cap_nodes                        */
+    int64_t cap_nodes;
+    int64_t *h_off;      /* host, capacity cap_iters + 1                      */
+    int64_t cap_iters;
+    int64_t n_iters;     /* out */
+    int32_t overflow;    /* out */
+    int32_t pad;
+} flsg_trace;
+
+int flsg_version(void);
+const char *flsg_last_error(void);
+
+/* Device properties the host side needs (CU count, free memory). */
+int flsg_device_info(int32_t *num_cus, int64_t *free_bytes, int64_t *total_bytes);
+
+/* Graph handle over a caller-owned device CSR.  `slack` = 1 marks a slack
+ * CSR (live row length in degree[]).  Replaces the whole-graph host->device
+ * copy the reference does inside every GPU query (ppr_solver.py:103-105,
+ * katz_solver.py:102-104): the graph is uploaded once and stays resident. */
+int flsg_graph_create(int64_t n, int64_t nnz_cap, const int32_t *d_indptr, int32_t *d_indices,
+                      int32_t *d_degree, int32_t slack, flsg_graph *out);
+int flsg_graph_destroy(flsg_graph g);
+
+/* Single-source solve on the whole GPU.  Replaces, by params:
+ *   PPR  GD  local : ppr_gd(..., local=True)   ppr_solver.py:208-288,321-328
+ *                    and ppr_gd_local_gpu        ppr_solver.py:82-146
+ *   PPR  GD  global: ppr_gd(..., local=False)  ppr_solver.py:290-320
+ *                    and ppr_gd_standard_gpu     ppr_solver.py:26-79
+ *   PPR  CHEB      : ppr_cheby                  ppr_solver.py:582-661
+ *   KATZ GD        : katz_gd / katz_gd_*_gpu    katz_solver.py:24-248
+ *   KATZ CHEB      : katz_cheby                 katz_solver.py:340-431
+ *   HK             : hk_sor_queue(omega=1)      hk_solver.py:206-295
+ * d_x, d_r: dense FP64 outputs of n entries (HK: d_r holds 2n, the two level
+ * residuals as the reference returns them).  trace may be NULL. */
+int flsg_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, double *d_r,
+               flsg_result *h_res, flsg_trace *trace, void *stream);
+
+/* Heat kernel with the reference's extra outputs: the Taylor order N
+ * (hk_solver.py:225) in *est_n, and per level j the sums
+ * h_lvl_sums[2j] = sum |r_j,u| pushed, h_lvl_sums[2j+1] = residual mass sent to
+ * level j+1 (the reference's beta_t diagnostic, hk_solver.py:250,286-290);
+ * h_lvl_sums (host, 2N doubles) may be NULL.  Replaces hk_sor_queue with
+ * omega=1, local=True (hk_solver.py:206-295). */
+int flsg_hk_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, double *d_r,
+                  flsg_result *h_res, double *h_lvl_sums, int64_t *est_n, flsg_trace *trace, void *stream);
+
+/* Batched solve: many seeds in one persistent launch (one workgroup owns a
+ * query at a time, queries are claimed dynamically).  Replaces the
+ * reference's per-seed fan-out solve_a_graph_all / solve_a_graph_gpu
+ * (ppr_solver.py:149-164,511-579; katz_solver.py:149-164,605-666).
+ * Outputs are sparse: query q's estimate is
+ *   d_out_ids/d_out_vals[h_res[q].. offsets ..]  -- d_out_off[q] (device,
+ *   int64) is the start and h_res[q].supp the length.
+ * out_cap bounds the total number of (id, value) pairs.  PPR/KATZ GD only. */
+int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_t *d_seeds, int64_t n_queries,
+                     int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off, int64_t out_cap,
+                     flsg_result *h_res, void *stream);
+
+/* Size of the per-query device state a batched launch keeps resident, in
+ * bytes per workgroup slot (the unit of memory-footprint admission). */
+int64_t flsg_batch_slot_bytes(int64_t n);
+
+/* Number of resident workgroup slots the handle uses for batched solves
+ * (0 = choose from CU count and free memory). */
+int flsg_batch_set_slots(flsg_graph g, int32_t slots);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLSG_H */

@@ -1,0 +1,249 @@
+"""GPU parity of the push kernels against the oracle (C restatement of the
+reference's numba solvers).
+
+Bars (written in each test):
+  * frontier sets per iteration: identical to the oracle's (integer work is
+    bit-exact; a node whose residual sits within rounding of its threshold
+    could legitimately flip -- such nodes would be counted and reported);
+  * operation counts: equal;
+  * FP64 estimate: relative L1 difference <= 1e-12;
+  * FP32 batched path: every query certified in FP64 (max |r_v|/thr_v < 1,
+    recomputed independently on the host), and the L1 distance to the oracle
+    bounded by the two certified residuals (||x - x*||_1 <= ||r||_1 / alpha).
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from flsg import _abi, engine, graph  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+ALPHA = 0.1
+REL_L1 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def dev_small(small_graph):
+    return engine.DeviceGraph(small_graph)
+
+
+def _rel_l1(a, b):
+    return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+def _compare_traces(gpu_tr, ora_tr):
+    """Number of iterations and per-iteration symmetric-difference sizes."""
+    assert len(gpu_tr) == len(ora_tr), f"iteration count {len(gpu_tr)} != oracle {len(ora_tr)}"
+    diffs = []
+    for t, (a, b) in enumerate(zip(gpu_tr, ora_tr)):
+        sa, sb = np.sort(a), np.sort(b)
+        assert np.unique(sa).size == sa.size, f"duplicate frontier entries at iteration {t}"
+        if sa.size != sb.size or not np.array_equal(sa, sb):
+            diffs.append((t, np.setxor1d(sa, sb).size))
+    return diffs
+
+
+def test_ppr_local_gd_parity_small(small_graph, dev_small):
+    g = small_graph
+    seeds = graph.uniform_seeds(g, 20, seed=0)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-6)
+    for s in seeds:
+        o = O.ppr_gd(g, s, ALPHA, 1e-6, local=True, trace=True)
+        out = engine.solve(dev_small, p, int(s), trace=True)
+        diffs = _compare_traces(out.trace, o["trace"])
+        assert diffs == [], f"seed {s}: frontier set differences {diffs}"
+        assert out.result["oper"] == o["oper"], f"seed {s}"
+        x = out.x.cpu().numpy()
+        r = out.r.cpu().numpy()
+        assert _rel_l1(x, o["x"]) <= REL_L1, f"seed {s}: rel L1 {_rel_l1(x, o['x'])}"
+        assert np.abs(r - o["r"]).sum() <= REL_L1 * np.abs(o["r"]).sum() + 1e-300
+        # certificate: |r_v| < eps*alpha*d_v for every v, recomputed in FP64
+        rr = O.ppr_residual(g, s, ALPHA, x)
+        thr = 1e-6 * ALPHA * np.maximum(g.degree, 1)
+        assert (np.abs(rr) / thr).max() < 1.0 + 1e-9
+
+
+def test_ppr_local_gd_alpha_roundtrip(small_graph, dev_small):
+    """The harness alpha (demo.py:31-35): 0.1 -> 0.10000000000000002."""
+    a, _ = graph.alpha_roundtrip(0.1)
+    assert a == 0.10000000000000002
+    g = small_graph
+    s = int(graph.uniform_seeds(g, 1, seed=3)[0])
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=a, eps=1e-6)
+    o = O.ppr_gd(g, s, a, 1e-6, local=True, trace=True)
+    out = engine.solve(dev_small, p, s, trace=True)
+    assert _compare_traces(out.trace, o["trace"]) == []
+    assert out.result["oper"] == o["oper"]
+    assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
+
+
+@pytest.mark.parametrize("method", ["gd", "cheb"])
+def test_ppr_global_parity_small(small_graph, dev_small, method):
+    g = small_graph
+    m = _abi.FLSG_GD if method == "gd" else _abi.FLSG_CHEB
+    p = engine.make_params(_abi.FLSG_PPR, m, False, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-6)
+    fn = O.ppr_gd if method == "gd" else O.ppr_cheby
+    for s in graph.uniform_seeds(g, 3, seed=1):
+        o = fn(g, s, ALPHA, 1e-6, local=False)
+        out = engine.solve(dev_small, p, int(s))
+        assert out.result["oper"] == o["oper"], f"seed {s}"
+        assert out.result["oper"] % (g.n + g.m) == 0
+        x = out.x.cpu().numpy()
+        # sequential row-order pull sums: bit-identical to the oracle
+        assert np.array_equal(x, o["x"]), f"seed {s}: rel L1 {_rel_l1(x, o['x'])}"
+        assert np.array_equal(out.r.cpu().numpy(), o["r"])
+
+
+@pytest.mark.parametrize("local", [True, False])
+def test_katz_gd_parity_small(small_graph, dev_small, local):
+    g = small_graph
+    beta = 1.0 / (g.d_max + 1)
+    eps = 1e-7
+    p = engine.make_params(_abi.FLSG_KATZ, _abi.FLSG_GD, local, _abi.FLSG_FP64, alpha=beta, eps=eps)
+    for s in graph.uniform_seeds(g, 5, seed=2):
+        o = O.katz_gd(g, s, beta, eps, local=local, trace=local)
+        out = engine.solve(dev_small, p, int(s), trace=local)
+        if local:
+            assert _compare_traces(out.trace, o["trace"]) == []
+        assert out.result["oper"] == o["oper"]
+        assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
+
+
+def test_katz_global_cheb_parity_small(small_graph, dev_small):
+    g = small_graph
+    beta = 1.0 / (g.d_max + 1)
+    import scipy.sparse.linalg as sla
+    A = O.adjacency(g)
+    lam_max = float(sla.eigsh(A, 1, which="LA")[0][0])
+    lam_min = float(sla.eigsh(A, 1, which="SA")[0][0])
+    l_max, mu_min = 1 - beta * lam_min, 1 - beta * lam_max
+    p = engine.make_params(_abi.FLSG_KATZ, _abi.FLSG_CHEB, False, _abi.FLSG_FP64, alpha=beta, eps=1e-7,
+                           l_max=l_max, mu_min=mu_min)
+    for s in graph.uniform_seeds(g, 2, seed=4):
+        o = O.katz_cheby(g, s, beta, l_max, mu_min, 1e-7, local=False)
+        out = engine.solve(dev_small, p, int(s))
+        assert out.result["oper"] == o["oper"]
+        assert np.array_equal(out.x.cpu().numpy(), o["x"])
+
+
+@pytest.mark.parametrize("n_terms", [0, 40])
+def test_hk_push_parity_small(small_graph, dev_small, n_terms):
+    g = small_graph
+    tau, eps = 5.0, 1e-7
+    p = engine.make_params(_abi.FLSG_HK, _abi.FLSG_GD, True, _abi.FLSG_FP64, eps=eps, tau=tau, n_terms=n_terms)
+    for s in graph.uniform_seeds(g, 5, seed=5):
+        o = O.hk_push(g, s, tau, eps, local=True, n_terms=n_terms, trace=True)
+        out = engine.solve(dev_small, p, int(s), trace=True)
+        assert out.extra["est_n"] == o["est_n"]
+        assert _compare_traces(out.trace, o["trace"]) == []
+        assert out.result["oper"] == int(o["oper"])
+        assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
+        rg = out.r.cpu().numpy()
+        assert np.abs(rg - o["r"]).sum() <= REL_L1 * max(np.abs(o["r"]).sum(), 1e-300)
+
+
+def _check_batch(g, seeds, bo, eps, alpha=ALPHA, n_check=None):
+    n_check = len(seeds) if n_check is None else n_check
+    thr = eps * alpha * np.maximum(g.degree, 1)
+    worst = 0.0
+    for q in range(n_check):
+        res = bo.results[q]
+        assert res["status"] == 0
+        assert res["cert"] < 1.0
+        x = bo.dense(q, g.n)
+        rr = O.ppr_residual(g, int(seeds[q]), alpha, x)
+        ratio = float((np.abs(rr) / thr).max())
+        worst = max(worst, ratio)
+        # host FP64 recompute agrees with the device certificate up to rounding
+        assert ratio < 1.0 + 1e-9, f"query {q}: host residual ratio {ratio}"
+    return worst
+
+
+def test_batch_fp32_certified_small(small_graph, dev_small):
+    g = small_graph
+    eps = 1e-6
+    seeds = graph.mixed_seeds(g, 64, seed=7)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
+    bo = engine.batch_solve(dev_small, p, seeds)
+    _check_batch(g, seeds, bo, eps)
+    for q in range(8):
+        s = int(seeds[q])
+        o = O.ppr_gd(g, s, ALPHA, eps, local=True)
+        x = bo.dense(q, g.n)
+        rr = O.ppr_residual(g, s, ALPHA, x)
+        bound = (np.abs(rr).sum() + np.abs(o["r"]).sum()) / ALPHA
+        err = np.abs(x - o["x"]).sum()
+        assert err <= bound * (1 + 1e-9) + 1e-15, f"query {q}: L1 {err} > bound {bound}"
+        # the FP32 frontier work tracks the FP64 one closely
+        assert abs(bo.results[q]["oper"] - o["oper"]) <= 0.05 * o["oper"] + 100
+
+
+def test_batch_same_alone_and_in_mix(small_graph, dev_small):
+    """A query's certified result does not depend on what else is in the batch."""
+    g = small_graph
+    eps = 1e-6
+    seeds = graph.mixed_seeds(g, 32, seed=9)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
+    mix = engine.batch_solve(dev_small, p, seeds)
+    for q in (0, 5, 17):
+        s = int(seeds[q])
+        alone = engine.batch_solve(dev_small, p, seeds[q:q + 1])
+        a = alone.dense(0, g.n)
+        b = mix.dense(q, g.n)
+        # both are certified solutions of the same system: they differ by at
+        # most the sum of their residual bounds, ||x - x*||_1 <= ||r||_1 / alpha
+        ra = np.abs(O.ppr_residual(g, s, ALPHA, a)).sum()
+        rb = np.abs(O.ppr_residual(g, s, ALPHA, b)).sum()
+        assert np.abs(a - b).sum() <= (ra + rb) / ALPHA * (1 + 1e-9)
+        assert alone.results[0]["supp"] > 0
+
+
+def test_batch_katz_certified_small(small_graph, dev_small):
+    g = small_graph
+    beta = 1.0 / (g.d_max + 1)
+    eps = 1e-7
+    seeds = graph.uniform_seeds(g, 32, seed=11)
+    p = engine.make_params(_abi.FLSG_KATZ, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=beta, eps=eps)
+    bo = engine.batch_solve(dev_small, p, seeds)
+    for q in range(len(seeds)):
+        assert bo.results[q]["status"] == 0 and bo.results[q]["cert"] < 1.0
+        o = O.katz_gd(g, int(seeds[q]), beta, eps, local=True)
+        x = bo.dense(q, g.n)
+        x[int(seeds[q])] -= 1.0   # katz_gd returns x - e_s
+        assert np.abs(x - o["x"]).sum() <= 1e-5 * max(np.abs(o["x"]).sum(), 1e-12) + 1e-9
+
+
+@pytest.fixture(scope="module")
+def dev_large(large_graph):
+    return engine.DeviceGraph(large_graph)
+
+
+def test_ppr_local_gd_parity_large(large_graph, dev_large):
+    """Full-size graph (2.39M nodes), eps = 1e-7: oracle parity on 3 seeds."""
+    g = large_graph
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-7)
+    for s in graph.uniform_seeds(g, 3, seed=21):
+        o = O.ppr_gd(g, s, ALPHA, 1e-7, local=True, trace=True)
+        out = engine.solve(dev_large, p, int(s), trace=True, trace_cap_nodes=64 * g.n)
+        diffs = _compare_traces(out.trace, o["trace"])
+        assert diffs == [], f"seed {s}: {diffs}"
+        assert out.result["oper"] == o["oper"]
+        assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
+
+
+def test_batch_certified_large(large_graph, dev_large):
+    """1024 mixed seeds at full size: every query certified on the device;
+    16 of them re-certified by an independent host FP64 recompute."""
+    g = large_graph
+    eps = 1e-7
+    seeds = graph.mixed_seeds(g, 1024, seed=0)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
+    bo = engine.batch_solve(dev_large, p, seeds)
+    assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo.results)
+    _check_batch(g, seeds, bo, eps, n_check=16)
