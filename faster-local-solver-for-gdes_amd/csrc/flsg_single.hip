@@ -317,11 +317,12 @@ template <int KIND>
 __global__ __launch_bounds__(kNT) void k_gsweep_node(const int32_t *degree, double *x, const double *r,
                                                      double *val, int64_t n, double alpha, double om_alpha,
                                                      SolveCounters *c, int p) {
-    if (c->act[p] == 0) return;
+    const unsigned int act = c->act[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c->act[p ^ 1] = 0;
-        c->iters = c->iters + 1;
+        c->act[p ^ 1] = 0;   // a finished solve stays finished in later launches
+        if (act) c->iters = c->iters + 1;
     }
+    if (act == 0) return;
     for (int64_t u = blockIdx.x * (int64_t)kNT + threadIdx.x; u < n; u += (int64_t)gridDim.x * kNT) {
         const double ru = r[u];
         x[u] = x[u] + ru;
@@ -378,11 +379,12 @@ __global__ __launch_bounds__(kNT) void k_csweep_node(const int32_t *degree, doub
                                                      double *dx, double *val, int64_t n, const double *coef,
                                                      double alpha, double om_alpha, SolveCounters *c, int p,
                                                      int64_t sweep) {
-    if (c->act[p] == 0) return;
+    const unsigned int act = c->act[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c->act[p ^ 1] = 0;
-        c->iters = c->iters + 1;
+        c->act[p ^ 1] = 0;   // a finished solve stays finished in later launches
+        if (act) c->iters = c->iters + 1;
     }
+    if (act == 0) return;
     const double b1 = coef[2 * sweep], b2 = coef[2 * sweep + 1];
     for (int64_t u = blockIdx.x * (int64_t)kNT + threadIdx.x; u < n; u += (int64_t)gridDim.x * kNT) {
         double t;

@@ -93,7 +93,7 @@ def _csr(g):
 
 
 def _mk_trace(n: int, cap_nodes: int | None, cap_iters: int = 100000):
-    cap_nodes = cap_nodes if cap_nodes is not None else max(8 * n, 1 << 16)
+    cap_nodes = cap_nodes if cap_nodes is not None else max(64 * n, 1 << 20)
     off = np.zeros(cap_iters + 1, dtype=np.int64)
     nodes = np.zeros(cap_nodes, dtype=np.int32)
     tr = _Trace(off=_p(off), nodes=_p(nodes), cap_iters=cap_iters, cap_nodes=cap_nodes)

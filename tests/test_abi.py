@@ -1,0 +1,38 @@
+"""The C ABI library: loads without a GPU and exports every entry point
+include/flsg.h declares (no compute calls here)."""
+import ctypes
+import os
+import re
+
+from flsg import _abi, _build
+
+
+def declared_symbols():
+    with open(os.path.join(_build.INCLUDE, "flsg.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(flsg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_abi():
+    syms = declared_symbols()
+    assert "flsg_solve" in syms and "flsg_batch_solve" in syms and "flsg_graph_create" in syms
+    assert set(syms) == set(_abi.SIGNATURES), "ctypes mirror out of sync with include/flsg.h"
+
+
+def test_library_exports_every_symbol():
+    lib = ctypes.CDLL(_build.LIB_PATH if os.path.exists(_build.LIB_PATH) else _build.build_lib())
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert missing == []
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(_abi.Params) == 4 * 4 + 5 * 8 + 2 * 8
+    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 2 * 8 + 2 * 4
+    assert ctypes.sizeof(_abi.Trace) == 5 * 8 + 2 * 4
+
+
+def test_version_without_gpu():
+    lib = _build.load_lib()
+    assert lib.flsg_version() >= 1
+    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 48
