@@ -135,6 +135,9 @@ int flsg_graph_destroy(flsg_graph g) {
     hipFree(g->trace_dev_off);
     if (g->h_ctr) hipHostFree(g->h_ctr);
     batch_free(g);
+    cheb_free(g);
+    ignn_free(g);
+    dyn_free(g);
     delete g;
     return FLSG_OK;
 }

@@ -70,6 +70,12 @@ struct flsg_graph_s {
     flsg::SolveCounters *h_ctr = nullptr;   // pinned mirror
     int64_t *trace_dev_off = nullptr;       // device per-iteration offsets
     int64_t trace_dev_off_cap = 0;
+    // ordered local Chebyshev workspace (flsg_cheb.hip)
+    void *cheb = nullptr;
+    // InstantGNN propagation workspace (flsg_ignn.hip)
+    void *ignn = nullptr;
+    // edge-insertion workspace (flsg_dyn.hip)
+    void *dyn = nullptr;
     // batched workspace
     flsg::BatchArena *arena = nullptr;
     int32_t batch_slots = 0;
@@ -81,6 +87,9 @@ int ensure_single_workspace(flsg_graph g);
 int recompute_m(flsg_graph g, hipStream_t st);
 int check_graph(flsg_graph g);
 void batch_free(flsg_graph g);
+void cheb_free(flsg_graph g);
+void ignn_free(flsg_graph g);
+void dyn_free(flsg_graph g);
 
 // ---------------------------------------------------------------- device helpers
 __device__ __forceinline__ unsigned int lane_id() { return __lane_id(); }

@@ -58,6 +58,12 @@ SIGNATURES = {
                                    C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Result), C.c_void_p]),
     "flsg_batch_slot_bytes": (C.c_int64, [C.c_int64]),
     "flsg_batch_set_slots": (C.c_int, [C.c_void_p, C.c_int32]),
+    "flsg_graph_insert_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                          C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+    "flsg_ignn_propagate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int64, C.POINTER(Result), C.c_void_p]),
+    "flsg_ignn_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_int64, C.c_void_p, C.c_void_p]),
 }
 
 

@@ -1,0 +1,129 @@
+"""InstantGNN feature propagation and dynamic edge inserts on the GPU (host side).
+
+State: P, R (FP64, n x C node-major) and the features X stay resident in HBM
+with a slack-CSR graph handle; each snapshot of inserted edges is applied in
+place on the device, P and R of the touched endpoints are corrected so the
+propagation invariant holds on the new graph, and the push resumes from them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .engine import DeviceGraph, _check, _ptr, _stream_ptr
+from .graph import SlackGraph
+
+
+class IgnnParams(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("degrees_changed", C.c_int32), ("cols", C.c_int64),
+                ("alpha", C.c_double), ("eps", C.c_double), ("beta", C.c_double), ("max_rounds", C.c_int64)]
+
+
+@dataclass
+class PropagationStats:
+    ms: float
+    rounds: int
+    rounds_fp64: int
+    oper: int
+    oper_cert: int
+    cert: float
+    keys: int = 0
+    inserted: int = 0
+
+
+class DynamicPropagation:
+    """P ~ sum_k alpha (1-alpha)^k (D^-b A D^(b-1))^k X for all columns of X,
+    maintained under edge-insert snapshots (InstantGNN.py:46-71, 298-359)."""
+
+    def __init__(self, g: SlackGraph, X: np.ndarray, alpha: float, eps: float, beta: float,
+                 precision: int = _abi.FLSG_FP32, device: str = "cuda:0", R0: np.ndarray | None = None,
+                 P0: np.ndarray | None = None):
+        self.dg = DeviceGraph(g, device)
+        torch = self.dg.torch
+        self.torch = torch
+        self.n = g.n
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        if X.ndim == 1:
+            X = X[:, None]
+        assert X.shape[0] == g.n
+        self.cols = X.shape[1]
+        dev = self.dg.device
+        self.X = torch.from_numpy(X).to(dev)
+        self.R = torch.from_numpy(np.ascontiguousarray(R0, dtype=np.float64)).to(dev) if R0 is not None \
+            else self.X.clone()
+        self.P = torch.from_numpy(np.ascontiguousarray(P0, dtype=np.float64)).to(dev) if P0 is not None \
+            else torch.zeros_like(self.X)
+        self.params = IgnnParams(precision=precision, degrees_changed=1, cols=self.cols, alpha=alpha, eps=eps,
+                                 beta=beta, max_rounds=0)
+        self.keys = torch.empty(max(2 * int(g.us.size) if g.us is not None else 0, 1), dtype=torch.int32,
+                                device=dev)
+        self.old_degree = torch.empty(g.n, dtype=torch.int32, device=dev)
+
+    def _propagate(self, nodes=None, n_nodes=0) -> PropagationStats:
+        res = _abi.Result()
+        lib = self.dg.lib
+        _check(lib.flsg_ignn_propagate(self.dg.handle, C.byref(self.params), _ptr(self.X), _ptr(self.P),
+                                       _ptr(self.R), _ptr(nodes) if nodes is not None else None, int(n_nodes),
+                                       C.byref(res), _stream_ptr(self.torch)))
+        self.params.degrees_changed = 0
+        return PropagationStats(ms=res.ms, rounds=res.iters, rounds_fp64=res.n_corr, oper=res.oper,
+                                oper_cert=res.oper_cert, cert=res.cert)
+
+    def initial(self) -> PropagationStats:
+        """instantGNNParallel: push from every node above threshold."""
+        return self._propagate()
+
+    def insert(self, us, vs) -> PropagationStats:
+        """One snapshot: in-place edge insertion, invariant correction of the
+        endpoints (instantGNNUpdateBatch), then the push from them."""
+        torch = self.torch
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        d_us = torch.from_numpy(np.ascontiguousarray(us, dtype=np.int32)).to(self.dg.device)
+        d_vs = torch.from_numpy(np.ascontiguousarray(vs, dtype=np.int32)).to(self.dg.device)
+        k = int(d_us.numel())
+        if self.keys.numel() < 2 * k:
+            self.keys = torch.empty(2 * k, dtype=torch.int32, device=self.dg.device)
+        nk = C.c_int64(0)
+        ni = C.c_int64(0)
+        lib = self.dg.lib
+        _check(lib.flsg_graph_insert_edges(self.dg.handle, _ptr(d_us), _ptr(d_vs), k, _ptr(self.keys),
+                                           C.byref(nk), _ptr(self.old_degree), C.byref(ni),
+                                           _stream_ptr(torch)))
+        self.params.degrees_changed = 1
+        _check(lib.flsg_ignn_update(self.dg.handle, C.byref(self.params), _ptr(self.X), _ptr(self.P),
+                                    _ptr(self.R), _ptr(self.keys), nk.value, _ptr(self.old_degree),
+                                    _stream_ptr(torch)))
+        st = self._propagate(self.keys, nk.value)
+        e1.record()
+        torch.cuda.synchronize()
+        st.ms = e0.elapsed_time(e1)
+        st.keys = nk.value
+        st.inserted = ni.value
+        return st
+
+    def graph_arrays(self):
+        """(indices, degree) of the device slack CSR, on the host."""
+        return self.dg.indices.cpu().numpy(), self.dg.degree.cpu().numpy()
+
+    def close(self):
+        self.dg.close()
+
+
+def ignn_residual(indptr, indices, degree, X, P, alpha, beta):
+    """Host FP64 recompute R = X - P/alpha + ((1-alpha)/alpha) M P of the
+    propagation invariant (M_vu = 1/(d_v^b d_u^(1-b)) over live rows)."""
+    import scipy.sparse as sp
+    n = degree.size
+    rows = np.repeat(np.arange(n), degree)
+    cols = np.concatenate([indices[indptr[u]:indptr[u] + degree[u]] for u in range(n)]) if n else np.zeros(0, int)
+    d = degree.astype(np.float64)
+    with np.errstate(divide="ignore"):
+        w = 1.0 / (d[rows] ** beta * d[cols] ** (1 - beta))
+    M = sp.csr_matrix((w, (rows, cols)), shape=(n, n))
+    R = X - P / alpha + ((1 - alpha) / alpha) * (M @ P)
+    return R

@@ -154,6 +154,50 @@ int64_t flsg_batch_slot_bytes(int64_t n);
  * (0 = choose from CU count and free memory). */
 int flsg_batch_set_slots(flsg_graph g, int32_t slots);
 
+/* ---------------------------------------------------------------- dynamic graphs */
+
+/* In-place insertion of k undirected edges (d_us[i], d_vs[i]) into a slack-CSR
+ * handle.  Replaces updetaGraph(onlyinsert=True) (InstantGNN.py:137-180,
+ * ppr_solver.py:723-766): an edge already present (or repeated in the batch)
+ * is skipped.  If any row's slack would overflow nothing is written and
+ * FLSG_ECAP is returned.  Outputs (any may be NULL): d_keys_out (device, up to
+ * 2k entries) the endpoints that gained edges, *n_keys their number,
+ * d_old_degree (device, n) the degrees before the batch, *n_inserted the
+ * number of edges added.  The new neighbours of u are
+ * indices[indptr[u]+old_degree[u] : indptr[u]+degree[u]]. */
+int flsg_graph_insert_edges(flsg_graph g, const int32_t *d_us, const int32_t *d_vs, int64_t k, int32_t *d_keys_out,
+                            int64_t *n_keys, int32_t *d_old_degree, int64_t *n_inserted, void *stream);
+
+/* ---------------------------------------------------------------- InstantGNN */
+
+typedef struct {
+    int32_t precision;       /* FLSG_FP64 plain / FLSG_FP32 push + FP64 certificate */
+    int32_t degrees_changed; /* recompute d^beta after an edge insertion          */
+    int64_t cols;            /* feature columns C (row-major n x C state)         */
+    double alpha;
+    double eps;              /* threshold eps * d_u^(1-beta)                      */
+    double beta;
+    int64_t max_rounds;      /* <= 0: 1e6                                         */
+} flsg_ignn_params;
+
+/* Feature propagation P ~ sum_k alpha (1-alpha)^k (D^-b A D^(b-1))^k X for all
+ * C columns at once.  d_P, d_R: FP64 n x C row-major state, updated in place
+ * (initially P = 0, R = X).  The frontier starts from d_nodes (device,
+ * n_nodes entries) or from all nodes when d_nodes is NULL.  FP32 precision runs
+ * the rounds in FP32 and then restores |R| < eps d^(1-b) in FP64 from the
+ * recomputed residual R = X - P/alpha + ((1-alpha)/alpha) M P (needs d_X,
+ * FP64 n x C).  Replaces instantGNNPush / instantGNNParallel
+ * (InstantGNN.py:12-71). */
+int flsg_ignn_propagate(flsg_graph g, const flsg_ignn_params *p, const double *d_X, double *d_P, double *d_R,
+                        const int32_t *d_nodes, int64_t n_nodes, flsg_result *h_res, void *stream);
+
+/* After flsg_graph_insert_edges: rescale / correct P and R of the endpoints
+ * (d_keys) so the propagation invariant holds on the new graph; follow with
+ * flsg_ignn_propagate(..., d_keys, n_keys, ...).  Replaces the adjustment part
+ * of instantGNNUpdateBatch (InstantGNN.py:298-318). */
+int flsg_ignn_update(flsg_graph g, const flsg_ignn_params *p, const double *d_X, double *d_P, double *d_R,
+                     const int32_t *d_keys, int64_t n_keys, const int32_t *d_old_degree, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -247,3 +247,53 @@ def test_batch_certified_large(large_graph, dev_large):
     bo = engine.batch_solve(dev_large, p, seeds)
     assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo.results)
     _check_batch(g, seeds, bo, eps, n_check=16)
+
+
+def test_ppr_local_cheb_exact_small(small_graph, dev_small):
+    """Local Chebyshev is order-dependent (residuals change sign); the GPU
+    replays the reference's sequential order, so queues match in ORDER and x,
+    r are bit-identical to the oracle."""
+    g = small_graph
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_CHEB, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-6)
+    for s in graph.uniform_seeds(g, 10, seed=6):
+        o = O.ppr_cheby(g, s, ALPHA, 1e-6, local=True, trace=True)
+        out = engine.solve(dev_small, p, int(s), trace=True)
+        assert len(out.trace) == len(o["trace"]), f"seed {s}"
+        for t, (a, b) in enumerate(zip(out.trace, o["trace"])):
+            assert np.array_equal(a, b), f"seed {s}: queue order differs at iteration {t}"
+        assert out.result["oper"] == o["oper"]
+        assert np.array_equal(out.x.cpu().numpy(), o["x"])
+        assert np.array_equal(out.r.cpu().numpy(), o["r"])
+
+
+def test_katz_local_cheb_exact_small(small_graph, dev_small):
+    g = small_graph
+    beta = 1.0 / (g.d_max + 1)
+    import scipy.sparse.linalg as sla
+    A = O.adjacency(g)
+    lam_max = float(sla.eigsh(A, 1, which="LA")[0][0])
+    lam_min = float(sla.eigsh(A, 1, which="SA")[0][0])
+    l_max, mu_min = 1 - beta * lam_min, 1 - beta * lam_max
+    p = engine.make_params(_abi.FLSG_KATZ, _abi.FLSG_CHEB, True, _abi.FLSG_FP64, alpha=beta, eps=1e-8,
+                           l_max=l_max, mu_min=mu_min)
+    for s in graph.uniform_seeds(g, 5, seed=8):
+        o = O.katz_cheby(g, s, beta, l_max, mu_min, 1e-8, local=True, trace=True)
+        out = engine.solve(dev_small, p, int(s), trace=True)
+        assert len(out.trace) == len(o["trace"])
+        for a, b in zip(out.trace, o["trace"]):
+            assert np.array_equal(a, b)
+        assert out.result["oper"] == o["oper"]
+        assert np.array_equal(out.x.cpu().numpy(), o["x"])
+
+
+def test_ppr_local_cheb_exact_large(large_graph, dev_large):
+    g = large_graph
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_CHEB, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-7)
+    for s in graph.uniform_seeds(g, 2, seed=22):
+        o = O.ppr_cheby(g, s, ALPHA, 1e-7, local=True, trace=True)
+        out = engine.solve(dev_large, p, int(s), trace=True, trace_cap_nodes=64 * g.n)
+        assert len(out.trace) == len(o["trace"])
+        for a, b in zip(out.trace, o["trace"]):
+            assert np.array_equal(a, b)
+        assert out.result["oper"] == o["oper"]
+        assert np.array_equal(out.x.cpu().numpy(), o["x"])
