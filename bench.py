@@ -52,7 +52,22 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="seeds timed on the CPU (0 = 2 per core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (3 times): the known-size reference "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (queries would shard with no "
@@ -105,16 +120,18 @@ def main():
 
     # ------------------------------------------------------------ plain FP64 single-source
     nplain = min(args.plain_queries, args.nq)
-    engine.solve(dg, p64, int(seeds[0]))   # warm
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    plain_oper = 0
-    for q in range(nplain):
-        out = engine.solve(dg, p64, int(seeds[q]))
-        plain_oper += out.result["oper"]
-    torch.cuda.synchronize()
-    plain_qps = nplain / (time.perf_counter() - tp)
-    log(f"plain FP64: {plain_qps:.1f} q/s over {nplain} seeds")
+    plain_qps = None
+    if nplain > 0:
+        engine.solve(dg, p64, int(seeds[0]))   # warm
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        plain_oper = 0
+        for q in range(nplain):
+            out = engine.solve(dg, p64, int(seeds[q]))
+            plain_oper += out.result["oper"]
+        torch.cuda.synchronize()
+        plain_qps = nplain / (time.perf_counter() - tp)
+        log(f"plain FP64: {plain_qps:.1f} q/s over {nplain} seeds")
 
     # ------------------------------------------------------------ CPU baseline (oracle, OpenMP over seeds)
     cpu = None
@@ -158,7 +175,7 @@ def main():
         "certified_all": certified,
         "max_cert": max(r["cert"] for r in res),
         "plain_fp64_qps": plain_qps,
-        "speedup_vs_plain_fp64": qps / plain_qps,
+        "speedup_vs_plain_fp64": (qps / plain_qps) if plain_qps else None,
         "mean_oper": oper / args.nq,
         "mean_corrections": sum(r["n_corr"] for r in res) / args.nq,
     }
