@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="seeds timed on the CPU (0 = 2 per core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0, help="batched workgroup size (256/512/1024; 0 = default)")
+    ap.add_argument("--no-relabel", action="store_true", help="keep the generator's node ids on the device")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times): the known-size reference "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -81,22 +83,25 @@ def main():
     t0 = time.time()
     g = graph.powerlaw_graph_large()
     log(f"graph n={g.n} nnz={g.m} dmax={g.d_max} isolated={(g.degree == 0).mean():.3f} ({time.time() - t0:.1f}s)")
-    dg = engine.DeviceGraph(g)
+    # node ids relabelled by degree on the device (outputs map back to the
+    # generator's ids); the plain FP64 path is timed on the same resident graph
+    dg = engine.DeviceGraph(g, relabel=not args.no_relabel)
     seeds = graph.mixed_seeds(g, args.nq, seed=0)
     p32 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=args.alpha, eps=args.eps)
     p64 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=args.alpha, eps=args.eps)
 
     # ------------------------------------------------------------ batched FP32 (the measured step)
     for w in range(args.warmup):
-        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots)
+        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots, block=args.block)
         log(f"warmup {w}: {bo.ms:.1f} ms")
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     runs = []
     for k in range(args.steps):
-        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots)
+        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots, block=args.block)
         runs.append(bo)
-        log(f"step {k}: {bo.ms:.1f} ms")
+        log(f"step {k}: {bo.ms:.1f} ms (per-query push {np.mean([r['ms_push'] for r in bo.results]):.2f} ms, "
+            f"total {np.mean([r['ms'] for r in bo.results]):.2f} ms)")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     res = runs[-1].results
@@ -172,6 +177,7 @@ def main():
             "kernel": "k_batch", "kernel_ms": kern_ms, "algorithmic_bytes": bytes_step,
         },
         "cpu_baseline": cpu,
+        "relabelled": not args.no_relabel,
         "certified_all": certified,
         "max_cert": max(r["cert"] for r in res),
         "plain_fp64_qps": plain_qps,

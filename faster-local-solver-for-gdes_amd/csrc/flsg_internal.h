@@ -79,6 +79,8 @@ struct flsg_graph_s {
     // batched workspace
     flsg::BatchArena *arena = nullptr;
     int32_t batch_slots = 0;
+    int32_t batch_block = 1024;
+    const int32_t *out_map = nullptr;       // internal id -> caller id for batched outputs
 };
 
 namespace flsg {

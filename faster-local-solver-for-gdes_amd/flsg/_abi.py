@@ -26,12 +26,17 @@ class Params(C.Structure):
 class Result(C.Structure):
     _fields_ = [
         ("oper", C.c_int64), ("iters", C.c_int64), ("oper_cert", C.c_int64), ("n_corr", C.c_int64),
-        ("supp", C.c_int64), ("nodes", C.c_int64), ("cert", C.c_double), ("ms", C.c_double), ("status", C.c_int32),
-        ("pad", C.c_int32),
+        ("supp", C.c_int64), ("nodes", C.c_int64), ("cert", C.c_double), ("ms", C.c_double),
+        ("ms_push", C.c_double), ("status", C.c_int32), ("pad", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class Query(C.Structure):
+    _fields_ = [("seed", C.c_int64), ("kind", C.c_int32), ("pad", C.c_int32), ("alpha", C.c_double),
+                ("eps", C.c_double)]
 
 
 class Trace(C.Structure):
@@ -58,6 +63,13 @@ SIGNATURES = {
                                    C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Result), C.c_void_p]),
     "flsg_batch_slot_bytes": (C.c_int64, [C.c_int64]),
     "flsg_batch_set_slots": (C.c_int, [C.c_void_p, C.c_int32]),
+    "flsg_batch_set_block": (C.c_int, [C.c_void_p, C.c_int32]),
+    "flsg_batch_reserve": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "flsg_graph_set_out_map": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "flsg_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                    C.c_void_p]),
+    "flsg_batch_refresh_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "flsg_graph_insert_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                           C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "flsg_ignn_propagate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

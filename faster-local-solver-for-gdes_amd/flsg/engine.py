@@ -54,7 +54,7 @@ class DeviceGraph:
     ``indices[indptr[u] : indptr[u] + degree[u]]``.
     """
 
-    def __init__(self, g: CSRGraph | SlackGraph, device: str = "cuda:0"):
+    def __init__(self, g: CSRGraph | SlackGraph, device: str = "cuda:0", relabel: bool = False):
         torch = _torch()
         self.torch = torch
         self.device = torch.device(device)
@@ -62,6 +62,15 @@ class DeviceGraph:
         if int(g.indptr[-1]) >= 2 ** 31:
             raise ValueError("graph too large for int32 CSR")
         self.slack = isinstance(g, SlackGraph)
+        self.perm = None          # internal id -> caller id (relabelled graphs)
+        self.iperm = None         # caller id -> internal id
+        if relabel:
+            if self.slack:
+                raise ValueError("relabelling is for static CSR graphs")
+            g, perm, iperm = relabel_by_degree(g)
+            self.perm = torch.from_numpy(perm.astype(np.int32)).to(self.device)
+            self.iperm = torch.from_numpy(iperm.astype(np.int64)).to(self.device)
+            self.perm_np, self.iperm_np = perm, iperm
         self.indptr = torch.from_numpy(np.ascontiguousarray(g.indptr, dtype=np.int32)).to(self.device)
         self.indices = torch.from_numpy(np.ascontiguousarray(g.indices, dtype=np.int32)).to(self.device)
         self.degree = torch.from_numpy(np.ascontiguousarray(g.degree, dtype=np.int32)).to(self.device)
@@ -73,6 +82,12 @@ class DeviceGraph:
                                           _ptr(self.degree), 1 if self.slack else 0, C.byref(h)))
         self.handle = h
         self.d_max = int(g.degree.max()) if self.n else 0
+        if self.perm is not None:
+            _check(self.lib.flsg_graph_set_out_map(self.handle, _ptr(self.perm)))
+
+    def to_internal(self, seeds):
+        s = np.asarray(seeds, dtype=np.int64)
+        return s if self.perm is None else self.iperm_np[s]
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -84,6 +99,27 @@ class DeviceGraph:
             self.close()
         except Exception:
             pass
+
+
+def relabel_by_degree(g: CSRGraph):
+    """Relabel nodes by non-increasing degree (ties by id): hubs, where most
+    pushes land in a power-law graph, become a contiguous prefix of every
+    per-node array.  Returns (graph, perm, iperm) with perm[new] = old."""
+    n = g.n
+    deg = np.asarray(g.degree, dtype=np.int64)
+    perm = np.argsort(-deg, kind="stable").astype(np.int64)
+    iperm = np.empty(n, dtype=np.int64)
+    iperm[perm] = np.arange(n, dtype=np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), deg)
+    src = iperm[rows]
+    dst = iperm[np.asarray(g.indices, dtype=np.int64)]
+    order = np.lexsort((dst, src))
+    ndeg = deg[perm]
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(ndeg, out=indptr[1:])
+    h = CSRGraph(n=n, indptr=indptr, indices=dst[order].astype(np.int32), degree=ndeg.astype(np.int32),
+                 name=g.name + "-bydegree")
+    return h, perm, iperm
 
 
 def make_params(kind: int, method: int = _abi.FLSG_GD, local: bool = True, precision: int = _abi.FLSG_FP64,
@@ -120,6 +156,7 @@ def solve(dg: DeviceGraph, params: _abi.Params, seed: int, trace: bool = False,
                         cap_iters=trace_cap_iters, n_iters=0, overflow=0)
         tr_ptr = C.byref(tr)
     extra = {}
+    seed = int(dg.to_internal([seed])[0])
     if params.kind == _abi.FLSG_HK:
         est_n = C.c_int64(0)
         lvl = np.zeros(2 * 100000, dtype=np.float64)
@@ -137,7 +174,12 @@ def solve(dg: DeviceGraph, params: _abi.Params, seed: int, trace: bool = False,
         if tr.overflow:
             raise FlsgError(_abi.FLSG_ECAP, "trace capacity exceeded")
         nodes = d_nodes[: int(h_off[tr.n_iters])].cpu().numpy()
+        if dg.perm is not None:
+            nodes = dg.perm_np[nodes].astype(np.int32)
         frontiers = [nodes[h_off[t]:h_off[t + 1]] for t in range(tr.n_iters)]
+    if dg.perm is not None:
+        x = x[dg.iperm]
+        r = r[..., dg.iperm]
     return SolveOut(x=x, r=r, result=res.as_dict(), trace=frontiers, extra=extra)
 
 
@@ -163,11 +205,11 @@ class BatchOut:
 
 
 def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None = None,
-                slots: int = 0) -> BatchOut:
+                slots: int = 0, block: int = 0) -> BatchOut:
     """Many single-source local solves in one persistent launch (FP32 push,
     FP64 threshold and FP64 certificate)."""
     torch = dg.torch
-    seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
+    seeds = np.ascontiguousarray(dg.to_internal(seeds))
     nq = int(seeds.size)
     if out_cap is None:
         # supports of eps=1e-7 PPR queries on the 2.39M-node graph are ~0.4-0.6M
@@ -179,6 +221,8 @@ def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None
     res = (_abi.Result * max(nq, 1))()
     if slots:
         _check(dg.lib.flsg_batch_set_slots(dg.handle, int(slots)))
+    if block:
+        _check(dg.lib.flsg_batch_set_block(dg.handle, int(block)))
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()

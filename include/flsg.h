@@ -80,6 +80,7 @@ typedef struct {
     int64_t nodes;       /* sum over iterations of the frontier size          */
     double cert;         /* max_v |r_v| / threshold_v after the solve (< 1)   */
     double ms;           /* device time of the solve, milliseconds            */
+    double ms_push;      /* batched: time of the FP32 push (before the FP64 certificate) */
     int32_t status;      /* FLSG_OK or error code for this query              */
     int32_t pad;
 } flsg_result;
@@ -151,8 +152,42 @@ int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_t *d_seeds,
 int64_t flsg_batch_slot_bytes(int64_t n);
 
 /* Number of resident workgroup slots the handle uses for batched solves
- * (0 = choose from CU count and free memory). */
+ * (0 = choose from CU count and free memory) and the workgroup size
+ * (256, 512 or 1024).  No counterpart in the reference (tuning knobs). */
 int flsg_batch_set_slots(flsg_graph g, int32_t slots);
+int flsg_batch_set_block(flsg_graph g, int32_t block);
+
+/* Allocate the slot arena (slots <= 0: the set / default count). */
+int flsg_batch_reserve(flsg_graph g, int32_t slots, void *stream);
+
+/* Internal-to-caller node id map applied to batched output ids (device,
+ * n entries; NULL = identity) -- used when the graph was relabelled (e.g. by
+ * degree) before upload.  The map must outlive the handle's use of it. */
+int flsg_graph_set_out_map(flsg_graph g, const int32_t *d_out_map);
+
+/* One batched query for flsg_batch_launch. */
+typedef struct {
+    int64_t seed;
+    int32_t kind;        /* FLSG_PPR or FLSG_KATZ (local GD)                  */
+    int32_t pad;
+    double alpha;        /* PPR alpha / Katz beta                             */
+    double eps;
+} flsg_query;
+
+/* Asynchronous batched launch over slots [slot_base, slot_base + n_slots) of
+ * the reserved arena: queries may mix PPR and Katz and their parameters.
+ * d_ctl: 2 uint64 of device scratch owned by this launch; d_res (device,
+ * n_queries records) receives the results; nothing is synchronised.  Launches
+ * whose slot ranges are disjoint may run concurrently on different streams
+ * (the serving loop's continuous batching).  The graph must not be modified
+ * while a launch runs. */
+int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queries, int32_t slot_base,
+                      int32_t n_slots, int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off, int64_t out_cap,
+                      uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters, void *stream);
+
+/* After flsg_graph_insert_edges on a handle with a slot arena: refresh the
+ * degree copy the slots keep for the listed nodes. */
+int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, int64_t n_keys, void *stream);
 
 /* ---------------------------------------------------------------- dynamic graphs */
 

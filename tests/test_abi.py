@@ -28,11 +28,12 @@ def test_library_exports_every_symbol():
 
 def test_struct_layouts():
     assert ctypes.sizeof(_abi.Params) == 4 * 4 + 5 * 8 + 2 * 8
-    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 2 * 8 + 2 * 4
+    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 3 * 8 + 2 * 4
+    assert ctypes.sizeof(_abi.Query) == 8 + 2 * 4 + 2 * 8
     assert ctypes.sizeof(_abi.Trace) == 5 * 8 + 2 * 4
 
 
 def test_version_without_gpu():
     lib = _build.load_lib()
     assert lib.flsg_version() >= 1
-    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 48
+    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 52

@@ -297,3 +297,24 @@ def test_ppr_local_cheb_exact_large(large_graph, dev_large):
             assert np.array_equal(a, b)
         assert out.result["oper"] == o["oper"]
         assert np.array_equal(out.x.cpu().numpy(), o["x"])
+
+
+def test_relabelled_graph_parity_small(small_graph):
+    """A degree-relabelled device graph returns results in the caller's ids:
+    FP64 local GD still matches the oracle's frontier sets and op count, and
+    batched outputs map back (certified against the original graph)."""
+    g = small_graph
+    dgr = engine.DeviceGraph(g, relabel=True)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-6)
+    for s in graph.uniform_seeds(g, 4, seed=30):
+        o = O.ppr_gd(g, s, ALPHA, 1e-6, local=True, trace=True)
+        out = engine.solve(dgr, p, int(s), trace=True)
+        assert _compare_traces(out.trace, o["trace"]) == []
+        assert out.result["oper"] == o["oper"]
+        assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
+    seeds = graph.mixed_seeds(g, 48, seed=31)
+    p32 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=1e-6)
+    for block in (256, 512, 1024):
+        bo = engine.batch_solve(dgr, p32, seeds, block=block)
+        _check_batch(g, seeds, bo, 1e-6)
+    dgr.close()
