@@ -21,24 +21,30 @@ int check_graph(flsg_graph g) {
     return FLSG_OK;
 }
 
-__global__ void k_sum_degree(const int32_t *degree, int64_t n, unsigned long long *out) {
-    unsigned long long t = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+// out[0] = sum of degree, out[1] = largest row capacity indptr[u+1] - indptr[u]
+__global__ void k_sum_degree(const int32_t *indptr, const int32_t *degree, int64_t n, unsigned long long *out) {
+    unsigned long long t = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         t += (unsigned long long)degree[i];
+        const unsigned long long c = (unsigned long long)(indptr[i + 1] - indptr[i]);
+        m = c > m ? c : m;
+    }
     t = wave_sum(t);
-    if (__lane_id() == 0 && t) atomicAdd(out, t);
+    if (__lane_id() == 0 && t) atomicAdd(&out[0], t);
+    if (m) atomicMax(&out[1], m);
 }
 
 int recompute_m(flsg_graph g, hipStream_t st) {
     unsigned long long *d = nullptr;
-    unsigned long long h = 0;
-    FLSG_HIP(hipMalloc((void **)&d, sizeof(unsigned long long)));
-    FLSG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_sum_degree, dim3(1024), dim3(256), 0, st, g->degree, g->n, d);
-    FLSG_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+    unsigned long long h[2] = {0, 0};
+    FLSG_HIP(hipMalloc((void **)&d, sizeof(unsigned long long) * 2));
+    FLSG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long) * 2, st));
+    hipLaunchKernelGGL(k_sum_degree, dim3(1024), dim3(256), 0, st, g->indptr, g->degree, g->n, d);
+    FLSG_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st));
     FLSG_HIP(hipStreamSynchronize(st));
     FLSG_HIP(hipFree(d));
-    g->m_live = (int64_t)h;
+    g->m_live = (int64_t)h[0];
+    g->d_max_hint = (int64_t)h[1];
     return FLSG_OK;
 }
 

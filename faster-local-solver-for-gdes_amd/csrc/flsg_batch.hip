@@ -37,16 +37,20 @@
 
 namespace flsg {
 
+constexpr int kMaxHkTerms = 96;      // heat-kernel Taylor order cap of a batched launch
 constexpr unsigned kInQ = 1u;        // flags: in the correction queue
 constexpr unsigned kSeen = 2u;       // flags: listed in the support
 
 struct __attribute__((aligned(32))) NodeRec {
-    float r;
-    float x;
-    int32_t d;
-    uint32_t fl;
-    double r64;
-    double x64;
+    float r;            // residual (FP32 push); heat kernel: even-level residual
+    float x;            // estimate (FP32 push)
+    int32_t d;          // degree (copy of the graph's, same line as r)
+    uint32_t fl;        // certificate flags
+    union {
+        double r64;     // FP64 residual (certificate)
+        float rh[2];    // heat kernel: rh[0] = odd-level residual
+    };
+    double x64;         // FP64 estimate (certificate); heat kernel: push value
 };
 static_assert(sizeof(NodeRec) == 32, "node record must be 32 bytes");
 
@@ -58,6 +62,7 @@ struct BatchArena {
     int32_t *q1 = nullptr;
     double *vq = nullptr;        // slots * n
     int32_t *sup = nullptr;      // slots * n
+    uint32_t *pcs = nullptr;     // slots * n: hub pieces of the current list
     uint64_t *ctl = nullptr;     // launch control for the synchronous entry point
     flsg_result *d_res = nullptr;
     int64_t d_res_cap = 0;
@@ -76,6 +81,8 @@ struct BatchArgs {
     int32_t *q1;
     double *vq;
     int32_t *sup;
+    uint32_t *pcs;
+    int32_t piece_w;             // edges per hub piece
     int32_t slot_base;
     uint64_t *ctl;               // [0] next query (low 32 bits used), [1] output entries used
     const int32_t *out_map;      // internal id -> caller id (NULL = identity)
@@ -85,6 +92,11 @@ struct BatchArgs {
     int64_t out_cap;
     flsg_result *res;
     int64_t max_iters;
+    // heat kernel (shared by the launch's HK queries)
+    int32_t hk_n;                // Taylor order N
+    double hk_tau;
+    double hk_ete;               // exp(tau) * eps
+    double hk_psi[kMaxHkTerms + 1];
 };
 
 struct QParams {
@@ -140,26 +152,327 @@ __device__ __forceinline__ uint32_t fl_and(uint32_t *p, uint32_t v) {
     return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+
+// Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), FP32
+// levels with FP64 threshold tests.  Level j's residual lives in rec.r (j even)
+// or rec.rh[0] (j odd); each level keeps a list of the nodes it touched so the
+// level array can be cleared afterwards (the reference zeroes the whole level,
+// hk_solver.py:282-285); the per-entry push value is parked in rec.x64.
+template <int NT>
+__device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, NodeRec *rec, int32_t *qa,
+                         int32_t *qb, double *vq, int32_t *sup, int *s_incl, int *s_start, double *s_val,
+                         int *s_scan, unsigned long long *s_red, unsigned int *s_cnt, unsigned int *s_ncnt,
+                         unsigned int *s_sup, unsigned int *s_tn, unsigned int *s_tnn, unsigned long long *s_base,
+                         uint64_t t_start) {
+    const int tid = threadIdx.x;
+    const int N = a.hk_n;
+    int32_t *tl_cur = (int32_t *)vq;          // nodes with a residual at the current level
+    int32_t *tl_nxt = (int32_t *)vq + a.n;    // nodes touched for the next level
+    const int s = (int)qq.seed;
+    if (tid == 0) {
+        rec[s].r = 1.0f;                       // hk_solver.py:223
+        qa[0] = s;
+        *s_cnt = 1;
+        *s_sup = 0;
+        tl_cur[0] = s;
+        *s_tn = 1;
+    }
+    __syncthreads();
+    int32_t *qc = qa, *qn = qb;
+    unsigned long long oper = 0, fnodes = 0;
+    int64_t levels = 0;
+    for (int j = 0; j < N; j++) {
+        const unsigned int cnt = *s_cnt;
+        if (cnt == 0) break;                   // hk_solver.py:237
+        levels++;
+        fnodes += cnt;
+        const bool odd = (j & 1) != 0;
+        const bool last = (j + 1 == N);
+        const double coef = a.hk_tau / ((double)j + 1.);
+        const double psic = a.hk_psi[j], psin = a.hk_psi[j + 1];
+        __syncthreads();
+        if (tid == 0) {
+            *s_ncnt = 0;
+            *s_tnn = 0;
+        }
+        unsigned long long vol = 0;
+        for (unsigned int i = tid; i < cnt; i += NT) {
+            const int u = qc[i];
+            NodeRec &ru = rec[u];
+            const int d = ru.d;
+            float &rc = odd ? ru.rh[0] : ru.r;
+            const float tmp = rc;
+            double v = 0.0;
+            if (!(fabs((double)tmp) < ((a.hk_ete * (double)d) / (double)N) / psic)) {   // hk_solver.py:244
+                const float xo = ru.x;
+                ru.x = xo + tmp;
+                if (xo == 0.0f) sup[atomicAdd(s_sup, 1u)] = u;
+                rc = 0.0f;
+                v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
+                vol += (unsigned long long)d;
+            }
+            ru.x64 = v;
+        }
+        oper += block_sum_u64<NT>(vol, s_red);
+        for (unsigned int base = 0; base < cnt; base += NT) {
+            const unsigned int i = base + tid;
+            int d = 0, st = 0;
+            double v = 0.0;
+            if (i < cnt) {
+                const int u = qc[i];
+                NodeRec &ru = rec[u];
+                v = ru.x64;
+                ru.x64 = 0.0;
+                if (v != 0.0) {
+                    d = ru.d;
+                    st = a.indptr[u];
+                }
+            }
+            int tot;
+            const int incl = block_incl_scan<NT>(d, s_scan, &tot);
+            s_incl[tid] = incl;
+            s_start[tid] = st;
+            s_val[tid] = v;
+            __syncthreads();
+            for (int e0 = 0; e0 < tot; e0 += NT) {
+                const int e = e0 + tid;
+                bool cross = false, first = false;
+                int vtx = 0;
+                if (e < tot) {
+                    const int k = upper_bound_lds(s_incl, 0, NT, e);
+                    const int o = e - (k ? s_incl[k - 1] : 0);
+                    vtx = a.indices[s_start[k] + o];
+                    const float val = (float)s_val[k];
+                    NodeRec &rv = rec[vtx];
+                    if (last) {
+                        const float old = atomic_add_wg(&rv.x, val);   // hk_solver.py:254-256
+                        if (old == 0.0f) sup[atomicAdd(s_sup, 1u)] = vtx;
+                    } else {
+                        float *rn = odd ? &rv.r : &rv.rh[0];
+                        const float old = atomic_add_wg(rn, val);
+                        const float nw = old + val;
+                        const double thr = ((a.hk_ete * (double)rv.d) / (double)N) / psin;
+                        cross = (double)old < thr && thr <= (double)nw;      // hk_solver.py:258
+                        first = old == 0.0f;
+                    }
+                }
+                const long long slot = wave_append(cross, s_ncnt);
+                if (slot >= 0) qn[slot] = vtx;
+                const long long ts = wave_append(first, s_tnn);
+                if (ts >= 0) tl_nxt[ts] = vtx;
+            }
+            __syncthreads();
+        }
+        // drop what is left of level j (hk_solver.py:282-285)
+        const unsigned int tn = *s_tn;
+        for (unsigned int i = tid; i < tn; i += NT) {
+            NodeRec &rv = rec[tl_cur[i]];
+            if (odd) rv.rh[0] = 0.0f;
+            else rv.r = 0.0f;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            *s_cnt = *s_ncnt;
+            *s_tn = *s_tnn;
+        }
+        int32_t *t = qc;
+        qc = qn;
+        qn = t;
+        int32_t *tt = tl_cur;
+        tl_cur = tl_nxt;
+        tl_nxt = tt;
+        __syncthreads();
+    }
+    // the level the loop stopped at still holds sub-threshold residual
+    {
+        const unsigned int tn = *s_tn;
+        const bool odd = (levels & 1) != 0;
+        for (unsigned int i = tid; i < tn; i += NT) {
+            NodeRec &rv = rec[tl_cur[i]];
+            if (odd) rv.rh[0] = 0.0f;
+            else rv.r = 0.0f;
+        }
+    }
+    __syncthreads();
+    const unsigned int S = *s_sup;
+    if (tid == 0) *s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
+    __syncthreads();
+    const unsigned long long obase = *s_base;
+    const bool fits = obase + S <= (unsigned long long)a.out_cap;
+    const double et = exp(a.hk_tau);
+    for (unsigned int i = tid; i < S; i += NT) {
+        const int u = sup[i];
+        NodeRec &ru = rec[u];
+        if (fits) {
+            a.out_ids[obase + i] = a.out_map ? a.out_map[u] : u;
+            a.out_vals[obase + i] = (double)ru.x / et;                         // hk_solver.py:291
+        }
+        ru.x = 0.0f;
+    }
+    if (tid == 0) {
+        const uint64_t t_end = now_ticks();
+        flsg_result rr{};
+        rr.oper = (int64_t)oper;
+        rr.iters = levels;
+        rr.supp = (int64_t)S;
+        rr.nodes = (int64_t)fnodes;
+        rr.ms = (double)(t_end - t_start) * 1e-5;
+        rr.ms_push = rr.ms;
+        rr.status = fits ? FLSG_OK : FLSG_ECAP;
+        a.res[q] = rr;
+        a.out_off[q] = fits ? (int64_t)obase : -1;
+    }
+    __syncthreads();
+}
+
+constexpr int kUnroll = 4;           // edges in flight per lane
+constexpr int kHubW = 512;           // entries with more edges go through pieces
+
+struct WaveLds {
+    int incl[kWave];
+    int start[kWave];
+    double val[kWave];
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Edges of the entries [0, cnt) of a list, wavefront by wavefront with no
+// block barrier: a wave claims either a group of 64 consecutive entries (its
+// edges are flattened by a wave prefix sum; entries with more than kHubW edges
+// contribute nothing there) or one piece of a hub entry (pcs[], built by the
+// caller: (entry << 8) | piece).  Each lane keeps kUnroll edges in flight.
+//   load(i, st, d, v): row start, degree (0 = nothing to push) and value of entry i
+//   edge(vtx, v) -> bool: process one edge; true appends vtx to next[] via *s_next
+template <typename Load, typename Edge>
+__device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w,
+                                           unsigned int *s_work, WaveLds &W, const int32_t *indices, Load load,
+                                           Edge edge, int32_t *next, unsigned int *s_next) {
+    const int lane = __lane_id();
+    const unsigned int ngroups = (cnt + kWave - 1) / kWave;
+    const unsigned int nitems = ngroups + npc;
+    for (;;) {
+        unsigned int item = 0;
+        if (lane == 0) item = atomicAdd(s_work, 1u);
+        item = __shfl(item, 0);
+        if (item >= nitems) break;
+        if (item < ngroups) {
+            const unsigned int i = item * kWave + lane;
+            int st = 0, d = 0;
+            double v = 0.0;
+            if (i < cnt) {
+                load(i, st, d, v);
+                if (d > kHubW) d = 0;
+            }
+            int incl = d;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            const int tot = __shfl(incl, kWave - 1);
+            W.incl[lane] = incl;
+            W.start[lane] = st;
+            W.val[lane] = v;
+            wave_sync();
+            for (int e0 = 0; e0 < tot; e0 += kWave * kUnroll) {
+                int vtx[kUnroll];
+                double vv[kUnroll];
+                bool act[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    const int e = e0 + u * kWave + lane;
+                    act[u] = e < tot;
+                    vtx[u] = 0;
+                    vv[u] = 0.0;
+                    if (act[u]) {
+                        const int k = upper_bound_lds(W.incl, 0, kWave, e);
+                        const int o = e - (k ? W.incl[k - 1] : 0);
+                        vtx[u] = indices[W.start[k] + o];
+                        vv[u] = W.val[k];
+                    }
+                }
+                bool cr[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) cr[u] = act[u] ? edge(vtx[u], vv[u]) : false;
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    const long long slot = wave_append(cr[u], s_next);
+                    if (slot >= 0) next[slot] = vtx[u];
+                }
+            }
+            wave_sync();
+        } else {
+            const uint32_t pc = pcs[item - ngroups];
+            int st = 0, d = 0;
+            double v = 0.0;
+            load(pc >> 8, st, d, v);
+            const int beg = (int)(pc & 255u) * piece_w;
+            const int end = min(d, beg + piece_w);
+            for (int e0 = beg; e0 < end; e0 += kWave * kUnroll) {
+                int vtx[kUnroll];
+                bool act[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    const int e = e0 + u * kWave + lane;
+                    act[u] = e < end;
+                    vtx[u] = act[u] ? indices[st + e] : 0;
+                }
+                bool cr[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) cr[u] = act[u] ? edge(vtx[u], v) : false;
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    const long long slot = wave_append(cr[u], s_next);
+                    if (slot >= 0) next[slot] = vtx[u];
+                }
+            }
+        }
+    }
+}
+
+// hub pieces of the entries [0, cnt) of a list whose degree is given by deg(i)
+template <int NT, typename Deg>
+__device__ __forceinline__ void build_pieces(unsigned int cnt, int piece_w, Deg deg, uint32_t *pcs,
+                                             unsigned int *s_npc) {
+    for (unsigned int i = threadIdx.x; i < cnt; i += NT) {
+        const int d = deg(i);
+        if (d > kHubW) {
+            const unsigned int np = (unsigned int)((d + piece_w - 1) / piece_w);
+            const unsigned int b = atomicAdd(s_npc, np);
+            for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
+        }
+    }
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
+    __shared__ WaveLds s_w[NT / kWave];
+    __shared__ unsigned long long s_red[NT / kWave];
+    __shared__ double s_redd[NT / kWave];
+    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_tnn, s_work, s_npc;
+    __shared__ unsigned long long s_base, s_vol;
+    // the heat-kernel path keeps its block-chunked form
     __shared__ int s_incl[NT];
     __shared__ int s_start[NT];
     __shared__ double s_val[NT];
     __shared__ int s_scan[NT / kWave];
-    __shared__ unsigned long long s_red[NT / kWave];
-    __shared__ double s_redd[NT / kWave];
-    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup;
-    __shared__ unsigned long long s_base;
 
     const int tid = threadIdx.x;
+    WaveLds &W = s_w[tid / kWave];
     const int64_t off = (int64_t)(a.slot_base + (int)blockIdx.x) * a.n;
     NodeRec *rec = a.rec + off;
     int32_t *qa = a.q0 + off;
     int32_t *qb = a.q1 + off;
     double *vq = a.vq + off;
     int32_t *sup = a.sup + off;
+    uint32_t *pcs = a.pcs + off;
     const int32_t *indptr = a.indptr;
     const int32_t *indices = a.indices;
+    const int pw = a.piece_w;
 
     for (;;) {
         if (tid == 0) s_q = atomicAdd((unsigned int *)&a.ctl[0], 1u);
@@ -169,6 +482,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         if ((int64_t)q >= a.nq) break;
         const uint64_t t_start = now_ticks();
         const flsg_query qq = a.queries[q];
+        if (qq.kind == FLSG_HK) {
+            hk_query<NT>(a, qq, q, rec, qa, qb, vq, sup, s_incl, s_start, s_val, s_scan, s_red, &s_cnt, &s_ncnt,
+                         &s_sup, &s_tn, &s_tnn, &s_base, t_start);
+            continue;
+        }
         const QParams P = qparams(qq);
         const int s = (int)qq.seed;
         if (tid == 0) {
@@ -176,10 +494,10 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             qa[0] = s;
             s_cnt = 1;
             s_sup = 0;
+            s_vol = 0;
         }
         __syncthreads();
         int32_t *qc = qa, *qn = qb;
-        unsigned long long oper = 0;   // identical in all threads
         unsigned long long fnodes = 0;
         int64_t iters = 0;
         int status = FLSG_OK;
@@ -195,60 +513,62 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             iters++;
             fnodes += cnt;
             __syncthreads();
-            if (tid == 0) s_ncnt = 0;
+            if (tid == 0) {
+                s_ncnt = 0;
+                s_npc = 0;
+                s_work = 0;
+            }
+            __syncthreads();
+            // node phase: r_u -> 0, x_u += r_u, push value (ppr_solver.py:273-282)
             unsigned long long vol = 0;
-            for (unsigned int i = tid; i < cnt; i += NT) {
-                const int u = qc[i];
-                NodeRec &ru = rec[u];
-                const int d = ru.d;
-                const float tmp = ru.r;
-                ru.r = 0.0f;
-                const float xo = ru.x;
-                ru.x = xo + tmp;
-                if (xo == 0.0f) {
-                    const unsigned int pos = atomicAdd(&s_sup, 1u);
-                    sup[pos] = u;
+            for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
+                int uu[kUnroll];
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    const unsigned int i = i0 + j * NT;
+                    uu[j] = i < cnt ? qc[i] : -1;
                 }
-                vq[i] = (double)(float)bpush_value(P, (double)tmp, d);
-                vol += (unsigned long long)d;
-            }
-            oper += block_sum_u64<NT>(vol, s_red);   // contains __syncthreads
-            for (unsigned int base = 0; base < cnt; base += NT) {
-                const unsigned int i = base + tid;
-                int d = 0, st = 0;
-                double v = 0.0;
-                if (i < cnt) {
-                    const int u = qc[i];
-                    d = rec[u].d;
-                    st = indptr[u];
-                    v = vq[i];
-                }
-                int tot;
-                const int incl = block_incl_scan<NT>(d, s_scan, &tot);
-                s_incl[tid] = incl;
-                s_start[tid] = st;
-                s_val[tid] = v;
-                __syncthreads();
-                for (int e0 = 0; e0 < tot; e0 += NT) {
-                    const int e = e0 + tid;
-                    bool cross = false;
-                    int vtx = 0;
-                    if (e < tot) {
-                        const int k = upper_bound_lds(s_incl, 0, NT, e);
-                        const int o = e - (k ? s_incl[k - 1] : 0);
-                        vtx = indices[s_start[k] + o];
-                        const float val = (float)s_val[k];
-                        NodeRec &rv = rec[vtx];
-                        const float old = atomic_add_wg(&rv.r, val);
-                        const float nw = old + val;
-                        const double thr = P.thr_scale * (double)rv.d;
-                        cross = (double)old < thr && thr <= (double)nw;
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    if (uu[j] < 0) continue;
+                    const unsigned int i = i0 + j * NT;
+                    NodeRec &ru = rec[uu[j]];
+                    const int d = ru.d;
+                    const float tmp = ru.r;
+                    ru.r = 0.0f;
+                    const float xo = ru.x;
+                    ru.x = xo + tmp;
+                    if (xo == 0.0f) sup[atomicAdd(&s_sup, 1u)] = uu[j];
+                    vq[i] = (double)(float)bpush_value(P, (double)tmp, d);
+                    vol += (unsigned long long)d;
+                    if (d > kHubW) {
+                        const unsigned int np = (unsigned int)((d + pw - 1) / pw);
+                        const unsigned int b = atomicAdd(&s_npc, np);
+                        for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
                     }
-                    const long long slot = wave_append(cross, &s_ncnt);
-                    if (slot >= 0) qn[slot] = vtx;
                 }
-                __syncthreads();
             }
+            if (vol) atomicAdd(&s_vol, vol);
+            __syncthreads();
+            // edge phase: v joins the next frontier iff this atomic moved r_v across thr_v
+            wave_edges(
+                cnt, pcs, s_npc, pw, &s_work, W, indices,
+                [&](unsigned int i, int &st, int &d, double &v) {
+                    const int u = qc[i];
+                    st = indptr[u];
+                    d = rec[u].d;
+                    v = vq[i];
+                },
+                [&](int vtx, double v) -> bool {
+                    const float val = (float)v;
+                    NodeRec &rv = rec[vtx];
+                    const float old = atomic_add_wg(&rv.r, val);
+                    const float nw = old + val;
+                    const double thr = P.thr_scale * (double)rv.d;
+                    return (double)old < thr && thr <= (double)nw;
+                },
+                qn, &s_ncnt);
+            __syncthreads();
             if (tid == 0) s_cnt = s_ncnt;
             int32_t *t = qc;
             qc = qn;
@@ -256,37 +576,36 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             __syncthreads();
         }
         __syncthreads();
+        const unsigned long long oper = s_vol;
         const uint64_t t_push = now_ticks();
         const unsigned int S32 = s_sup;
-
-        // ---------------- 2a. sweep A: r64 = pushes of x32 over supp; reset r32
         unsigned long long cert_ops = 0;
-        for (unsigned int base = 0; base < S32; base += NT) {
-            const unsigned int i = base + tid;
-            int d = 0, st = 0;
-            double v = 0.0;
-            if (i < S32) {
-                const int u = sup[i];
-                d = rec[u].d;
-                st = indptr[u];
-                v = bpush_value(P, (double)rec[u].x, d);
-            }
-            int tot;
-            const int incl = block_incl_scan<NT>(d, s_scan, &tot);
-            s_incl[tid] = incl;
-            s_start[tid] = st;
-            s_val[tid] = v;
-            __syncthreads();
-            cert_ops += (unsigned long long)tot;
-            for (int e = tid; e < tot; e += NT) {
-                const int k = upper_bound_lds(s_incl, 0, NT, e);
-                const int o = e - (k ? s_incl[k - 1] : 0);
-                NodeRec &rv = rec[indices[s_start[k] + o]];
-                atomic_add_wg(&rv.r64, s_val[k]);
-                rv.r = 0.0f;
-            }
-            __syncthreads();
+        if (tid == 0) {
+            s_npc = 0;
+            s_work = 0;
         }
+        __syncthreads();
+        // pieces of the support's hubs (sweeps A and B)
+        build_pieces<NT>(S32, pw, [&](unsigned int i) { return rec[sup[i]].d; }, pcs, &s_npc);
+        __syncthreads();
+        const unsigned int npc_sup = s_npc;
+        // ---------------- 2a. sweep A: r64 = pushes of x32 over supp; reset r32
+        wave_edges(
+            S32, pcs, npc_sup, pw, &s_work, W, indices,
+            [&](unsigned int i, int &st, int &d, double &v) {
+                const int u = sup[i];
+                st = indptr[u];
+                d = rec[u].d;
+                v = bpush_value(P, (double)rec[u].x, d);
+            },
+            [&](int vtx, double v) -> bool {
+                NodeRec &rv = rec[vtx];
+                atomic_add_wg(&rv.r64, v);
+                rv.r = 0.0f;
+                return false;
+            },
+            qn, &s_ncnt);
+        __syncthreads();
         // 2b. r64_u -= x_u on the support; x64 = x32
         for (unsigned int i = tid; i < S32; i += NT) {
             NodeRec &ru = rec[sup[i]];
@@ -295,47 +614,45 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             ru.x64 = xd;
             ru.x = 0.0f;
             ru.fl = kSeen;
+            cert_ops += (unsigned long long)ru.d;
         }
         __syncthreads();
         if (tid == 0) {
             rec[s].r64 = rec[s].r64 + P.r0;
             rec[s].r = 0.0f;
             s_cnt = 0;
+            s_work = 0;
         }
         __syncthreads();
         // 2c. sweep B: queue every candidate (supp and its neighbours) with |r64| >= thr
-        for (unsigned int base = 0; base < S32; base += NT) {
-            const unsigned int i = base + tid;
-            int d = 0, st = 0;
+        auto check = [&](int vtx, double) -> bool {
+            NodeRec &rv = rec[vtx];
+            if (fabs(rv.r64) >= P.thr_scale * (double)rv.d) return (fl_or(&rv.fl, kInQ) & kInQ) == 0;
+            return false;
+        };
+        for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
+            const unsigned int i = i0 + tid;
+            bool enq = false;
+            int u = 0;
             if (i < S32) {
-                const int u = sup[i];
-                d = rec[u].d + 1;      // the node itself + its row
-                st = indptr[u];
-                s_val[tid] = (double)u;
+                u = sup[i];
+                enq = check(u, 0.0);
             }
-            int tot;
-            const int incl = block_incl_scan<NT>(d, s_scan, &tot);
-            s_incl[tid] = incl;
-            s_start[tid] = st;
-            __syncthreads();
-            cert_ops += (unsigned long long)tot;
-            for (int e0 = 0; e0 < tot; e0 += NT) {
-                const int e = e0 + tid;
-                bool enq = false;
-                int vtx = 0;
-                if (e < tot) {
-                    const int k = upper_bound_lds(s_incl, 0, NT, e);
-                    const int o = e - (k ? s_incl[k - 1] : 0);
-                    vtx = (o == 0) ? (int)s_val[k] : indices[s_start[k] + o - 1];
-                    NodeRec &rv = rec[vtx];
-                    if (fabs(rv.r64) >= P.thr_scale * (double)rv.d) enq = (fl_or(&rv.fl, kInQ) & kInQ) == 0;
-                }
-                const long long slot = wave_append(enq, &s_cnt);
-                if (slot >= 0) qc[slot] = vtx;
-            }
-            __syncthreads();
+            const long long slot = wave_append(enq, &s_cnt);
+            if (slot >= 0) qc[slot] = u;
         }
-        // 2d. FP64 correction push
+        wave_edges(
+            S32, pcs, npc_sup, pw, &s_work, W, indices,
+            [&](unsigned int i, int &st, int &d, double &v) {
+                const int u = sup[i];
+                st = indptr[u];
+                d = rec[u].d;
+                v = 1.0;
+            },
+            check, qc, &s_cnt);
+        __syncthreads();
+        cert_ops *= 2;
+        // 2d. FP64 correction push (mark-based queue)
         unsigned long long n_corr = 0;
         int64_t corr_iters = 0;
         for (;;) {
@@ -347,8 +664,14 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             }
             corr_iters++;
             __syncthreads();
-            if (tid == 0) s_ncnt = 0;
-            unsigned long long pushed = 0;
+            if (tid == 0) {
+                s_ncnt = 0;
+                s_npc = 0;
+                s_work = 0;
+                s_vol = 0;
+            }
+            __syncthreads();
+            unsigned long long pushed = 0, cvol = 0;
             for (unsigned int i = tid; i < cnt; i += NT) {
                 const int u = qc[i];
                 NodeRec &ru = rec[u];
@@ -361,53 +684,42 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     ru.x64 = ru.x64 + tmp;
                     if ((fl & kSeen) == 0) {
                         fl_or(&ru.fl, kSeen);
-                        const unsigned int pos = atomicAdd(&s_sup, 1u);
-                        sup[pos] = u;
+                        sup[atomicAdd(&s_sup, 1u)] = u;
                     }
                     v = bpush_value(P, tmp, d);
                     pushed += 1;
+                    cvol += (unsigned long long)d;
+                    if (d > kHubW) {
+                        const unsigned int np = (unsigned int)((d + pw - 1) / pw);
+                        const unsigned int b = atomicAdd(&s_npc, np);
+                        for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
+                    }
                 }
                 vq[i] = v;
             }
-            n_corr += block_sum_u64<NT>(pushed, s_red);
-            for (unsigned int base = 0; base < cnt; base += NT) {
-                const unsigned int i = base + tid;
-                int d = 0, st = 0;
-                double v = 0.0;
-                if (i < cnt) {
+            if (pushed) atomicAdd(&s_vol, pushed);
+            cert_ops += cvol;
+            __syncthreads();
+            n_corr += s_vol;
+            wave_edges(
+                cnt, pcs, s_npc, pw, &s_work, W, indices,
+                [&](unsigned int i, int &st, int &d, double &v) {
                     v = vq[i];
                     if (v != 0.0) {
                         const int u = qc[i];
-                        d = rec[u].d;
                         st = indptr[u];
+                        d = rec[u].d;
                     }
-                }
-                int tot;
-                const int incl = block_incl_scan<NT>(d, s_scan, &tot);
-                s_incl[tid] = incl;
-                s_start[tid] = st;
-                s_val[tid] = v;
-                __syncthreads();
-                cert_ops += (unsigned long long)tot;
-                for (int e0 = 0; e0 < tot; e0 += NT) {
-                    const int e = e0 + tid;
-                    bool enq = false;
-                    int vtx = 0;
-                    if (e < tot) {
-                        const int k = upper_bound_lds(s_incl, 0, NT, e);
-                        const int o = e - (k ? s_incl[k - 1] : 0);
-                        vtx = indices[s_start[k] + o];
-                        const double val = s_val[k];
-                        NodeRec &rv = rec[vtx];
-                        const double old = atomic_add_wg(&rv.r64, val);
-                        const double nw = old + val;
-                        if (fabs(nw) >= P.thr_scale * (double)rv.d) enq = (fl_or(&rv.fl, kInQ) & kInQ) == 0;
-                    }
-                    const long long slot = wave_append(enq, &s_ncnt);
-                    if (slot >= 0) qn[slot] = vtx;
-                }
-                __syncthreads();
-            }
+                },
+                [&](int vtx, double val) -> bool {
+                    NodeRec &rv = rec[vtx];
+                    const double old = atomic_add_wg(&rv.r64, val);
+                    const double nw = old + val;
+                    if (fabs(nw) >= P.thr_scale * (double)rv.d) return (fl_or(&rv.fl, kInQ) & kInQ) == 0;
+                    return false;
+                },
+                qn, &s_ncnt);
+            __syncthreads();
             if (tid == 0) s_cnt = s_ncnt;
             int32_t *t = qc;
             qc = qn;
@@ -417,51 +729,58 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         __syncthreads();
         const unsigned int S = s_sup;
         // ---------------- 3. sweep C: output, certificate value and reset
-        if (tid == 0) s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
+        if (tid == 0) {
+            s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
+            s_npc = 0;
+            s_work = 0;
+        }
         __syncthreads();
+        build_pieces<NT>(S, pw, [&](unsigned int i) { return rec[sup[i]].d; }, pcs, &s_npc);
         const unsigned long long obase = s_base;
         const bool fits = obase + S <= (unsigned long long)a.out_cap;
         double cmax = 0.0;
-        for (unsigned int base = 0; base < S; base += NT) {
-            const unsigned int i = base + tid;
-            int d = 0, st = 0;
-            if (i < S) {
-                const int u = sup[i];
-                d = rec[u].d + 1;
-                st = indptr[u];
-                s_val[tid] = (double)u;
-                if (fits) {
-                    a.out_ids[obase + i] = a.out_map ? a.out_map[u] : u;
-                    a.out_vals[obase + i] = rec[u].x64;
-                }
+        // a node visited more than once: the first visit reads its residual,
+        // later ones see zero
+        auto visit = [&](int vtx, double) -> bool {
+            NodeRec &rv = rec[vtx];
+            const double r64 = rv.r64;
+            if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)rv.d));
+            rv.r64 = 0.0;
+            rv.fl = 0u;
+            return false;
+        };
+        for (unsigned int i = tid; i < S; i += NT) {
+            const int u = sup[i];
+            if (fits) {
+                a.out_ids[obase + i] = a.out_map ? a.out_map[u] : u;
+                a.out_vals[obase + i] = rec[u].x64;
             }
-            int tot;
-            const int incl = block_incl_scan<NT>(d, s_scan, &tot);
-            s_incl[tid] = incl;
-            s_start[tid] = st;
-            __syncthreads();
-            cert_ops += (unsigned long long)tot;
-            // a neighbour shared by several support nodes is visited more than
-            // once: the first visit reads its residual, later ones see zero
-            for (int e = tid; e < tot; e += NT) {
-                const int k = upper_bound_lds(s_incl, 0, NT, e);
-                const int o = e - (k ? s_incl[k - 1] : 0);
-                NodeRec &rv = rec[(o == 0) ? (int)s_val[k] : indices[s_start[k] + o - 1]];
-                const double r64 = rv.r64;
-                if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)rv.d));
-                rv.r64 = 0.0;
-                rv.fl = 0u;
-            }
-            __syncthreads();
         }
+        __syncthreads();
+        for (unsigned int i = tid; i < S; i += NT) {
+            const int u = sup[i];
+            cert_ops += (unsigned long long)rec[u].d;
+            visit(u, 0.0);
+        }
+        wave_edges(
+            S, pcs, s_npc, pw, &s_work, W, indices,
+            [&](unsigned int i, int &st, int &d, double &v) {
+                const int u = sup[i];
+                st = indptr[u];
+                d = rec[u].d;
+                v = 1.0;
+            },
+            visit, qn, &s_ncnt);
+        __syncthreads();
         for (unsigned int i = tid; i < S; i += NT) rec[sup[i]].x64 = 0.0;
         cmax = block_max<NT>(cmax, s_redd);
+        const unsigned long long cops = block_sum_u64<NT>(cert_ops, s_red);
         if (tid == 0) {
             const uint64_t t_end = now_ticks();
             flsg_result rr{};
             rr.oper = (int64_t)oper;
             rr.iters = iters;
-            rr.oper_cert = (int64_t)cert_ops;
+            rr.oper_cert = (int64_t)cops + 2 * (int64_t)S;
             rr.n_corr = (int64_t)n_corr;
             rr.supp = (int64_t)S;
             rr.nodes = (int64_t)fnodes;
@@ -517,6 +836,7 @@ void batch_free(flsg_graph g) {
     hipFree(A->q1);
     hipFree(A->vq);
     hipFree(A->sup);
+    hipFree(A->pcs);
     hipFree(A->ctl);
     hipFree(A->d_res);
     hipFree(A->d_q);
@@ -525,8 +845,8 @@ void batch_free(flsg_graph g) {
 }
 
 int64_t slot_bytes(int64_t n) {
-    // 32 B node record + q0 q1 sup (4 B) + vq (8 B)
-    return n * (32 + 3 * 4 + 8);
+    // 32 B node record + q0 q1 sup pcs (4 B) + vq (8 B)
+    return n * (32 + 4 * 4 + 8);
 }
 
 static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
@@ -543,6 +863,7 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     FLSG_HIP(hipMalloc((void **)&A->q1, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->vq, sizeof(double) * ns));
     FLSG_HIP(hipMalloc((void **)&A->sup, sizeof(int32_t) * ns));
+    FLSG_HIP(hipMalloc((void **)&A->pcs, sizeof(uint32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
     hipLaunchKernelGGL(k_slot_init, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree, g->n,
                        (int64_t)ns);
@@ -564,7 +885,7 @@ static int32_t default_slots(flsg_graph g) {
 static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_base, int32_t n_slots,
                   int32_t block, const int32_t *d_out_map, int32_t *d_out_ids, double *d_out_vals,
                   int64_t *d_out_off, int64_t out_cap, uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters,
-                  hipStream_t st) {
+                  const flsg_hk_config *hk, hipStream_t st) {
     BatchArena *A = g->arena;
     if (!A) return fail(FLSG_EARG, "no batch arena: call flsg_batch_reserve first");
     if (slot_base < 0 || n_slots <= 0 || slot_base + n_slots > A->slots)
@@ -581,6 +902,12 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.q1 = A->q1;
     a.vq = A->vq;
     a.sup = A->sup;
+    a.pcs = A->pcs;
+    // hub pieces are encoded (entry << 8) | piece: entries < 2^24, < 256 pieces per node
+    if (g->n >= (1 << 24)) return fail(FLSG_EARG, "batched solves need n < 2^24");
+    int pw = 1024;
+    while ((int64_t)pw * 255 < (int64_t)g->d_max_hint) pw <<= 1;
+    a.piece_w = pw;
     a.slot_base = slot_base;
     a.ctl = d_ctl;
     a.out_map = d_out_map;
@@ -590,6 +917,16 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.out_cap = out_cap;
     a.res = d_res;
     a.max_iters = max_iters > 0 ? max_iters : 20000;
+    if (hk) {
+        if (hk->tau < 1.0 || !(hk->eps > 0.0)) return fail(FLSG_EARG, "heat kernel: tau >= 1 and eps > 0");
+        const int64_t N = hk->n_terms > 0 ? hk->n_terms : host_estimate_n(hk->tau, .5 * hk->eps);   // hk_solver.py:225
+        if (N > kMaxHkTerms) return fail(FLSG_EARG, "heat kernel: Taylor order above the batched cap (96)");
+        a.hk_n = (int32_t)N;
+        a.hk_tau = hk->tau;
+        a.hk_ete = exp(hk->tau) * hk->eps;
+        a.hk_psi[N] = 1.;
+        for (int64_t i = N - 1; i >= 0; i--) a.hk_psi[i] = a.hk_psi[i + 1] * hk->tau / ((double)i + 1.) + 1.;
+    }
     const int grid = n_slots;
     if (block == 1024) hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), 0, st, a);
     else if (block == 256) hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), 0, st, a);
@@ -649,7 +986,7 @@ extern "C" int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, i
 extern "C" int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queries, int32_t slot_base,
                                  int32_t n_slots, int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off,
                                  int64_t out_cap, uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters,
-                                 void *stream) {
+                                 const flsg_hk_config *hk, void *stream) {
     int rc = check_graph(g);
     if (rc) return rc;
     if (!d_queries || !d_ctl || !d_res || !d_out_off || (out_cap > 0 && (!d_out_ids || !d_out_vals)))
@@ -657,7 +994,7 @@ extern "C" int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int6
     if (n_queries <= 0) return FLSG_OK;
     if (n_queries >= (int64_t)UINT32_MAX / 2) return fail(FLSG_EARG, "too many queries");
     return launch(g, d_queries, n_queries, slot_base, n_slots, g->batch_block, g->out_map, d_out_ids, d_out_vals,
-                  d_out_off, out_cap, d_ctl, d_res, max_iters, (hipStream_t)stream);
+                  d_out_off, out_cap, d_ctl, d_res, max_iters, hk, (hipStream_t)stream);
 }
 
 extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_t *d_seeds, int64_t n_queries,
@@ -691,7 +1028,7 @@ extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_
                        p->eps);
     const int32_t used = (int32_t)(n_queries < slots ? n_queries : slots);
     rc = launch(g, A->d_q, n_queries, 0, used, g->batch_block, g->out_map, d_out_ids, d_out_vals, d_out_off, out_cap,
-                A->ctl, A->d_res, p->max_iters, st);
+                A->ctl, A->d_res, p->max_iters, nullptr, st);
     if (rc) return rc;
     FLSG_HIP(hipMemcpyAsync(h_res, A->d_res, sizeof(flsg_result) * n_queries, hipMemcpyDeviceToHost, st));
     FLSG_HIP(hipStreamSynchronize(st));

@@ -54,6 +54,7 @@ struct flsg_graph_s {
     int32_t *degree = nullptr;
     int32_t slack = 0;
     int64_t m_live = 0;             // sum of degree (the reference's m)
+    int64_t d_max_hint = 0;         // largest row capacity (bounds every live degree)
     int device = 0;
     int num_cus = 0;
     // single-query workspace (lazily allocated)
@@ -87,6 +88,7 @@ namespace flsg {
 
 int ensure_single_workspace(flsg_graph g);
 int recompute_m(flsg_graph g, hipStream_t st);
+int64_t host_estimate_n(double t, double eps);
 int check_graph(flsg_graph g);
 void batch_free(flsg_graph g);
 void cheb_free(flsg_graph g);

@@ -39,6 +39,10 @@ class Query(C.Structure):
                 ("eps", C.c_double)]
 
 
+class HkConfig(C.Structure):
+    _fields_ = [("tau", C.c_double), ("eps", C.c_double), ("n_terms", C.c_int64)]
+
+
 class Trace(C.Structure):
     _fields_ = [
         ("d_nodes", C.c_void_p), ("cap_nodes", C.c_int64), ("h_off", C.c_void_p), ("cap_iters", C.c_int64),
@@ -68,7 +72,7 @@ SIGNATURES = {
     "flsg_graph_set_out_map": (C.c_int, [C.c_void_p, C.c_void_p]),
     "flsg_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
-                                    C.c_void_p]),
+                                    C.c_void_p, C.c_void_p]),
     "flsg_batch_refresh_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "flsg_graph_insert_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                           C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),

@@ -168,11 +168,20 @@ int flsg_graph_set_out_map(flsg_graph g, const int32_t *d_out_map);
 /* One batched query for flsg_batch_launch. */
 typedef struct {
     int64_t seed;
-    int32_t kind;        /* FLSG_PPR or FLSG_KATZ (local GD)                  */
+    int32_t kind;        /* FLSG_PPR / FLSG_KATZ (local GD) / FLSG_HK         */
     int32_t pad;
-    double alpha;        /* PPR alpha / Katz beta                             */
-    double eps;
+    double alpha;        /* PPR alpha / Katz beta (unused for HK)             */
+    double eps;          /* unused for HK (the launch's flsg_hk_config)        */
 } flsg_query;
+
+/* Heat-kernel parameters shared by the HK queries of one batched launch
+ * (FP32 levels, FP64 thresholds; no certificate -- the level push drops
+ * sub-threshold residual by design, hk_solver.py:282-285). */
+typedef struct {
+    double tau;
+    double eps;
+    int64_t n_terms;     /* <= 0: estimate_n(tau, eps/2); at most 96          */
+} flsg_hk_config;
 
 /* Asynchronous batched launch over slots [slot_base, slot_base + n_slots) of
  * the reserved arena: queries may mix PPR and Katz and their parameters.
@@ -183,7 +192,8 @@ typedef struct {
  * while a launch runs. */
 int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queries, int32_t slot_base,
                       int32_t n_slots, int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off, int64_t out_cap,
-                      uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters, void *stream);
+                      uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters, const flsg_hk_config *hk,
+                      void *stream);
 
 /* After flsg_graph_insert_edges on a handle with a slot arena: refresh the
  * degree copy the slots keep for the listed nodes. */
