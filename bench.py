@@ -51,6 +51,7 @@ def main():
     ap.add_argument("--plain-queries", type=int, default=32)
     ap.add_argument("--cpu-sample", type=int, default=0, help="seeds timed on the CPU (0 = 2 per core)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-batched", type=int, default=1, help="also time the batched FP64 push (1) or not (0)")
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--block", type=int, default=0, help="batched workgroup size (256/512/1024; 0 = default)")
     ap.add_argument("--no-relabel", action="store_true", help="keep the generator's node ids on the device")
@@ -123,6 +124,18 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    # ------------------------------------------------------------ batched FP64 push (same launch, FP64 records)
+    b64_qps = None
+    if args.fp64_batched:
+        bo64 = engine.batch_solve(dg, p64, seeds, slots=args.slots, block=args.block)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        bo64 = engine.batch_solve(dg, p64, seeds, slots=args.slots, block=args.block)
+        torch.cuda.synchronize()
+        b64_qps = args.nq / (time.perf_counter() - tb)
+        ok64 = all(r["status"] == 0 and r["cert"] < 1.0 for r in bo64.results)
+        log(f"batched FP64: {b64_qps:.1f} q/s (certified {ok64}, kernel {bo64.ms:.1f} ms)")
+
     # ------------------------------------------------------------ plain FP64 single-source
     nplain = min(args.plain_queries, args.nq)
     plain_qps = None
@@ -181,6 +194,7 @@ def main():
         "certified_all": certified,
         "max_cert": max(r["cert"] for r in res),
         "plain_fp64_qps": plain_qps,
+        "batched_fp64_qps": b64_qps,
         "speedup_vs_plain_fp64": (qps / plain_qps) if plain_qps else None,
         "mean_oper": oper / args.nq,
         "mean_corrections": sum(r["n_corr"] for r in res) / args.nq,

@@ -40,6 +40,7 @@ namespace flsg {
 constexpr int kMaxHkTerms = 96;      // heat-kernel Taylor order cap of a batched launch
 constexpr unsigned kInQ = 1u;        // flags: in the correction queue
 constexpr unsigned kSeen = 2u;       // flags: listed in the support
+constexpr unsigned kTouched = 4u;    // flags: in the query's touched list (reset at the end)
 
 struct __attribute__((aligned(32))) NodeRec {
     float r;            // residual (FP32 push); heat kernel: even-level residual
@@ -63,6 +64,7 @@ struct BatchArena {
     double *vq = nullptr;        // slots * n
     int32_t *sup = nullptr;      // slots * n
     uint32_t *pcs = nullptr;     // slots * n: hub pieces of the current list
+    int32_t *tl = nullptr;       // slots * n: records touched by the current query
     uint64_t *ctl = nullptr;     // launch control for the synchronous entry point
     flsg_result *d_res = nullptr;
     int64_t d_res_cap = 0;
@@ -82,6 +84,7 @@ struct BatchArgs {
     double *vq;
     int32_t *sup;
     uint32_t *pcs;
+    int32_t *tl;
     int32_t piece_w;             // edges per hub piece
     int32_t slot_base;
     uint64_t *ctl;               // [0] next query (low 32 bits used), [1] output entries used
@@ -101,12 +104,14 @@ struct BatchArgs {
 
 struct QParams {
     int kind;
+    bool fp64;                   // FP64 push (certificate by construction) instead of FP32 + recompute
     double alpha, om_alpha, thr_scale, r0;
 };
 
 __device__ __forceinline__ QParams qparams(const flsg_query &q) {
     QParams p;
     p.kind = q.kind;
+    p.fp64 = q.precision == FLSG_FP64;
     p.alpha = q.alpha;
     p.om_alpha = 1.0 - q.alpha;
     p.thr_scale = q.kind == FLSG_PPR ? q.eps * q.alpha : q.eps;   // ppr_solver.py:221 / katz_solver.py:179
@@ -346,11 +351,13 @@ __device__ __forceinline__ void wave_sync() {
 // contribute nothing there) or one piece of a hub entry (pcs[], built by the
 // caller: (entry << 8) | piece).  Each lane keeps kUnroll edges in flight.
 //   load(i, st, d, v): row start, degree (0 = nothing to push) and value of entry i
-//   edge(vtx, v) -> bool: process one edge; true appends vtx to next[] via *s_next
+//   edge(vtx, v) -> int: process one edge; bit 0 appends vtx to next[] (counter
+//   *s_next), bit 1 appends it to tl[] (counter *s_tl)
 template <typename Load, typename Edge>
 __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w,
                                            unsigned int *s_work, WaveLds &W, const int32_t *indices, Load load,
-                                           Edge edge, int32_t *next, unsigned int *s_next) {
+                                           Edge edge, int32_t *next, unsigned int *s_next, int32_t *tl,
+                                           unsigned int *s_tl) {
     const int lane = __lane_id();
     const unsigned int ngroups = (cnt + kWave - 1) / kWave;
     const unsigned int nitems = ngroups + npc;
@@ -359,10 +366,11 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
         if (lane == 0) item = atomicAdd(s_work, 1u);
         item = __shfl(item, 0);
         if (item >= nitems) break;
-        if (item < ngroups) {
+        int st = 0, d = 0, beg = 0, end = 0, tot = 0;
+        double v = 0.0;
+        const bool group = item < ngroups;
+        if (group) {
             const unsigned int i = item * kWave + lane;
-            int st = 0, d = 0;
-            double v = 0.0;
             if (i < cnt) {
                 load(i, st, d, v);
                 if (d > kHubW) d = 0;
@@ -373,65 +381,57 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
                 const int y = __shfl_up(incl, o);
                 if (lane >= o) incl += y;
             }
-            const int tot = __shfl(incl, kWave - 1);
+            tot = __shfl(incl, kWave - 1);
             W.incl[lane] = incl;
             W.start[lane] = st;
             W.val[lane] = v;
             wave_sync();
-            for (int e0 = 0; e0 < tot; e0 += kWave * kUnroll) {
-                int vtx[kUnroll];
-                double vv[kUnroll];
-                bool act[kUnroll];
+            end = tot;
+        } else {
+            const uint32_t pc = pcs[item - ngroups];
+            load(pc >> 8, st, d, v);
+            beg = (int)(pc & 255u) * piece_w;
+            end = min(d, beg + piece_w);
+        }
+        for (int e0 = beg; e0 < end; e0 += kWave * kUnroll) {
+            int vtx[kUnroll];
+            double vv[kUnroll];
+            bool act[kUnroll];
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    const int e = e0 + u * kWave + lane;
-                    act[u] = e < tot;
-                    vtx[u] = 0;
-                    vv[u] = 0.0;
-                    if (act[u]) {
+            for (int u = 0; u < kUnroll; u++) {
+                const int e = e0 + u * kWave + lane;
+                act[u] = e < end;
+                vtx[u] = 0;
+                vv[u] = v;
+                if (act[u]) {
+                    if (group) {
                         const int k = upper_bound_lds(W.incl, 0, kWave, e);
                         const int o = e - (k ? W.incl[k - 1] : 0);
                         vtx[u] = indices[W.start[k] + o];
                         vv[u] = W.val[k];
+                    } else {
+                        vtx[u] = indices[st + e];
                     }
                 }
-                bool cr[kUnroll];
-#pragma unroll
-                for (int u = 0; u < kUnroll; u++) cr[u] = act[u] ? edge(vtx[u], vv[u]) : false;
-#pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    const long long slot = wave_append(cr[u], s_next);
-                    if (slot >= 0) next[slot] = vtx[u];
-                }
             }
-            wave_sync();
-        } else {
-            const uint32_t pc = pcs[item - ngroups];
-            int st = 0, d = 0;
-            double v = 0.0;
-            load(pc >> 8, st, d, v);
-            const int beg = (int)(pc & 255u) * piece_w;
-            const int end = min(d, beg + piece_w);
-            for (int e0 = beg; e0 < end; e0 += kWave * kUnroll) {
-                int vtx[kUnroll];
-                bool act[kUnroll];
+            int cr[kUnroll];
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    const int e = e0 + u * kWave + lane;
-                    act[u] = e < end;
-                    vtx[u] = act[u] ? indices[st + e] : 0;
-                }
-                bool cr[kUnroll];
+            for (int u = 0; u < kUnroll; u++) cr[u] = act[u] ? edge(vtx[u], vv[u]) : 0;
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++) cr[u] = act[u] ? edge(vtx[u], v) : false;
-#pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    const long long slot = wave_append(cr[u], s_next);
-                    if (slot >= 0) next[slot] = vtx[u];
-                }
+            for (int u = 0; u < kUnroll; u++) {
+                const long long slot = wave_append((cr[u] & 1) != 0, s_next);
+                if (slot >= 0) next[slot] = vtx[u];
+                const long long ts = wave_append((cr[u] & 2) != 0, s_tl);
+                if (ts >= 0) tl[ts] = vtx[u];
             }
         }
+        if (group) wave_sync();
     }
+}
+
+// first touch of a record in this query: list it for the final reset
+__device__ __forceinline__ int touch(NodeRec &rv) {
+    return (fl_or(&rv.fl, kTouched) & kTouched) ? 0 : 2;
 }
 
 // hub pieces of the entries [0, cnt) of a list whose degree is given by deg(i)
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __shared__ WaveLds s_w[NT / kWave];
     __shared__ unsigned long long s_red[NT / kWave];
     __shared__ double s_redd[NT / kWave];
-    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_tnn, s_work, s_npc;
+    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_tnn, s_work, s_npc, s_tl;
     __shared__ unsigned long long s_base, s_vol;
     // the heat-kernel path keeps its block-chunked form
     __shared__ int s_incl[NT];
@@ -470,6 +470,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     double *vq = a.vq + off;
     int32_t *sup = a.sup + off;
     uint32_t *pcs = a.pcs + off;
+    int32_t *tl = a.tl + off;
     const int32_t *indptr = a.indptr;
     const int32_t *indices = a.indices;
     const int pw = a.piece_w;
@@ -490,10 +491,14 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const QParams P = qparams(qq);
         const int s = (int)qq.seed;
         if (tid == 0) {
-            rec[s].r = (float)P.r0;
+            if (P.fp64) rec[s].r64 = P.r0;
+            else rec[s].r = (float)P.r0;
+            rec[s].fl = kTouched;
+            tl[0] = s;
             qa[0] = s;
             s_cnt = 1;
             s_sup = 0;
+            s_tl = 1;
             s_vol = 0;
         }
         __syncthreads();
@@ -502,7 +507,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         int64_t iters = 0;
         int status = FLSG_OK;
 
-        // ---------------- 1. FP32 local push
+        // ---------------- 1. local push (ppr_solver.py:246-288, katz_solver.py:183-224)
         for (;;) {
             const unsigned int cnt = s_cnt;
             if (cnt == 0) break;
@@ -534,12 +539,23 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     const unsigned int i = i0 + j * NT;
                     NodeRec &ru = rec[uu[j]];
                     const int d = ru.d;
-                    const float tmp = ru.r;
-                    ru.r = 0.0f;
-                    const float xo = ru.x;
-                    ru.x = xo + tmp;
-                    if (xo == 0.0f) sup[atomicAdd(&s_sup, 1u)] = uu[j];
-                    vq[i] = (double)(float)bpush_value(P, (double)tmp, d);
+                    bool first;
+                    if (P.fp64) {
+                        const double tmp = ru.r64;
+                        ru.r64 = 0.0;
+                        const double xo = ru.x64;
+                        ru.x64 = xo + tmp;
+                        first = xo == 0.0;
+                        vq[i] = bpush_value(P, tmp, d);
+                    } else {
+                        const float tmp = ru.r;
+                        ru.r = 0.0f;
+                        const float xo = ru.x;
+                        ru.x = xo + tmp;
+                        first = xo == 0.0f;
+                        vq[i] = (double)(float)bpush_value(P, (double)tmp, d);
+                    }
+                    if (first) sup[atomicAdd(&s_sup, 1u)] = uu[j];
                     vol += (unsigned long long)d;
                     if (d > kHubW) {
                         const unsigned int np = (unsigned int)((d + pw - 1) / pw);
@@ -551,23 +567,36 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             if (vol) atomicAdd(&s_vol, vol);
             __syncthreads();
             // edge phase: v joins the next frontier iff this atomic moved r_v across thr_v
-            wave_edges(
-                cnt, pcs, s_npc, pw, &s_work, W, indices,
-                [&](unsigned int i, int &st, int &d, double &v) {
-                    const int u = qc[i];
-                    st = indptr[u];
-                    d = rec[u].d;
-                    v = vq[i];
-                },
-                [&](int vtx, double v) -> bool {
-                    const float val = (float)v;
-                    NodeRec &rv = rec[vtx];
-                    const float old = atomic_add_wg(&rv.r, val);
-                    const float nw = old + val;
-                    const double thr = P.thr_scale * (double)rv.d;
-                    return (double)old < thr && thr <= (double)nw;
-                },
-                qn, &s_ncnt);
+            auto load = [&](unsigned int i, int &st, int &d, double &v) {
+                const int u = qc[i];
+                st = indptr[u];
+                d = rec[u].d;
+                v = vq[i];
+            };
+            if (P.fp64) {
+                wave_edges(
+                    cnt, pcs, s_npc, pw, &s_work, W, indices, load,
+                    [&](int vtx, double val) -> int {
+                        NodeRec &rv = rec[vtx];
+                        const double old = atomic_add_wg(&rv.r64, val);
+                        const double nw = old + val;
+                        const double thr = P.thr_scale * (double)rv.d;
+                        return ((old < thr && thr <= nw) ? 1 : 0) | (old == 0.0 ? touch(rv) : 0);
+                    },
+                    qn, &s_ncnt, tl, &s_tl);
+            } else {
+                wave_edges(
+                    cnt, pcs, s_npc, pw, &s_work, W, indices, load,
+                    [&](int vtx, double v) -> int {
+                        const float val = (float)v;
+                        NodeRec &rv = rec[vtx];
+                        const float old = atomic_add_wg(&rv.r, val);
+                        const float nw = old + val;
+                        const double thr = P.thr_scale * (double)rv.d;
+                        return (((double)old < thr && thr <= (double)nw) ? 1 : 0) | (old == 0.0f ? touch(rv) : 0);
+                    },
+                    qn, &s_ncnt, tl, &s_tl);
+            }
             __syncthreads();
             if (tid == 0) s_cnt = s_ncnt;
             int32_t *t = qc;
@@ -578,177 +607,161 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         __syncthreads();
         const unsigned long long oper = s_vol;
         const uint64_t t_push = now_ticks();
-        const unsigned int S32 = s_sup;
         unsigned long long cert_ops = 0;
-        if (tid == 0) {
-            s_npc = 0;
-            s_work = 0;
-        }
-        __syncthreads();
-        // pieces of the support's hubs (sweeps A and B)
-        build_pieces<NT>(S32, pw, [&](unsigned int i) { return rec[sup[i]].d; }, pcs, &s_npc);
-        __syncthreads();
-        const unsigned int npc_sup = s_npc;
-        // ---------------- 2a. sweep A: r64 = pushes of x32 over supp; reset r32
-        wave_edges(
-            S32, pcs, npc_sup, pw, &s_work, W, indices,
-            [&](unsigned int i, int &st, int &d, double &v) {
-                const int u = sup[i];
-                st = indptr[u];
-                d = rec[u].d;
-                v = bpush_value(P, (double)rec[u].x, d);
-            },
-            [&](int vtx, double v) -> bool {
-                NodeRec &rv = rec[vtx];
-                atomic_add_wg(&rv.r64, v);
-                rv.r = 0.0f;
-                return false;
-            },
-            qn, &s_ncnt);
-        __syncthreads();
-        // 2b. r64_u -= x_u on the support; x64 = x32
-        for (unsigned int i = tid; i < S32; i += NT) {
-            NodeRec &ru = rec[sup[i]];
-            const double xd = (double)ru.x;
-            ru.r64 = ru.r64 - xd;
-            ru.x64 = xd;
-            ru.x = 0.0f;
-            ru.fl = kSeen;
-            cert_ops += (unsigned long long)ru.d;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            rec[s].r64 = rec[s].r64 + P.r0;
-            rec[s].r = 0.0f;
-            s_cnt = 0;
-            s_work = 0;
-        }
-        __syncthreads();
-        // 2c. sweep B: queue every candidate (supp and its neighbours) with |r64| >= thr
-        auto check = [&](int vtx, double) -> bool {
-            NodeRec &rv = rec[vtx];
-            if (fabs(rv.r64) >= P.thr_scale * (double)rv.d) return (fl_or(&rv.fl, kInQ) & kInQ) == 0;
-            return false;
-        };
-        for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
-            const unsigned int i = i0 + tid;
-            bool enq = false;
-            int u = 0;
-            if (i < S32) {
-                u = sup[i];
-                enq = check(u, 0.0);
-            }
-            const long long slot = wave_append(enq, &s_cnt);
-            if (slot >= 0) qc[slot] = u;
-        }
-        wave_edges(
-            S32, pcs, npc_sup, pw, &s_work, W, indices,
-            [&](unsigned int i, int &st, int &d, double &v) {
-                const int u = sup[i];
-                st = indptr[u];
-                d = rec[u].d;
-                v = 1.0;
-            },
-            check, qc, &s_cnt);
-        __syncthreads();
-        cert_ops *= 2;
-        // 2d. FP64 correction push (mark-based queue)
         unsigned long long n_corr = 0;
-        int64_t corr_iters = 0;
-        for (;;) {
-            const unsigned int cnt = s_cnt;
-            if (cnt == 0) break;
-            if (corr_iters >= a.max_iters) {
-                status = FLSG_ENOCONV;
-                break;
-            }
-            corr_iters++;
-            __syncthreads();
+        if (!P.fp64) {
+            // ---------------- 2. FP64 certificate of the FP32 result
+            const unsigned int S32 = s_sup;
             if (tid == 0) {
-                s_ncnt = 0;
                 s_npc = 0;
                 s_work = 0;
-                s_vol = 0;
             }
             __syncthreads();
-            unsigned long long pushed = 0, cvol = 0;
-            for (unsigned int i = tid; i < cnt; i += NT) {
-                const int u = qc[i];
-                NodeRec &ru = rec[u];
-                const int d = ru.d;
-                const uint32_t fl = fl_and(&ru.fl, ~kInQ);
-                const double tmp = ru.r64;
-                double v = 0.0;
-                if (fabs(tmp) >= P.thr_scale * (double)d) {
-                    ru.r64 = 0.0;
-                    ru.x64 = ru.x64 + tmp;
-                    if ((fl & kSeen) == 0) {
-                        fl_or(&ru.fl, kSeen);
-                        sup[atomicAdd(&s_sup, 1u)] = u;
-                    }
-                    v = bpush_value(P, tmp, d);
-                    pushed += 1;
-                    cvol += (unsigned long long)d;
-                    if (d > kHubW) {
-                        const unsigned int np = (unsigned int)((d + pw - 1) / pw);
-                        const unsigned int b = atomicAdd(&s_npc, np);
-                        for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
-                    }
-                }
-                vq[i] = v;
-            }
-            if (pushed) atomicAdd(&s_vol, pushed);
-            cert_ops += cvol;
+            build_pieces<NT>(S32, pw, [&](unsigned int i) { return rec[sup[i]].d; }, pcs, &s_npc);
             __syncthreads();
-            n_corr += s_vol;
+            const unsigned int npc_sup = s_npc;
+            // 2a. sweep A: r64 = pushes of x32 over supp (every target is already touched)
             wave_edges(
-                cnt, pcs, s_npc, pw, &s_work, W, indices,
+                S32, pcs, npc_sup, pw, &s_work, W, indices,
                 [&](unsigned int i, int &st, int &d, double &v) {
-                    v = vq[i];
-                    if (v != 0.0) {
-                        const int u = qc[i];
-                        st = indptr[u];
-                        d = rec[u].d;
+                    const int u = sup[i];
+                    st = indptr[u];
+                    d = rec[u].d;
+                    v = bpush_value(P, (double)rec[u].x, d);
+                },
+                [&](int vtx, double v) -> int {
+                    atomic_add_wg(&rec[vtx].r64, v);
+                    return 0;
+                },
+                qn, &s_ncnt, tl, &s_tl);
+            __syncthreads();
+            // 2b. r64_u -= x_u on the support; x64 = x32
+            for (unsigned int i = tid; i < S32; i += NT) {
+                NodeRec &ru = rec[sup[i]];
+                const double xd = (double)ru.x;
+                ru.r64 = ru.r64 - xd;
+                ru.x64 = xd;
+                fl_or(&ru.fl, kSeen);
+                cert_ops += 2ull * (unsigned long long)ru.d + 1ull;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                rec[s].r64 = rec[s].r64 + P.r0;
+                s_cnt = 0;
+                s_work = 0;
+            }
+            __syncthreads();
+            // 2c. sweep B: queue every candidate (supp and its neighbours) with |r64| >= thr
+            auto check = [&](int vtx, double) -> int {
+                NodeRec &rv = rec[vtx];
+                if (fabs(rv.r64) >= P.thr_scale * (double)rv.d) return (fl_or(&rv.fl, kInQ) & kInQ) == 0 ? 1 : 0;
+                return 0;
+            };
+            for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
+                const unsigned int i = i0 + tid;
+                bool enq = false;
+                int u = 0;
+                if (i < S32) {
+                    u = sup[i];
+                    enq = check(u, 0.0) != 0;
+                }
+                const long long slot = wave_append(enq, &s_cnt);
+                if (slot >= 0) qc[slot] = u;
+            }
+            wave_edges(
+                S32, pcs, npc_sup, pw, &s_work, W, indices,
+                [&](unsigned int i, int &st, int &d, double &v) {
+                    const int u = sup[i];
+                    st = indptr[u];
+                    d = rec[u].d;
+                    v = 1.0;
+                },
+                check, qc, &s_cnt, tl, &s_tl);
+            __syncthreads();
+            // 2d. FP64 correction push (mark-based queue: the recomputed residual has both signs)
+            int64_t corr_iters = 0;
+            for (;;) {
+                const unsigned int cnt = s_cnt;
+                if (cnt == 0) break;
+                if (corr_iters >= a.max_iters) {
+                    status = FLSG_ENOCONV;
+                    break;
+                }
+                corr_iters++;
+                __syncthreads();
+                if (tid == 0) {
+                    s_ncnt = 0;
+                    s_npc = 0;
+                    s_work = 0;
+                    s_vol = 0;
+                }
+                __syncthreads();
+                unsigned long long pushed = 0, cvol = 0;
+                for (unsigned int i = tid; i < cnt; i += NT) {
+                    const int u = qc[i];
+                    NodeRec &ru = rec[u];
+                    const int d = ru.d;
+                    const uint32_t fl = fl_and(&ru.fl, ~kInQ);
+                    const double tmp = ru.r64;
+                    double v = 0.0;
+                    if (fabs(tmp) >= P.thr_scale * (double)d) {
+                        ru.r64 = 0.0;
+                        ru.x64 = ru.x64 + tmp;
+                        if ((fl & kSeen) == 0) {
+                            fl_or(&ru.fl, kSeen);
+                            sup[atomicAdd(&s_sup, 1u)] = u;
+                        }
+                        v = bpush_value(P, tmp, d);
+                        pushed += 1;
+                        cvol += (unsigned long long)d;
+                        if (d > kHubW) {
+                            const unsigned int np = (unsigned int)((d + pw - 1) / pw);
+                            const unsigned int b = atomicAdd(&s_npc, np);
+                            for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
+                        }
                     }
-                },
-                [&](int vtx, double val) -> bool {
-                    NodeRec &rv = rec[vtx];
-                    const double old = atomic_add_wg(&rv.r64, val);
-                    const double nw = old + val;
-                    if (fabs(nw) >= P.thr_scale * (double)rv.d) return (fl_or(&rv.fl, kInQ) & kInQ) == 0;
-                    return false;
-                },
-                qn, &s_ncnt);
-            __syncthreads();
-            if (tid == 0) s_cnt = s_ncnt;
-            int32_t *t = qc;
-            qc = qn;
-            qn = t;
+                    vq[i] = v;
+                }
+                if (pushed) atomicAdd(&s_vol, pushed);
+                cert_ops += cvol;
+                __syncthreads();
+                n_corr += s_vol;
+                wave_edges(
+                    cnt, pcs, s_npc, pw, &s_work, W, indices,
+                    [&](unsigned int i, int &st, int &d, double &v) {
+                        v = vq[i];
+                        if (v != 0.0) {
+                            const int u = qc[i];
+                            st = indptr[u];
+                            d = rec[u].d;
+                        }
+                    },
+                    [&](int vtx, double val) -> int {
+                        NodeRec &rv = rec[vtx];
+                        const double old = atomic_add_wg(&rv.r64, val);
+                        const double nw = old + val;
+                        // an untouched record has r64 == 0, so only such edges can be a first touch
+                        int res = old == 0.0 ? touch(rv) : 0;
+                        if (fabs(nw) >= P.thr_scale * (double)rv.d && (fl_or(&rv.fl, kInQ) & kInQ) == 0) res |= 1;
+                        return res;
+                    },
+                    qn, &s_ncnt, tl, &s_tl);
+                __syncthreads();
+                if (tid == 0) s_cnt = s_ncnt;
+                int32_t *t = qc;
+                qc = qn;
+                qn = t;
+                __syncthreads();
+            }
             __syncthreads();
         }
-        __syncthreads();
         const unsigned int S = s_sup;
-        // ---------------- 3. sweep C: output, certificate value and reset
-        if (tid == 0) {
-            s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
-            s_npc = 0;
-            s_work = 0;
-        }
+        const unsigned int T = s_tl;
+        // ---------------- 3. output, certificate value, reset of every touched record
+        if (tid == 0) s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
         __syncthreads();
-        build_pieces<NT>(S, pw, [&](unsigned int i) { return rec[sup[i]].d; }, pcs, &s_npc);
         const unsigned long long obase = s_base;
         const bool fits = obase + S <= (unsigned long long)a.out_cap;
-        double cmax = 0.0;
-        // a node visited more than once: the first visit reads its residual,
-        // later ones see zero
-        auto visit = [&](int vtx, double) -> bool {
-            NodeRec &rv = rec[vtx];
-            const double r64 = rv.r64;
-            if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)rv.d));
-            rv.r64 = 0.0;
-            rv.fl = 0u;
-            return false;
-        };
         for (unsigned int i = tid; i < S; i += NT) {
             const int u = sup[i];
             if (fits) {
@@ -757,22 +770,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             }
         }
         __syncthreads();
-        for (unsigned int i = tid; i < S; i += NT) {
-            const int u = sup[i];
-            cert_ops += (unsigned long long)rec[u].d;
-            visit(u, 0.0);
+        double cmax = 0.0;
+        for (unsigned int i = tid; i < T; i += NT) {
+            NodeRec &rv = rec[tl[i]];
+            const double r64 = rv.r64;
+            if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)rv.d));
+            const int d = rv.d;
+            NodeRec z{};
+            z.d = d;
+            rv = z;
         }
-        wave_edges(
-            S, pcs, s_npc, pw, &s_work, W, indices,
-            [&](unsigned int i, int &st, int &d, double &v) {
-                const int u = sup[i];
-                st = indptr[u];
-                d = rec[u].d;
-                v = 1.0;
-            },
-            visit, qn, &s_ncnt);
-        __syncthreads();
-        for (unsigned int i = tid; i < S; i += NT) rec[sup[i]].x64 = 0.0;
         cmax = block_max<NT>(cmax, s_redd);
         const unsigned long long cops = block_sum_u64<NT>(cert_ops, s_red);
         if (tid == 0) {
@@ -780,7 +787,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             flsg_result rr{};
             rr.oper = (int64_t)oper;
             rr.iters = iters;
-            rr.oper_cert = (int64_t)cops + 2 * (int64_t)S;
+            rr.oper_cert = (int64_t)cops;
             rr.n_corr = (int64_t)n_corr;
             rr.supp = (int64_t)S;
             rr.nodes = (int64_t)fnodes;
@@ -816,12 +823,13 @@ __global__ void k_slot_degrees(NodeRec *rec, const int32_t *degree, int64_t n, i
     }
 }
 
-__global__ void k_fill_queries(flsg_query *q, const int64_t *seeds, int64_t nq, int32_t kind, double alpha, double eps) {
+__global__ void k_fill_queries(flsg_query *q, const int64_t *seeds, int64_t nq, int32_t kind, double alpha, double eps,
+                               int32_t precision) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
         flsg_query x;
         x.seed = seeds[i];
         x.kind = kind;
-        x.pad = 0;
+        x.precision = precision;
         x.alpha = alpha;
         x.eps = eps;
         q[i] = x;
@@ -837,6 +845,7 @@ void batch_free(flsg_graph g) {
     hipFree(A->vq);
     hipFree(A->sup);
     hipFree(A->pcs);
+    hipFree(A->tl);
     hipFree(A->ctl);
     hipFree(A->d_res);
     hipFree(A->d_q);
@@ -845,8 +854,8 @@ void batch_free(flsg_graph g) {
 }
 
 int64_t slot_bytes(int64_t n) {
-    // 32 B node record + q0 q1 sup pcs (4 B) + vq (8 B)
-    return n * (32 + 4 * 4 + 8);
+    // 32 B node record + q0 q1 sup pcs tl (4 B) + vq (8 B)
+    return n * (32 + 5 * 4 + 8);
 }
 
 static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
@@ -864,6 +873,7 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     FLSG_HIP(hipMalloc((void **)&A->vq, sizeof(double) * ns));
     FLSG_HIP(hipMalloc((void **)&A->sup, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->pcs, sizeof(uint32_t) * ns));
+    FLSG_HIP(hipMalloc((void **)&A->tl, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
     hipLaunchKernelGGL(k_slot_init, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree, g->n,
                        (int64_t)ns);
@@ -903,6 +913,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.vq = A->vq;
     a.sup = A->sup;
     a.pcs = A->pcs;
+    a.tl = A->tl;
     // hub pieces are encoded (entry << 8) | piece: entries < 2^24, < 256 pieces per node
     if (g->n >= (1 << 24)) return fail(FLSG_EARG, "batched solves need n < 2^24");
     int pw = 1024;
@@ -1008,7 +1019,7 @@ extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_
     if (n_queries >= (int64_t)UINT32_MAX / 2) return fail(FLSG_EARG, "too many queries");
     if (p->kind != FLSG_PPR && p->kind != FLSG_KATZ) return fail(FLSG_EARG, "batched solves: PPR or Katz");
     if (p->method != FLSG_GD || !p->local) return fail(FLSG_EARG, "batched solves: local gradient descent");
-    if (p->precision != FLSG_FP32) return fail(FLSG_EARG, "batched solves run the FP32 + FP64-certificate path");
+    if (p->precision != FLSG_FP32 && p->precision != FLSG_FP64) return fail(FLSG_EARG, "unknown precision");
     hipStream_t st = (hipStream_t)stream;
     const int32_t slots = g->batch_slots > 0 ? g->batch_slots : default_slots(g);
     rc = ensure_arena(g, slots, st);
@@ -1025,7 +1036,7 @@ extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_
         A->d_q_cap = n_queries;
     }
     hipLaunchKernelGGL(k_fill_queries, dim3(64), dim3(256), 0, st, A->d_q, d_seeds, n_queries, p->kind, p->alpha,
-                       p->eps);
+                       p->eps, p->precision);
     const int32_t used = (int32_t)(n_queries < slots ? n_queries : slots);
     rc = launch(g, A->d_q, n_queries, 0, used, g->batch_block, g->out_map, d_out_ids, d_out_vals, d_out_off, out_cap,
                 A->ctl, A->d_res, p->max_iters, nullptr, st);

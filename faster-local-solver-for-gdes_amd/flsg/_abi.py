@@ -35,7 +35,7 @@ class Result(C.Structure):
 
 
 class Query(C.Structure):
-    _fields_ = [("seed", C.c_int64), ("kind", C.c_int32), ("pad", C.c_int32), ("alpha", C.c_double),
+    _fields_ = [("seed", C.c_int64), ("kind", C.c_int32), ("precision", C.c_int32), ("alpha", C.c_double),
                 ("eps", C.c_double)]
 
 

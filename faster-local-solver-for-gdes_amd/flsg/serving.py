@@ -1,0 +1,344 @@
+"""Serving loop: continuous batching of PPR / Katz / heat-kernel queries on a
+graph held resident in HBM, with in-place edge-insert snapshots.
+
+The reference has no serving path; its per-query GPU functions copy the whole
+graph to the device on every call (ppr_solver.py:101-107).  Here:
+
+* the graph (relabelled by degree, stored as a slack CSR with per-row headroom
+  for inserts) and a slot arena stay resident;
+* the slot arena is split into lanes -- each lane has its own HIP stream, slot
+  range, query / result / output buffers -- so launches on different lanes run
+  concurrently on the device;
+* requests that arrive while lanes are busy wait in a queue and are admitted
+  into the next free lane's launch as long as the lane's output arena can hold
+  their expected footprint (a per-kind support estimate, raised whenever a
+  query overflows; an overflowing query is re-queued);
+* edge-insert snapshots are applied between launches: the loop stops admitting,
+  lets in-flight launches finish, inserts on the device and refreshes the slots'
+  degree copies;
+* latency = completion (the host observes the lane's completion event) minus
+  arrival; p50 / p99 and queries/s are reported for the FP32-batched path and
+  for the plain FP64 single-source path serving the same arrival stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .engine import DeviceGraph, _check, _ptr, make_params, relabel_by_degree
+from .graph import CSRGraph, SlackGraph
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: int                 # FLSG_PPR / FLSG_KATZ / FLSG_HK
+    seed: int                 # caller node id
+    arrival: float            # seconds from the start of the run
+    alpha: float = 0.1        # PPR alpha / Katz beta
+    eps: float = 1e-7
+    done: float = -1.0
+    status: int = -1
+    oper: int = 0
+    supp: int = 0
+    cert: float = 0.0
+    device_ms: float = 0.0
+    result: tuple | None = None   # (ids, values) when collected
+
+
+@dataclass
+class InsertBatch:
+    at: float                 # seconds from the start of the run
+    us: np.ndarray
+    vs: np.ndarray
+    applied: float = -1.0
+    inserted: int = 0
+
+
+def mixed_requests(g: CSRGraph, rate: float, count: int, seed: int = 0, d_max: int | None = None,
+                   eps: float = 1e-7, alpha: float = 0.1, mix=(0.7, 0.2, 0.1)) -> list:
+    """Poisson arrivals at `rate`/s: PPR (alpha, eps) 70%, Katz
+    (beta = 1/(d_max+1), eps) 20%, heat kernel 10%, seeded uniformly at
+    non-isolated nodes."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    pool = np.flatnonzero(g.degree >= 1)
+    seeds = pool[rng.integers(0, pool.size, size=count)]
+    kinds = rng.choice([_abi.FLSG_PPR, _abi.FLSG_KATZ, _abi.FLSG_HK], size=count, p=list(mix))
+    beta = 1.0 / ((d_max if d_max is not None else g.d_max) + 1)
+    out = []
+    for i in range(count):
+        k = int(kinds[i])
+        out.append(Request(rid=i, kind=k, seed=int(seeds[i]), arrival=float(t[i]),
+                           alpha=alpha if k == _abi.FLSG_PPR else beta, eps=eps))
+    return out
+
+
+def random_insert_batches(g: CSRGraph, every: float, size: int, count: int, seed: int = 1) -> list:
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        us = rng.integers(0, g.n, size=size).astype(np.int32)
+        vs = rng.integers(0, g.n, size=size).astype(np.int32)
+        out.append(InsertBatch(at=every * (b + 1), us=us, vs=vs))
+    return out
+
+
+def slack_with_headroom(g: CSRGraph, headroom: int) -> SlackGraph:
+    """Slack CSR (data_process_GNN.py:92-119) holding g's rows with `headroom`
+    free slots per row for in-place inserts."""
+    cap = g.degree.astype(np.int64) + headroom
+    indptr = np.zeros(g.n + 1, dtype=np.int64)
+    np.cumsum(cap, out=indptr[1:])
+    indices = np.full(int(indptr[-1]), g.n, dtype=np.int32)
+    rows = np.repeat(np.arange(g.n, dtype=np.int64), g.degree)
+    pos = indptr[rows] + (np.arange(rows.size) - np.repeat(g.indptr[:-1], g.degree))
+    indices[pos] = g.indices
+    return SlackGraph(n=g.n, indptr=indptr, indices=indices, degree=g.degree.astype(np.int32).copy(),
+                      us=np.zeros((0, 0), np.int32), vs=np.zeros((0, 0), np.int32), name=g.name + "-slack")
+
+
+class _Lane:
+    def __init__(self, srv: "Server", idx: int, slot_base: int, n_slots: int):
+        torch = srv.torch
+        dev = srv.dg.device
+        self.idx = idx
+        self.slot_base = slot_base
+        self.n_slots = n_slots
+        self.stream = torch.cuda.Stream(device=dev)
+        self.qcap = srv.max_batch
+        self.h_q = (_abi.Query * self.qcap)()
+        self.d_q = torch.empty(self.qcap * C.sizeof(_abi.Query), dtype=torch.uint8, device=dev)
+        self.d_res = torch.empty(self.qcap * C.sizeof(_abi.Result), dtype=torch.uint8, device=dev)
+        self.h_res = torch.empty(self.qcap * C.sizeof(_abi.Result), dtype=torch.uint8).pin_memory()
+        self.d_off = torch.empty(self.qcap, dtype=torch.int64, device=dev)
+        self.h_off = torch.empty(self.qcap, dtype=torch.int64).pin_memory()
+        self.d_ctl = torch.zeros(2, dtype=torch.int64, device=dev)
+        self.out_cap = srv.out_per_lane
+        self.ids = torch.empty(self.out_cap, dtype=torch.int32, device=dev)
+        self.vals = torch.empty(self.out_cap, dtype=torch.float64, device=dev)
+        self.busy = False
+        self.event = None
+        self.reqs: list = []
+
+
+class Server:
+    def __init__(self, g: CSRGraph, lanes: int = 4, slots: int = 0, block: int = 1024, headroom: int = 8,
+                 max_batch: int = 256, out_per_lane: int | None = None, hk_tau: float = 5.0,
+                 hk_eps: float = 1e-7, hk_terms: int = 40, precision: int = _abi.FLSG_FP32,
+                 device: str = "cuda:0"):
+        h, perm, iperm = relabel_by_degree(g)
+        self.g = g
+        self.precision = precision
+        self.iperm = iperm
+        self.perm = perm
+        sg = slack_with_headroom(h, headroom)
+        self.dg = DeviceGraph(sg, device)
+        torch = self.dg.torch
+        self.torch = torch
+        self.d_perm = torch.from_numpy(perm.astype(np.int32)).to(self.dg.device)
+        lib = self.dg.lib
+        self.lib = lib
+        _check(lib.flsg_graph_set_out_map(self.dg.handle, _ptr(self.d_perm)))
+        _check(lib.flsg_batch_set_block(self.dg.handle, int(block)))
+        if slots <= 0:
+            nc = C.c_int32(0)
+            free = C.c_int64(0)
+            tot = C.c_int64(0)
+            _check(lib.flsg_device_info(C.byref(nc), C.byref(free), C.byref(tot)))
+            slots = int(nc.value)
+        self.slots = slots
+        _check(lib.flsg_batch_reserve(self.dg.handle, int(slots), None))
+        torch.cuda.synchronize()
+        self.max_batch = max_batch
+        self.out_per_lane = out_per_lane or max(1 << 20, (g.n // 2) * max(1, slots // lanes))
+        per = slots // lanes
+        self.lanes = [_Lane(self, i, i * per, per if i < lanes - 1 else slots - i * per) for i in range(lanes)]
+        self.hk = _abi.HkConfig(tau=hk_tau, eps=hk_eps, n_terms=hk_terms)
+        # per-kind support estimate used for admission (raised on overflow)
+        self.est = {_abi.FLSG_PPR: min(g.n, 400_000), _abi.FLSG_KATZ: min(g.n, 50_000),
+                    _abi.FLSG_HK: min(g.n, 300_000)}
+        self.update_stream = torch.cuda.Stream(device=self.dg.device)
+
+    # ------------------------------------------------------------------ plumbing
+    def _launch(self, lane: _Lane, reqs: list, t0: float):
+        torch = self.torch
+        for j, r in enumerate(reqs):
+            q = lane.h_q[j]
+            q.seed = int(self.iperm[r.seed])
+            q.kind = r.kind
+            q.precision = self.precision
+            q.alpha = r.alpha
+            q.eps = r.eps
+        nbytes = len(reqs) * C.sizeof(_abi.Query)
+        src = torch.frombuffer(bytearray(bytes(lane.h_q)[:nbytes]), dtype=torch.uint8)
+        with torch.cuda.stream(lane.stream):
+            lane.d_q[:nbytes].copy_(src, non_blocking=False)
+            _check(self.lib.flsg_batch_launch(self.dg.handle, _ptr(lane.d_q), len(reqs), lane.slot_base,
+                                              min(lane.n_slots, len(reqs)), _ptr(lane.ids), _ptr(lane.vals),
+                                              _ptr(lane.d_off), lane.out_cap, _ptr(lane.d_ctl), _ptr(lane.d_res),
+                                              0, C.byref(self.hk), C.c_void_p(lane.stream.cuda_stream)))
+            rb = len(reqs) * C.sizeof(_abi.Result)
+            lane.h_res[:rb].copy_(lane.d_res[:rb], non_blocking=True)
+            lane.h_off[:len(reqs)].copy_(lane.d_off[:len(reqs)], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(lane.stream)
+        lane.event = ev
+        lane.busy = True
+        lane.reqs = reqs
+
+    def _complete(self, lane: _Lane, now: float, collect: bool, requeue: deque) -> int:
+        res = (_abi.Result * len(lane.reqs)).from_buffer_copy(
+            lane.h_res[:len(lane.reqs) * C.sizeof(_abi.Result)].numpy().tobytes())
+        offs = lane.h_off[:len(lane.reqs)].numpy()
+        finished = 0
+        for j, r in enumerate(lane.reqs):
+            rr = res[j]
+            if rr.status == _abi.FLSG_ECAP:
+                self.est[r.kind] = max(self.est[r.kind], int(rr.supp * 1.25) + 1)
+                requeue.append(r)
+                continue
+            finished += 1
+            r.done = now
+            r.status = rr.status
+            r.oper = rr.oper
+            r.supp = rr.supp
+            r.cert = rr.cert
+            r.device_ms = rr.ms
+            self.est[r.kind] = max(self.est[r.kind], int(rr.supp * 1.1) + 1)
+            if collect and rr.status == _abi.FLSG_OK:
+                o, k = int(offs[j]), int(rr.supp)
+                r.result = (lane.ids[o:o + k].cpu().numpy(), lane.vals[o:o + k].cpu().numpy())
+        lane.busy = False
+        lane.reqs = []
+        return finished
+
+    def _apply_insert(self, ins: InsertBatch):
+        torch = self.torch
+        lib = self.lib
+        dev = self.dg.device
+        us = torch.from_numpy(self.iperm[ins.us].astype(np.int32)).to(dev)
+        vs = torch.from_numpy(self.iperm[ins.vs].astype(np.int32)).to(dev)
+        keys = torch.empty(max(2 * us.numel(), 1), dtype=torch.int32, device=dev)
+        nk, ni = C.c_int64(0), C.c_int64(0)
+        with torch.cuda.stream(self.update_stream):
+            _check(lib.flsg_graph_insert_edges(self.dg.handle, _ptr(us), _ptr(vs), int(us.numel()), _ptr(keys),
+                                               C.byref(nk), None, C.byref(ni),
+                                               C.c_void_p(self.update_stream.cuda_stream)))
+            _check(lib.flsg_batch_refresh_degrees(self.dg.handle, _ptr(keys), nk.value,
+                                                  C.c_void_p(self.update_stream.cuda_stream)))
+        self.update_stream.synchronize()
+        ins.inserted = int(ni.value)
+
+    # ------------------------------------------------------------------ loop
+    def run(self, requests: list, inserts: list | None = None, collect: bool = False,
+            time_scale: float = 1.0) -> dict:
+        """Serve `requests` (arrival times in seconds, replayed in real time
+        scaled by time_scale) and apply `inserts` at their times."""
+        inserts = sorted(inserts or [], key=lambda b: b.at)
+        order = sorted(requests, key=lambda r: r.arrival)
+        pending: deque = deque()
+        requeue: deque = deque()
+        nxt = 0
+        ins_i = 0
+        self.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        total = len(order)
+        launches = 0
+        while done < total or ins_i < len(inserts):
+            now = (time.perf_counter() - t0) / time_scale
+            while nxt < total and order[nxt].arrival <= now:
+                pending.append(order[nxt])
+                nxt += 1
+            for lane in self.lanes:
+                if lane.busy and lane.event.query():
+                    done += self._complete(lane, now, collect, requeue)
+            while requeue:
+                pending.appendleft(requeue.pop())
+            due = ins_i < len(inserts) and inserts[ins_i].at <= now
+            if due:
+                if any(l.busy for l in self.lanes):
+                    continue          # drain before mutating the graph
+                self._apply_insert(inserts[ins_i])
+                inserts[ins_i].applied = (time.perf_counter() - t0) / time_scale
+                ins_i += 1
+                continue
+            if not pending:
+                if nxt >= total and not any(l.busy for l in self.lanes) and ins_i >= len(inserts):
+                    break
+                time.sleep(0)
+                continue
+            for lane in self.lanes:
+                if lane.busy or not pending:
+                    continue
+                batch, budget = [], lane.out_cap
+                while pending and len(batch) < lane.qcap:
+                    need = self.est[pending[0].kind]
+                    if batch and need > budget:
+                        break
+                    budget -= need
+                    batch.append(pending.popleft())
+                self._launch(lane, batch, t0)
+                launches += 1
+        wall = time.perf_counter() - t0
+        return self._stats(order, wall, launches, inserts)
+
+    @staticmethod
+    def _stats(order: list, wall: float, launches: int, inserts: list) -> dict:
+        lat = np.array([r.done - r.arrival for r in order if r.done >= 0])
+        ok = [r for r in order if r.status == _abi.FLSG_OK]
+        certified = all(r.cert < 1.0 for r in ok if r.kind != _abi.FLSG_HK)
+        span = max(r.done for r in order) - min(r.arrival for r in order) if order else 0.0
+        return {
+            "queries": len(order), "ok": len(ok), "launches": launches, "wall_s": wall,
+            "qps": len(ok) / span if span > 0 else 0.0,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
+            "device_ms_mean": float(np.mean([r.device_ms for r in ok])) if ok else None,
+            "certified": certified,
+            "inserts_applied": sum(1 for b in inserts if b.applied >= 0),
+            "edges_inserted": sum(b.inserted for b in inserts),
+        }
+
+    def close(self):
+        self.dg.close()
+
+
+def run_plain(g: CSRGraph, requests: list, hk_tau: float = 5.0, hk_eps: float = 1e-7, hk_terms: int = 40,
+              device: str = "cuda:0", dg: DeviceGraph | None = None, time_scale: float = 1.0) -> dict:
+    """The same arrival stream served one query at a time by the plain FP64
+    single-source path (whole GPU per query, FIFO)."""
+    from .engine import solve
+    own = dg is None
+    dg = dg or DeviceGraph(g, device, relabel=True)
+    torch = dg.torch
+    order = sorted(requests, key=lambda r: r.arrival)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in order:
+        now = (time.perf_counter() - t0) / time_scale
+        if r.arrival > now:
+            time.sleep((r.arrival - now) * time_scale)
+        if r.kind == _abi.FLSG_HK:
+            p = make_params(_abi.FLSG_HK, _abi.FLSG_GD, True, _abi.FLSG_FP64, eps=hk_eps, tau=hk_tau,
+                            n_terms=hk_terms)
+        else:
+            p = make_params(r.kind, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=r.alpha, eps=r.eps)
+        out = solve(dg, p, r.seed)
+        torch.cuda.synchronize()
+        r.done = (time.perf_counter() - t0) / time_scale
+        r.status = 0
+        r.oper = out.result["oper"]
+        r.device_ms = out.result["ms"]
+    wall = time.perf_counter() - t0
+    if own:
+        dg.close()
+    st = Server._stats(order, wall, len(order), [])
+    st["certified"] = True
+    return st

@@ -169,7 +169,9 @@ int flsg_graph_set_out_map(flsg_graph g, const int32_t *d_out_map);
 typedef struct {
     int64_t seed;
     int32_t kind;        /* FLSG_PPR / FLSG_KATZ (local GD) / FLSG_HK         */
-    int32_t pad;
+    int32_t precision;   /* PPR / Katz: FLSG_FP32 (FP32 push, FP64 threshold
+                            tests, FP64 recompute + correction push) or FLSG_FP64
+                            (FP64 push); HK levels are FP32 either way       */
     double alpha;        /* PPR alpha / Katz beta (unused for HK)             */
     double eps;          /* unused for HK (the launch's flsg_hk_config)        */
 } flsg_query;

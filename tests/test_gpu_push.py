@@ -219,6 +219,35 @@ def test_batch_katz_certified_small(small_graph, dev_small):
         assert np.abs(x - o["x"]).sum() <= 1e-5 * max(np.abs(o["x"]).sum(), 1e-12) + 1e-9
 
 
+@pytest.mark.parametrize("kind", [_abi.FLSG_PPR, _abi.FLSG_KATZ])
+def test_batch_fp64_parity_small(small_graph, dev_small, kind):
+    """Batched FP64 push: the same frontier semantics as the plain FP64 path,
+    so the operation count equals the oracle's and x agrees to rel L1 1e-12
+    (only the order of the atomic sums differs)."""
+    g = small_graph
+    if kind == _abi.FLSG_PPR:
+        a, eps = ALPHA, 1e-6
+    else:
+        a, eps = 1.0 / (g.d_max + 1), 1e-7
+    seeds = graph.mixed_seeds(g, 48, seed=13)
+    p = engine.make_params(kind, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=a, eps=eps)
+    bo = engine.batch_solve(dev_small, p, seeds)
+    for q in range(len(seeds)):
+        s = int(seeds[q])
+        res = bo.results[q]
+        assert res["status"] == 0 and res["cert"] < 1.0 and res["n_corr"] == 0
+        x = bo.dense(q, g.n)
+        if kind == _abi.FLSG_PPR:
+            o = O.ppr_gd(g, s, a, eps, local=True)
+        else:
+            o = O.katz_gd(g, s, a, eps, local=True)
+            x[s] -= 1.0   # katz_gd returns x - e_s
+        assert res["oper"] == o["oper"], f"query {q}"
+        assert _rel_l1(x, o["x"]) <= REL_L1, f"query {q}: {_rel_l1(x, o['x'])}"
+    if kind == _abi.FLSG_PPR:
+        _check_batch(g, seeds, bo, eps, n_check=8)
+
+
 @pytest.fixture(scope="module")
 def dev_large(large_graph):
     return engine.DeviceGraph(large_graph)
@@ -247,6 +276,21 @@ def test_batch_certified_large(large_graph, dev_large):
     bo = engine.batch_solve(dev_large, p, seeds)
     assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo.results)
     _check_batch(g, seeds, bo, eps, n_check=16)
+
+
+def test_batch_fp64_large(large_graph, dev_large):
+    """Full size, eps = 1e-7: 256 mixed seeds through the batched FP64 push,
+    every query certified on the device, 3 against the oracle."""
+    g = large_graph
+    eps = 1e-7
+    seeds = graph.mixed_seeds(g, 256, seed=2)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=eps)
+    bo = engine.batch_solve(dev_large, p, seeds)
+    assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo.results)
+    for q in range(3):
+        o = O.ppr_gd(g, int(seeds[q]), ALPHA, eps, local=True)
+        assert bo.results[q]["oper"] == o["oper"]
+        assert _rel_l1(bo.dense(q, g.n), o["x"]) <= REL_L1
 
 
 def test_ppr_local_cheb_exact_small(small_graph, dev_small):

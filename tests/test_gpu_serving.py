@@ -1,0 +1,112 @@
+"""Serving loop: continuous batching on the GPU.
+
+Bars: every PPR / Katz result is certified (|r_v| < thr_v in FP64, recomputed
+on the host on the graph current at its launch); a query's result is the same
+alone and in the mix -- for PPR / Katz within the two certified residuals
+(||x_a - x_b||_1 <= (||r_a||_1 + ||r_b||_1) / alpha), for the heat kernel
+within FP32 rounding (relative L1 1e-5); edge-insert snapshots land in place.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from flsg import _abi, graph, serving  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def _dense(r, n):
+    x = np.zeros(n)
+    ids, vals = r.result
+    x[ids] = vals
+    return x
+
+
+def _katz_residual(g, s, beta, x):
+    A = O.adjacency(g)
+    es = np.zeros(g.n)
+    es[s] = 1.0
+    return es - (x - beta * (A @ x))
+
+
+@pytest.fixture(scope="module")
+def srv_small(small_graph):
+    srv = serving.Server(small_graph, lanes=3, slots=24, block=256, max_batch=32)
+    yield srv
+    srv.close()
+
+
+def test_serving_mix_certified_and_same_alone(small_graph, srv_small):
+    g = small_graph
+    eps = 1e-6
+    reqs = serving.mixed_requests(g, rate=2000.0, count=120, seed=3, eps=eps)
+    st = srv_small.run(reqs, collect=True)
+    assert st["ok"] == len(reqs) and st["certified"]
+    assert st["launches"] >= 2
+    for r in reqs:
+        x = _dense(r, g.n)
+        if r.kind == _abi.FLSG_PPR:
+            rr = O.ppr_residual(g, r.seed, r.alpha, x)
+            assert (np.abs(rr) / (eps * r.alpha * np.maximum(g.degree, 1))).max() < 1 + 1e-9
+        elif r.kind == _abi.FLSG_KATZ:
+            rr = _katz_residual(g, r.seed, r.alpha, x)
+            assert (np.abs(rr) / (eps * np.maximum(g.degree, 1))).max() < 1 + 1e-9
+    # alone vs in the mix
+    picks = {k: next(r for r in reqs if r.kind == k) for k in (_abi.FLSG_PPR, _abi.FLSG_KATZ, _abi.FLSG_HK)}
+    for k, r in picks.items():
+        alone = serving.Request(rid=0, kind=r.kind, seed=r.seed, arrival=0.0, alpha=r.alpha, eps=r.eps)
+        srv_small.run([alone], collect=True)
+        a, b = _dense(alone, g.n), _dense(r, g.n)
+        if k == _abi.FLSG_HK:
+            assert np.abs(a - b).sum() <= 1e-5 * np.abs(a).sum()
+        else:
+            res = O.ppr_residual if k == _abi.FLSG_PPR else _katz_residual
+            ra = np.abs(res(g, r.seed, r.alpha, a)).sum()
+            rb = np.abs(res(g, r.seed, r.alpha, b)).sum()
+            scale = r.alpha if k == _abi.FLSG_PPR else 1.0 - r.alpha * g.d_max
+            assert np.abs(a - b).sum() <= (ra + rb) / scale * (1 + 1e-9) + 1e-15
+
+
+def test_serving_hk_matches_oracle(small_graph, srv_small):
+    """Batched FP32 heat kernel vs the oracle's FP64 level push (same N, same
+    thresholds): equal up to FP32 rounding."""
+    g = small_graph
+    reqs = [serving.Request(rid=i, kind=_abi.FLSG_HK, seed=int(s), arrival=0.0)
+            for i, s in enumerate(graph.uniform_seeds(g, 6, seed=8))]
+    srv_small.run(reqs, collect=True)
+    for r in reqs:
+        o = O.hk_push(g, r.seed, 5.0, 1e-7, local=True, n_terms=40)
+        x = _dense(r, g.n)
+        assert np.abs(x - o["x"]).sum() <= 1e-5 * np.abs(o["x"]).sum()
+        assert abs(r.oper - o["oper"]) <= 0.02 * o["oper"] + 50
+
+
+def test_serving_inserts_in_place(small_graph):
+    g = small_graph
+    srv = serving.Server(g, lanes=2, slots=16, block=256, max_batch=16, headroom=6)
+    try:
+        eps = 1e-6
+        ins = serving.random_insert_batches(g, every=0.01, size=300, count=2, seed=5)
+        reqs = serving.mixed_requests(g, rate=3000.0, count=60, seed=4, eps=eps, mix=(1.0, 0.0, 0.0))
+        st = srv.run(reqs, inserts=ins, collect=True)
+        assert st["inserts_applied"] == 2 and st["edges_inserted"] > 0
+        # queries served after the last insert are certified on the final graph
+        u = np.concatenate([g.indices.astype(np.int64)] + [b.us for b in ins] + [b.vs for b in ins])
+        rows = np.concatenate([np.repeat(np.arange(g.n), g.degree)] + [b.vs for b in ins] + [b.us for b in ins])
+        g2 = graph.CSRGraph.from_edges(g.n, rows, u)
+        last = max(b.applied for b in ins)
+        late = [r for r in reqs if r.arrival > last + 1e-3 and r.done > 0]
+        for r in late[:10]:
+            x = _dense(r, g.n)
+            rr = O.ppr_residual(g2, r.seed, r.alpha, x)
+            assert (np.abs(rr) / (eps * r.alpha * np.maximum(g2.degree, 1))).max() < 1 + 1e-9
+        ind = srv.dg.indices.cpu().numpy()
+        deg = srv.dg.degree.cpu().numpy()
+        assert int(deg.sum()) == g2.m
+    finally:
+        srv.close()
