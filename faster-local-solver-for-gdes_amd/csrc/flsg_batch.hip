@@ -41,6 +41,7 @@ constexpr int kMaxHkTerms = 96;      // heat-kernel Taylor order cap of a batche
 constexpr unsigned kInQ = 1u;        // flags: in the correction queue
 constexpr unsigned kSeen = 2u;       // flags: listed in the support
 constexpr unsigned kTouched = 4u;    // flags: in the query's touched list (reset at the end)
+constexpr int kSink = 64;            // per-slot sink records (d = 0) for the idle lanes of an edge batch
 
 struct __attribute__((aligned(32))) NodeRec {
     float r;            // residual (FP32 push); heat kernel: even-level residual
@@ -55,13 +56,21 @@ struct __attribute__((aligned(32))) NodeRec {
 };
 static_assert(sizeof(NodeRec) == 32, "node record must be 32 bytes");
 
+// A frontier entry as its node phase leaves it for the edge phase: row start,
+// edge count (0 = nothing to push) and push value -- one 16-byte load.
+struct __attribute__((aligned(16))) Ent {
+    int32_t st;
+    int32_t d;
+    double v;
+};
+
 struct BatchArena {
     int64_t n = 0;
     int32_t slots = 0;
-    NodeRec *rec = nullptr;      // slots * n
+    NodeRec *rec = nullptr;      // slots * (n + kSink): per slot n records, then kSink sink records
     int32_t *q0 = nullptr;       // slots * n
     int32_t *q1 = nullptr;
-    double *vq = nullptr;        // slots * n
+    Ent *ent = nullptr;          // slots * n: frontier entries (heat kernel: two touched lists)
     int32_t *sup = nullptr;      // slots * n
     uint32_t *pcs = nullptr;     // slots * n: hub pieces of the current list
     int32_t *tl = nullptr;       // slots * n: records touched by the current query
@@ -75,13 +84,14 @@ struct BatchArena {
 struct BatchArgs {
     const int32_t *indptr;
     const int32_t *indices;
+    const int2 *colx;            // {v, degree[v]} per column entry
     int64_t n;
     const flsg_query *queries;   // per-query (seed, kind, alpha, eps)
     int64_t nq;
-    NodeRec *rec;
+    NodeRec *rec;                // slot stride n + kSink
     int32_t *q0;
     int32_t *q1;
-    double *vq;
+    Ent *ent;
     int32_t *sup;
     uint32_t *pcs;
     int32_t *tl;
@@ -165,14 +175,14 @@ __device__ __forceinline__ uint32_t fl_and(uint32_t *p, uint32_t v) {
 // hk_solver.py:282-285); the per-entry push value is parked in rec.x64.
 template <int NT>
 __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, NodeRec *rec, int32_t *qa,
-                         int32_t *qb, double *vq, int32_t *sup, int *s_incl, int *s_start, double *s_val,
+                         int32_t *qb, int32_t *tls, int32_t *sup, int *s_incl, int *s_start, double *s_val,
                          int *s_scan, unsigned long long *s_red, unsigned int *s_cnt, unsigned int *s_ncnt,
                          unsigned int *s_sup, unsigned int *s_tn, unsigned int *s_tnn, unsigned long long *s_base,
                          uint64_t t_start) {
     const int tid = threadIdx.x;
     const int N = a.hk_n;
-    int32_t *tl_cur = (int32_t *)vq;          // nodes with a residual at the current level
-    int32_t *tl_nxt = (int32_t *)vq + a.n;    // nodes touched for the next level
+    int32_t *tl_cur = tls;                    // nodes with a residual at the current level
+    int32_t *tl_nxt = tls + a.n;              // nodes touched for the next level
     const int s = (int)qq.seed;
     if (tid == 0) {
         rec[s].r = 1.0f;                       // hk_solver.py:223
@@ -349,15 +359,26 @@ __device__ __forceinline__ void wave_sync() {
 // block barrier: a wave claims either a group of 64 consecutive entries (its
 // edges are flattened by a wave prefix sum; entries with more than kHubW edges
 // contribute nothing there) or one piece of a hub entry (pcs[], built by the
-// caller: (entry << 8) | piece).  Each lane keeps kUnroll edges in flight.
-//   load(i, st, d, v): row start, degree (0 = nothing to push) and value of entry i
-//   edge(vtx, v) -> int: process one edge; bit 0 appends vtx to next[] (counter
-//   *s_next), bit 1 appends it to tl[] (counter *s_tl)
-template <typename Load, typename Edge>
+// caller: (entry << 8) | piece).
+// Column entries are read from colx ({v, degree[v]}, 8 bytes, coalesced), so
+// the threshold test of an edge needs no load of its own.
+//   load(i, st, d, v): row start, edge count (0 = nothing) and value of entry i
+//   pre(vtx, v) -> S: issue one edge's memory operations, return their results
+//   post(vtx, dv, v, S, act) -> int: bit 0 appends vtx to next[] (counter *s_next),
+//                           bit 1 appends it to tl[] (counter *s_tl); the
+//                           result is masked by act, side effects must test it
+// A lane issues pre() for all its kUnroll edges before post() consumes any
+// result, so kUnroll atomics per lane are in flight at once.  Both run
+// unpredicated (a predicated atomic whose result is used makes the compiler
+// wait inside the predicated block, one round trip per edge): an idle lane
+// aims at its own sink record (id sink + lane, degree 0) with value -0.0,
+// which leaves any value unchanged under IEEE addition.
+template <typename Load, typename Pre, typename Post>
 __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w,
-                                           unsigned int *s_work, WaveLds &W, const int32_t *indices, Load load,
-                                           Edge edge, int32_t *next, unsigned int *s_next, int32_t *tl,
-                                           unsigned int *s_tl) {
+                                           unsigned int *s_work, WaveLds &W, const int2 *colx, Load load,
+                                           Pre pre, Post post, int32_t *next, unsigned int *s_next, int32_t *tl,
+                                           unsigned int *s_tl, int sink) {
+    using S = decltype(pre(0, 0.0));
     const int lane = __lane_id();
     const unsigned int ngroups = (cnt + kWave - 1) / kWave;
     const unsigned int nitems = ngroups + npc;
@@ -394,29 +415,35 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
             end = min(d, beg + piece_w);
         }
         for (int e0 = beg; e0 < end; e0 += kWave * kUnroll) {
-            int vtx[kUnroll];
+            int vtx[kUnroll], dv[kUnroll];
             double vv[kUnroll];
             bool act[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
                 const int e = e0 + u * kWave + lane;
                 act[u] = e < end;
-                vtx[u] = 0;
-                vv[u] = v;
+                int2 c = make_int2(sink + lane, 0);
+                vv[u] = -0.0;
                 if (act[u]) {
+                    vv[u] = v;
                     if (group) {
                         const int k = upper_bound_lds(W.incl, 0, kWave, e);
                         const int o = e - (k ? W.incl[k - 1] : 0);
-                        vtx[u] = indices[W.start[k] + o];
+                        c = colx[W.start[k] + o];
                         vv[u] = W.val[k];
                     } else {
-                        vtx[u] = indices[st + e];
+                        c = colx[st + e];
                     }
                 }
+                vtx[u] = c.x;
+                dv[u] = c.y;
             }
+            S res[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) res[u] = pre(vtx[u], vv[u]);
             int cr[kUnroll];
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) cr[u] = act[u] ? edge(vtx[u], vv[u]) : 0;
+            for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 3 : 0);
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
                 const long long slot = wave_append((cr[u] & 1) != 0, s_next);
@@ -427,11 +454,6 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
         }
         if (group) wave_sync();
     }
-}
-
-// first touch of a record in this query: list it for the final reset
-__device__ __forceinline__ int touch(NodeRec &rv) {
-    return (fl_or(&rv.fl, kTouched) & kTouched) ? 0 : 2;
 }
 
 // hub pieces of the entries [0, cnt) of a list whose degree is given by deg(i)
@@ -447,6 +469,23 @@ __device__ __forceinline__ void build_pieces(unsigned int cnt, int piece_w, Deg 
         }
     }
 }
+
+__device__ __forceinline__ void add_pieces(unsigned int i, int d, int pw, uint32_t *pcs, unsigned int *s_npc) {
+    if (d > kHubW) {
+        const unsigned int np = (unsigned int)((d + pw - 1) / pw);
+        const unsigned int b = atomicAdd(s_npc, np);
+        for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
+    }
+}
+
+// Touched-record bookkeeping of the main push: a record untouched by the
+// current query has r == +0.0 (bit pattern 0); the node phase leaves -0.0 in
+// the residual it moves to x and pushes are >= 0, so r never returns to +0.0
+// once it has left it: "old == +0.0 and new != 0" in the edge atomic's return
+// marks exactly the first touch, with no flag atomic.  The correction push (residuals of both signs) falls back to the
+// flag for records the main push never reached.
+__device__ __forceinline__ bool pristine32(float old) { return __float_as_uint(old) == 0u; }
+__device__ __forceinline__ bool pristine64(double old) { return __double_as_longlong(old) == 0ll; }
 
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
@@ -464,15 +503,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     const int tid = threadIdx.x;
     WaveLds &W = s_w[tid / kWave];
     const int64_t off = (int64_t)(a.slot_base + (int)blockIdx.x) * a.n;
-    NodeRec *rec = a.rec + off;
+    NodeRec *rec = a.rec + (int64_t)(a.slot_base + (int)blockIdx.x) * (a.n + kSink);
+    const int sink = (int)a.n;
     int32_t *qa = a.q0 + off;
     int32_t *qb = a.q1 + off;
-    double *vq = a.vq + off;
+    Ent *ent = a.ent + off;
     int32_t *sup = a.sup + off;
     uint32_t *pcs = a.pcs + off;
     int32_t *tl = a.tl + off;
     const int32_t *indptr = a.indptr;
-    const int32_t *indices = a.indices;
+    const int2 *colx = a.colx;
     const int pw = a.piece_w;
 
     for (;;) {
@@ -484,8 +524,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const uint64_t t_start = now_ticks();
         const flsg_query qq = a.queries[q];
         if (qq.kind == FLSG_HK) {
-            hk_query<NT>(a, qq, q, rec, qa, qb, vq, sup, s_incl, s_start, s_val, s_scan, s_red, &s_cnt, &s_ncnt,
-                         &s_sup, &s_tn, &s_tnn, &s_base, t_start);
+            hk_query<NT>(a, qq, q, rec, qa, qb, (int32_t *)ent, sup, s_incl, s_start, s_val, s_scan, s_red, &s_cnt,
+                         &s_ncnt, &s_sup, &s_tn, &s_tnn, &s_base, t_start);
             continue;
         }
         const QParams P = qparams(qq);
@@ -493,7 +533,6 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         if (tid == 0) {
             if (P.fp64) rec[s].r64 = P.r0;
             else rec[s].r = (float)P.r0;
-            rec[s].fl = kTouched;
             tl[0] = s;
             qa[0] = s;
             s_cnt = 1;
@@ -506,6 +545,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         unsigned long long fnodes = 0;
         int64_t iters = 0;
         int status = FLSG_OK;
+        auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
+            const Ent e = ent[i];
+            st = e.st;
+            d = e.d;
+            v = e.v;
+        };
 
         // ---------------- 1. local push (ppr_solver.py:246-288, katz_solver.py:183-224)
         for (;;) {
@@ -524,7 +569,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 s_work = 0;
             }
             __syncthreads();
-            // node phase: r_u -> 0, x_u += r_u, push value (ppr_solver.py:273-282)
+            // node phase: r_u -> 0, x_u += r_u, push value (ppr_solver.py:273-282);
+            // every record of the unrolled entries is read before any is written
             unsigned long long vol = 0;
             for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
                 int uu[kUnroll];
@@ -533,69 +579,79 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     const unsigned int i = i0 + j * NT;
                     uu[j] = i < cnt ? qc[i] : -1;
                 }
+                double rr[kUnroll], xx[kUnroll];
+                int dd[kUnroll], ss[kUnroll];
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    rr[j] = 0.0;
+                    xx[j] = 0.0;
+                    dd[j] = 0;
+                    ss[j] = 0;
+                    if (uu[j] >= 0) {
+                        // 16-byte record halves: one request each
+                        const char *pr = reinterpret_cast<const char *>(rec + uu[j]);
+                        float4 lo;
+                        __builtin_memcpy(&lo, pr, 16);
+                        dd[j] = __float_as_int(lo.z);
+                        if (P.fp64) {
+                            double2 hi;
+                            __builtin_memcpy(&hi, pr + 16, 16);
+                            rr[j] = hi.x;
+                            xx[j] = hi.y;
+                        } else {
+                            rr[j] = (double)lo.x;
+                            xx[j] = (double)lo.y;
+                        }
+                        ss[j] = indptr[uu[j]];
+                    }
+                }
 #pragma unroll
                 for (int j = 0; j < kUnroll; j++) {
                     if (uu[j] < 0) continue;
                     const unsigned int i = i0 + j * NT;
-                    NodeRec &ru = rec[uu[j]];
-                    const int d = ru.d;
-                    bool first;
+                    char *pr = reinterpret_cast<char *>(rec + uu[j]);
+                    Ent e;
+                    e.st = ss[j];
+                    e.d = dd[j];
                     if (P.fp64) {
-                        const double tmp = ru.r64;
-                        ru.r64 = 0.0;
-                        const double xo = ru.x64;
-                        ru.x64 = xo + tmp;
-                        first = xo == 0.0;
-                        vq[i] = bpush_value(P, tmp, d);
+                        const double2 hv = make_double2(-0.0, xx[j] + rr[j]);
+                        __builtin_memcpy(pr + 16, &hv, 16);
+                        e.v = bpush_value(P, rr[j], dd[j]);
                     } else {
-                        const float tmp = ru.r;
-                        ru.r = 0.0f;
-                        const float xo = ru.x;
-                        ru.x = xo + tmp;
-                        first = xo == 0.0f;
-                        vq[i] = (double)(float)bpush_value(P, (double)tmp, d);
+                        const float2 fv = make_float2(-0.0f, (float)xx[j] + (float)rr[j]);
+                        __builtin_memcpy(pr, &fv, 8);
+                        e.v = (double)(float)bpush_value(P, rr[j], dd[j]);
                     }
-                    if (first) sup[atomicAdd(&s_sup, 1u)] = uu[j];
-                    vol += (unsigned long long)d;
-                    if (d > kHubW) {
-                        const unsigned int np = (unsigned int)((d + pw - 1) / pw);
-                        const unsigned int b = atomicAdd(&s_npc, np);
-                        for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
-                    }
+                    ent[i] = e;
+                    if (xx[j] == 0.0) sup[atomicAdd(&s_sup, 1u)] = uu[j];
+                    vol += (unsigned long long)dd[j];
+                    add_pieces(i, dd[j], pw, pcs, &s_npc);
                 }
             }
             if (vol) atomicAdd(&s_vol, vol);
             __syncthreads();
             // edge phase: v joins the next frontier iff this atomic moved r_v across thr_v
-            auto load = [&](unsigned int i, int &st, int &d, double &v) {
-                const int u = qc[i];
-                st = indptr[u];
-                d = rec[u].d;
-                v = vq[i];
-            };
             if (P.fp64) {
                 wave_edges(
-                    cnt, pcs, s_npc, pw, &s_work, W, indices, load,
-                    [&](int vtx, double val) -> int {
-                        NodeRec &rv = rec[vtx];
-                        const double old = atomic_add_wg(&rv.r64, val);
+                    cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
+                    [&](int vtx, double val) { return atomic_add_wg(&rec[vtx].r64, val); },
+                    [&](int, int dv, double val, double old, bool) -> int {
                         const double nw = old + val;
-                        const double thr = P.thr_scale * (double)rv.d;
-                        return ((old < thr && thr <= nw) ? 1 : 0) | (old == 0.0 ? touch(rv) : 0);
+                        const double thr = P.thr_scale * (double)dv;
+                        return ((old < thr && thr <= nw) ? 1 : 0) | ((pristine64(old) && nw != 0.0) ? 2 : 0);
                     },
-                    qn, &s_ncnt, tl, &s_tl);
+                    qn, &s_ncnt, tl, &s_tl, sink);
             } else {
                 wave_edges(
-                    cnt, pcs, s_npc, pw, &s_work, W, indices, load,
-                    [&](int vtx, double v) -> int {
-                        const float val = (float)v;
-                        NodeRec &rv = rec[vtx];
-                        const float old = atomic_add_wg(&rv.r, val);
-                        const float nw = old + val;
-                        const double thr = P.thr_scale * (double)rv.d;
-                        return (((double)old < thr && thr <= (double)nw) ? 1 : 0) | (old == 0.0f ? touch(rv) : 0);
+                    cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
+                    [&](int vtx, double val) { return atomic_add_wg(&rec[vtx].r, (float)val); },
+                    [&](int, int dv, double val, float old, bool) -> int {
+                        const float nw = old + (float)val;
+                        const double thr = P.thr_scale * (double)dv;
+                        return (((double)old < thr && thr <= (double)nw) ? 1 : 0) |
+                               ((pristine32(old) && nw != 0.0f) ? 2 : 0);
                     },
-                    qn, &s_ncnt, tl, &s_tl);
+                    qn, &s_ncnt, tl, &s_tl, sink);
             }
             __syncthreads();
             if (tid == 0) s_cnt = s_ncnt;
@@ -620,20 +676,21 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             build_pieces<NT>(S32, pw, [&](unsigned int i) { return rec[sup[i]].d; }, pcs, &s_npc);
             __syncthreads();
             const unsigned int npc_sup = s_npc;
-            // 2a. sweep A: r64 = pushes of x32 over supp (every target is already touched)
+            // source u of a support sweep: v = push value of x32_u
+            auto load_sup = [&](unsigned int i, int &st, int &d, double &v) {
+                const int u = sup[i];
+                st = indptr[u];
+                d = rec[u].d;
+                v = bpush_value(P, (double)rec[u].x, d);
+            };
+            // 2a. sweep A: r64 = pushes of x32 over supp (every target is already listed)
             wave_edges(
-                S32, pcs, npc_sup, pw, &s_work, W, indices,
-                [&](unsigned int i, int &st, int &d, double &v) {
-                    const int u = sup[i];
-                    st = indptr[u];
-                    d = rec[u].d;
-                    v = bpush_value(P, (double)rec[u].x, d);
-                },
-                [&](int vtx, double v) -> int {
+                S32, pcs, npc_sup, pw, &s_work, W, colx, load_sup,
+                [&](int vtx, double v) {
                     atomic_add_wg(&rec[vtx].r64, v);
                     return 0;
                 },
-                qn, &s_ncnt, tl, &s_tl);
+                [&](int, int, double, int, bool) -> int { return 0; }, qn, &s_ncnt, tl, &s_tl, sink);
             __syncthreads();
             // 2b. r64_u -= x_u on the support; x64 = x32
             for (unsigned int i = tid; i < S32; i += NT) {
@@ -652,31 +709,27 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             }
             __syncthreads();
             // 2c. sweep B: queue every candidate (supp and its neighbours) with |r64| >= thr
-            auto check = [&](int vtx, double) -> int {
-                NodeRec &rv = rec[vtx];
-                if (fabs(rv.r64) >= P.thr_scale * (double)rv.d) return (fl_or(&rv.fl, kInQ) & kInQ) == 0 ? 1 : 0;
-                return 0;
-            };
             for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
                 const unsigned int i = i0 + tid;
                 bool enq = false;
                 int u = 0;
                 if (i < S32) {
                     u = sup[i];
-                    enq = check(u, 0.0) != 0;
+                    NodeRec &ru = rec[u];
+                    if (fabs(ru.r64) >= P.thr_scale * (double)ru.d) enq = (fl_or(&ru.fl, kInQ) & kInQ) == 0;
                 }
                 const long long slot = wave_append(enq, &s_cnt);
                 if (slot >= 0) qc[slot] = u;
             }
             wave_edges(
-                S32, pcs, npc_sup, pw, &s_work, W, indices,
-                [&](unsigned int i, int &st, int &d, double &v) {
-                    const int u = sup[i];
-                    st = indptr[u];
-                    d = rec[u].d;
-                    v = 1.0;
+                S32, pcs, npc_sup, pw, &s_work, W, colx, load_sup,
+                [&](int vtx, double) { return rec[vtx].r64; },
+                [&](int vtx, int dv, double, double r64, bool act) -> int {
+                    if (act && fabs(r64) >= P.thr_scale * (double)dv)
+                        return (fl_or(&rec[vtx].fl, kInQ) & kInQ) == 0 ? 1 : 0;
+                    return 0;
                 },
-                check, qc, &s_cnt, tl, &s_tl);
+                qc, &s_cnt, tl, &s_tl, sink);
             __syncthreads();
             // 2d. FP64 correction push (mark-based queue: the recomputed residual has both signs)
             int64_t corr_iters = 0;
@@ -703,7 +756,10 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     const int d = ru.d;
                     const uint32_t fl = fl_and(&ru.fl, ~kInQ);
                     const double tmp = ru.r64;
-                    double v = 0.0;
+                    Ent e;
+                    e.st = 0;
+                    e.d = 0;
+                    e.v = 0.0;
                     if (fabs(tmp) >= P.thr_scale * (double)d) {
                         ru.r64 = 0.0;
                         ru.x64 = ru.x64 + tmp;
@@ -711,41 +767,33 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                             fl_or(&ru.fl, kSeen);
                             sup[atomicAdd(&s_sup, 1u)] = u;
                         }
-                        v = bpush_value(P, tmp, d);
+                        e.st = indptr[u];
+                        e.d = d;
+                        e.v = bpush_value(P, tmp, d);
                         pushed += 1;
                         cvol += (unsigned long long)d;
-                        if (d > kHubW) {
-                            const unsigned int np = (unsigned int)((d + pw - 1) / pw);
-                            const unsigned int b = atomicAdd(&s_npc, np);
-                            for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
-                        }
+                        add_pieces(i, d, pw, pcs, &s_npc);
                     }
-                    vq[i] = v;
+                    ent[i] = e;
                 }
                 if (pushed) atomicAdd(&s_vol, pushed);
                 cert_ops += cvol;
                 __syncthreads();
                 n_corr += s_vol;
                 wave_edges(
-                    cnt, pcs, s_npc, pw, &s_work, W, indices,
-                    [&](unsigned int i, int &st, int &d, double &v) {
-                        v = vq[i];
-                        if (v != 0.0) {
-                            const int u = qc[i];
-                            st = indptr[u];
-                            d = rec[u].d;
-                        }
-                    },
-                    [&](int vtx, double val) -> int {
+                    cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
+                    [&](int vtx, double val) { return atomic_add_wg(&rec[vtx].r64, val); },
+                    [&](int vtx, int dv, double val, double old, bool act) -> int {
+                        if (!act) return 0;
                         NodeRec &rv = rec[vtx];
-                        const double old = atomic_add_wg(&rv.r64, val);
                         const double nw = old + val;
-                        // an untouched record has r64 == 0, so only such edges can be a first touch
-                        int res = old == 0.0 ? touch(rv) : 0;
-                        if (fabs(nw) >= P.thr_scale * (double)rv.d && (fl_or(&rv.fl, kInQ) & kInQ) == 0) res |= 1;
+                        int res = 0;
+                        // records the main push never reached still hold r == +0.0
+                        if (old == 0.0 && pristine32(rv.r) && (fl_or(&rv.fl, kTouched) & kTouched) == 0) res = 2;
+                        if (fabs(nw) >= P.thr_scale * (double)dv && (fl_or(&rv.fl, kInQ) & kInQ) == 0) res |= 1;
                         return res;
                     },
-                    qn, &s_ncnt, tl, &s_tl);
+                    qn, &s_ncnt, tl, &s_tl, sink);
                 __syncthreads();
                 if (tid == 0) s_cnt = s_ncnt;
                 int32_t *t = qc;
@@ -774,8 +822,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         for (unsigned int i = tid; i < T; i += NT) {
             NodeRec &rv = rec[tl[i]];
             const double r64 = rv.r64;
-            if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)rv.d));
             const int d = rv.d;
+            if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)d));
             NodeRec z{};
             z.d = d;
             rv = z;
@@ -802,11 +850,13 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     }
 }
 
-// records of every slot: zero, d = degree
+// records of every slot: zero, d = degree (sink records: d = 0)
 __global__ void k_slot_init(NodeRec *rec, const int32_t *degree, int64_t n, int64_t total) {
+    const int64_t rn = n + kSink;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = i % rn;
         NodeRec z{};
-        z.d = degree[i % n];
+        z.d = u < n ? degree[u] : 0;
         rec[i] = z;
     }
 }
@@ -819,7 +869,7 @@ __global__ void k_slot_degrees(NodeRec *rec, const int32_t *degree, int64_t n, i
         const int64_t k = i / slots;
         const int64_t sl = i - k * slots;
         const int u = keys[k];
-        rec[sl * n + u].d = degree[u];
+        rec[sl * (n + kSink) + u].d = degree[u];
     }
 }
 
@@ -836,13 +886,43 @@ __global__ void k_fill_queries(flsg_query *q, const int64_t *seeds, int64_t nq, 
     }
 }
 
+// column entries with the neighbour's degree; slack entries (id outside
+// [0, n)) carry degree 0 and are never read by a live row
+__global__ void k_colx(const int32_t *indices, const int32_t *degree, int64_t n, int64_t nnz, int2 *colx) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int v = indices[e];
+        colx[e] = make_int2(v, (v >= 0 && v < n) ? degree[v] : 0);
+    }
+}
+
+int colx_refresh(flsg_graph g, hipStream_t st) {
+    if (!g->colx || g->nnz_cap == 0) return FLSG_OK;
+    hipLaunchKernelGGL(k_colx, dim3(g->num_cus * 8), dim3(256), 0, st, (const int32_t *)g->indices,
+                       (const int32_t *)g->degree, g->n, g->nnz_cap, g->colx);
+    FLSG_HIP(hipGetLastError());
+    return FLSG_OK;
+}
+
+// built once per handle (synchronously, so launches on other streams see it);
+// edge insertions refresh it in place
+static int colx_ensure(flsg_graph g, hipStream_t st) {
+    if (g->colx) return FLSG_OK;
+    FLSG_HIP(hipMalloc((void **)&g->colx, sizeof(int2) * (size_t)(g->nnz_cap > 0 ? g->nnz_cap : 1)));
+    int rc = colx_refresh(g, st);
+    if (rc) return rc;
+    FLSG_HIP(hipStreamSynchronize(st));
+    return FLSG_OK;
+}
+
 void batch_free(flsg_graph g) {
+    hipFree(g->colx);
+    g->colx = nullptr;
     BatchArena *A = g->arena;
     if (!A) return;
     hipFree(A->rec);
     hipFree(A->q0);
     hipFree(A->q1);
-    hipFree(A->vq);
+    hipFree(A->ent);
     hipFree(A->sup);
     hipFree(A->pcs);
     hipFree(A->tl);
@@ -854,8 +934,8 @@ void batch_free(flsg_graph g) {
 }
 
 int64_t slot_bytes(int64_t n) {
-    // 32 B node record + q0 q1 sup pcs tl (4 B) + vq (8 B)
-    return n * (32 + 5 * 4 + 8);
+    // 32 B node record + q0 q1 sup pcs tl (4 B) + frontier entry (16 B), kSink sink records
+    return n * (32 + 5 * 4 + 16) + (int64_t)kSink * 32;
 }
 
 static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
@@ -867,16 +947,17 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     A->n = g->n;
     A->slots = slots;
     const size_t ns = (size_t)g->n * (size_t)slots;
-    FLSG_HIP(hipMalloc((void **)&A->rec, sizeof(NodeRec) * ns));
+    const size_t nrec = ((size_t)g->n + kSink) * (size_t)slots;
+    FLSG_HIP(hipMalloc((void **)&A->rec, sizeof(NodeRec) * nrec));
     FLSG_HIP(hipMalloc((void **)&A->q0, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->q1, sizeof(int32_t) * ns));
-    FLSG_HIP(hipMalloc((void **)&A->vq, sizeof(double) * ns));
+    FLSG_HIP(hipMalloc((void **)&A->ent, sizeof(Ent) * ns));
     FLSG_HIP(hipMalloc((void **)&A->sup, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->pcs, sizeof(uint32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->tl, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
     hipLaunchKernelGGL(k_slot_init, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree, g->n,
-                       (int64_t)ns);
+                       (int64_t)nrec);
     FLSG_HIP(hipGetLastError());
     return FLSG_OK;
 }
@@ -900,17 +981,20 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     if (!A) return fail(FLSG_EARG, "no batch arena: call flsg_batch_reserve first");
     if (slot_base < 0 || n_slots <= 0 || slot_base + n_slots > A->slots)
         return fail(FLSG_EARG, "slot range outside the reserved arena");
+    int rc = colx_ensure(g, st);
+    if (rc) return rc;
     FLSG_HIP(hipMemsetAsync(d_ctl, 0, sizeof(uint64_t) * 2, st));
     BatchArgs a{};
     a.indptr = g->indptr;
     a.indices = g->indices;
+    a.colx = g->colx;
     a.n = g->n;
     a.queries = d_q;
     a.nq = nq;
     a.rec = A->rec;
     a.q0 = A->q0;
     a.q1 = A->q1;
-    a.vq = A->vq;
+    a.ent = A->ent;
     a.sup = A->sup;
     a.pcs = A->pcs;
     a.tl = A->tl;
@@ -980,7 +1064,9 @@ extern "C" int flsg_batch_reserve(flsg_graph g, int32_t slots, void *stream) {
     if (rc) return rc;
     if (slots <= 0) slots = g->batch_slots > 0 ? g->batch_slots : default_slots(g);
     g->batch_slots = slots;
-    return ensure_arena(g, slots, (hipStream_t)stream);
+    rc = ensure_arena(g, slots, (hipStream_t)stream);
+    if (rc) return rc;
+    return colx_ensure(g, (hipStream_t)stream);
 }
 
 extern "C" int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, int64_t n_keys, void *stream) {

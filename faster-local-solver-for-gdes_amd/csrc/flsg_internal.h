@@ -78,6 +78,7 @@ struct flsg_graph_s {
     // edge-insertion workspace (flsg_dyn.hip)
     void *dyn = nullptr;
     // batched workspace
+    int2 *colx = nullptr;                   // nnz_cap column entries {v, degree[v]} (batched edge phases)
     flsg::BatchArena *arena = nullptr;
     int32_t batch_slots = 0;
     int32_t batch_block = 1024;
@@ -91,6 +92,7 @@ int recompute_m(flsg_graph g, hipStream_t st);
 int64_t host_estimate_n(double t, double eps);
 int check_graph(flsg_graph g);
 void batch_free(flsg_graph g);
+int colx_refresh(flsg_graph g, hipStream_t st);
 void cheb_free(flsg_graph g);
 void ignn_free(flsg_graph g);
 void dyn_free(flsg_graph g);

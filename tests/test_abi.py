@@ -36,4 +36,4 @@ def test_struct_layouts():
 def test_version_without_gpu():
     lib = _build.load_lib()
     assert lib.flsg_version() >= 1
-    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 60
+    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 68 + 64 * 32
