@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--block", type=int, default=0, help="batched workgroup size (256/512/1024; 0 = default)")
     ap.add_argument("--no-relabel", action="store_true", help="keep the generator's node ids on the device")
+    ap.add_argument("--hub-bytes", type=int, default=None,
+                    help="LDS hub cache bytes per workgroup (-1 default, 0 off)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times): the known-size reference "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -93,13 +95,13 @@ def main():
 
     # ------------------------------------------------------------ batched FP32 (the measured step)
     for w in range(args.warmup):
-        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots, block=args.block)
+        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots, block=args.block, hub_bytes=args.hub_bytes)
         log(f"warmup {w}: {bo.ms:.1f} ms")
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     runs = []
     for k in range(args.steps):
-        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots, block=args.block)
+        bo = engine.batch_solve(dg, p32, seeds, slots=args.slots, block=args.block, hub_bytes=args.hub_bytes)
         runs.append(bo)
         log(f"step {k}: {bo.ms:.1f} ms (per-query push {np.mean([r['ms_push'] for r in bo.results]):.2f} ms, "
             f"total {np.mean([r['ms'] for r in bo.results]):.2f} ms)")
@@ -127,10 +129,10 @@ def main():
     # ------------------------------------------------------------ batched FP64 push (same launch, FP64 records)
     b64_qps = None
     if args.fp64_batched:
-        bo64 = engine.batch_solve(dg, p64, seeds, slots=args.slots, block=args.block)
+        bo64 = engine.batch_solve(dg, p64, seeds, slots=args.slots, block=args.block, hub_bytes=args.hub_bytes)
         torch.cuda.synchronize()
         tb = time.perf_counter()
-        bo64 = engine.batch_solve(dg, p64, seeds, slots=args.slots, block=args.block)
+        bo64 = engine.batch_solve(dg, p64, seeds, slots=args.slots, block=args.block, hub_bytes=args.hub_bytes)
         torch.cuda.synchronize()
         b64_qps = args.nq / (time.perf_counter() - tb)
         ok64 = all(r["status"] == 0 and r["cert"] < 1.0 for r in bo64.results)

@@ -96,6 +96,7 @@ struct BatchArgs {
     uint32_t *pcs;
     int32_t *tl;
     int32_t piece_w;             // edges per hub piece
+    int32_t hub_bytes;           // dynamic LDS of the hub cache (0 = none)
     int32_t slot_base;
     uint64_t *ctl;               // [0] next query (low 32 bits used), [1] output entries used
     const int32_t *out_map;      // internal id -> caller id (NULL = identity)
@@ -499,6 +500,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __shared__ int s_start[NT];
     __shared__ double s_val[NT];
     __shared__ int s_scan[NT / kWave];
+    // hub cache: the residuals of nodes [0, H) live here for the whole query
+    // (ids are degree-ranked on relabelled graphs, so these take the largest
+    // share of the pushes): LDS atomics instead of memory-side ones
+    extern __shared__ double s_hub[];
+    float *hub32 = reinterpret_cast<float *>(s_hub);
+    double *hub64 = s_hub;
 
     const int tid = threadIdx.x;
     WaveLds &W = s_w[tid / kWave];
@@ -530,9 +537,18 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         }
         const QParams P = qparams(qq);
         const int s = (int)qq.seed;
+        const int hub_n = (int)min((int64_t)(P.fp64 ? a.hub_bytes / 8 : a.hub_bytes / 4), a.n);
+        for (int i = tid; i < a.hub_bytes / 8; i += NT) s_hub[i] = 0.0;
+        __syncthreads();
         if (tid == 0) {
-            if (P.fp64) rec[s].r64 = P.r0;
-            else rec[s].r = (float)P.r0;
+            if (s < hub_n) {
+                if (P.fp64) hub64[s] = P.r0;
+                else hub32[s] = (float)P.r0;
+            } else if (P.fp64) {
+                rec[s].r64 = P.r0;
+            } else {
+                rec[s].r = (float)P.r0;
+            }
             tl[0] = s;
             qa[0] = s;
             s_cnt = 1;
@@ -596,10 +612,10 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         if (P.fp64) {
                             double2 hi;
                             __builtin_memcpy(&hi, pr + 16, 16);
-                            rr[j] = hi.x;
+                            rr[j] = uu[j] < hub_n ? hub64[uu[j]] : hi.x;
                             xx[j] = hi.y;
                         } else {
-                            rr[j] = (double)lo.x;
+                            rr[j] = (double)(uu[j] < hub_n ? hub32[uu[j]] : lo.x);
                             xx[j] = (double)lo.y;
                         }
                         ss[j] = indptr[uu[j]];
@@ -616,10 +632,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     if (P.fp64) {
                         const double2 hv = make_double2(-0.0, xx[j] + rr[j]);
                         __builtin_memcpy(pr + 16, &hv, 16);
+                        if (uu[j] < hub_n) hub64[uu[j]] = -0.0;
                         e.v = bpush_value(P, rr[j], dd[j]);
                     } else {
                         const float2 fv = make_float2(-0.0f, (float)xx[j] + (float)rr[j]);
                         __builtin_memcpy(pr, &fv, 8);
+                        if (uu[j] < hub_n) hub32[uu[j]] = -0.0f;
                         e.v = (double)(float)bpush_value(P, rr[j], dd[j]);
                     }
                     ent[i] = e;
@@ -634,7 +652,9 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             if (P.fp64) {
                 wave_edges(
                     cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
-                    [&](int vtx, double val) { return atomic_add_wg(&rec[vtx].r64, val); },
+                    [&](int vtx, double val) {
+                        return vtx < hub_n ? atomicAdd(&hub64[vtx], val) : atomic_add_wg(&rec[vtx].r64, val);
+                    },
                     [&](int, int dv, double val, double old, bool) -> int {
                         const double nw = old + val;
                         const double thr = P.thr_scale * (double)dv;
@@ -644,7 +664,9 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             } else {
                 wave_edges(
                     cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
-                    [&](int vtx, double val) { return atomic_add_wg(&rec[vtx].r, (float)val); },
+                    [&](int vtx, double val) {
+                        return vtx < hub_n ? atomicAdd(&hub32[vtx], (float)val) : atomic_add_wg(&rec[vtx].r, (float)val);
+                    },
                     [&](int, int dv, double val, float old, bool) -> int {
                         const float nw = old + (float)val;
                         const double thr = P.thr_scale * (double)dv;
@@ -659,6 +681,17 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             qc = qn;
             qn = t;
             __syncthreads();
+        }
+        __syncthreads();
+        // hub residuals back into their records (touched hubs have nonzero bits)
+        for (int v = tid; v < hub_n; v += NT) {
+            if (P.fp64) {
+                const double h = hub64[v];
+                if (!pristine64(h)) rec[v].r64 = h;
+            } else {
+                const float h = hub32[v];
+                if (!pristine32(h)) rec[v].r = h;
+            }
         }
         __syncthreads();
         const unsigned long long oper = s_vol;
@@ -683,53 +716,68 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 d = rec[u].d;
                 v = bpush_value(P, (double)rec[u].x, d);
             };
-            // 2a. sweep A: r64 = pushes of x32 over supp (every target is already listed)
+            // 2a. sweep A: r64 = pushes of x32 over supp (every target is already
+            // listed; r64 is still zero everywhere); hub targets sum in LDS
+            const int hub64_n = (int)min((int64_t)(a.hub_bytes / 8), a.n);
+            for (int i = tid; i < hub64_n; i += NT) hub64[i] = 0.0;
+            __syncthreads();
             wave_edges(
                 S32, pcs, npc_sup, pw, &s_work, W, colx, load_sup,
                 [&](int vtx, double v) {
-                    atomic_add_wg(&rec[vtx].r64, v);
+                    if (vtx < hub64_n) atomicAdd(&hub64[vtx], v);
+                    else atomic_add_wg(&rec[vtx].r64, v);
                     return 0;
                 },
                 [&](int, int, double, int, bool) -> int { return 0; }, qn, &s_ncnt, tl, &s_tl, sink);
             __syncthreads();
-            // 2b. r64_u -= x_u on the support; x64 = x32
-            for (unsigned int i = tid; i < S32; i += NT) {
-                NodeRec &ru = rec[sup[i]];
-                const double xd = (double)ru.x;
-                ru.r64 = ru.r64 - xd;
-                ru.x64 = xd;
-                fl_or(&ru.fl, kSeen);
-                cert_ops += 2ull * (unsigned long long)ru.d + 1ull;
+            for (int v = tid; v < hub64_n; v += NT) {
+                const double h = hub64[v];
+                if (!pristine64(h)) rec[v].r64 = h;
             }
             __syncthreads();
-            if (tid == 0) {
-                rec[s].r64 = rec[s].r64 + P.r0;
-                s_cnt = 0;
-                s_work = 0;
-            }
+            // 2b. one pass over the touched list, a superset of supp(x) and its
+            // neighbours (the only nodes whose residual can be nonzero):
+            // r64 = b - x + (pushes of x) in FP64, x64 = x32, and every node with
+            // |r64| >= thr queued for the correction push
+            const unsigned int T0 = s_tl;
+            if (tid == 0) s_cnt = 0;
             __syncthreads();
-            // 2c. sweep B: queue every candidate (supp and its neighbours) with |r64| >= thr
-            for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
-                const unsigned int i = i0 + tid;
-                bool enq = false;
-                int u = 0;
-                if (i < S32) {
-                    u = sup[i];
-                    NodeRec &ru = rec[u];
-                    if (fabs(ru.r64) >= P.thr_scale * (double)ru.d) enq = (fl_or(&ru.fl, kInQ) & kInQ) == 0;
+            for (unsigned int i0 = 0; i0 < T0; i0 += NT * kUnroll) {
+                int vv[kUnroll];
+                float4 lo[kUnroll];
+                double rs[kUnroll];
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    const unsigned int i = i0 + j * NT + tid;
+                    vv[j] = i < T0 ? tl[i] : -1;
                 }
-                const long long slot = wave_append(enq, &s_cnt);
-                if (slot >= 0) qc[slot] = u;
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    rs[j] = 0.0;
+                    if (vv[j] >= 0) {
+                        __builtin_memcpy(&lo[j], rec + vv[j], 16);
+                        rs[j] = rec[vv[j]].r64;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    bool enq = false;
+                    if (vv[j] >= 0) {
+                        const double xd = (double)lo[j].y;
+                        const int d = __float_as_int(lo[j].z);
+                        double rt = rs[j] - xd;
+                        if (vv[j] == s) rt = rt + P.r0;
+                        enq = fabs(rt) >= P.thr_scale * (double)d;
+                        const double2 hv = make_double2(rt, xd);
+                        __builtin_memcpy(reinterpret_cast<char *>(rec + vv[j]) + 16, &hv, 16);
+                        rec[vv[j]].fl = (xd != 0.0 ? kSeen : 0u) | (enq ? kInQ : 0u);
+                        cert_ops += (xd != 0.0 ? (unsigned long long)d : 0ull) + 1ull;
+                    }
+                    const long long slot = wave_append(enq, &s_cnt);
+                    if (slot >= 0) qc[slot] = vv[j];
+                }
             }
-            wave_edges(
-                S32, pcs, npc_sup, pw, &s_work, W, colx, load_sup,
-                [&](int vtx, double) { return rec[vtx].r64; },
-                [&](int vtx, int dv, double, double r64, bool act) -> int {
-                    if (act && fabs(r64) >= P.thr_scale * (double)dv)
-                        return (fl_or(&rec[vtx].fl, kInQ) & kInQ) == 0 ? 1 : 0;
-                    return 0;
-                },
-                qc, &s_cnt, tl, &s_tl, sink);
             __syncthreads();
             // 2d. FP64 correction push (mark-based queue: the recomputed residual has both signs)
             int64_t corr_iters = 0;
@@ -1023,9 +1071,16 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
         for (int64_t i = N - 1; i >= 0; i--) a.hk_psi[i] = a.hk_psi[i + 1] * hk->tau / ((double)i + 1.) + 1.;
     }
     const int grid = n_slots;
-    if (block == 1024) hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), 0, st, a);
-    else if (block == 256) hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), 0, st, a);
+    int hub = g->batch_hub_bytes;
+    if (hub < 0) hub = block == 1024 ? 120 * 1024 : block == 512 ? 48 * 1024 : 24 * 1024;
+    a.hub_bytes = hub;
+    const void *fn = block == 1024  ? reinterpret_cast<const void *>(&k_batch<1024>)
+                     : block == 256 ? reinterpret_cast<const void *>(&k_batch<256>)
+                                    : reinterpret_cast<const void *>(&k_batch<512>);
+    FLSG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, hub));
+    if (block == 1024) hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), hub, st, a);
+    else if (block == 256) hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), hub, st, a);
+    else hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), hub, st, a);
     FLSG_HIP(hipGetLastError());
     return FLSG_OK;
 }
@@ -1041,6 +1096,15 @@ extern "C" int flsg_batch_set_slots(flsg_graph g, int32_t slots) {
     if (rc) return rc;
     if (slots < 0) return fail(FLSG_EARG, "slots < 0");
     g->batch_slots = slots;
+    return FLSG_OK;
+}
+
+extern "C" int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (bytes < -1 || bytes > 120 * 1024 || (bytes > 0 && bytes % 8 != 0))
+        return fail(FLSG_EARG, "hub cache bytes: -1, 0 or a multiple of 8 up to 122880");
+    g->batch_hub_bytes = bytes;
     return FLSG_OK;
 }
 

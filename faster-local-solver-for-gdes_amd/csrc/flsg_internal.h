@@ -82,6 +82,7 @@ struct flsg_graph_s {
     flsg::BatchArena *arena = nullptr;
     int32_t batch_slots = 0;
     int32_t batch_block = 1024;
+    int32_t batch_hub_bytes = -1;           // LDS hub cache per workgroup (-1 = default for the block)
     const int32_t *out_map = nullptr;       // internal id -> caller id for batched outputs
 };
 

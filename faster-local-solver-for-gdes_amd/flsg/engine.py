@@ -205,7 +205,7 @@ class BatchOut:
 
 
 def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None = None,
-                slots: int = 0, block: int = 0) -> BatchOut:
+                slots: int = 0, block: int = 0, hub_bytes: int | None = None) -> BatchOut:
     """Many single-source local solves in one persistent launch (FP32 push,
     FP64 threshold and FP64 certificate)."""
     torch = dg.torch
@@ -223,6 +223,8 @@ def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None
         _check(dg.lib.flsg_batch_set_slots(dg.handle, int(slots)))
     if block:
         _check(dg.lib.flsg_batch_set_block(dg.handle, int(block)))
+    if hub_bytes is not None:
+        _check(dg.lib.flsg_batch_set_hub_cache(dg.handle, int(hub_bytes)))
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()

@@ -157,6 +157,14 @@ int64_t flsg_batch_slot_bytes(int64_t n);
 int flsg_batch_set_slots(flsg_graph g, int32_t slots);
 int flsg_batch_set_block(flsg_graph g, int32_t block);
 
+/* LDS bytes per workgroup that hold the residuals of nodes [0, bytes/4)
+ * (FP32 push) or [0, bytes/8) (FP64 push and certificate sums) for a whole
+ * query: on degree-relabelled graphs these are the hubs, whose pushes then
+ * use LDS atomics.  -1 = default for the workgroup size (120 KiB at 1024,
+ * 48 KiB at 512, 24 KiB at 256), 0 = off, else a multiple of 8 up to
+ * 120 KiB.  No counterpart in the reference (tuning knob). */
+int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes);
+
 /* Allocate the slot arena (slots <= 0: the set / default count). */
 int flsg_batch_reserve(flsg_graph g, int32_t slots, void *stream);
 
