@@ -364,7 +364,8 @@ __device__ __forceinline__ void wave_sync() {
 // Column entries are read from colx ({v, degree[v]}, 8 bytes, coalesced), so
 // the threshold test of an edge needs no load of its own.
 //   load(i, st, d, v): row start, edge count (0 = nothing) and value of entry i
-//   pre(vtx, v) -> S: issue one edge's memory operations, return their results
+//   pre(vtx, v, S &, stage): issue one edge's LDS (stage 0) or memory-side
+//                           (stage 1) operations, results into S
 //   post(vtx, dv, v, S, act) -> int: bit 0 appends vtx to next[] (counter *s_next),
 //                           bit 1 appends it to tl[] (counter *s_tl); the
 //                           result is masked by act, side effects must test it
@@ -374,12 +375,11 @@ __device__ __forceinline__ void wave_sync() {
 // wait inside the predicated block, one round trip per edge): an idle lane
 // aims at its own sink record (id sink + lane, degree 0) with value -0.0,
 // which leaves any value unchanged under IEEE addition.
-template <typename Load, typename Pre, typename Post>
+template <typename S, typename Load, typename Pre, typename Post>
 __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w,
                                            unsigned int *s_work, WaveLds &W, const int2 *colx, Load load,
                                            Pre pre, Post post, int32_t *next, unsigned int *s_next, int32_t *tl,
                                            unsigned int *s_tl, int sink) {
-    using S = decltype(pre(0, 0.0));
     const int lane = __lane_id();
     const unsigned int ngroups = (cnt + kWave - 1) / kWave;
     const unsigned int nitems = ngroups + npc;
@@ -439,9 +439,15 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
                 vtx[u] = c.x;
                 dv[u] = c.y;
             }
+            // stage 0 issues the LDS-side operations of all kUnroll edges, stage 1
+            // the memory-side ones: a result register the compiler shares between
+            // the two kinds then only ever waits for an LDS op, never for memory
             S res[kUnroll];
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) res[u] = pre(vtx[u], vv[u]);
+            for (int st2 = 0; st2 < 2; st2++) {
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) pre(vtx[u], vv[u], res[u], st2);
+            }
             int cr[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 3 : 0);
@@ -485,6 +491,16 @@ __device__ __forceinline__ void add_pieces(unsigned int i, int d, int pw, uint32
 // once it has left it: "old == +0.0 and new != 0" in the edge atomic's return
 // marks exactly the first touch, with no flag atomic.  The correction push (residuals of both signs) falls back to the
 // flag for records the main push never reached.
+// an edge atomic's result (from the hub cache or from memory)
+struct Res32 {
+    float v;
+    bool lds;
+};
+struct Res64 {
+    double v;
+    bool lds;
+};
+
 __device__ __forceinline__ bool pristine32(float old) { return __float_as_uint(old) == 0u; }
 __device__ __forceinline__ bool pristine64(double old) { return __double_as_longlong(old) == 0ll; }
 
@@ -537,7 +553,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         }
         const QParams P = qparams(qq);
         const int s = (int)qq.seed;
-        const int hub_n = (int)min((int64_t)(P.fp64 ? a.hub_bytes / 8 : a.hub_bytes / 4), a.n);
+        // cache capacity in elements; its last kWave entries take the identity
+        // adds of idle lanes (so they cost no memory-side atomic)
+        const int hub_cap = P.fp64 ? a.hub_bytes / 8 : a.hub_bytes / 4;
+        const int hub_n = hub_cap > kWave ? (int)min((int64_t)(hub_cap - kWave), a.n) : 0;
+        const int hub_sink = hub_cap > kWave ? hub_cap - kWave - sink : INT32_MIN;   // + vtx of an idle lane
         for (int i = tid; i < a.hub_bytes / 8; i += NT) s_hub[i] = 0.0;
         __syncthreads();
         if (tid == 0) {
@@ -650,24 +670,38 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             __syncthreads();
             // edge phase: v joins the next frontier iff this atomic moved r_v across thr_v
             if (P.fp64) {
-                wave_edges(
+                wave_edges<Res64>(
                     cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
-                    [&](int vtx, double val) {
-                        return vtx < hub_n ? atomicAdd(&hub64[vtx], val) : atomic_add_wg(&rec[vtx].r64, val);
+                    [&](int vtx, double val, Res64 &r, int stage) {
+                        const int li = vtx < hub_n ? vtx : vtx >= sink ? vtx + hub_sink : -1;
+                        if (stage == 0) {
+                            r.lds = li >= 0;
+                            if (li >= 0) r.v = atomicAdd(&hub64[li], val);
+                        } else if (li < 0) {
+                            r.v = atomic_add_wg(&rec[vtx].r64, val);
+                        }
                     },
-                    [&](int, int dv, double val, double old, bool) -> int {
+                    [&](int, int dv, double val, const Res64 &r, bool) -> int {
+                        const double old = r.v;
                         const double nw = old + val;
                         const double thr = P.thr_scale * (double)dv;
                         return ((old < thr && thr <= nw) ? 1 : 0) | ((pristine64(old) && nw != 0.0) ? 2 : 0);
                     },
                     qn, &s_ncnt, tl, &s_tl, sink);
             } else {
-                wave_edges(
+                wave_edges<Res32>(
                     cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
-                    [&](int vtx, double val) {
-                        return vtx < hub_n ? atomicAdd(&hub32[vtx], (float)val) : atomic_add_wg(&rec[vtx].r, (float)val);
+                    [&](int vtx, double val, Res32 &r, int stage) {
+                        const int li = vtx < hub_n ? vtx : vtx >= sink ? vtx + hub_sink : -1;
+                        if (stage == 0) {
+                            r.lds = li >= 0;
+                            if (li >= 0) r.v = atomicAdd(&hub32[li], (float)val);
+                        } else if (li < 0) {
+                            r.v = atomic_add_wg(&rec[vtx].r, (float)val);
+                        }
                     },
-                    [&](int, int dv, double val, float old, bool) -> int {
+                    [&](int, int dv, double val, const Res32 &r, bool) -> int {
+                        const float old = r.v;
                         const float nw = old + (float)val;
                         const double thr = P.thr_scale * (double)dv;
                         return (((double)old < thr && thr <= (double)nw) ? 1 : 0) |
@@ -718,15 +752,17 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             };
             // 2a. sweep A: r64 = pushes of x32 over supp (every target is already
             // listed; r64 is still zero everywhere); hub targets sum in LDS
-            const int hub64_n = (int)min((int64_t)(a.hub_bytes / 8), a.n);
-            for (int i = tid; i < hub64_n; i += NT) hub64[i] = 0.0;
+            const int hub64_cap = a.hub_bytes / 8;
+            const int hub64_n = hub64_cap > kWave ? (int)min((int64_t)(hub64_cap - kWave), a.n) : 0;
+            const int hub64_sink = hub64_cap > kWave ? hub64_cap - kWave - sink : INT32_MIN;
+            for (int i = tid; i < hub64_cap; i += NT) hub64[i] = 0.0;
             __syncthreads();
-            wave_edges(
+            wave_edges<int>(
                 S32, pcs, npc_sup, pw, &s_work, W, colx, load_sup,
-                [&](int vtx, double v) {
-                    if (vtx < hub64_n) atomicAdd(&hub64[vtx], v);
-                    else atomic_add_wg(&rec[vtx].r64, v);
-                    return 0;
+                [&](int vtx, double v, int &, int stage) {
+                    const int li = vtx < hub64_n ? vtx : vtx >= sink ? vtx + hub64_sink : -1;
+                    if (stage == 0 && li >= 0) atomicAdd(&hub64[li], v);
+                    if (stage == 1 && li < 0) atomic_add_wg(&rec[vtx].r64, v);
                 },
                 [&](int, int, double, int, bool) -> int { return 0; }, qn, &s_ncnt, tl, &s_tl, sink);
             __syncthreads();
@@ -828,9 +864,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 cert_ops += cvol;
                 __syncthreads();
                 n_corr += s_vol;
-                wave_edges(
+                wave_edges<double>(
                     cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
-                    [&](int vtx, double val) { return atomic_add_wg(&rec[vtx].r64, val); },
+                    [&](int vtx, double val, double &old, int stage) {
+                        if (stage == 1) old = atomic_add_wg(&rec[vtx].r64, val);
+                    },
                     [&](int vtx, int dv, double val, double old, bool act) -> int {
                         if (!act) return 0;
                         NodeRec &rv = rec[vtx];
