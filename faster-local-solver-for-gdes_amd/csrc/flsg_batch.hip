@@ -463,20 +463,7 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
     }
 }
 
-// hub pieces of the entries [0, cnt) of a list whose degree is given by deg(i)
-template <int NT, typename Deg>
-__device__ __forceinline__ void build_pieces(unsigned int cnt, int piece_w, Deg deg, uint32_t *pcs,
-                                             unsigned int *s_npc) {
-    for (unsigned int i = threadIdx.x; i < cnt; i += NT) {
-        const int d = deg(i);
-        if (d > kHubW) {
-            const unsigned int np = (unsigned int)((d + piece_w - 1) / piece_w);
-            const unsigned int b = atomicAdd(s_npc, np);
-            for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
-        }
-    }
-}
-
+// hub pieces of list entry i (degree d): (i << 8) | piece
 __device__ __forceinline__ void add_pieces(unsigned int i, int d, int pw, uint32_t *pcs, unsigned int *s_npc) {
     if (d > kHubW) {
         const unsigned int np = (unsigned int)((d + pw - 1) / pw);
@@ -740,16 +727,38 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 s_work = 0;
             }
             __syncthreads();
-            build_pieces<NT>(S32, pw, [&](unsigned int i) { return rec[sup[i]].d; }, pcs, &s_npc);
+            // entries of the support sweep: row, degree and push value of x32_u
+            for (unsigned int i0 = tid; i0 < S32; i0 += NT * kUnroll) {
+                int uu[kUnroll], ss[kUnroll];
+                float4 lo[kUnroll];
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    const unsigned int i = i0 + j * NT;
+                    uu[j] = i < S32 ? sup[i] : -1;
+                }
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    ss[j] = 0;
+                    if (uu[j] >= 0) {
+                        __builtin_memcpy(&lo[j], rec + uu[j], 16);
+                        ss[j] = indptr[uu[j]];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    if (uu[j] < 0) continue;
+                    const unsigned int i = i0 + j * NT;
+                    Ent e;
+                    e.st = ss[j];
+                    e.d = __float_as_int(lo[j].z);
+                    e.v = bpush_value(P, (double)lo[j].y, e.d);
+                    ent[i] = e;
+                    add_pieces(i, e.d, pw, pcs, &s_npc);
+                }
+            }
             __syncthreads();
             const unsigned int npc_sup = s_npc;
-            // source u of a support sweep: v = push value of x32_u
-            auto load_sup = [&](unsigned int i, int &st, int &d, double &v) {
-                const int u = sup[i];
-                st = indptr[u];
-                d = rec[u].d;
-                v = bpush_value(P, (double)rec[u].x, d);
-            };
             // 2a. sweep A: r64 = pushes of x32 over supp (every target is already
             // listed; r64 is still zero everywhere); hub targets sum in LDS
             const int hub64_cap = a.hub_bytes / 8;
@@ -758,7 +767,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             for (int i = tid; i < hub64_cap; i += NT) hub64[i] = 0.0;
             __syncthreads();
             wave_edges<int>(
-                S32, pcs, npc_sup, pw, &s_work, W, colx, load_sup,
+                S32, pcs, npc_sup, pw, &s_work, W, colx, load_ent,
                 [&](int vtx, double v, int &, int stage) {
                     const int li = vtx < hub64_n ? vtx : vtx >= sink ? vtx + hub64_sink : -1;
                     if (stage == 0 && li >= 0) atomicAdd(&hub64[li], v);
@@ -896,23 +905,58 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         __syncthreads();
         const unsigned long long obase = s_base;
         const bool fits = obase + S <= (unsigned long long)a.out_cap;
-        for (unsigned int i = tid; i < S; i += NT) {
-            const int u = sup[i];
-            if (fits) {
-                a.out_ids[obase + i] = a.out_map ? a.out_map[u] : u;
-                a.out_vals[obase + i] = rec[u].x64;
-            }
-        }
+        // one pass over the touched list: certificate value, output of the
+        // support (listed by flag in the FP32 path, by x != 0 in the FP64 path,
+        // whose x only grows; in any order), record reset; the loads of kUnroll
+        // entries are issued before any is used
+        if (tid == 0) s_ncnt = 0;
         __syncthreads();
         double cmax = 0.0;
-        for (unsigned int i = tid; i < T; i += NT) {
-            NodeRec &rv = rec[tl[i]];
-            const double r64 = rv.r64;
-            const int d = rv.d;
-            if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)d));
-            NodeRec z{};
-            z.d = d;
-            rv = z;
+        for (unsigned int i0 = 0; i0 < T; i0 += NT * kUnroll) {
+            int vv[kUnroll];
+            float4 lo[kUnroll];
+            double2 hi[kUnroll];
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                const unsigned int i = i0 + j * NT + tid;
+                vv[j] = i < T ? tl[i] : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                hi[j] = make_double2(0.0, 0.0);
+                if (vv[j] >= 0) {
+                    const char *pr = reinterpret_cast<const char *>(rec + vv[j]);
+                    __builtin_memcpy(&lo[j], pr, 16);
+                    __builtin_memcpy(&hi[j], pr + 16, 16);
+                }
+            }
+            bool is_sup[kUnroll];
+            int id[kUnroll];
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                is_sup[j] = false;
+                id[j] = vv[j];
+                if (vv[j] < 0) continue;
+                const int d = __float_as_int(lo[j].z);
+                const double r64 = hi[j].x;
+                if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)d));
+                is_sup[j] = P.fp64 ? hi[j].y != 0.0 : (__float_as_uint(lo[j].w) & kSeen) != 0u;
+                if (is_sup[j] && a.out_map) id[j] = a.out_map[vv[j]];
+                char *pr = reinterpret_cast<char *>(rec + vv[j]);
+                const float4 zlo = make_float4(0.f, 0.f, lo[j].z, 0.f);   // keep d
+                const double2 zhi = make_double2(0.0, 0.0);
+                __builtin_memcpy(pr, &zlo, 16);
+                __builtin_memcpy(pr + 16, &zhi, 16);
+            }
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                const long long k = wave_append(is_sup[j], &s_ncnt);
+                if (k >= 0 && fits && (unsigned long long)k < S) {
+                    a.out_ids[obase + k] = id[j];
+                    a.out_vals[obase + k] = hi[j].y;
+                }
+            }
         }
         cmax = block_max<NT>(cmax, s_redd);
         const unsigned long long cops = block_sum_u64<NT>(cert_ops, s_red);
@@ -923,7 +967,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             rr.iters = iters;
             rr.oper_cert = (int64_t)cops;
             rr.n_corr = (int64_t)n_corr;
-            rr.supp = (int64_t)S;
+            rr.supp = (int64_t)min(s_ncnt, S);   // entries written (== S: every support record is listed)
             rr.nodes = (int64_t)fnodes;
             rr.cert = cmax;
             rr.ms = (double)(t_end - t_start) * 1e-5;       // 100 MHz constant clock
