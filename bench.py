@@ -26,13 +26,15 @@ for _p in (PKG, REPO):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
-# algorithmic bytes (FP32 push / certificate), per the documented model:
-#   edge = 4 col-idx + 8 r_v RMW + 4 d_v;  node = 8 indptr + 4 r_u R + 4 r_u W
-#          + 8 x_u RMW + 4 d_u + 4 frontier id R + 4 next-frontier W
-B_EDGE_FP32, B_NODE_FP32 = 16, 36
-#   certificate: per support edge 4 col-idx + 4 x gather; per support node
-#   8 indptr + 4 x + 4 d + 8 r W
-B_CERT_EDGE, B_CERT_NODE = 8, 24
+# algorithmic bytes of the batched FP32 path, per unit (the model the
+# roofline's "achieved" uses):
+#   push, per frontier entry: 4 queue id + 16 record read (r, x, d) + 8 record
+#     write (r, x) + 4 row start + 4 next-frontier id                     = 36
+#   push, per edge: 8 column entry {v, d_v} + 4 r_v read + 4 r_v write    = 16
+#   certificate, per support / correction edge: 8 column entry + 16 r64 RMW = 24
+#   per touched record: 16 read (r64, x) + 16 reset write                 = 32
+#   per output entry: 4 id + 8 value                                       = 12
+B_NODE_FP32, B_EDGE_FP32, B_CERT_EDGE, B_TOUCHED, B_OUT = 36, 16, 24, 32, 12
 HBM_PEAK_GBS = 8000.0
 
 
@@ -116,7 +118,9 @@ def main():
     nodes = sum(r["nodes"] for r in res)
     cert_edges = sum(r["oper_cert"] for r in res)
     supp = sum(r["supp"] for r in res)
-    bytes_step = B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * cert_edges + B_CERT_NODE * supp
+    touched = sum(r["touched"] for r in res)
+    bytes_step = (B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * cert_edges + B_TOUCHED * touched
+                  + B_OUT * supp)
     kern_ms = float(np.mean([b.ms for b in runs]))
     achieved = bytes_step / (kern_ms * 1e-3) / 1e9
 

@@ -33,6 +33,10 @@
 // across workgroups.
 #include <math.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "flsg_internal.h"
 
 namespace flsg {
@@ -817,7 +821,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         const double2 hv = make_double2(rt, xd);
                         __builtin_memcpy(reinterpret_cast<char *>(rec + vv[j]) + 16, &hv, 16);
                         rec[vv[j]].fl = (xd != 0.0 ? kSeen : 0u) | (enq ? kInQ : 0u);
-                        cert_ops += (xd != 0.0 ? (unsigned long long)d : 0ull) + 1ull;
+                        if (xd != 0.0) cert_ops += (unsigned long long)d;
                     }
                     const long long slot = wave_append(enq, &s_cnt);
                     if (slot >= 0) qc[slot] = vv[j];
@@ -973,6 +977,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             rr.ms = (double)(t_end - t_start) * 1e-5;       // 100 MHz constant clock
             rr.ms_push = (double)(t_push - t_start) * 1e-5;
             rr.status = fits ? status : FLSG_ECAP;
+            rr.touched = (int32_t)T;
             a.res[q] = rr;
             a.out_off[q] = fits ? (int64_t)obase : -1;
         }
@@ -1156,10 +1161,22 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     int hub = g->batch_hub_bytes;
     if (hub < 0) hub = block == 1024 ? 120 * 1024 : block == 512 ? 48 * 1024 : 24 * 1024;
     a.hub_bytes = hub;
-    const void *fn = block == 1024  ? reinterpret_cast<const void *>(&k_batch<1024>)
-                     : block == 256 ? reinterpret_cast<const void *>(&k_batch<256>)
-                                    : reinterpret_cast<const void *>(&k_batch<512>);
-    FLSG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, hub));
+    const int bi = block == 1024 ? 0 : block == 256 ? 1 : 2;
+    const void *fn = bi == 0   ? reinterpret_cast<const void *>(&k_batch<1024>)
+                     : bi == 1 ? reinterpret_cast<const void *>(&k_batch<256>)
+                               : reinterpret_cast<const void *>(&k_batch<512>);
+    // the dynamic-LDS limit is raised once per device and kernel (a call per
+    // launch would sit on the serving loop's launch path)
+    static std::mutex attr_mu;
+    static std::map<std::pair<int, int>, int> attr_set;
+    {
+        std::lock_guard<std::mutex> lk(attr_mu);
+        int &have = attr_set[std::make_pair(g->device, bi)];
+        if (hub > have) {
+            FLSG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, hub));
+            have = hub;
+        }
+    }
     if (block == 1024) hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), hub, st, a);
     else if (block == 256) hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), hub, st, a);
     else hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), hub, st, a);

@@ -27,11 +27,11 @@ class Result(C.Structure):
     _fields_ = [
         ("oper", C.c_int64), ("iters", C.c_int64), ("oper_cert", C.c_int64), ("n_corr", C.c_int64),
         ("supp", C.c_int64), ("nodes", C.c_int64), ("cert", C.c_double), ("ms", C.c_double),
-        ("ms_push", C.c_double), ("status", C.c_int32), ("pad", C.c_int32),
+        ("ms_push", C.c_double), ("status", C.c_int32), ("touched", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Query(C.Structure):

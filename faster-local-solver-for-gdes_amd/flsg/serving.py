@@ -112,7 +112,9 @@ class _Lane:
         self.n_slots = n_slots
         self.stream = torch.cuda.Stream(device=dev)
         self.qcap = srv.max_batch
-        self.h_q = (_abi.Query * self.qcap)()
+        # queries are staged in pinned memory and uploaded asynchronously on the lane's stream
+        self.h_qbuf = torch.empty(self.qcap * C.sizeof(_abi.Query), dtype=torch.uint8).pin_memory()
+        self.h_q = (_abi.Query * self.qcap).from_address(self.h_qbuf.data_ptr())
         self.d_q = torch.empty(self.qcap * C.sizeof(_abi.Query), dtype=torch.uint8, device=dev)
         self.d_res = torch.empty(self.qcap * C.sizeof(_abi.Result), dtype=torch.uint8, device=dev)
         self.h_res = torch.empty(self.qcap * C.sizeof(_abi.Result), dtype=torch.uint8).pin_memory()
@@ -176,9 +178,8 @@ class Server:
             q.alpha = r.alpha
             q.eps = r.eps
         nbytes = len(reqs) * C.sizeof(_abi.Query)
-        src = torch.frombuffer(bytearray(bytes(lane.h_q)[:nbytes]), dtype=torch.uint8)
         with torch.cuda.stream(lane.stream):
-            lane.d_q[:nbytes].copy_(src, non_blocking=False)
+            lane.d_q[:nbytes].copy_(lane.h_qbuf[:nbytes], non_blocking=True)
             _check(self.lib.flsg_batch_launch(self.dg.handle, _ptr(lane.d_q), len(reqs), lane.slot_base,
                                               min(lane.n_slots, len(reqs)), _ptr(lane.ids), _ptr(lane.vals),
                                               _ptr(lane.d_off), lane.out_cap, _ptr(lane.d_ctl), _ptr(lane.d_res),
@@ -342,3 +343,75 @@ def run_plain(g: CSRGraph, requests: list, hk_tau: float = 5.0, hk_eps: float = 
     st = Server._stats(order, wall, len(order), [])
     st["certified"] = True
     return st
+
+
+def serving_bench(g: CSRGraph | None = None, rates=(1000, 2000, 5000, 10000, 20000, 50000), seconds: float = 2.0,
+                  max_queries: int = 4000, plain_queries: int = 600, insert_every: float = 1.0,
+                  insert_size: int = 10000, precisions=(_abi.FLSG_FP32, _abi.FLSG_FP64), log=print) -> list:
+    """The serving mix on the 2,394,385-node power-law graph: Poisson arrivals
+    at each offered rate (PPR 70% alpha=0.1, Katz 20% beta=1/(d_max+1), heat
+    kernel 10% t=5 order 40; eps=1e-7), min(rate*seconds, max_queries)
+    requests per rate, a batch of insert_size random edge inserts every
+    insert_every seconds (at least one per rate), served by the batched path
+    at each precision and -- plain_queries requests per rate -- by the plain
+    FP64 single-source path.  Returns one dict per (path, rate)."""
+    from . import graph as _graph
+    g = g if g is not None else _graph.powerlaw_graph_large()
+    out = []
+    for prec in precisions:
+        name = "batched-fp32" if prec == _abi.FLSG_FP32 else "batched-fp64"
+        srv = Server(g, precision=prec)
+        try:
+            srv.run(mixed_requests(g, rate=1e7, count=256, seed=99))      # warm-up (not reported)
+            for rate in rates:
+                count = int(min(rate * seconds, max_queries))
+                reqs = mixed_requests(g, rate=float(rate), count=count, seed=int(rate))
+                span = reqs[-1].arrival
+                ins = random_insert_batches(g, every=insert_every, size=insert_size,
+                                            count=max(1, int(span // insert_every)), seed=int(rate) + 7)
+                st = srv.run(reqs, inserts=ins)
+                rec = {"path": name, "offered_qps": rate, **st}
+                out.append(rec)
+                log(rec)
+        finally:
+            srv.close()
+    if plain_queries > 0:
+        dg = DeviceGraph(g, relabel=True)
+        try:
+            run_plain(g, mixed_requests(g, rate=1e7, count=8, seed=98), dg=dg)      # warm-up
+            for rate in rates:
+                reqs = mixed_requests(g, rate=float(rate), count=plain_queries, seed=int(rate))
+                st = run_plain(g, reqs, dg=dg)
+                rec = {"path": "plain-fp64", "offered_qps": rate, **st}
+                out.append(rec)
+                log(rec)
+        finally:
+            dg.close()
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="serving-mix benchmark (one GPU)")
+    ap.add_argument("--rates", default="1000,2000,5000,10000,20000,50000")
+    ap.add_argument("--seconds", type=float, default=2.0)
+    ap.add_argument("--max-queries", type=int, default=4000)
+    ap.add_argument("--plain-queries", type=int, default=600)
+    ap.add_argument("--out", default=None, help="JSON-lines file for the results")
+    args = ap.parse_args()
+    fh = open(args.out, "w") if args.out else None
+
+    def _log(rec):
+        line = json.dumps(rec)
+        print(line, flush=True)
+        if fh:
+            fh.write(line + "\n")
+            fh.flush()
+
+    serving_bench(rates=tuple(int(r) for r in args.rates.split(",")), seconds=args.seconds,
+                  max_queries=args.max_queries, plain_queries=args.plain_queries, log=_log)
+    if fh:
+        fh.close()
+    sys.exit(0)

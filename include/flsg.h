@@ -74,7 +74,8 @@ typedef struct {
 typedef struct {
     int64_t oper;        /* the reference's operation count                   */
     int64_t iters;       /* frontier iterations / global sweeps / HK levels   */
-    int64_t oper_cert;   /* edge visits of the FP64 certificate + correction  */
+    int64_t oper_cert;   /* batched FP32: support edges of the FP64 residual
+                            recompute + edges of the FP64 correction push     */
     int64_t n_corr;      /* nodes pushed by the FP64 correction               */
     int64_t supp;        /* |supp x|                                          */
     int64_t nodes;       /* sum over iterations of the frontier size          */
@@ -82,7 +83,7 @@ typedef struct {
     double ms;           /* device time of the solve, milliseconds            */
     double ms_push;      /* batched: time of the FP32 push (before the FP64 certificate) */
     int32_t status;      /* FLSG_OK or error code for this query              */
-    int32_t pad;
+    int32_t touched;     /* batched: records the query touched (and reset)    */
 } flsg_result;
 
 /* Optional per-iteration frontier trace (device buffers): iteration t
