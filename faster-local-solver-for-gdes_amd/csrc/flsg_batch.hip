@@ -173,178 +173,6 @@ __device__ __forceinline__ uint32_t fl_and(uint32_t *p, uint32_t v) {
 }
 
 
-// Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), FP32
-// levels with FP64 threshold tests.  Level j's residual lives in rec.r (j even)
-// or rec.rh[0] (j odd); each level keeps a list of the nodes it touched so the
-// level array can be cleared afterwards (the reference zeroes the whole level,
-// hk_solver.py:282-285); the per-entry push value is parked in rec.x64.
-template <int NT>
-__device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, NodeRec *rec, int32_t *qa,
-                         int32_t *qb, int32_t *tls, int32_t *sup, int *s_incl, int *s_start, double *s_val,
-                         int *s_scan, unsigned long long *s_red, unsigned int *s_cnt, unsigned int *s_ncnt,
-                         unsigned int *s_sup, unsigned int *s_tn, unsigned int *s_tnn, unsigned long long *s_base,
-                         uint64_t t_start) {
-    const int tid = threadIdx.x;
-    const int N = a.hk_n;
-    int32_t *tl_cur = tls;                    // nodes with a residual at the current level
-    int32_t *tl_nxt = tls + a.n;              // nodes touched for the next level
-    const int s = (int)qq.seed;
-    if (tid == 0) {
-        rec[s].r = 1.0f;                       // hk_solver.py:223
-        qa[0] = s;
-        *s_cnt = 1;
-        *s_sup = 0;
-        tl_cur[0] = s;
-        *s_tn = 1;
-    }
-    __syncthreads();
-    int32_t *qc = qa, *qn = qb;
-    unsigned long long oper = 0, fnodes = 0;
-    int64_t levels = 0;
-    for (int j = 0; j < N; j++) {
-        const unsigned int cnt = *s_cnt;
-        if (cnt == 0) break;                   // hk_solver.py:237
-        levels++;
-        fnodes += cnt;
-        const bool odd = (j & 1) != 0;
-        const bool last = (j + 1 == N);
-        const double coef = a.hk_tau / ((double)j + 1.);
-        const double psic = a.hk_psi[j], psin = a.hk_psi[j + 1];
-        __syncthreads();
-        if (tid == 0) {
-            *s_ncnt = 0;
-            *s_tnn = 0;
-        }
-        unsigned long long vol = 0;
-        for (unsigned int i = tid; i < cnt; i += NT) {
-            const int u = qc[i];
-            NodeRec &ru = rec[u];
-            const int d = ru.d;
-            float &rc = odd ? ru.rh[0] : ru.r;
-            const float tmp = rc;
-            double v = 0.0;
-            if (!(fabs((double)tmp) < ((a.hk_ete * (double)d) / (double)N) / psic)) {   // hk_solver.py:244
-                const float xo = ru.x;
-                ru.x = xo + tmp;
-                if (xo == 0.0f) sup[atomicAdd(s_sup, 1u)] = u;
-                rc = 0.0f;
-                v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
-                vol += (unsigned long long)d;
-            }
-            ru.x64 = v;
-        }
-        oper += block_sum_u64<NT>(vol, s_red);
-        for (unsigned int base = 0; base < cnt; base += NT) {
-            const unsigned int i = base + tid;
-            int d = 0, st = 0;
-            double v = 0.0;
-            if (i < cnt) {
-                const int u = qc[i];
-                NodeRec &ru = rec[u];
-                v = ru.x64;
-                ru.x64 = 0.0;
-                if (v != 0.0) {
-                    d = ru.d;
-                    st = a.indptr[u];
-                }
-            }
-            int tot;
-            const int incl = block_incl_scan<NT>(d, s_scan, &tot);
-            s_incl[tid] = incl;
-            s_start[tid] = st;
-            s_val[tid] = v;
-            __syncthreads();
-            for (int e0 = 0; e0 < tot; e0 += NT) {
-                const int e = e0 + tid;
-                bool cross = false, first = false;
-                int vtx = 0;
-                if (e < tot) {
-                    const int k = upper_bound_lds(s_incl, 0, NT, e);
-                    const int o = e - (k ? s_incl[k - 1] : 0);
-                    vtx = a.indices[s_start[k] + o];
-                    const float val = (float)s_val[k];
-                    NodeRec &rv = rec[vtx];
-                    if (last) {
-                        const float old = atomic_add_wg(&rv.x, val);   // hk_solver.py:254-256
-                        if (old == 0.0f) sup[atomicAdd(s_sup, 1u)] = vtx;
-                    } else {
-                        float *rn = odd ? &rv.r : &rv.rh[0];
-                        const float old = atomic_add_wg(rn, val);
-                        const float nw = old + val;
-                        const double thr = ((a.hk_ete * (double)rv.d) / (double)N) / psin;
-                        cross = (double)old < thr && thr <= (double)nw;      // hk_solver.py:258
-                        first = old == 0.0f;
-                    }
-                }
-                const long long slot = wave_append(cross, s_ncnt);
-                if (slot >= 0) qn[slot] = vtx;
-                const long long ts = wave_append(first, s_tnn);
-                if (ts >= 0) tl_nxt[ts] = vtx;
-            }
-            __syncthreads();
-        }
-        // drop what is left of level j (hk_solver.py:282-285)
-        const unsigned int tn = *s_tn;
-        for (unsigned int i = tid; i < tn; i += NT) {
-            NodeRec &rv = rec[tl_cur[i]];
-            if (odd) rv.rh[0] = 0.0f;
-            else rv.r = 0.0f;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            *s_cnt = *s_ncnt;
-            *s_tn = *s_tnn;
-        }
-        int32_t *t = qc;
-        qc = qn;
-        qn = t;
-        int32_t *tt = tl_cur;
-        tl_cur = tl_nxt;
-        tl_nxt = tt;
-        __syncthreads();
-    }
-    // the level the loop stopped at still holds sub-threshold residual
-    {
-        const unsigned int tn = *s_tn;
-        const bool odd = (levels & 1) != 0;
-        for (unsigned int i = tid; i < tn; i += NT) {
-            NodeRec &rv = rec[tl_cur[i]];
-            if (odd) rv.rh[0] = 0.0f;
-            else rv.r = 0.0f;
-        }
-    }
-    __syncthreads();
-    const unsigned int S = *s_sup;
-    if (tid == 0) *s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
-    __syncthreads();
-    const unsigned long long obase = *s_base;
-    const bool fits = obase + S <= (unsigned long long)a.out_cap;
-    const double et = exp(a.hk_tau);
-    for (unsigned int i = tid; i < S; i += NT) {
-        const int u = sup[i];
-        NodeRec &ru = rec[u];
-        if (fits) {
-            a.out_ids[obase + i] = a.out_map ? a.out_map[u] : u;
-            a.out_vals[obase + i] = (double)ru.x / et;                         // hk_solver.py:291
-        }
-        ru.x = 0.0f;
-    }
-    if (tid == 0) {
-        const uint64_t t_end = now_ticks();
-        flsg_result rr{};
-        rr.oper = (int64_t)oper;
-        rr.iters = levels;
-        rr.supp = (int64_t)S;
-        rr.nodes = (int64_t)fnodes;
-        rr.ms = (double)(t_end - t_start) * 1e-5;
-        rr.ms_push = rr.ms;
-        rr.status = fits ? FLSG_OK : FLSG_ECAP;
-        a.res[q] = rr;
-        a.out_off[q] = fits ? (int64_t)obase : -1;
-    }
-    __syncthreads();
-}
-
 constexpr int kUnroll = 4;           // edges in flight per lane
 constexpr int kHubW = 512;           // entries with more edges go through pieces
 
@@ -495,6 +323,207 @@ struct Res64 {
 __device__ __forceinline__ bool pristine32(float old) { return __float_as_uint(old) == 0u; }
 __device__ __forceinline__ bool pristine64(double old) { return __double_as_longlong(old) == 0ll; }
 
+
+// Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), FP32
+// levels with FP64 threshold tests.  Level j's residual lives in rec.r (j even)
+// or rec.rh[0] (j odd) -- for the hub cache's nodes in one of two LDS level
+// arrays.  Each level keeps the list of nodes it touched; the node phase walks
+// that list, pushes every node at or above the level threshold (level
+// residuals only grow, so this is the set the reference's crossing queue
+// holds, hk_solver.py:258) and clears every entry (the reference zeroes the
+// whole level, hk_solver.py:282-285), so no separate clearing pass is needed.
+template <int NT>
+__device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, NodeRec *rec, int32_t *qa,
+                         int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs, WaveLds &W, float *hub32,
+                         unsigned int *s_cnt, unsigned int *s_ncnt, unsigned int *s_sup, unsigned int *s_npc,
+                         unsigned int *s_work, unsigned int *s_tn, unsigned long long *s_vol,
+                         unsigned long long *s_base, uint64_t t_start, int sink) {
+    const int tid = threadIdx.x;
+    const int N = a.hk_n;
+    const int s = (int)qq.seed;
+    const int pw = a.piece_w;
+    const int32_t *indptr = a.indptr;
+    // hub cache: two level arrays of hub_n floats, then kWave sink slots
+    const int cap = a.hub_bytes / 4;
+    const int hub_n = cap > 3 * kWave ? (int)min((int64_t)((cap - kWave) / 2), a.n) : 0;
+    const int hsink = hub_n > 0 ? 2 * hub_n - sink : INT32_MIN;   // LDS index of an idle lane = vtx + hsink
+    for (int i = tid; i < a.hub_bytes / 8; i += NT) reinterpret_cast<double *>(hub32)[i] = 0.0;
+    __syncthreads();
+    if (tid == 0) {
+        if (s < hub_n) hub32[s] = 1.0f;   // hk_solver.py:223
+        else rec[s].r = 1.0f;
+        qa[0] = s;
+        *s_cnt = 1;
+        *s_sup = 0;
+        *s_vol = 0;
+        *s_tn = 0;
+    }
+    __syncthreads();
+    int32_t *lc = qa, *ln = qb;
+    int64_t levels = 0;
+    auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
+        const Ent e = ent[i];
+        st = e.st;
+        d = e.d;
+        v = e.v;
+    };
+    for (int j = 0; j < N; j++) {
+        const unsigned int cnt = *s_cnt;
+        if (cnt == 0) break;   // hk_solver.py:237
+        levels++;
+        const bool odd = (j & 1) != 0;
+        const bool last = (j + 1 == N);
+        const double coef = a.hk_tau / ((double)j + 1.);
+        const double psic = a.hk_psi[j];
+        float *hc = hub32 + (odd ? hub_n : 0);
+        const int nb = odd ? 0 : hub_n;   // LDS base of level j + 1
+        __syncthreads();
+        if (tid == 0) {
+            *s_ncnt = 0;
+            *s_npc = 0;
+            *s_work = 0;
+        }
+        __syncthreads();
+        unsigned long long vol = 0;
+        unsigned int pushers = 0;
+        for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
+            int uu[kUnroll], ss[kUnroll];
+            float4 lo[kUnroll];
+            float ro[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                const unsigned int i = i0 + k * NT;
+                uu[k] = i < cnt ? lc[i] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                lo[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                ro[k] = 0.f;
+                ss[k] = 0;
+                if (uu[k] >= 0) {
+                    __builtin_memcpy(&lo[k], rec + uu[k], 16);
+                    ro[k] = uu[k] < hub_n ? hc[uu[k]] : odd ? rec[uu[k]].rh[0] : lo[k].x;
+                    ss[k] = indptr[uu[k]];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                if (uu[k] < 0) continue;
+                const unsigned int i = i0 + k * NT;
+                const int u = uu[k];
+                const int d = __float_as_int(lo[k].z);
+                const float tmp = ro[k];
+                if (u < hub_n) hc[u] = 0.0f;            // hk_solver.py:282-285
+                else if (odd) rec[u].rh[0] = 0.0f;
+                else rec[u].r = 0.0f;
+                Ent e;
+                e.st = 0;
+                e.d = 0;
+                e.v = 0.0;
+                if (!(fabs((double)tmp) < ((a.hk_ete * (double)d) / (double)N) / psic)) {   // hk_solver.py:244
+                    const float xo = lo[k].y;
+                    rec[u].x = xo + tmp;
+                    if (__float_as_uint(xo) == 0u) sup[atomicAdd(s_sup, 1u)] = u;
+                    e.st = ss[k];
+                    e.d = d;
+                    e.v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
+                    vol += (unsigned long long)d;
+                    pushers++;
+                    add_pieces(i, d, pw, pcs, s_npc);
+                }
+                ent[i] = e;
+            }
+        }
+        if (vol) atomicAdd(s_vol, vol);
+        if (pushers) atomicAdd(s_tn, pushers);
+        __syncthreads();
+        if (!last) {
+            // next level's residual; first touch lists the node for level j + 1
+            wave_edges<Res32>(
+                cnt, pcs, *s_npc, pw, s_work, W, a.colx, load_ent,
+                [&](int vtx, double val, Res32 &r, int stage) {
+                    const int li = vtx < hub_n ? nb + vtx : vtx >= sink ? vtx + hsink : -1;
+                    if (stage == 0) {
+                        r.lds = li >= 0;
+                        if (li >= 0) r.v = atomicAdd(&hub32[li], (float)val);
+                    } else if (li < 0) {
+                        r.v = atomic_add_wg(odd ? &rec[vtx].r : &rec[vtx].rh[0], (float)val);
+                    }
+                },
+                [&](int, int, double val, const Res32 &r, bool) -> int {
+                    return (pristine32(r.v) && r.v + (float)val != 0.0f) ? 2 : 0;
+                },
+                ln, s_ncnt, ln, s_ncnt, sink);
+        } else {
+            // last level pushes straight into x (hk_solver.py:254-256)
+            wave_edges<float>(
+                cnt, pcs, *s_npc, pw, s_work, W, a.colx, load_ent,
+                [&](int vtx, double val, float &old, int stage) {
+                    if (stage == 1) old = atomic_add_wg(&rec[vtx].x, (float)val);
+                },
+                [&](int, int, double val, float old, bool) -> int {
+                    return (pristine32(old) && old + (float)val != 0.0f) ? 2 : 0;
+                },
+                ln, s_ncnt, sup, s_sup, sink);
+        }
+        __syncthreads();
+        if (tid == 0) *s_cnt = *s_ncnt;
+        int32_t *t = lc;
+        lc = ln;
+        ln = t;
+        __syncthreads();
+    }
+    const unsigned int S = *s_sup;
+    if (tid == 0) *s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
+    __syncthreads();
+    const unsigned long long obase = *s_base;
+    const bool fits = obase + S <= (unsigned long long)a.out_cap;
+    const double et = exp(a.hk_tau);
+    for (unsigned int i0 = tid; i0 < S; i0 += NT * kUnroll) {
+        int uu[kUnroll], id[kUnroll];
+        float xv[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            const unsigned int i = i0 + k * NT;
+            uu[k] = i < S ? sup[i] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            id[k] = 0;
+            xv[k] = 0.f;
+            if (uu[k] >= 0) {
+                xv[k] = rec[uu[k]].x;
+                id[k] = a.out_map ? a.out_map[uu[k]] : uu[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            if (uu[k] < 0) continue;
+            const unsigned int i = i0 + k * NT;
+            if (fits) {
+                a.out_ids[obase + i] = id[k];
+                a.out_vals[obase + i] = (double)xv[k] / et;   // hk_solver.py:291
+            }
+            rec[uu[k]].x = 0.0f;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint64_t t_end = now_ticks();
+        flsg_result rr{};
+        rr.oper = (int64_t)*s_vol;
+        rr.iters = levels;
+        rr.supp = (int64_t)S;
+        rr.nodes = (int64_t)*s_tn;
+        rr.ms = (double)(t_end - t_start) * 1e-5;
+        rr.ms_push = rr.ms;
+        rr.status = fits ? FLSG_OK : FLSG_ECAP;
+        a.res[q] = rr;
+        a.out_off[q] = fits ? (int64_t)obase : -1;
+    }
+    __syncthreads();
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __shared__ WaveLds s_w[NT / kWave];
@@ -502,11 +531,6 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __shared__ double s_redd[NT / kWave];
     __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_tnn, s_work, s_npc, s_tl;
     __shared__ unsigned long long s_base, s_vol;
-    // the heat-kernel path keeps its block-chunked form
-    __shared__ int s_incl[NT];
-    __shared__ int s_start[NT];
-    __shared__ double s_val[NT];
-    __shared__ int s_scan[NT / kWave];
     // hub cache: the residuals of nodes [0, H) live here for the whole query
     // (ids are degree-ranked on relabelled graphs, so these take the largest
     // share of the pushes): LDS atomics instead of memory-side ones
@@ -538,8 +562,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const uint64_t t_start = now_ticks();
         const flsg_query qq = a.queries[q];
         if (qq.kind == FLSG_HK) {
-            hk_query<NT>(a, qq, q, rec, qa, qb, (int32_t *)ent, sup, s_incl, s_start, s_val, s_scan, s_red, &s_cnt,
-                         &s_ncnt, &s_sup, &s_tn, &s_tnn, &s_base, t_start);
+            hk_query<NT>(a, qq, q, rec, qa, qb, ent, sup, pcs, W, hub32, &s_cnt, &s_ncnt, &s_sup, &s_npc, &s_work,
+                         &s_tn, &s_vol, &s_base, t_start, sink);
             continue;
         }
         const QParams P = qparams(qq);

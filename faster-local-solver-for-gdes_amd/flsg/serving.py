@@ -104,10 +104,11 @@ def slack_with_headroom(g: CSRGraph, headroom: int) -> SlackGraph:
 
 
 class _Lane:
-    def __init__(self, srv: "Server", idx: int, slot_base: int, n_slots: int):
+    def __init__(self, srv: "Server", idx: int, slot_base: int, n_slots: int, hk: bool = False):
         torch = srv.torch
         dev = srv.dg.device
         self.idx = idx
+        self.hk = hk                 # heat-kernel lane (takes short queries only when no HK query waits)
         self.slot_base = slot_base
         self.n_slots = n_slots
         self.stream = torch.cuda.Stream(device=dev)
@@ -121,7 +122,9 @@ class _Lane:
         self.d_off = torch.empty(self.qcap, dtype=torch.int64, device=dev)
         self.h_off = torch.empty(self.qcap, dtype=torch.int64).pin_memory()
         self.d_ctl = torch.zeros(2, dtype=torch.int64, device=dev)
-        self.out_cap = srv.out_per_lane
+        # output arena in (id, value) entries: sized to the lane's slots and the
+        # kind it serves (heat-kernel supports cover most of the graph)
+        self.out_cap = srv.out_per_lane or max(1 << 20, int(n_slots * srv.g.n * (0.9 if hk else 0.5)))
         self.ids = torch.empty(self.out_cap, dtype=torch.int32, device=dev)
         self.vals = torch.empty(self.out_cap, dtype=torch.float64, device=dev)
         self.busy = False
@@ -133,7 +136,7 @@ class Server:
     def __init__(self, g: CSRGraph, lanes: int = 4, slots: int = 0, block: int = 1024, headroom: int = 8,
                  max_batch: int = 256, out_per_lane: int | None = None, hk_tau: float = 5.0,
                  hk_eps: float = 1e-7, hk_terms: int = 40, precision: int = _abi.FLSG_FP32,
-                 device: str = "cuda:0"):
+                 hk_lanes: int | None = None, hk_slot_share: float = 0.5, device: str = "cuda:0"):
         h, perm, iperm = relabel_by_degree(g)
         self.g = g
         self.precision = precision
@@ -158,13 +161,28 @@ class Server:
         _check(lib.flsg_batch_reserve(self.dg.handle, int(slots), None))
         torch.cuda.synchronize()
         self.max_batch = max_batch
-        self.out_per_lane = out_per_lane or max(1 << 20, (g.n // 2) * max(1, slots // lanes))
-        per = slots // lanes
-        self.lanes = [_Lane(self, i, i * per, per if i < lanes - 1 else slots - i * per) for i in range(lanes)]
+        self.out_per_lane = out_per_lane
+        # A launch ends with its slowest query, and a heat-kernel query does ~7x
+        # the work of a PPR query: heat-kernel queries get lanes (and a slot
+        # share) of their own so they never hold PPR / Katz slots idle.
+        if hk_lanes is None:
+            hk_lanes = 1 if lanes >= 2 else 0
+        hk_lanes = max(0, min(hk_lanes, lanes - 1))
+        short = lanes - hk_lanes
+        hk_slots = int(round(slots * hk_slot_share)) if hk_lanes else 0
+        hk_slots = min(max(hk_slots, hk_lanes), slots - short)
+        sizes = [((slots - hk_slots) // short) + (1 if i < (slots - hk_slots) % short else 0) for i in range(short)]
+        if hk_lanes:
+            sizes += [(hk_slots // hk_lanes) + (1 if i < hk_slots % hk_lanes else 0) for i in range(hk_lanes)]
+        self.lanes = []
+        base = 0
+        for i, sz in enumerate(sizes):
+            self.lanes.append(_Lane(self, i, base, sz, hk=i >= short))
+            base += sz
         self.hk = _abi.HkConfig(tau=hk_tau, eps=hk_eps, n_terms=hk_terms)
         # per-kind support estimate used for admission (raised on overflow)
-        self.est = {_abi.FLSG_PPR: min(g.n, 400_000), _abi.FLSG_KATZ: min(g.n, 50_000),
-                    _abi.FLSG_HK: min(g.n, 300_000)}
+        self.est = {_abi.FLSG_PPR: max(1, g.n // 4), _abi.FLSG_KATZ: min(g.n, 50_000),
+                    _abi.FLSG_HK: max(1, (g.n * 4) // 5)}
         self.update_stream = torch.cuda.Stream(device=self.dg.device)
 
     # ------------------------------------------------------------------ plumbing
@@ -243,8 +261,10 @@ class Server:
         scaled by time_scale) and apply `inserts` at their times."""
         inserts = sorted(inserts or [], key=lambda b: b.at)
         order = sorted(requests, key=lambda r: r.arrival)
-        pending: deque = deque()
+        pend_short: deque = deque()
+        pend_hk: deque = deque()
         requeue: deque = deque()
+        has_hk_lane = any(l.hk for l in self.lanes)
         nxt = 0
         ins_i = 0
         self.torch.cuda.synchronize()
@@ -255,13 +275,15 @@ class Server:
         while done < total or ins_i < len(inserts):
             now = (time.perf_counter() - t0) / time_scale
             while nxt < total and order[nxt].arrival <= now:
-                pending.append(order[nxt])
+                r = order[nxt]
+                (pend_hk if has_hk_lane and r.kind == _abi.FLSG_HK else pend_short).append(r)
                 nxt += 1
             for lane in self.lanes:
                 if lane.busy and lane.event.query():
                     done += self._complete(lane, now, collect, requeue)
             while requeue:
-                pending.appendleft(requeue.pop())
+                r = requeue.pop()
+                (pend_hk if has_hk_lane and r.kind == _abi.FLSG_HK else pend_short).appendleft(r)
             due = ins_i < len(inserts) and inserts[ins_i].at <= now
             if due:
                 if any(l.busy for l in self.lanes):
@@ -270,13 +292,17 @@ class Server:
                 inserts[ins_i].applied = (time.perf_counter() - t0) / time_scale
                 ins_i += 1
                 continue
-            if not pending:
+            if not pend_short and not pend_hk:
                 if nxt >= total and not any(l.busy for l in self.lanes) and ins_i >= len(inserts):
                     break
                 time.sleep(0)
                 continue
             for lane in self.lanes:
-                if lane.busy or not pending:
+                if lane.busy:
+                    continue
+                # heat-kernel lanes fall back to short queries when no HK query waits
+                pending = pend_hk if (lane.hk and pend_hk) else pend_short
+                if not pending:
                     continue
                 batch, budget = [], lane.out_cap
                 while pending and len(batch) < lane.qcap:

@@ -110,3 +110,51 @@ def test_serving_inserts_in_place(small_graph):
         assert int(deg.sum()) == g2.m
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("hub_bytes", [0, -1])
+def test_serving_hk_and_ppr_with_and_without_hub_cache(small_graph, hub_bytes):
+    """Batched heat kernel and PPR with the LDS hub cache off (every level
+    residual in the node records) and on (the small graph fits in it whole):
+    heat kernel equal to the oracle up to FP32 rounding, PPR certified."""
+    g = small_graph
+    srv = serving.Server(g, lanes=2, slots=8, block=256, max_batch=16)
+    try:
+        assert srv.lib.flsg_batch_set_hub_cache(srv.dg.handle, hub_bytes) == 0
+        seeds = graph.uniform_seeds(g, 4, seed=17)
+        reqs = [serving.Request(rid=i, kind=_abi.FLSG_HK, seed=int(s), arrival=0.0) for i, s in enumerate(seeds)]
+        reqs += [serving.Request(rid=10 + i, kind=_abi.FLSG_PPR, seed=int(s), arrival=0.0, eps=1e-6)
+                 for i, s in enumerate(seeds)]
+        st = srv.run(reqs, collect=True)
+        assert st["ok"] == len(reqs) and st["certified"]
+        for r in reqs:
+            x = _dense(r, g.n)
+            if r.kind == _abi.FLSG_HK:
+                o = O.hk_push(g, r.seed, 5.0, 1e-7, local=True, n_terms=40)
+                assert np.abs(x - o["x"]).sum() <= 1e-5 * np.abs(o["x"]).sum()
+                assert abs(r.oper - o["oper"]) <= 0.02 * o["oper"] + 50
+            else:
+                rr = O.ppr_residual(g, r.seed, r.alpha, x)
+                assert (np.abs(rr) / (1e-6 * r.alpha * np.maximum(g.degree, 1))).max() < 1 + 1e-9
+    finally:
+        srv.close()
+
+
+def test_serving_hk_large(large_graph):
+    """Full-size graph (2.39M nodes, t = 5, order 40, eps = 1e-7): batched heat
+    kernel (hubs in LDS, the rest in node records) against the oracle's FP64
+    level push, 3 seeds."""
+    g = large_graph
+    srv = serving.Server(g, lanes=1, slots=4, max_batch=4)
+    try:
+        seeds = graph.uniform_seeds(g, 3, seed=23)
+        reqs = [serving.Request(rid=i, kind=_abi.FLSG_HK, seed=int(s), arrival=0.0) for i, s in enumerate(seeds)]
+        st = srv.run(reqs, collect=True)
+        assert st["ok"] == len(reqs)
+        for r in reqs:
+            o = O.hk_push(g, r.seed, 5.0, 1e-7, local=True, n_terms=40)
+            x = _dense(r, g.n)
+            assert np.abs(x - o["x"]).sum() <= 1e-5 * np.abs(o["x"]).sum()
+            assert abs(r.oper - o["oper"]) <= 0.02 * o["oper"] + 50
+    finally:
+        srv.close()
