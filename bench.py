@@ -159,6 +159,7 @@ def main():
 
     # ------------------------------------------------------------ CPU baseline (oracle, OpenMP over seeds)
     cpu = None
+    l1 = None
     if not args.no_cpu:
         from oracle import oracle as O
         cores = min(os.cpu_count() or 1, 16)
@@ -171,6 +172,15 @@ def main():
                "sample": f"first {sample} of the 1024 seeds, PPR local GD FP64 (C restatement, OpenMP over "
                          f"seeds), median {float(np.median(times)):.3f} s/query single-thread"}
         log(f"cpu: {cpu['value']:.2f} q/s on {cores} threads")
+        # L1 error of batched FP32 results against the FP64 oracle (a few seeds)
+        errs = []
+        for q in range(min(4, args.nq)):
+            o = O.ppr_gd(g, int(seeds[q]), args.alpha, args.eps, local=True)
+            xb = runs[-1].dense(q, g.n)
+            errs.append((float(np.abs(xb - o["x"]).sum()), float(np.abs(o["x"]).sum())))
+        l1 = {"seeds": len(errs), "max_l1": max(e for e, _ in errs),
+              "max_rel_l1": max(e / t for e, t in errs), "mean_l1": float(np.mean([e for e, _ in errs]))}
+        log(f"batched FP32 vs FP64 oracle: {l1}")
 
     line = {
         "metric": "certified PPR queries/s, batched FP32 local GD (FP64 threshold + FP64 certificate), eps=1e-7",
@@ -202,6 +212,7 @@ def main():
         "plain_fp64_qps": plain_qps,
         "batched_fp64_qps": b64_qps,
         "speedup_vs_plain_fp64": (qps / plain_qps) if plain_qps else None,
+        "l1_vs_fp64_oracle": l1,
         "mean_oper": oper / args.nq,
         "mean_corrections": sum(r["n_corr"] for r in res) / args.nq,
     }

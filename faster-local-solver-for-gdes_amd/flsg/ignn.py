@@ -127,3 +127,76 @@ def ignn_residual(indptr, indices, degree, X, P, alpha, beta):
     M = sp.csr_matrix((w, (rows, cols)), shape=(n, n))
     R = X - P / alpha + ((1 - alpha) / alpha) * (M @ P)
     return R
+
+
+def snapshot_bench(snapshots: int = 16, cols: int = 128, alpha: float = 0.1, eps: float = 1e-7, beta: float = 0.5,
+                   plain_cols: int = 8, log=print) -> list:
+    """The dynamic feature-propagation workload: the 169,343-node citation
+    graph with 950,000 edges held out and re-inserted in `snapshots` batches,
+    X = 128 row-normalised N(0,1) FP32 feature columns.  After the initial
+    propagation every snapshot is inserted in place and P updated
+    incrementally: all columns at once in FP32 with the FP64 certificate, all
+    columns at once in FP64, and -- the per-column form of the reference,
+    InstantGNN.py:332-359 -- one FP64 propagation per column for `plain_cols`
+    columns (its per-snapshot time scaled to all columns is reported as such).
+    Returns one dict per (path, snapshot)."""
+    from . import graph as _graph
+    sg = _graph.citation_dynamic_graph(snapshots=snapshots)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((sg.n, cols))
+    X /= np.linalg.norm(X, axis=1, keepdims=True)
+    X = X.astype(np.float32).astype(np.float64)
+    out = []
+
+    def rec(path, k, st, scale=1.0):
+        d = {"path": path, "snapshot": k, "ms": st.ms * scale, "rounds": st.rounds, "rounds_fp64": st.rounds_fp64,
+             "oper": st.oper, "oper_cert": st.oper_cert, "cert": st.cert, "keys": st.keys, "inserted": st.inserted}
+        out.append(d)
+        log(d)
+
+    for prec, name in ((_abi.FLSG_FP32, "batched-fp32-cert"), (_abi.FLSG_FP64, "batched-fp64")):
+        prop = DynamicPropagation(sg.copy(), X, alpha, eps, beta, precision=prec)
+        try:
+            rec(name, -1, prop.initial())
+            for k in range(snapshots):
+                rec(name, k, prop.insert(sg.us[k], sg.vs[k]))
+        finally:
+            prop.close()
+    if plain_cols > 0:
+        props = [DynamicPropagation(sg.copy(), X[:, c:c + 1], alpha, eps, beta, precision=_abi.FLSG_FP64)
+                 for c in range(plain_cols)]
+        try:
+            for k in range(-1, snapshots):
+                sts = [p.initial() if k < 0 else p.insert(sg.us[k], sg.vs[k]) for p in props]
+                agg = PropagationStats(ms=sum(s.ms for s in sts), rounds=max(s.rounds for s in sts),
+                                       rounds_fp64=0, oper=sum(s.oper for s in sts), oper_cert=0,
+                                       cert=max(s.cert for s in sts), keys=sts[0].keys, inserted=sts[0].inserted)
+                rec(f"per-column-fp64 (x{cols}/{plain_cols})", k, agg, scale=cols / plain_cols)
+        finally:
+            for p in props:
+                p.close()
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="dynamic feature-propagation snapshot benchmark (one GPU)")
+    ap.add_argument("--snapshots", type=int, default=16)
+    ap.add_argument("--plain-cols", type=int, default=8)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    fh = open(args.out, "w") if args.out else None
+
+    def _log(d):
+        line = json.dumps(d)
+        print(line, flush=True)
+        if fh:
+            fh.write(line + "\n")
+            fh.flush()
+
+    snapshot_bench(snapshots=args.snapshots, plain_cols=args.plain_cols, log=_log)
+    if fh:
+        fh.close()
+    sys.exit(0)
