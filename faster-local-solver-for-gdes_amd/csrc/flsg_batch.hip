@@ -48,15 +48,20 @@ constexpr unsigned kTouched = 4u;    // flags: in the query's touched list (rese
 constexpr int kSink = 64;            // per-slot sink records (d = 0) for the idle lanes of an edge batch
 
 struct __attribute__((aligned(32))) NodeRec {
-    float r;            // residual (FP32 push); heat kernel: even-level residual
-    float x;            // estimate (FP32 push)
+    union {
+        struct {
+            float r;    // residual (FP32 push); FP32 heat kernel: even-level residual
+            float x;    // estimate (FP32 push and FP32 heat kernel)
+        };
+        double hx;      // FP64 heat kernel: estimate
+    };
     int32_t d;          // degree (copy of the graph's, same line as r)
     uint32_t fl;        // certificate flags
     union {
-        double r64;     // FP64 residual (certificate)
-        float rh[2];    // heat kernel: rh[0] = odd-level residual
+        double r64;     // FP64 residual; FP64 heat kernel: even-level residual
+        float rh[2];    // FP32 heat kernel: rh[0] = odd-level residual
     };
-    double x64;         // FP64 estimate (certificate); heat kernel: push value
+    double x64;         // FP64 estimate; FP64 heat kernel: odd-level residual
 };
 static_assert(sizeof(NodeRec) == 32, "node record must be 32 bytes");
 
@@ -324,34 +329,64 @@ __device__ __forceinline__ bool pristine32(float old) { return __float_as_uint(o
 __device__ __forceinline__ bool pristine64(double old) { return __double_as_longlong(old) == 0ll; }
 
 
-// Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), FP32
-// levels with FP64 threshold tests.  Level j's residual lives in rec.r (j even)
-// or rec.rh[0] (j odd) -- for the hub cache's nodes in one of two LDS level
-// arrays.  Each level keeps the list of nodes it touched; the node phase walks
-// that list, pushes every node at or above the level threshold (level
-// residuals only grow, so this is the set the reference's crossing queue
-// holds, hk_solver.py:258) and clears every entry (the reference zeroes the
-// whole level, hk_solver.py:282-285), so no separate clearing pass is needed.
-template <int NT>
+// Heat-kernel record fields by precision: level residuals (parity 0 / 1)
+// and the estimate.
+template <bool F64>
+struct HkF;
+template <>
+struct HkF<false> {
+    using T = float;
+    static __device__ __forceinline__ float *lvl(NodeRec &r, bool odd) { return odd ? &r.rh[0] : &r.r; }
+    static __device__ __forceinline__ float *xp(NodeRec &r) { return &r.x; }
+};
+template <>
+struct HkF<true> {
+    using T = double;
+    static __device__ __forceinline__ double *lvl(NodeRec &r, bool odd) { return odd ? &r.x64 : &r.r64; }
+    static __device__ __forceinline__ double *xp(NodeRec &r) { return &r.hx; }
+};
+
+__device__ __forceinline__ bool pristine(float v) { return __float_as_uint(v) == 0u; }
+__device__ __forceinline__ bool pristine(double v) { return __double_as_longlong(v) == 0ll; }
+
+template <typename T>
+struct ResT {
+    T v;
+    bool lds;
+};
+
+// Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), with
+// FP32 or FP64 level residuals (the query's precision) and FP64 threshold
+// tests.  Level j's residual lives in the record (HkF::lvl, parity j & 1) --
+// for the hub cache's nodes in one of two LDS level arrays.  Each level keeps
+// the list of nodes it touched; the node phase walks that list, pushes every
+// node at or above the level threshold (level residuals only grow, so this is
+// the set the reference's crossing queue holds, hk_solver.py:258) and clears
+// every entry (the reference zeroes the whole level, hk_solver.py:282-285),
+// so no separate clearing pass is needed.
+template <int NT, bool F64>
 __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, NodeRec *rec, int32_t *qa,
-                         int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs, WaveLds &W, float *hub32,
+                         int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs, WaveLds &W, double *hub_raw,
                          unsigned int *s_cnt, unsigned int *s_ncnt, unsigned int *s_sup, unsigned int *s_npc,
                          unsigned int *s_work, unsigned int *s_tn, unsigned long long *s_vol,
                          unsigned long long *s_base, uint64_t t_start, int sink) {
+    using F = HkF<F64>;
+    using T = typename F::T;
     const int tid = threadIdx.x;
     const int N = a.hk_n;
     const int s = (int)qq.seed;
     const int pw = a.piece_w;
     const int32_t *indptr = a.indptr;
-    // hub cache: two level arrays of hub_n floats, then kWave sink slots
-    const int cap = a.hub_bytes / 4;
+    T *hub = reinterpret_cast<T *>(hub_raw);
+    // hub cache: two level arrays of hub_n entries, then kWave sink slots
+    const int cap = a.hub_bytes / (int)sizeof(T);
     const int hub_n = cap > 3 * kWave ? (int)min((int64_t)((cap - kWave) / 2), a.n) : 0;
     const int hsink = hub_n > 0 ? 2 * hub_n - sink : INT32_MIN;   // LDS index of an idle lane = vtx + hsink
-    for (int i = tid; i < a.hub_bytes / 8; i += NT) reinterpret_cast<double *>(hub32)[i] = 0.0;
+    for (int i = tid; i < a.hub_bytes / 8; i += NT) hub_raw[i] = 0.0;
     __syncthreads();
     if (tid == 0) {
-        if (s < hub_n) hub32[s] = 1.0f;   // hk_solver.py:223
-        else rec[s].r = 1.0f;
+        if (s < hub_n) hub[s] = (T)1;   // hk_solver.py:223
+        else *F::lvl(rec[s], false) = (T)1;
         qa[0] = s;
         *s_cnt = 1;
         *s_sup = 0;
@@ -375,7 +410,7 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
         const bool last = (j + 1 == N);
         const double coef = a.hk_tau / ((double)j + 1.);
         const double psic = a.hk_psi[j];
-        float *hc = hub32 + (odd ? hub_n : 0);
+        T *hc = hub + (odd ? hub_n : 0);
         const int nb = odd ? 0 : hub_n;   // LDS base of level j + 1
         __syncthreads();
         if (tid == 0) {
@@ -387,9 +422,8 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
         unsigned long long vol = 0;
         unsigned int pushers = 0;
         for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
-            int uu[kUnroll], ss[kUnroll];
-            float4 lo[kUnroll];
-            float ro[kUnroll];
+            int uu[kUnroll], ss[kUnroll], dd[kUnroll];
+            T ro[kUnroll], xo[kUnroll];
 #pragma unroll
             for (int k = 0; k < kUnroll; k++) {
                 const unsigned int i = i0 + k * NT;
@@ -397,12 +431,25 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
             }
 #pragma unroll
             for (int k = 0; k < kUnroll; k++) {
-                lo[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                ro[k] = 0.f;
+                ro[k] = (T)0;
+                xo[k] = (T)0;
                 ss[k] = 0;
+                dd[k] = 0;
                 if (uu[k] >= 0) {
-                    __builtin_memcpy(&lo[k], rec + uu[k], 16);
-                    ro[k] = uu[k] < hub_n ? hc[uu[k]] : odd ? rec[uu[k]].rh[0] : lo[k].x;
+                    NodeRec &ru = rec[uu[k]];
+                    float4 lo;
+                    __builtin_memcpy(&lo, &ru, 16);
+                    dd[k] = __float_as_int(lo.z);
+                    if constexpr (F64) {
+                        double2 hi;
+                        __builtin_memcpy(&hi, reinterpret_cast<const char *>(&ru) + 16, 16);
+                        __builtin_memcpy(&xo[k], &lo, 8);
+                        ro[k] = odd ? hi.y : hi.x;
+                    } else {
+                        xo[k] = lo.y;
+                        ro[k] = odd ? ru.rh[0] : lo.x;
+                    }
+                    if (uu[k] < hub_n) ro[k] = hc[uu[k]];
                     ss[k] = indptr[uu[k]];
                 }
             }
@@ -411,19 +458,17 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
                 if (uu[k] < 0) continue;
                 const unsigned int i = i0 + k * NT;
                 const int u = uu[k];
-                const int d = __float_as_int(lo[k].z);
-                const float tmp = ro[k];
-                if (u < hub_n) hc[u] = 0.0f;            // hk_solver.py:282-285
-                else if (odd) rec[u].rh[0] = 0.0f;
-                else rec[u].r = 0.0f;
+                const int d = dd[k];
+                const T tmp = ro[k];
+                if (u < hub_n) hc[u] = (T)0;            // hk_solver.py:282-285
+                else *F::lvl(rec[u], odd) = (T)0;
                 Ent e;
                 e.st = 0;
                 e.d = 0;
                 e.v = 0.0;
                 if (!(fabs((double)tmp) < ((a.hk_ete * (double)d) / (double)N) / psic)) {   // hk_solver.py:244
-                    const float xo = lo[k].y;
-                    rec[u].x = xo + tmp;
-                    if (__float_as_uint(xo) == 0u) sup[atomicAdd(s_sup, 1u)] = u;
+                    *F::xp(rec[u]) = xo[k] + tmp;
+                    if (pristine(xo[k])) sup[atomicAdd(s_sup, 1u)] = u;
                     e.st = ss[k];
                     e.d = d;
                     e.v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
@@ -439,30 +484,30 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
         __syncthreads();
         if (!last) {
             // next level's residual; first touch lists the node for level j + 1
-            wave_edges<Res32>(
+            wave_edges<ResT<T>>(
                 cnt, pcs, *s_npc, pw, s_work, W, a.colx, load_ent,
-                [&](int vtx, double val, Res32 &r, int stage) {
+                [&](int vtx, double val, ResT<T> &r, int stage) {
                     const int li = vtx < hub_n ? nb + vtx : vtx >= sink ? vtx + hsink : -1;
                     if (stage == 0) {
                         r.lds = li >= 0;
-                        if (li >= 0) r.v = atomicAdd(&hub32[li], (float)val);
+                        if (li >= 0) r.v = atomicAdd(&hub[li], (T)val);
                     } else if (li < 0) {
-                        r.v = atomic_add_wg(odd ? &rec[vtx].r : &rec[vtx].rh[0], (float)val);
+                        r.v = atomic_add_wg(F::lvl(rec[vtx], !odd), (T)val);
                     }
                 },
-                [&](int, int, double val, const Res32 &r, bool) -> int {
-                    return (pristine32(r.v) && r.v + (float)val != 0.0f) ? 2 : 0;
+                [&](int, int, double val, const ResT<T> &r, bool) -> int {
+                    return (pristine(r.v) && r.v + (T)val != (T)0) ? 2 : 0;
                 },
                 ln, s_ncnt, ln, s_ncnt, sink);
         } else {
             // last level pushes straight into x (hk_solver.py:254-256)
-            wave_edges<float>(
+            wave_edges<T>(
                 cnt, pcs, *s_npc, pw, s_work, W, a.colx, load_ent,
-                [&](int vtx, double val, float &old, int stage) {
-                    if (stage == 1) old = atomic_add_wg(&rec[vtx].x, (float)val);
+                [&](int vtx, double val, T &old, int stage) {
+                    if (stage == 1) old = atomic_add_wg(F::xp(rec[vtx]), (T)val);
                 },
-                [&](int, int, double val, float old, bool) -> int {
-                    return (pristine32(old) && old + (float)val != 0.0f) ? 2 : 0;
+                [&](int, int, double val, T old, bool) -> int {
+                    return (pristine(old) && old + (T)val != (T)0) ? 2 : 0;
                 },
                 ln, s_ncnt, sup, s_sup, sink);
         }
@@ -481,7 +526,7 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
     const double et = exp(a.hk_tau);
     for (unsigned int i0 = tid; i0 < S; i0 += NT * kUnroll) {
         int uu[kUnroll], id[kUnroll];
-        float xv[kUnroll];
+        T xv[kUnroll];
 #pragma unroll
         for (int k = 0; k < kUnroll; k++) {
             const unsigned int i = i0 + k * NT;
@@ -490,9 +535,9 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
 #pragma unroll
         for (int k = 0; k < kUnroll; k++) {
             id[k] = 0;
-            xv[k] = 0.f;
+            xv[k] = (T)0;
             if (uu[k] >= 0) {
-                xv[k] = rec[uu[k]].x;
+                xv[k] = *F::xp(rec[uu[k]]);
                 id[k] = a.out_map ? a.out_map[uu[k]] : uu[k];
             }
         }
@@ -504,7 +549,7 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
                 a.out_ids[obase + i] = id[k];
                 a.out_vals[obase + i] = (double)xv[k] / et;   // hk_solver.py:291
             }
-            rec[uu[k]].x = 0.0f;
+            *F::xp(rec[uu[k]]) = (T)0;
         }
     }
     __syncthreads();
@@ -562,8 +607,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const uint64_t t_start = now_ticks();
         const flsg_query qq = a.queries[q];
         if (qq.kind == FLSG_HK) {
-            hk_query<NT>(a, qq, q, rec, qa, qb, ent, sup, pcs, W, hub32, &s_cnt, &s_ncnt, &s_sup, &s_npc, &s_work,
-                         &s_tn, &s_vol, &s_base, t_start, sink);
+            if (qq.precision == FLSG_FP64)
+                hk_query<NT, true>(a, qq, q, rec, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
+                                   &s_work, &s_tn, &s_vol, &s_base, t_start, sink);
+            else
+                hk_query<NT, false>(a, qq, q, rec, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
+                                    &s_work, &s_tn, &s_vol, &s_base, t_start, sink);
             continue;
         }
         const QParams P = qparams(qq);
@@ -788,41 +837,52 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             __syncthreads();
             const unsigned int npc_sup = s_npc;
             // 2a. sweep A: r64 = pushes of x32 over supp (every target is already
-            // listed; r64 is still zero everywhere); hub targets sum in LDS
+            // listed; r64 is still zero everywhere); hub targets sum in LDS.
+            // Outside the support r64 is the whole recomputed residual and only
+            // grows here, so the atomic that carries it across thr_v queues v for
+            // the correction push -- exactly one does, as in the push itself.
             const int hub64_cap = a.hub_bytes / 8;
             const int hub64_n = hub64_cap > kWave ? (int)min((int64_t)(hub64_cap - kWave), a.n) : 0;
             const int hub64_sink = hub64_cap > kWave ? hub64_cap - kWave - sink : INT32_MIN;
             for (int i = tid; i < hub64_cap; i += NT) hub64[i] = 0.0;
+            if (tid == 0) s_cnt = 0;
             __syncthreads();
-            wave_edges<int>(
+            wave_edges<Res64>(
                 S32, pcs, npc_sup, pw, &s_work, W, colx, load_ent,
-                [&](int vtx, double v, int &, int stage) {
+                [&](int vtx, double v, Res64 &r, int stage) {
                     const int li = vtx < hub64_n ? vtx : vtx >= sink ? vtx + hub64_sink : -1;
-                    if (stage == 0 && li >= 0) atomicAdd(&hub64[li], v);
-                    if (stage == 1 && li < 0) atomic_add_wg(&rec[vtx].r64, v);
+                    if (stage == 0) {
+                        r.lds = li >= 0;
+                        if (li >= 0) r.v = atomicAdd(&hub64[li], v);
+                    } else if (li < 0) {
+                        r.v = atomic_add_wg(&rec[vtx].r64, v);
+                    }
                 },
-                [&](int, int, double, int, bool) -> int { return 0; }, qn, &s_ncnt, tl, &s_tl, sink);
+                [&](int vtx, int dv, double v, const Res64 &r, bool act) -> int {
+                    const double thr = P.thr_scale * (double)dv;
+                    if (!act || !(r.v < thr && thr <= r.v + v)) return 0;
+                    NodeRec &rv = rec[vtx];
+                    if (!pristine32(rv.x)) return 0;      // support: finished in 2b
+                    fl_or(&rv.fl, kInQ);
+                    return 1;
+                },
+                qc, &s_cnt, tl, &s_tl, sink);
             __syncthreads();
             for (int v = tid; v < hub64_n; v += NT) {
                 const double h = hub64[v];
                 if (!pristine64(h)) rec[v].r64 = h;
             }
             __syncthreads();
-            // 2b. one pass over the touched list, a superset of supp(x) and its
-            // neighbours (the only nodes whose residual can be nonzero):
-            // r64 = b - x + (pushes of x) in FP64, x64 = x32, and every node with
-            // |r64| >= thr queued for the correction push
-            const unsigned int T0 = s_tl;
-            if (tid == 0) s_cnt = 0;
-            __syncthreads();
-            for (unsigned int i0 = 0; i0 < T0; i0 += NT * kUnroll) {
+            // 2b. the support: r64 = b - x + (pushes of x) in FP64, x64 = x32,
+            // queued for the correction push if |r64| >= thr
+            for (unsigned int i0 = 0; i0 < S32; i0 += NT * kUnroll) {
                 int vv[kUnroll];
                 float4 lo[kUnroll];
                 double rs[kUnroll];
 #pragma unroll
                 for (int j = 0; j < kUnroll; j++) {
                     const unsigned int i = i0 + j * NT + tid;
-                    vv[j] = i < T0 ? tl[i] : -1;
+                    vv[j] = i < S32 ? sup[i] : -1;
                 }
 #pragma unroll
                 for (int j = 0; j < kUnroll; j++) {
@@ -844,8 +904,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         enq = fabs(rt) >= P.thr_scale * (double)d;
                         const double2 hv = make_double2(rt, xd);
                         __builtin_memcpy(reinterpret_cast<char *>(rec + vv[j]) + 16, &hv, 16);
-                        rec[vv[j]].fl = (xd != 0.0 ? kSeen : 0u) | (enq ? kInQ : 0u);
-                        if (xd != 0.0) cert_ops += (unsigned long long)d;
+                        rec[vv[j]].fl = kSeen | (enq ? kInQ : 0u);
+                        cert_ops += (unsigned long long)d;
                     }
                     const long long slot = wave_append(enq, &s_cnt);
                     if (slot >= 0) qc[slot] = vv[j];
