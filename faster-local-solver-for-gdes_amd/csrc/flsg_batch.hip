@@ -1,34 +1,37 @@
-// flsg_batch.hip -- many queries in one persistent launch, FP32 push with an
-// FP64 threshold test and an FP64 residual certificate.
+// flsg_batch.hip -- many queries in one persistent launch: PPR / Katz local
+// push (FP32 with an FP64 threshold test and FP64 certificate, or FP64) and the
+// heat-kernel level push, mixed freely within a launch.
 //
 // Layout.  Workgroup b of a launch owns slot (slot_base + b) for the whole
 // launch.  A slot holds, per node v, one 32-byte record
 //     { float r; float x; int32 d; uint32 flags; double r64; double x64; }
-// so the residual atomic of a push, the degree its threshold needs and the
-// certificate's FP64 fields of that node share one cache line (separate
-// per-field arrays moved ~19x their algorithmic bytes: every random 4-8 byte
-// access pulled its own line), plus the frontier double buffer, the per-entry
-// push values and the support list.  Records are all-zero (except d) between
-// queries: every query resets exactly the entries it touched.  A slot is only
-// ever touched by its own workgroup (one CU, one XCD), so its atomics are
-// workgroup-scoped and execute in that XCD's L2; kernel boundaries publish slot
-// state to the next launch.  Launches with disjoint slot ranges may run
+// so the residual atomic of a push, the degree the node phase needs and the
+// FP64 fields of that node share one line, 64 sink records (degree 0) for
+// idle lanes, the frontier double buffer, the support and touched lists, the
+// hub-piece list and the 16-byte frontier entries.  Records are all-zero
+// (except d) between queries: every query lists each record it touches once
+// and resets exactly those.  Launches with disjoint slot ranges may run
 // concurrently on different streams.
 //
-// Per query (workgroup-local, barriers are __syncthreads only):
-//   1. FP32 local push (gradient descent = Jacobi push, ppr_solver.py:246-288 /
-//      katz_solver.py:183-224): node phase (r_u -> 0, x_u += r_u, push value),
-//      then an edge phase whose edges are flattened with an LDS prefix sum over
-//      chunks of the frontier; v joins the next frontier iff the atomic that
-//      moved r_v from below to above thr_v = eps*alpha*d_v (FP64 compare) was
-//      this one (pushes are >= 0, so exactly one atomic sees it).
-//   2. FP64 certificate: r64 = b - Q x recomputed from the FP32 x over supp(x)
-//      and its neighbours (sweep A), every node with |r64_v| >= thr_v queued
-//      (sweep B), FP64 correction push (mark-based queue: the recomputed
-//      residual has both signs) until none is left -- on return
-//      |r_v| < thr_v for every v, in FP64, on every query.
-//   3. Output (support ids + FP64 values), certificate value and slot reset in
-//      one sweep (sweep C).
+// The residuals of the first ids (the hubs, on a degree-ranked graph) live in
+// an LDS hub cache for the whole query, so their pushes are LDS atomics: every
+// global atomic on this part is executed on the memory side, whatever its
+// locality.
+//
+// Per PPR / Katz query (workgroup-local, barriers are __syncthreads only):
+//   1. Local push (gradient descent = Jacobi push, ppr_solver.py:246-288 /
+//      katz_solver.py:183-224): node phase (r_u -> x_u, push value into the
+//      frontier entry), then an edge phase that streams the column entries
+//      {v, d_v}; v joins the next frontier iff the atomic that moved r_v from
+//      below to above thr_v = eps*alpha*d_v (FP64 compare) was this one
+//      (pushes are >= 0, so exactly one atomic sees it).
+//   2. FP32 precision only -- FP64 certificate: r64 = b - Q x recomputed from
+//      the FP32 x (support sweep, which also queues the crossings outside the
+//      support; then the support itself), FP64 correction push (mark-based
+//      queue: the recomputed residual has both signs) until none is left -- on
+//      return |r_v| < thr_v for every v, in FP64, on every query.
+//   3. One pass over the touched list: certificate value, output (support ids
+//      + FP64 values), record reset.
 // Queries are claimed through a per-launch counter, so uneven queries balance
 // across workgroups.
 #include <math.h>
