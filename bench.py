@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--no-relabel", action="store_true", help="keep the generator's node ids on the device")
     ap.add_argument("--hub-bytes", type=int, default=None,
                     help="LDS hub cache bytes per workgroup (-1 default, 0 off)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (workgroup size / slots / hub cache / degree bins)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times): the known-size reference "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -94,6 +96,16 @@ def main():
     seeds = graph.mixed_seeds(g, args.nq, seed=0)
     p32 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=args.alpha, eps=args.eps)
     p64 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=args.alpha, eps=args.eps)
+
+    # ------------------------------------------------------------ autotune (untimed)
+    tuned = None
+    if not args.no_autotune and not (args.slots or args.block or args.hub_bytes is not None):
+        from flsg import autotune
+        t_at = time.perf_counter()
+        tn = autotune.tune_batched(dg, p32, seeds[:min(args.nq, 512)], log=log)
+        tuned = {**tn["best"], "sample_qps": tn["qps"], "trials": len(tn["trials"]),
+                 "seconds": time.perf_counter() - t_at}
+        log(f"autotune: {tuned}")
 
     # ------------------------------------------------------------ batched FP32 (the measured step)
     for w in range(args.warmup):
@@ -207,6 +219,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "relabelled": not args.no_relabel,
+        "tuned": tuned,
         "certified_all": certified,
         "max_cert": max(r["cert"] for r in res),
         "plain_fp64_qps": plain_qps,

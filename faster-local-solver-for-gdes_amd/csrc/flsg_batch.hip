@@ -84,7 +84,8 @@ struct BatchArena {
     int32_t *q1 = nullptr;
     Ent *ent = nullptr;          // slots * n: frontier entries (heat kernel: two touched lists)
     int32_t *sup = nullptr;      // slots * n
-    uint32_t *pcs = nullptr;     // slots * n: hub pieces of the current list
+    uint32_t *pcs = nullptr;     // slots * pcs_stride: hub pieces of the current list
+    int64_t pcs_stride = 0;
     int32_t *tl = nullptr;       // slots * n: records touched by the current query
     uint64_t *ctl = nullptr;     // launch control for the synchronous entry point
     flsg_result *d_res = nullptr;
@@ -106,8 +107,10 @@ struct BatchArgs {
     Ent *ent;
     int32_t *sup;
     uint32_t *pcs;
+    int64_t pcs_stride;
     int32_t *tl;
     int32_t piece_w;             // edges per hub piece
+    int32_t hub_w;               // entries with more edges go through pieces
     int32_t hub_bytes;           // dynamic LDS of the hub cache (0 = none)
     int32_t slot_base;
     uint64_t *ctl;               // [0] next query (low 32 bits used), [1] output entries used
@@ -182,7 +185,6 @@ __device__ __forceinline__ uint32_t fl_and(uint32_t *p, uint32_t v) {
 
 
 constexpr int kUnroll = 4;           // edges in flight per lane
-constexpr int kHubW = 512;           // entries with more edges go through pieces
 
 struct WaveLds {
     int incl[kWave];
@@ -198,7 +200,7 @@ __device__ __forceinline__ void wave_sync() {
 
 // Edges of the entries [0, cnt) of a list, wavefront by wavefront with no
 // block barrier: a wave claims either a group of 64 consecutive entries (its
-// edges are flattened by a wave prefix sum; entries with more than kHubW edges
+// edges are flattened by a wave prefix sum; entries with more than hub_w edges
 // contribute nothing there) or one piece of a hub entry (pcs[], built by the
 // caller: (entry << 8) | piece).
 // Column entries are read from colx ({v, degree[v]}, 8 bytes, coalesced), so
@@ -216,7 +218,7 @@ __device__ __forceinline__ void wave_sync() {
 // aims at its own sink record (id sink + lane, degree 0) with value -0.0,
 // which leaves any value unchanged under IEEE addition.
 template <typename S, typename Load, typename Pre, typename Post>
-__device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w,
+__device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w, int hub_w,
                                            unsigned int *s_work, WaveLds &W, const int2 *colx, Load load,
                                            Pre pre, Post post, int32_t *next, unsigned int *s_next, int32_t *tl,
                                            unsigned int *s_tl, int sink) {
@@ -235,7 +237,7 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
             const unsigned int i = item * kWave + lane;
             if (i < cnt) {
                 load(i, st, d, v);
-                if (d > kHubW) d = 0;
+                if (d > hub_w) d = 0;
             }
             int incl = d;
 #pragma unroll
@@ -304,8 +306,9 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
 }
 
 // hub pieces of list entry i (degree d): (i << 8) | piece
-__device__ __forceinline__ void add_pieces(unsigned int i, int d, int pw, uint32_t *pcs, unsigned int *s_npc) {
-    if (d > kHubW) {
+__device__ __forceinline__ void add_pieces(unsigned int i, int d, int pw, int hw, uint32_t *pcs,
+                                           unsigned int *s_npc) {
+    if (d > hw) {
         const unsigned int np = (unsigned int)((d + pw - 1) / pw);
         const unsigned int b = atomicAdd(s_npc, np);
         for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
@@ -379,6 +382,7 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
     const int N = a.hk_n;
     const int s = (int)qq.seed;
     const int pw = a.piece_w;
+    const int hw = a.hub_w;
     const int32_t *indptr = a.indptr;
     T *hub = reinterpret_cast<T *>(hub_raw);
     // hub cache: two level arrays of hub_n entries, then kWave sink slots
@@ -477,7 +481,7 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
                     e.v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
                     vol += (unsigned long long)d;
                     pushers++;
-                    add_pieces(i, d, pw, pcs, s_npc);
+                    add_pieces(i, d, pw, hw, pcs, s_npc);
                 }
                 ent[i] = e;
             }
@@ -488,7 +492,7 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
         if (!last) {
             // next level's residual; first touch lists the node for level j + 1
             wave_edges<ResT<T>>(
-                cnt, pcs, *s_npc, pw, s_work, W, a.colx, load_ent,
+                cnt, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
                 [&](int vtx, double val, ResT<T> &r, int stage) {
                     const int li = vtx < hub_n ? nb + vtx : vtx >= sink ? vtx + hsink : -1;
                     if (stage == 0) {
@@ -505,7 +509,7 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
         } else {
             // last level pushes straight into x (hk_solver.py:254-256)
             wave_edges<T>(
-                cnt, pcs, *s_npc, pw, s_work, W, a.colx, load_ent,
+                cnt, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
                 [&](int vtx, double val, T &old, int stage) {
                     if (stage == 1) old = atomic_add_wg(F::xp(rec[vtx]), (T)val);
                 },
@@ -595,11 +599,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     int32_t *qb = a.q1 + off;
     Ent *ent = a.ent + off;
     int32_t *sup = a.sup + off;
-    uint32_t *pcs = a.pcs + off;
+    uint32_t *pcs = a.pcs + (int64_t)(a.slot_base + (int)blockIdx.x) * a.pcs_stride;
     int32_t *tl = a.tl + off;
     const int32_t *indptr = a.indptr;
     const int2 *colx = a.colx;
     const int pw = a.piece_w;
+    const int hw = a.hub_w;
 
     for (;;) {
         if (tid == 0) s_q = atomicAdd((unsigned int *)&a.ctl[0], 1u);
@@ -730,7 +735,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     ent[i] = e;
                     if (xx[j] == 0.0) sup[atomicAdd(&s_sup, 1u)] = uu[j];
                     vol += (unsigned long long)dd[j];
-                    add_pieces(i, dd[j], pw, pcs, &s_npc);
+                    add_pieces(i, dd[j], pw, hw, pcs, &s_npc);
                 }
             }
             if (vol) atomicAdd(&s_vol, vol);
@@ -738,7 +743,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             // edge phase: v joins the next frontier iff this atomic moved r_v across thr_v
             if (P.fp64) {
                 wave_edges<Res64>(
-                    cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
+                    cnt, pcs, s_npc, pw, hw, &s_work, W, colx, load_ent,
                     [&](int vtx, double val, Res64 &r, int stage) {
                         const int li = vtx < hub_n ? vtx : vtx >= sink ? vtx + hub_sink : -1;
                         if (stage == 0) {
@@ -757,7 +762,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     qn, &s_ncnt, tl, &s_tl, sink);
             } else {
                 wave_edges<Res32>(
-                    cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
+                    cnt, pcs, s_npc, pw, hw, &s_work, W, colx, load_ent,
                     [&](int vtx, double val, Res32 &r, int stage) {
                         const int li = vtx < hub_n ? vtx : vtx >= sink ? vtx + hub_sink : -1;
                         if (stage == 0) {
@@ -834,7 +839,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     e.d = __float_as_int(lo[j].z);
                     e.v = bpush_value(P, (double)lo[j].y, e.d);
                     ent[i] = e;
-                    add_pieces(i, e.d, pw, pcs, &s_npc);
+                    add_pieces(i, e.d, pw, hw, pcs, &s_npc);
                 }
             }
             __syncthreads();
@@ -851,7 +856,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             if (tid == 0) s_cnt = 0;
             __syncthreads();
             wave_edges<Res64>(
-                S32, pcs, npc_sup, pw, &s_work, W, colx, load_ent,
+                S32, pcs, npc_sup, pw, hw, &s_work, W, colx, load_ent,
                 [&](int vtx, double v, Res64 &r, int stage) {
                     const int li = vtx < hub64_n ? vtx : vtx >= sink ? vtx + hub64_sink : -1;
                     if (stage == 0) {
@@ -956,7 +961,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         e.v = bpush_value(P, tmp, d);
                         pushed += 1;
                         cvol += (unsigned long long)d;
-                        add_pieces(i, d, pw, pcs, &s_npc);
+                        add_pieces(i, d, pw, hw, pcs, &s_npc);
                     }
                     ent[i] = e;
                 }
@@ -965,7 +970,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 __syncthreads();
                 n_corr += s_vol;
                 wave_edges<double>(
-                    cnt, pcs, s_npc, pw, &s_work, W, colx, load_ent,
+                    cnt, pcs, s_npc, pw, hw, &s_work, W, colx, load_ent,
                     [&](int vtx, double val, double &old, int stage) {
                         if (stage == 1) old = atomic_add_wg(&rec[vtx].r64, val);
                     },
@@ -1157,6 +1162,7 @@ void batch_free(flsg_graph g) {
 
 int64_t slot_bytes(int64_t n) {
     // 32 B node record + q0 q1 sup pcs tl (4 B) + frontier entry (16 B), kSink sink records
+    // (the hub-piece list adds 4 B per 64 column slots of the graph)
     return n * (32 + 5 * 4 + 16) + (int64_t)kSink * 32;
 }
 
@@ -1175,7 +1181,10 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     FLSG_HIP(hipMalloc((void **)&A->q1, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ent, sizeof(Ent) * ns));
     FLSG_HIP(hipMalloc((void **)&A->sup, sizeof(int32_t) * ns));
-    FLSG_HIP(hipMalloc((void **)&A->pcs, sizeof(uint32_t) * ns));
+    // a list's entries are distinct nodes and every split entry has more than
+    // 64 edges per piece, so a list never needs more than n + nnz/64 pieces
+    A->pcs_stride = g->n + g->nnz_cap / kWave + 1;
+    FLSG_HIP(hipMalloc((void **)&A->pcs, sizeof(uint32_t) * (size_t)A->pcs_stride * (size_t)slots));
     FLSG_HIP(hipMalloc((void **)&A->tl, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
     hipLaunchKernelGGL(k_slot_init, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree, g->n,
@@ -1219,12 +1228,16 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.ent = A->ent;
     a.sup = A->sup;
     a.pcs = A->pcs;
+    a.pcs_stride = A->pcs_stride;
     a.tl = A->tl;
     // hub pieces are encoded (entry << 8) | piece: entries < 2^24, < 256 pieces per node
     if (g->n >= (1 << 24)) return fail(FLSG_EARG, "batched solves need n < 2^24");
-    int pw = 1024;
+    // degree bins: entries with more than hub_w edges are split into pieces of
+    // piece_w edges (at most 255 per entry)
+    int pw = g->batch_piece_w > 0 ? g->batch_piece_w : 1024;
     while ((int64_t)pw * 255 < (int64_t)g->d_max_hint) pw <<= 1;
     a.piece_w = pw;
+    a.hub_w = g->batch_hub_w > 0 ? g->batch_hub_w : 512;
     a.slot_base = slot_base;
     a.ctl = d_ctl;
     a.out_map = d_out_map;
@@ -1282,6 +1295,18 @@ extern "C" int flsg_batch_set_slots(flsg_graph g, int32_t slots) {
     if (rc) return rc;
     if (slots < 0) return fail(FLSG_EARG, "slots < 0");
     g->batch_slots = slots;
+    return FLSG_OK;
+}
+
+extern "C" int flsg_batch_set_degree_bins(flsg_graph g, int32_t split_edges, int32_t piece_edges) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (split_edges != 0 && (split_edges < kWave || split_edges > (1 << 20)))
+        return fail(FLSG_EARG, "split_edges: 0 or 64..2^20");
+    if (piece_edges != 0 && (piece_edges < kWave || piece_edges > (1 << 24)))
+        return fail(FLSG_EARG, "piece_edges: 0 or 64..2^24");
+    g->batch_hub_w = split_edges;
+    g->batch_piece_w = piece_edges;
     return FLSG_OK;
 }
 

@@ -83,6 +83,8 @@ struct flsg_graph_s {
     int32_t batch_slots = 0;
     int32_t batch_block = 1024;
     int32_t batch_hub_bytes = -1;           // LDS hub cache per workgroup (-1 = default for the block)
+    int32_t batch_hub_w = 0;                // degree bin: entries with more edges are split (0 = 512)
+    int32_t batch_piece_w = 0;              // edges per piece (0 = 1024, doubled to <= 255 pieces per entry)
     const int32_t *out_map = nullptr;       // internal id -> caller id for batched outputs
 };
 

@@ -1,0 +1,91 @@
+"""Autotuner of the batched kernel: workgroup size and slot count, LDS hub
+cache size and the degree bins of the edge phases, picked by measuring the
+throughput of a sample of the caller's own queries on the resident graph.
+
+Coordinate descent: starting from the defaults, each knob in turn is swept
+over its candidates with the others fixed, and the fastest value is kept.
+Every trial runs the same seeds in one batched launch and is timed with HIP
+events around the launch; the tuned setting is left applied on the handle.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _abi
+from .engine import DeviceGraph, _check, batch_solve
+
+
+def _apply(dg: DeviceGraph, cfg: dict) -> None:
+    lib = dg.lib
+    _check(lib.flsg_batch_set_block(dg.handle, int(cfg["block"])))
+    _check(lib.flsg_batch_set_slots(dg.handle, int(cfg["slots"])))
+    _check(lib.flsg_batch_set_hub_cache(dg.handle, int(cfg["hub_bytes"])))
+    _check(lib.flsg_batch_set_degree_bins(dg.handle, int(cfg["split_edges"]), int(cfg["piece_edges"])))
+
+
+def tune_batched(dg: DeviceGraph, params: _abi.Params, seeds, rounds: int = 2, log=None) -> dict:
+    """Return {"best": cfg, "qps": q/s of the best, "trials": [...]} for the
+    batched solve of `seeds` (a sample of the workload: ~2 queries per slot is
+    enough) with `params`; the best setting is applied to `dg`."""
+    nc = C.c_int32(0)
+    free = C.c_int64(0)
+    tot = C.c_int64(0)
+    _check(dg.lib.flsg_device_info(C.byref(nc), C.byref(free), C.byref(tot)))
+    cus = max(1, int(nc.value))
+    slot_b = int(dg.lib.flsg_batch_slot_bytes(dg.n))
+    shapes = [(1024, cus)]
+    if 2 * cus * slot_b < free.value // 2:
+        shapes.append((512, 2 * cus))
+    space = [
+        ("shape", shapes),
+        ("hub_bytes", [-1, 0, 64 * 1024, 122880]),
+        ("split_edges", [256, 512, 1024, 2048]),
+        ("piece_edges", [0, 2048, 4096, 8192]),
+    ]
+    cfg = {"block": 1024, "slots": cus, "hub_bytes": -1, "split_edges": 512, "piece_edges": 0}
+    trials = []
+    seeds = np.asarray(seeds)
+
+    def measure(c):
+        _apply(dg, c)
+        best = 0.0
+        for _ in range(rounds):
+            bo = batch_solve(dg, params, seeds)
+            ok = all(r["status"] == 0 for r in bo.results)
+            if not ok:
+                return 0.0
+            best = max(best, len(seeds) / (bo.ms * 1e-3))
+        return best
+
+    cur = measure(cfg)
+    trials.append({**cfg, "qps": cur})
+    for name, values in space:
+        best_v, best_q = None, cur
+        for v in values:
+            c = dict(cfg)
+            if name == "shape":
+                c["block"], c["slots"] = v
+                if (c["block"], c["slots"]) == (cfg["block"], cfg["slots"]):
+                    continue
+            else:
+                if c[name] == v:
+                    continue
+                c[name] = v
+            t0 = time.perf_counter()
+            q = measure(c)
+            trials.append({**c, "qps": q})
+            if log:
+                log(f"autotune {name}={v}: {q:.1f} q/s ({time.perf_counter() - t0:.1f} s)")
+            if q > best_q:
+                best_v, best_q = v, q
+        if best_v is not None:
+            if name == "shape":
+                cfg["block"], cfg["slots"] = best_v
+            else:
+                cfg[name] = best_v
+            cur = best_q
+    _apply(dg, cfg)
+    return {"best": cfg, "qps": cur, "trials": trials}

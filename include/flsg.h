@@ -166,6 +166,13 @@ int flsg_batch_set_block(flsg_graph g, int32_t block);
  * 120 KiB.  No counterpart in the reference (tuning knob). */
 int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes);
 
+/* Degree bins of the batched edge phases: a list entry with more than
+ * split_edges edges (0 = 512) is processed as pieces of piece_edges edges
+ * (0 = 1024; raised to keep at most 255 pieces per entry), each piece by one
+ * wavefront; smaller entries are flattened 64 at a time by a wavefront.  No
+ * counterpart in the reference (tuning knob, set by the autotuner). */
+int flsg_batch_set_degree_bins(flsg_graph g, int32_t split_edges, int32_t piece_edges);
+
 /* Allocate the slot arena (slots <= 0: the set / default count). */
 int flsg_batch_reserve(flsg_graph g, int32_t slots, void *stream);
 

@@ -184,6 +184,28 @@ def test_batch_fp32_certified_small(small_graph, dev_small):
         assert abs(bo.results[q]["oper"] - o["oper"]) <= 0.05 * o["oper"] + 100
 
 
+@pytest.mark.parametrize("split,piece", [(64, 64), (4096, 0), (64, 1 << 16)])
+def test_batch_degree_bins(small_graph, split, piece):
+    """Any degree bins (entries split into pieces from 64 edges up, pieces of
+    64 edges up) give certified results, in both precisions."""
+    g = small_graph
+    dg = engine.DeviceGraph(g)
+    try:
+        assert dg.lib.flsg_batch_set_degree_bins(dg.handle, split, piece) == 0
+        seeds = graph.mixed_seeds(g, 48, seed=31)
+        for prec in (_abi.FLSG_FP32, _abi.FLSG_FP64):
+            p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, prec, alpha=ALPHA, eps=1e-6)
+            bo = engine.batch_solve(dg, p, seeds)
+            _check_batch(g, seeds, bo, 1e-6, n_check=12)
+            if prec == _abi.FLSG_FP64:
+                for q in range(4):
+                    o = O.ppr_gd(g, int(seeds[q]), ALPHA, 1e-6, local=True)
+                    assert bo.results[q]["oper"] == o["oper"]
+        assert dg.lib.flsg_batch_set_degree_bins(dg.handle, 8, 0) != 0     # below one wavefront: rejected
+    finally:
+        dg.close()
+
+
 def test_batch_same_alone_and_in_mix(small_graph, dev_small):
     """A query's certified result does not depend on what else is in the batch."""
     g = small_graph
