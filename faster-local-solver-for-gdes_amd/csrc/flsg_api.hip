@@ -127,19 +127,19 @@ int flsg_graph_create(int64_t n, int64_t nnz_cap, const int32_t *d_indptr, int32
 
 int flsg_graph_destroy(flsg_graph g) {
     if (!g) return FLSG_OK;
-    hipSetDevice(g->device);
-    hipDeviceSynchronize();
-    hipFree(g->front[0]);
-    hipFree(g->front[1]);
-    hipFree(g->val);
-    hipFree(g->pieces);
-    hipFree(g->work_a);
-    hipFree(g->work_b);
-    hipFree(g->work_c);
-    hipFree(g->coef);
-    hipFree(g->ctr);
-    hipFree(g->trace_dev_off);
-    if (g->h_ctr) hipHostFree(g->h_ctr);
+    (void)hipSetDevice(g->device);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(g->front[0]);
+    (void)hipFree(g->front[1]);
+    (void)hipFree(g->val);
+    (void)hipFree(g->pieces);
+    (void)hipFree(g->work_a);
+    (void)hipFree(g->work_b);
+    (void)hipFree(g->work_c);
+    (void)hipFree(g->coef);
+    (void)hipFree(g->ctr);
+    (void)hipFree(g->trace_dev_off);
+    if (g->h_ctr) (void)hipHostFree(g->h_ctr);
     batch_free(g);
     cheb_free(g);
     ignn_free(g);

@@ -1142,20 +1142,20 @@ static int colx_ensure(flsg_graph g, hipStream_t st) {
 }
 
 void batch_free(flsg_graph g) {
-    hipFree(g->colx);
+    (void)hipFree(g->colx);
     g->colx = nullptr;
     BatchArena *A = g->arena;
     if (!A) return;
-    hipFree(A->rec);
-    hipFree(A->q0);
-    hipFree(A->q1);
-    hipFree(A->ent);
-    hipFree(A->sup);
-    hipFree(A->pcs);
-    hipFree(A->tl);
-    hipFree(A->ctl);
-    hipFree(A->d_res);
-    hipFree(A->d_q);
+    (void)hipFree(A->rec);
+    (void)hipFree(A->q0);
+    (void)hipFree(A->q1);
+    (void)hipFree(A->ent);
+    (void)hipFree(A->sup);
+    (void)hipFree(A->pcs);
+    (void)hipFree(A->tl);
+    (void)hipFree(A->ctl);
+    (void)hipFree(A->d_res);
+    (void)hipFree(A->d_q);
     delete A;
     g->arena = nullptr;
 }

@@ -193,20 +193,20 @@ __global__ void k_cheb_init_state(const int32_t *indptr, const int32_t *indices,
 
 static void free_cheb(ChebWork *w) {
     if (!w) return;
-    hipFree(w->keys[0]);
-    hipFree(w->keys[1]);
-    hipFree(w->vals[0]);
-    hipFree(w->vals[1]);
-    hipFree(w->P);
-    hipFree(w->dplus);
-    hipFree(w->cross[0]);
-    hipFree(w->cross[1]);
-    hipFree(w->queue);
-    hipFree(w->cnt);
-    hipFree(w->oper);
-    hipFree(w->tmp);
-    hipFree(w->dx);
-    hipFree(w->val);
+    (void)hipFree(w->keys[0]);
+    (void)hipFree(w->keys[1]);
+    (void)hipFree(w->vals[0]);
+    (void)hipFree(w->vals[1]);
+    (void)hipFree(w->P);
+    (void)hipFree(w->dplus);
+    (void)hipFree(w->cross[0]);
+    (void)hipFree(w->cross[1]);
+    (void)hipFree(w->queue);
+    (void)hipFree(w->cnt);
+    (void)hipFree(w->oper);
+    (void)hipFree(w->tmp);
+    (void)hipFree(w->dx);
+    (void)hipFree(w->val);
     delete w;
 }
 

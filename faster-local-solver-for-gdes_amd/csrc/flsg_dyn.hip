@@ -104,11 +104,11 @@ __global__ void k_dyn_keys(int32_t *cnt, int64_t n, int32_t *keys_out, uint32_t 
 void dyn_free(flsg_graph g) {
     DynWork *w = (DynWork *)g->dyn;
     if (!w) return;
-    hipFree(w->keys[0]);
-    hipFree(w->keys[1]);
-    hipFree(w->cnt);
-    hipFree(w->flags);
-    hipFree(w->tmp);
+    (void)hipFree(w->keys[0]);
+    (void)hipFree(w->keys[1]);
+    (void)hipFree(w->cnt);
+    (void)hipFree(w->flags);
+    (void)hipFree(w->tmp);
     delete w;
     g->dyn = nullptr;
 }

@@ -297,17 +297,17 @@ __global__ __launch_bounds__(kIT) void k_ignn_update_correct(IgnnArgs a, const i
 
 static void free_ignn(IgnnWork *w) {
     if (!w) return;
-    hipFree(w->dpb);
-    hipFree(w->dp1b);
-    hipFree(w->front[0]);
-    hipFree(w->front[1]);
-    hipFree(w->flag);
-    hipFree(w->P32);
-    hipFree(w->R32);
-    hipFree(w->V);
-    hipFree(w->cnt);
-    hipFree(w->oper);
-    hipFree(w->maxr);
+    (void)hipFree(w->dpb);
+    (void)hipFree(w->dp1b);
+    (void)hipFree(w->front[0]);
+    (void)hipFree(w->front[1]);
+    (void)hipFree(w->flag);
+    (void)hipFree(w->P32);
+    (void)hipFree(w->R32);
+    (void)hipFree(w->V);
+    (void)hipFree(w->cnt);
+    (void)hipFree(w->oper);
+    (void)hipFree(w->maxr);
     delete w;
 }
 
@@ -457,9 +457,9 @@ extern "C" int flsg_ignn_propagate(flsg_graph g, const flsg_ignn_params *p, cons
     FLSG_HIP(hipEventRecord(e1, st));
     FLSG_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     h_res->oper = fp32 ? (int64_t)oper32 : (int64_t)oper64;
     h_res->oper_cert = fp32 ? (int64_t)oper64 : 0;
     h_res->iters = rounds;

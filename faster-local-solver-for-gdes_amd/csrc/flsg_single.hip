@@ -782,12 +782,13 @@ extern "C" int flsg_solve(flsg_graph g, const flsg_params *p, int64_t seed, doub
     } else {
         rc = fail(FLSG_EARG, "unknown problem kind");
     }
-    hipEventRecord(e1, st);
-    hipEventSynchronize(e1);
+    hipError_t es = hipEventRecord(e1, st);
+    if (es == hipSuccess) es = hipEventSynchronize(e1);   // also reports a fault of the solve's kernels
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (es == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == FLSG_OK && es != hipSuccess) rc = fail(FLSG_EHIP, std::string("solve: ") + hipGetErrorString(es));
     h_res->ms = ms;
     h_res->status = rc;
     return rc;
@@ -813,12 +814,13 @@ extern "C" int flsg_hk_solve(flsg_graph g, const flsg_params *p, int64_t seed, d
     FLSG_HIP(hipEventCreate(&e1));
     FLSG_HIP(hipEventRecord(e0, st));
     rc = solve_hk(g, p, seed, d_x, d_r, h_res, trace, st, h_lvl_sums, est_n);
-    hipEventRecord(e1, st);
-    hipEventSynchronize(e1);
+    hipError_t es = hipEventRecord(e1, st);
+    if (es == hipSuccess) es = hipEventSynchronize(e1);   // also reports a fault of the solve's kernels
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (es == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == FLSG_OK && es != hipSuccess) rc = fail(FLSG_EHIP, std::string("solve: ") + hipGetErrorString(es));
     h_res->ms = ms;
     h_res->status = rc;
     return rc;
