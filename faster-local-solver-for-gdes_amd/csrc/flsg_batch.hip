@@ -97,6 +97,7 @@ struct BatchArena {
 struct BatchArgs {
     const int32_t *indptr;
     const int32_t *indices;
+    const int32_t *degree;
     const int2 *colx;            // {v, degree[v]} per column entry
     int64_t n;
     const flsg_query *queries;   // per-query (seed, kind, alpha, eps)
@@ -680,7 +681,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             // node phase: r_u -> 0, x_u += r_u, push value (ppr_solver.py:273-282);
             // every record of the unrolled entries is read before any is written
             unsigned long long vol = 0;
-            for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
+            for (unsigned int i00 = 0; i00 < cnt; i00 += NT * kUnroll) {
+                const unsigned int i0 = i00 + tid;
                 int uu[kUnroll];
 #pragma unroll
                 for (int j = 0; j < kUnroll; j++) {
@@ -733,9 +735,14 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         e.v = (double)(float)bpush_value(P, rr[j], dd[j]);
                     }
                     ent[i] = e;
-                    if (xx[j] == 0.0) sup[atomicAdd(&s_sup, 1u)] = uu[j];
                     vol += (unsigned long long)dd[j];
                     add_pieces(i, dd[j], pw, hw, pcs, &s_npc);
+                }
+                // first push of a node: list it in the support (one LDS atomic per wave)
+#pragma unroll
+                for (int j = 0; j < kUnroll; j++) {
+                    const long long k = wave_append(uu[j] >= 0 && xx[j] == 0.0, &s_sup);
+                    if (k >= 0) sup[k] = uu[j];
                 }
             }
             if (vol) atomicAdd(&s_vol, vol);
@@ -1017,13 +1024,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 const unsigned int i = i0 + j * NT + tid;
                 vv[j] = i < T ? tl[i] : -1;
             }
+            // the FP64 push never writes the FP32 half of a record: there only the
+            // FP64 half is read and cleared, the degree comes from the graph
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) {
                 lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
                 hi[j] = make_double2(0.0, 0.0);
                 if (vv[j] >= 0) {
                     const char *pr = reinterpret_cast<const char *>(rec + vv[j]);
-                    __builtin_memcpy(&lo[j], pr, 16);
+                    if (P.fp64) lo[j].z = __int_as_float(a.degree[vv[j]]);
+                    else __builtin_memcpy(&lo[j], pr, 16);
                     __builtin_memcpy(&hi[j], pr + 16, 16);
                 }
             }
@@ -1040,9 +1050,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 is_sup[j] = P.fp64 ? hi[j].y != 0.0 : (__float_as_uint(lo[j].w) & kSeen) != 0u;
                 if (is_sup[j] && a.out_map) id[j] = a.out_map[vv[j]];
                 char *pr = reinterpret_cast<char *>(rec + vv[j]);
-                const float4 zlo = make_float4(0.f, 0.f, lo[j].z, 0.f);   // keep d
                 const double2 zhi = make_double2(0.0, 0.0);
-                __builtin_memcpy(pr, &zlo, 16);
+                if (!P.fp64) {
+                    const float4 zlo = make_float4(0.f, 0.f, lo[j].z, 0.f);   // keep d
+                    __builtin_memcpy(pr, &zlo, 16);
+                }
                 __builtin_memcpy(pr + 16, &zhi, 16);
             }
 #pragma unroll
@@ -1218,6 +1230,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     BatchArgs a{};
     a.indptr = g->indptr;
     a.indices = g->indices;
+    a.degree = g->degree;
     a.colx = g->colx;
     a.n = g->n;
     a.queries = d_q;
