@@ -97,7 +97,6 @@ struct BatchArena {
 struct BatchArgs {
     const int32_t *indptr;
     const int32_t *indices;
-    const int32_t *degree;
     const int2 *colx;            // {v, degree[v]} per column entry
     int64_t n;
     const flsg_query *queries;   // per-query (seed, kind, alpha, eps)
@@ -1024,16 +1023,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 const unsigned int i = i0 + j * NT + tid;
                 vv[j] = i < T ? tl[i] : -1;
             }
-            // the FP64 push never writes the FP32 half of a record: there only the
-            // FP64 half is read and cleared, the degree comes from the graph
+            // (the FP64 push never writes the FP32 half of a record, so there only
+            // the FP64 half is cleared; the degree is read from the record's own
+            // line, which a load of the shared degree array would not hit)
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) {
                 lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
                 hi[j] = make_double2(0.0, 0.0);
                 if (vv[j] >= 0) {
                     const char *pr = reinterpret_cast<const char *>(rec + vv[j]);
-                    if (P.fp64) lo[j].z = __int_as_float(a.degree[vv[j]]);
-                    else __builtin_memcpy(&lo[j], pr, 16);
+                    __builtin_memcpy(&lo[j], pr, 16);
                     __builtin_memcpy(&hi[j], pr + 16, 16);
                 }
             }
@@ -1230,7 +1229,6 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     BatchArgs a{};
     a.indptr = g->indptr;
     a.indices = g->indices;
-    a.degree = g->degree;
     a.colx = g->colx;
     a.n = g->n;
     a.queries = d_q;
