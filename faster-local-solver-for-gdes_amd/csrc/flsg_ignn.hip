@@ -34,7 +34,10 @@ struct IgnnWork {
     int64_t n = 0;
     int64_t cols = 0;
     double *dpb = nullptr;       // d^beta
+    double *dpbi = nullptr;      // 1 / d^beta (0 for d = 0)
     double *dp1b = nullptr;      // d^(1-beta)
+    uint32_t *pcs = nullptr;     // pieces of long frontier rows: (entry << 8) | piece
+    unsigned int *pcnt = nullptr;  // [0] pieces, [1] work items claimed
     double beta = -1.0;
     int32_t *front[2] = {nullptr, nullptr};
     uint32_t *flag = nullptr;
@@ -51,7 +54,10 @@ struct IgnnArgs {
     const int32_t *indices;
     const int32_t *degree;
     const double *dpb;
+    const double *dpbi;
     const double *dp1b;
+    uint32_t *pcs;
+    unsigned int *pcnt;
     int64_t n;
     int cols;
     double alpha;
@@ -65,13 +71,19 @@ __device__ __forceinline__ bool ignn_special(const IgnnArgs &a, int u, int d) {
     return d == 0 || (d == 1 && a.indices[a.indptr[u]] == u);
 }
 
-__global__ void k_ignn_pow(const int32_t *degree, int64_t n, double beta, double *dpb, double *dp1b) {
+__global__ void k_ignn_pow(const int32_t *degree, int64_t n, double beta, double *dpb, double *dpbi, double *dp1b) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double d = (double)degree[i];
-        dpb[i] = pow(d, beta);
+        const double b = pow(d, beta);
+        dpb[i] = b;
+        dpbi[i] = b > 0.0 ? 1.0 / b : 0.0;
         dp1b[i] = pow(d, 1. - beta);
     }
 }
+
+// rows longer than this are split into pieces of this many edges (one
+// wavefront each) in the wide edge phase
+constexpr int kPieceI = 256;
 
 // frontier = nodes (all, or the listed ones) with some |R[u][c]| >= eps_u
 template <typename T>
@@ -105,8 +117,15 @@ __global__ __launch_bounds__(kIT) void k_ignn_node(IgnnArgs a, const int32_t *fr
         const int64_t i = t / C;
         const int c = (int)(t - i * C);
         const int u = front[i];
-        if (c == 0) a.flag[u] = 0u;
         const int d = a.degree[u];
+        if (c == 0) {
+            a.flag[u] = 0u;
+            if (a.pcs && d > kPieceI && !ignn_special(a, u, d)) {
+                const unsigned int np = (unsigned int)((d + kPieceI - 1) / kPieceI);
+                const unsigned int b = atomicAdd(&a.pcnt[0], np);
+                for (unsigned int q = 0; q < np; q++) a.pcs[b + q] = ((uint32_t)i << 8) | q;
+            }
+        }
         const int64_t o = (int64_t)u * C + c;
         const double r = (double)R[o];
         double v = 0.0;
@@ -194,6 +213,86 @@ __global__ __launch_bounds__(kIT) void k_ignn_push(IgnnArgs a, const int32_t *fr
             if (slot >= 0) next[slot] = v;
         }
         __syncthreads();
+    }
+}
+
+// Wide edge phase (C >= 32): one wavefront per frontier row (rows longer
+// than kPieceI edges in pieces), claimed through a work counter so every
+// wavefront of the grid shares the frontier whatever its size.  Lane l owns
+// columns l, l+64, ... of the row's push values; a wavefront issues the column
+// atomics of kUI edges before it consumes any result, and a target joins the
+// next frontier when one of its columns reached its threshold (first setter
+// of its flag appends it).
+constexpr int kUI = 4;
+
+template <typename T>
+__global__ __launch_bounds__(kIT) void k_ignn_push_wide(IgnnArgs a, const int32_t *front, unsigned int k, T *R,
+                                                        const double *V, int32_t *next, unsigned int *ncnt) {
+    const int lane = __lane_id();
+    const int C = a.cols;
+    const unsigned int nitems = k + a.pcnt[0];
+    for (;;) {
+        unsigned int item = 0;
+        if (lane == 0) item = atomicAdd(&a.pcnt[1], 1u);
+        item = __shfl(item, 0);
+        if (item >= nitems) break;
+        unsigned int i;
+        int beg = 0;
+        if (item < k) {
+            i = item;
+        } else {
+            const uint32_t pc = a.pcs[item - k];
+            i = pc >> 8;
+            beg = (int)(pc & 255u) * kPieceI;
+        }
+        const int u = front[i];
+        const int d = a.degree[u];
+        if (ignn_special(a, u, d)) continue;
+        if (item < k && d > kPieceI) continue;      // done by its pieces
+        const int end = item < k ? d : min(d, beg + kPieceI);
+        const int st = a.indptr[u];
+        const double *Vi = V + (int64_t)i * C;
+        for (int cb = 0; cb < C; cb += 2 * kWave) {
+            const int c0 = cb + lane, c1 = cb + kWave + lane;
+            const double pv0 = c0 < C ? Vi[c0] : 0.0;
+            const double pv1 = c1 < C ? Vi[c1] : 0.0;
+            if (__ballot(pv0 != 0.0 || pv1 != 0.0) == 0ull) continue;
+            for (int e0 = beg; e0 < end; e0 += kUI) {
+                int vv[kUI];
+                double inv[kUI], thr[kUI];
+#pragma unroll
+                for (int q = 0; q < kUI; q++) {
+                    const int e = e0 + q;
+                    vv[q] = e < end ? a.indices[st + e] : -1;
+                }
+#pragma unroll
+                for (int q = 0; q < kUI; q++) {
+                    inv[q] = vv[q] >= 0 ? a.dpbi[vv[q]] : 0.0;
+                    thr[q] = vv[q] >= 0 ? a.dp1b[vv[q]] * a.eps : 0.0;
+                }
+                T add0[kUI], add1[kUI], o0[kUI], o1[kUI];
+#pragma unroll
+                for (int q = 0; q < kUI; q++) {
+                    add0[q] = (T)(pv0 * inv[q]);                          // InstantGNN.py:32
+                    add1[q] = (T)(pv1 * inv[q]);
+                    o0[q] = (T)0;
+                    o1[q] = (T)0;
+                }
+#pragma unroll
+                for (int q = 0; q < kUI; q++) {
+                    if (vv[q] >= 0 && pv0 != 0.0) o0[q] = atomic_add_dev(&R[(int64_t)vv[q] * C + c0], add0[q]);
+                    if (vv[q] >= 0 && pv1 != 0.0) o1[q] = atomic_add_dev(&R[(int64_t)vv[q] * C + c1], add1[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < kUI; q++) {
+                    const bool hot = vv[q] >= 0 &&
+                                     ((pv0 != 0.0 && fabs((double)o0[q] + (double)add0[q]) >= thr[q]) ||
+                                      (pv1 != 0.0 && fabs((double)o1[q] + (double)add1[q]) >= thr[q]));
+                    if (__ballot(hot) != 0ull && lane == 0 && atomicOr(&a.flag[vv[q]], 1u) == 0u)
+                        next[atomicAdd(ncnt, 1u)] = vv[q];
+                }
+            }
+        }
     }
 }
 
@@ -298,7 +397,10 @@ __global__ __launch_bounds__(kIT) void k_ignn_update_correct(IgnnArgs a, const i
 static void free_ignn(IgnnWork *w) {
     if (!w) return;
     (void)hipFree(w->dpb);
+    (void)hipFree(w->dpbi);
     (void)hipFree(w->dp1b);
+    (void)hipFree(w->pcs);
+    (void)hipFree(w->pcnt);
     (void)hipFree(w->front[0]);
     (void)hipFree(w->front[1]);
     (void)hipFree(w->flag);
@@ -330,7 +432,11 @@ static int ensure_ignn(flsg_graph g, int64_t cols, bool fp32, hipStream_t st) {
         w->n = n;
         w->cols = cols;
         FLSG_HIP(hipMalloc((void **)&w->dpb, sizeof(double) * n));
+        FLSG_HIP(hipMalloc((void **)&w->dpbi, sizeof(double) * n));
         FLSG_HIP(hipMalloc((void **)&w->dp1b, sizeof(double) * n));
+        // frontier rows are distinct nodes: at most n + nnz/kPieceI pieces
+        FLSG_HIP(hipMalloc((void **)&w->pcs, sizeof(uint32_t) * (size_t)(n + g->nnz_cap / kPieceI + 1)));
+        FLSG_HIP(hipMalloc((void **)&w->pcnt, sizeof(unsigned int) * 2));
         FLSG_HIP(hipMalloc((void **)&w->front[0], sizeof(int32_t) * (n + 1)));
         FLSG_HIP(hipMalloc((void **)&w->front[1], sizeof(int32_t) * (n + 1)));
         FLSG_HIP(hipMalloc((void **)&w->flag, sizeof(uint32_t) * n));
@@ -353,7 +459,10 @@ static IgnnArgs ignn_args(flsg_graph g, IgnnWork *w, const flsg_ignn_params *p) 
     a.indices = g->indices;
     a.degree = g->degree;
     a.dpb = w->dpb;
+    a.dpbi = w->dpbi;
     a.dp1b = w->dp1b;
+    a.pcs = p->cols >= 32 ? w->pcs : nullptr;   // wide edge phase only
+    a.pcnt = w->pcnt;
     a.n = g->n;
     a.cols = (int)p->cols;
     a.alpha = p->alpha;
@@ -377,9 +486,14 @@ static int ignn_rounds(flsg_graph g, IgnnWork *w, const IgnnArgs &a, T *P, T *R,
     while (k > 0) {
         if (rounds >= max_rounds) return fail(FLSG_ENOCONV, "InstantGNN push: round cap reached");
         FLSG_HIP(hipMemsetAsync(w->cnt + (cur ^ 1), 0, sizeof(unsigned int), st));
+        FLSG_HIP(hipMemsetAsync(w->pcnt, 0, sizeof(unsigned int) * 2, st));
         hipLaunchKernelGGL(k_ignn_node<T>, dim3(grid), dim3(kIT), 0, st, a, w->front[cur], k, P, R, w->V);
-        hipLaunchKernelGGL(k_ignn_push<T>, dim3(grid), dim3(kIT), 0, st, a, w->front[cur], k, R, (const double *)w->V,
-                           w->front[cur ^ 1], w->cnt + (cur ^ 1));
+        if (a.pcs)
+            hipLaunchKernelGGL(k_ignn_push_wide<T>, dim3(grid), dim3(kIT), 0, st, a, w->front[cur], k, R,
+                               (const double *)w->V, w->front[cur ^ 1], w->cnt + (cur ^ 1));
+        else
+            hipLaunchKernelGGL(k_ignn_push<T>, dim3(grid), dim3(kIT), 0, st, a, w->front[cur], k, R,
+                               (const double *)w->V, w->front[cur ^ 1], w->cnt + (cur ^ 1));
         FLSG_HIP(hipGetLastError());
         FLSG_HIP(hipMemcpyAsync(&k, w->cnt + (cur ^ 1), sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         FLSG_HIP(hipStreamSynchronize(st));
@@ -411,7 +525,7 @@ extern "C" int flsg_ignn_propagate(flsg_graph g, const flsg_ignn_params *p, cons
     const int64_t n = g->n, NC = g->n * p->cols;
     const int grid = g->num_cus * 8;
     if (w->beta != p->beta || p->degrees_changed) {
-        hipLaunchKernelGGL(k_ignn_pow, dim3(grid), dim3(kIT), 0, st, g->degree, n, p->beta, w->dpb, w->dp1b);
+        hipLaunchKernelGGL(k_ignn_pow, dim3(grid), dim3(kIT), 0, st, g->degree, n, p->beta, w->dpb, w->dpbi, w->dp1b);
         w->beta = p->beta;
     }
     const int64_t max_rounds = p->max_rounds > 0 ? p->max_rounds : 1000000;
@@ -482,7 +596,7 @@ extern "C" int flsg_ignn_update(flsg_graph g, const flsg_ignn_params *p, const d
     if (rc) return rc;
     IgnnWork *w = (IgnnWork *)g->ignn;
     const int grid = g->num_cus * 8;
-    hipLaunchKernelGGL(k_ignn_pow, dim3(grid), dim3(kIT), 0, st, g->degree, g->n, p->beta, w->dpb, w->dp1b);
+    hipLaunchKernelGGL(k_ignn_pow, dim3(grid), dim3(kIT), 0, st, g->degree, g->n, p->beta, w->dpb, w->dpbi, w->dp1b);
     w->beta = p->beta;
     if (n_keys <= 0) return FLSG_OK;
     IgnnArgs a = ignn_args(g, w, p);
