@@ -384,3 +384,23 @@ def test_relabelled_graph_parity_small(small_graph):
         bo = engine.batch_solve(dgr, p32, seeds, block=block)
         _check_batch(g, seeds, bo, 1e-6)
     dgr.close()
+
+
+def test_autotune_small(small_graph):
+    """The autotuner measures its candidates, leaves the best applied, and the
+    tuned launches still return certified results."""
+    from flsg import autotune
+    g = small_graph
+    dg = engine.DeviceGraph(g)
+    try:
+        seeds = graph.mixed_seeds(g, 64, seed=37)
+        p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=1e-6)
+        out = autotune.tune_batched(dg, p, seeds, rounds=1)
+        best = out["best"]
+        assert out["qps"] > 0 and len(out["trials"]) >= 5
+        assert best["block"] in (256, 512, 1024) and best["split_edges"] >= 64
+        assert max(t["qps"] for t in out["trials"]) == pytest.approx(out["qps"])
+        bo = engine.batch_solve(dg, p, seeds)
+        _check_batch(g, seeds, bo, 1e-6, n_check=8)
+    finally:
+        dg.close()
