@@ -614,6 +614,25 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         if ((int64_t)q >= a.nq) break;
         const uint64_t t_start = now_ticks();
         const flsg_query qq = a.queries[q];
+        // a query the launch cannot run fails on its own record, the other
+        // queries of the launch go on: seed outside the graph, unknown kind,
+        // a heat-kernel query in a launch without heat-kernel settings, or PPR /
+        // Katz parameters outside the range where every push is >= 0
+        // (0 < alpha < 1 for PPR, beta > 0 for Katz, eps > 0)
+        const bool bad_kind = qq.kind != FLSG_PPR && qq.kind != FLSG_KATZ && qq.kind != FLSG_HK;
+        const bool bad_par = qq.kind == FLSG_HK ? a.hk_n <= 0
+                                                : !(qq.eps > 0.0) || !(qq.alpha > 0.0) ||
+                                                      (qq.kind == FLSG_PPR && !(qq.alpha < 1.0));
+        if (qq.seed < 0 || qq.seed >= a.n || bad_kind || bad_par) {
+            if (tid == 0) {
+                flsg_result rr{};
+                rr.status = FLSG_EARG;
+                a.res[q] = rr;
+                a.out_off[q] = -1;
+            }
+            __syncthreads();
+            continue;
+        }
         if (qq.kind == FLSG_HK) {
             if (qq.precision == FLSG_FP64)
                 hk_query<NT, true>(a, qq, q, rec, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,

@@ -205,9 +205,12 @@ class BatchOut:
 
 
 def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None = None,
-                slots: int = 0, block: int = 0, hub_bytes: int | None = None) -> BatchOut:
-    """Many single-source local solves in one persistent launch (FP32 push,
-    FP64 threshold and FP64 certificate)."""
+                slots: int = 0, block: int = 0, hub_bytes: int | None = None,
+                allow_failed: bool = False) -> BatchOut:
+    """Many single-source local solves in one persistent launch
+    (params.precision FLSG_FP32: FP32 push, FP64 threshold and FP64
+    certificate; FLSG_FP64: FP64 push).  A query that fails raises FlsgError
+    unless allow_failed, in which case its status is in results[q]."""
     torch = dg.torch
     seeds = np.ascontiguousarray(dg.to_internal(seeds))
     nq = int(seeds.size)
@@ -231,7 +234,9 @@ def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None
     rc = dg.lib.flsg_batch_solve(dg.handle, C.byref(params), _ptr(d_seeds), nq, _ptr(ids), _ptr(vals),
                                  _ptr(offs), int(out_cap), res, _stream_ptr(torch))
     e1.record()
-    _check(rc)
+    per_query = rc != _abi.FLSG_OK and any(res[i].status != _abi.FLSG_OK for i in range(nq))
+    if not (allow_failed and per_query):
+        _check(rc)
     torch.cuda.synchronize()
     return BatchOut(ids=ids, vals=vals, offsets=offs.cpu().numpy()[:nq], results=[res[i].as_dict() for i in range(nq)],
                     ms=e0.elapsed_time(e1))

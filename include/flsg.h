@@ -143,7 +143,11 @@ int flsg_hk_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x,
  * Outputs are sparse: query q's estimate is
  *   d_out_ids/d_out_vals[h_res[q].. offsets ..]  -- d_out_off[q] (device,
  *   int64) is the start and h_res[q].supp the length.
- * out_cap bounds the total number of (id, value) pairs.  PPR/KATZ GD only. */
+ * out_cap bounds the total number of (id, value) pairs.  PPR/KATZ GD only,
+ * p->precision FLSG_FP32 (FP32 push + FP64 certificate) or FLSG_FP64.  When
+ * some queries fail (bad seed or parameters: FLSG_EARG; output arena full:
+ * FLSG_ECAP; iteration cap: FLSG_ENOCONV) h_res is still filled for every
+ * query and the call returns the first failing query's status. */
 int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_t *d_seeds, int64_t n_queries,
                      int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off, int64_t out_cap,
                      flsg_result *h_res, void *stream);

@@ -206,6 +206,28 @@ def test_batch_degree_bins(small_graph, split, piece):
         dg.close()
 
 
+def test_batch_bad_seed_fails_alone(small_graph, dev_small):
+    """A query whose seed lies outside the graph fails with FLSG_EARG on its
+    own record (the kernel checks before touching a slot); the rest of the
+    launch is unaffected."""
+    g = small_graph
+    seeds = graph.mixed_seeds(g, 16, seed=41).astype(np.int64)
+    seeds[3] = g.n + 5
+    seeds[9] = -1
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=1e-6)
+    with pytest.raises(engine.FlsgError):
+        engine.batch_solve(dev_small, p, seeds)
+    bo = engine.batch_solve(dev_small, p, seeds, allow_failed=True)
+    for q in range(len(seeds)):
+        if q in (3, 9):
+            assert bo.results[q]["status"] == _abi.FLSG_EARG
+        else:
+            assert bo.results[q]["status"] == 0 and bo.results[q]["cert"] < 1.0
+    good = [q for q in range(len(seeds)) if q not in (3, 9)]
+    bo2 = engine.batch_solve(dev_small, p, seeds[good])
+    assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo2.results)
+
+
 def test_batch_same_alone_and_in_mix(small_graph, dev_small):
     """A query's certified result does not depend on what else is in the batch."""
     g = small_graph

@@ -72,6 +72,22 @@ def test_serving_mix_certified_and_same_alone(small_graph, srv_small):
             assert np.abs(a - b).sum() <= (ra + rb) / scale * (1 + 1e-9) + 1e-15
 
 
+def test_serving_rejects_bad_query(small_graph, srv_small):
+    """A request the kernel cannot run (PPR alpha outside (0, 1), eps <= 0)
+    fails alone with FLSG_EARG; the requests launched beside it are served."""
+    g = small_graph
+    s0, s1 = (int(v) for v in graph.uniform_seeds(g, 2, seed=43))
+    reqs = [serving.Request(rid=0, kind=_abi.FLSG_PPR, seed=s0, arrival=0.0, alpha=1.5, eps=1e-6),
+            serving.Request(rid=1, kind=_abi.FLSG_PPR, seed=s1, arrival=0.0, alpha=0.1, eps=1e-6),
+            serving.Request(rid=2, kind=_abi.FLSG_KATZ, seed=s1, arrival=0.0, alpha=1e-4, eps=0.0)]
+    srv_small.run(reqs, collect=True)
+    assert reqs[0].status == _abi.FLSG_EARG and reqs[2].status == _abi.FLSG_EARG
+    assert reqs[1].status == _abi.FLSG_OK and reqs[1].cert < 1.0
+    x = _dense(reqs[1], g.n)
+    rr = O.ppr_residual(g, s1, 0.1, x)
+    assert (np.abs(rr) / (1e-6 * 0.1 * np.maximum(g.degree, 1))).max() < 1 + 1e-9
+
+
 def test_serving_hk_matches_oracle(small_graph, srv_small):
     """Batched FP32 heat kernel vs the oracle's FP64 level push (same N, same
     thresholds): equal up to FP32 rounding."""
