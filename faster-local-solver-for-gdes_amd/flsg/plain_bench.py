@@ -1,0 +1,51 @@
+"""The plain FP64 single-source path on the 2,394,385-node power-law graph:
+local GD, local Chebyshev, global GD and global Chebyshev (PPR, alpha 0.1,
+eps 1e-7) over 50 uniformly random non-isolated seeds, one query at a time on
+the whole GPU -- the per-method operation counts and times to set beside the
+reference's published CPU / CuPy numbers (README.md:39-48, wiki-talk graph of
+the same size)."""
+from __future__ import annotations
+
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _abi, engine, graph
+
+
+def plain_bench(seeds: int = 50, eps: float = 1e-7, alpha: float = 0.1, log=print) -> list:
+    g = graph.powerlaw_graph_large()
+    dg = engine.DeviceGraph(g)
+    try:
+        ss = graph.uniform_seeds(g, seeds, seed=0)
+        out = []
+        for name, method, local in (("local GD", _abi.FLSG_GD, True), ("local Chebyshev", _abi.FLSG_CHEB, True),
+                                    ("global GD", _abi.FLSG_GD, False),
+                                    ("global Chebyshev", _abi.FLSG_CHEB, False)):
+            p = engine.make_params(_abi.FLSG_PPR, method, local, _abi.FLSG_FP64, alpha=alpha, eps=eps)
+            engine.solve(dg, p, int(ss[0]))      # warm-up
+            opers, ms, iters = [], [], []
+            t0 = time.perf_counter()
+            for s in ss:
+                o = engine.solve(dg, p, int(s))
+                opers.append(o.result["oper"])
+                ms.append(o.result["ms"])
+                iters.append(o.result["iters"])
+            wall = time.perf_counter() - t0
+            rec = {"method": name, "seeds": int(len(ss)), "mean_oper": float(np.mean(opers)),
+                   "mean_iters": float(np.mean(iters)), "mean_ms": float(np.mean(ms)),
+                   "median_ms": float(np.median(ms)), "qps": len(ss) / wall,
+                   "sweeps_equiv": float(np.mean(opers)) / (g.n + g.m)}
+            out.append(rec)
+            log(rec)
+        return out
+    finally:
+        dg.close()
+
+
+if __name__ == "__main__":
+    for r in plain_bench(seeds=int(sys.argv[1]) if len(sys.argv) > 1 else 50,
+                         log=lambda d: print(json.dumps(d), flush=True)):
+        pass

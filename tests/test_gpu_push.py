@@ -387,6 +387,36 @@ def test_ppr_local_cheb_exact_large(large_graph, dev_large):
         assert np.array_equal(out.x.cpu().numpy(), o["x"])
 
 
+def test_large_local_vs_fp64_global_solve(large_graph, dev_large):
+    """Full size (2.39M nodes, eps = 1e-7): local GD and local Chebyshev
+    estimates checked against an FP64 global GD solve to residual 1e-15
+    (thresholds 1e-15 * alpha * d), through what the mathematics fixes:
+      * GD: r >= 0 and x* - x = Q^-1 r, and 1^T Q^-1 = 1^T / alpha on the
+        non-isolated nodes, so ||x* - x||_1 = sum(r) / alpha exactly, and x <= x*;
+      * Chebyshev (r of both signs): ||x* - x||_1 <= ||r||_1 / alpha.
+    The global solve's own distance to x* is at most ||r_glob||_1 / alpha."""
+    g = large_graph
+    s = int(graph.uniform_seeds(g, 1, seed=27)[0])
+    pg = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, False, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-15)
+    ref = engine.solve(dev_large, pg, s)
+    xs = ref.x.cpu().numpy()
+    err_ref = np.abs(ref.r.cpu().numpy()).sum() / ALPHA
+    assert err_ref < 1e-8
+    for method in (_abi.FLSG_GD, _abi.FLSG_CHEB):
+        p = engine.make_params(_abi.FLSG_PPR, method, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-7)
+        out = engine.solve(dev_large, p, s)
+        x = out.x.cpu().numpy()
+        r = out.r.cpu().numpy()
+        l1 = np.abs(x - xs).sum()
+        if method == _abi.FLSG_GD:
+            assert (r >= 0).all()
+            expect = r.sum() / ALPHA
+            assert abs(l1 - expect) <= 1e-9 * expect + 2 * err_ref, (l1, expect)
+            assert (x <= xs + 1e-15).all()
+        else:
+            assert l1 <= np.abs(r).sum() / ALPHA * (1 + 1e-9) + err_ref
+
+
 def test_relabelled_graph_parity_small(small_graph):
     """A degree-relabelled device graph returns results in the caller's ids:
     FP64 local GD still matches the oracle's frontier sets and op count, and
