@@ -15,8 +15,9 @@
 //      key = (destination << pbits) | position;
 //   3. radix sort of the keys (hipCUB) groups every destination's events in
 //      the reference's order;
-//   4. one thread per destination replays its events sequentially in FP64
-//      (bit-identical sums) and records the position of the first crossing;
+//   4. every destination's events are replayed sequentially in FP64
+//      (bit-identical sums) by a thread, a wave or a block depending on
+//      how many there are, recording the position of the first crossing;
 //   5. the crossing nodes, sorted by that position, are the next queue (or,
 //      when there are more than n/4 of them, ascending ids, ppr_solver.py:639).
 #include <math.h>
@@ -38,7 +39,8 @@ struct ChebWork {
     int32_t *dplus = nullptr;      // n
     uint64_t *cross[2] = {nullptr, nullptr};   // n
     int32_t *queue = nullptr;      // n
-    unsigned int *cnt = nullptr;   // crossing count
+    unsigned int *cnt = nullptr;   // crossings, medium and heavy destinations (3)
+    uint32_t *seg[2] = {nullptr, nullptr};   // n + 1 each: medium / heavy destinations
     unsigned long long *oper = nullptr;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
@@ -114,42 +116,279 @@ __global__ __launch_bounds__(kCT) void k_cheb_events(const int32_t *indptr, cons
     }
 }
 
-// step 4: sequential replay per destination
-__global__ __launch_bounds__(kCT) void k_cheb_replay(const uint64_t *keys, const int32_t *vals, int64_t E,
-                                                     int pbits, const int64_t *P, const int32_t *degree,
-                                                     const double *val, const double *dx, double *r,
-                                                     double thr_scale, int vbits, uint64_t *cross,
-                                                     unsigned int *cnt) {
-    const uint64_t pmask = (pbits >= 64) ? ~0ull : ((1ull << pbits) - 1);
-    for (int64_t e = blockIdx.x * (int64_t)kCT + threadIdx.x; e < E; e += (int64_t)gridDim.x * kCT) {
-        const uint64_t key = keys[e];
-        const int v = (int)(key >> pbits);
-        if (e > 0 && (int)(keys[e - 1] >> pbits) == v) continue;   // not the head of v's events
-        const double thr = thr_scale * (double)degree[v];
-        double rv = r[v];
-        bool marked = false;
-        uint64_t first = 0;
-        for (int64_t f = e; f < E; f++) {
-            const uint64_t kf = keys[f];
-            if ((int)(kf >> pbits) != v) break;
-            const int64_t pos = (int64_t)(kf & pmask);
-            const int i = vals[f];
-            if (pos == P[i + 1] - 1) {
-                rv += -dx[v];                                   // ppr_solver.py:657
+// step 4: sequential replay per destination.  A destination's events must be
+// summed one after another in FP64 (bit-identical to the reference), but the
+// loads behind them need not be: destinations are split by event count into
+//   light  (<= kLight events): one thread, every load issued before the sum;
+//   medium (<= kBig):          one wave, 64 events loaded per chunk and
+//                              staged in LDS, summed from there in order;
+//   heavy  (>  kBig):          one 1024-thread block, 15 waves load rounds of
+//                              960 events into LDS while wave 0 sums the last.
+// Positions are < 2^31, so bit 31 of a position word marks a checked add.
+constexpr int kLight = 16;
+constexpr int kBig = 256;
+constexpr int kHB = 1024;
+constexpr int kRound = kHB - kWave;
+static_assert(kRound % kWave == 0, "a round is whole chunks");
+constexpr uint32_t kChk = 0x80000000u;
+
+struct ReplayArgs {
+    const uint64_t *keys;
+    const int32_t *vals;
+    int64_t E;
+    int pbits;
+    int vbits;
+    const int64_t *P;
+    const int32_t *degree;
+    const double *val;
+    const double *dx;
+    double *r;
+    double thr_scale;
+    uint64_t *cross;
+    unsigned int *cnt;   // [0] crossings [1] medium [2] heavy
+    uint32_t *med;       // event index of each medium destination's first event
+    uint32_t *big;       // same, heavy destinations
+};
+
+__device__ __forceinline__ void emit_cross(const ReplayArgs &a, int v, uint32_t first) {
+    const unsigned int slot = atomicAdd(a.cnt, 1u);
+    a.cross[slot] = ((uint64_t)first << a.vbits) | (uint64_t)(uint32_t)v;
+}
+
+// event f of destination v: its addend and position word; false past the segment
+__device__ __forceinline__ bool load_event(const ReplayArgs &a, int v, int64_t f, double dxv, double &ad,
+                                           uint32_t &pc) {
+    ad = -0.0;   // exact identity of FP64 addition, so padding never changes a sum
+    pc = 0;
+    if (f >= a.E) return false;
+    const uint64_t kf = a.keys[f];
+    if ((int)(kf >> a.pbits) != v) return false;
+    const uint32_t pos = (uint32_t)(kf & ((1ull << a.pbits) - 1));
+    const int i = a.vals[f];
+    if ((int64_t)pos == a.P[i + 1] - 1) {
+        ad = -dxv;                                              // ppr_solver.py:657
+        pc = pos;
+    } else {
+        ad = a.val[i];                                          // ppr_solver.py:652
+        pc = pos | kChk;                                        // checked at ppr_solver.py:653
+    }
+    return true;
+}
+
+// adds the wave's first cnt addends to rv in lane order; lane j keeps the
+// partial sum after its add, the value the reference tests
+__device__ __forceinline__ void wave_serial(double ad, uint32_t pc, int lane, int cnt, double thr, double &rv,
+                                            bool &marked, uint32_t &first) {
+    const int lo = __double2loint(ad), hi = __double2hiint(ad);
+    double mine = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < cnt; j++) {
+        rv += __hiloint2double(__builtin_amdgcn_readlane(hi, j), __builtin_amdgcn_readlane(lo, j));
+        mine = (lane == j) ? rv : mine;
+    }
+    const unsigned long long hit = __ballot(lane < cnt && (pc & kChk) && thr <= fabs(mine));
+    if (!marked && hit) {
+        marked = true;
+        first = (uint32_t)__builtin_amdgcn_readlane((int)pc, __ffsll((long long)hit) - 1) & ~kChk;
+    }
+}
+
+// the same sum with the addends read from LDS at uniform addresses (one
+// broadcast read per add instead of two readlanes) and only the running
+// maximum of |partial sum| kept; any crossing in the chunk shows as
+// max >= thr, and then the chunk is summed again lane by lane to find the
+// first checked crossing, which happens at most once per destination
+__device__ __forceinline__ void chunk_sum(const double *s_add, double ad, uint32_t pc, int lane, int cnt, double thr,
+                                          double &rv, bool &marked, uint32_t &first) {
+    // s_add[cnt .. round up to 8) holds -0.0, so whole groups of 8 are summed;
+    // two register sets alternate so each group's reads are in flight while
+    // the previous group is added
+    const double r0 = rv;
+    double m = 0.0;
+    const int ng = (cnt + 7) >> 3;
+    double ga[8], gb[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) ga[k] = s_add[k];
+    for (int g = 0; g < ng; g += 2) {
+        const int ob = (g + 1 < ng ? g + 1 : g) * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k++) gb[k] = s_add[ob + k];
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads ahead of the adds
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            rv += ga[k];
+            m = fmax(m, fabs(rv));
+        }
+        if (g + 1 >= ng) break;
+        const int oa = (g + 2 < ng ? g + 2 : g) * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k++) ga[k] = s_add[oa + k];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            rv += gb[k];
+            m = fmax(m, fabs(rv));
+        }
+    }
+    if (!marked && thr <= m) {
+        rv = r0;
+        wave_serial(ad, pc, lane, cnt, thr, rv, marked, first);
+    }
+}
+
+// heads of segments: light ones replayed here, the rest queued by size
+__global__ __launch_bounds__(kCT) void k_cheb_replay(ReplayArgs a) {
+    const int64_t e = blockIdx.x * (int64_t)kCT + threadIdx.x;
+    if (e >= a.E) return;
+    const uint64_t key = a.keys[e];
+    const int v = (int)(key >> a.pbits);
+    if (e > 0 && (int)(a.keys[e - 1] >> a.pbits) == v) return;   // not the head of v's events
+    uint64_t kk[kLight];
+    kk[0] = key;
+    int len = 1;
+#pragma unroll
+    for (int j = 1; j <= kLight; j++) {
+        const int64_t f = e + j;
+        uint64_t kf = 0;
+        if (f < a.E) kf = a.keys[f];
+        const bool same = f < a.E && (int)(kf >> a.pbits) == v;   // same-destination events form a prefix
+        len += same ? 1 : 0;
+        if (j < kLight) kk[j] = kf;
+    }
+    if (len > kLight) {
+        const int64_t f = e + kBig;
+        const bool heavy = f < a.E && (int)(a.keys[f] >> a.pbits) == v;
+        const unsigned int slot = atomicAdd(a.cnt + (heavy ? 2 : 1), 1u);
+        (heavy ? a.big : a.med)[slot] = (uint32_t)e;
+        return;
+    }
+    const uint64_t pmask = (1ull << a.pbits) - 1;
+    int iv[kLight];
+#pragma unroll
+    for (int j = 0; j < kLight; j++) iv[j] = (j < len) ? a.vals[e + j] : 0;
+    int64_t endp[kLight];
+    double vv[kLight];
+#pragma unroll
+    for (int j = 0; j < kLight; j++) {
+        endp[j] = (j < len) ? a.P[iv[j] + 1] - 1 : -1;
+        vv[j] = (j < len) ? a.val[iv[j]] : 0.0;
+    }
+    const double thr = a.thr_scale * (double)a.degree[v];
+    const double dxv = a.dx[v];
+    double rv = a.r[v];
+    bool marked = false;
+    uint32_t first = 0;
+#pragma unroll
+    for (int j = 0; j < kLight; j++) {
+        if (j < len) {
+            const int64_t pos = (int64_t)(kk[j] & pmask);
+            if (pos == endp[j]) {
+                rv += -dxv;                                     // ppr_solver.py:657
             } else {
-                rv += val[i];                                   // ppr_solver.py:652
+                rv += vv[j];                                    // ppr_solver.py:652
                 if (!marked && thr <= fabs(rv)) {               // ppr_solver.py:653
                     marked = true;
-                    first = (uint64_t)pos;
+                    first = (uint32_t)pos;
                 }
             }
         }
-        r[v] = rv;
-        if (marked) {
-            const unsigned int slot = atomicAdd(cnt, 1u);
-            cross[slot] = (first << vbits) | (uint64_t)(uint32_t)v;
+    }
+    a.r[v] = rv;
+    if (marked) emit_cross(a, v, first);
+}
+
+// one medium destination on one wave; s_w is the wave's own 64 LDS slots
+__device__ void replay_wave(const ReplayArgs &a, uint32_t e, int lane, double *s_w) {
+    const int v = (int)(a.keys[e] >> a.pbits);
+    const double thr = a.thr_scale * (double)a.degree[v];
+    const double dxv = a.dx[v];
+    double rv = a.r[v];
+    bool marked = false;
+    uint32_t first = 0;
+    for (int64_t f0 = e;; f0 += kWave) {
+        double ad;
+        uint32_t pc;
+        const unsigned long long ok = __ballot(load_event(a, v, f0 + lane, dxv, ad, pc));
+        s_w[lane] = ad;
+        __builtin_amdgcn_wave_barrier();
+        chunk_sum(s_w, ad, pc, lane, __popcll(ok), thr, rv, marked, first);
+        __builtin_amdgcn_wave_barrier();
+        if (ok != ~0ull) break;
+    }
+    if (lane == 0) {
+        a.r[v] = rv;
+        if (marked) emit_cross(a, v, first);
+    }
+}
+
+// heavy destinations (one per block), then medium ones (one per wave).  Both
+// schedules are static strides of uniform values: a work counter claimed by
+// thread 0 at the top of the loop would put "if (t == 0)" at both ends of the
+// body, and the compiler then threads the other threads' back edge straight
+// to the barrier, leaving thread 0 outside a loop that never sees a new item.
+__global__ __launch_bounds__(kHB) void k_cheb_replay_heavy(ReplayArgs a) {
+    __shared__ double s_ad[2][kRound];
+    __shared__ uint32_t s_pc[2][kRound];
+    __shared__ double s_w[kHB / kWave][kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
+    const unsigned int nbig = a.cnt[2];
+    for (unsigned int item = blockIdx.x; item < nbig; item += gridDim.x) {
+        const uint32_t e = a.big[item];
+        const int v = (int)(a.keys[e] >> a.pbits);
+        const double thr = a.thr_scale * (double)a.degree[v];
+        const double dxv = a.dx[v];
+        double rv = (wid == 0) ? a.r[v] : 0.0;
+        bool marked = false;
+        uint32_t first = 0;
+        bool ok = false;
+        if (wid > 0) {
+            double ad;
+            uint32_t pc;
+            ok = load_event(a, v, (int64_t)e + (t - kWave), dxv, ad, pc);
+            s_ad[0][t - kWave] = ad;   // -0.0 past the segment
+            s_pc[0][t - kWave] = pc;
+        }
+        int c = __syncthreads_count(ok);   // valid events of a round are a prefix
+        int64_t f0 = e;
+        int b = 0;
+        for (;;) {
+            ok = false;
+            if (wid > 0) {
+                if (c == kRound) {
+                    double ad;
+                    uint32_t pc;
+                    ok = load_event(a, v, f0 + kRound + (t - kWave), dxv, ad, pc);
+                    s_ad[b ^ 1][t - kWave] = ad;
+                    s_pc[b ^ 1][t - kWave] = pc;
+                }
+            } else {
+                for (int q = 0; q < c; q += kWave) {
+                    const int idx = q + lane;
+                    double ad = -0.0;
+                    uint32_t pc = 0;
+                    if (idx < c) {
+                        ad = s_ad[b][idx];
+                        pc = s_pc[b][idx];
+                    }
+                    chunk_sum(&s_ad[b][q], ad, pc, lane, c - q < kWave ? c - q : kWave, thr, rv, marked, first);
+                }
+            }
+            const int c2 = __syncthreads_count(ok);
+            if (c < kRound) break;
+            c = c2;
+            f0 += kRound;
+            b ^= 1;
+        }
+        if (t == 0) {
+            a.r[v] = rv;
+            if (marked) emit_cross(a, v, first);
         }
     }
+    const unsigned int nmed = a.cnt[1];
+    const unsigned int waves = gridDim.x * (kHB / kWave);
+    for (unsigned int item = blockIdx.x * (kHB / kWave) + wid; item < nmed; item += waves)
+        replay_wave(a, a.med[item], lane, s_w[wid]);
 }
 
 __global__ void k_cheb_extract(const uint64_t *sorted, unsigned int c, int vbits, int by_id, int32_t *queue) {
@@ -203,6 +442,8 @@ static void free_cheb(ChebWork *w) {
     (void)hipFree(w->cross[1]);
     (void)hipFree(w->queue);
     (void)hipFree(w->cnt);
+    (void)hipFree(w->seg[0]);
+    (void)hipFree(w->seg[1]);
     (void)hipFree(w->oper);
     (void)hipFree(w->tmp);
     (void)hipFree(w->dx);
@@ -233,7 +474,9 @@ static int ensure_cheb(flsg_graph g, hipStream_t st) {
     FLSG_HIP(hipMalloc((void **)&w->cross[0], sizeof(uint64_t) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->cross[1], sizeof(uint64_t) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->queue, sizeof(int32_t) * (n + 1)));
-    FLSG_HIP(hipMalloc((void **)&w->cnt, sizeof(unsigned int)));
+    FLSG_HIP(hipMalloc((void **)&w->cnt, sizeof(unsigned int) * 3));
+    FLSG_HIP(hipMalloc((void **)&w->seg[0], sizeof(uint32_t) * (n + 1)));
+    FLSG_HIP(hipMalloc((void **)&w->seg[1], sizeof(uint32_t) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->oper, sizeof(unsigned long long)));
     FLSG_HIP(hipMalloc((void **)&w->dx, sizeof(double) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->val, sizeof(double) * (n + 1)));
@@ -350,6 +593,7 @@ int solve_local_cheb(flsg_graph g, const flsg_params *p, int64_t s, double *d_x,
         FLSG_HIP(hipMemcpyAsync(&E, w->P + k, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         FLSG_HIP(hipStreamSynchronize(st));
         if (E > w->cap) return fail(FLSG_ECAP, "event buffer too small");
+        if (E >= (int64_t)kChk) return fail(FLSG_ECAP, "more than 2^31 events in one iteration");
         const int pbits = bits_for((uint64_t)E);
         if (pbits + vbits > 64) return fail(FLSG_ECAP, "event keys exceed 64 bits");
         hipLaunchKernelGGL(k_cheb_events, dim3(grid), dim3(kCT), 0, st, g->indptr, g->indices, g->degree, w->queue,
@@ -357,9 +601,25 @@ int solve_local_cheb(flsg_graph g, const flsg_params *p, int64_t s, double *d_x,
         tb = w->tmp_bytes;
         FLSG_HIP(hipcub::DeviceRadixSort::SortPairs(w->tmp, tb, w->keys[0], w->keys[1], w->vals[0], w->vals[1],
                                                     (int)E, 0, pbits + vbits, st));
-        FLSG_HIP(hipMemsetAsync(w->cnt, 0, sizeof(unsigned int), st));
-        hipLaunchKernelGGL(k_cheb_replay, dim3(grid), dim3(kCT), 0, st, w->keys[1], w->vals[1], E, pbits, w->P,
-                           g->degree, w->val, w->dx, d_r, thr_scale, vbits, w->cross[0], w->cnt);
+        FLSG_HIP(hipMemsetAsync(w->cnt, 0, sizeof(unsigned int) * 3, st));
+        ReplayArgs ra{};
+        ra.keys = w->keys[1];
+        ra.vals = w->vals[1];
+        ra.E = E;
+        ra.pbits = pbits;
+        ra.vbits = vbits;
+        ra.P = w->P;
+        ra.degree = g->degree;
+        ra.val = w->val;
+        ra.dx = w->dx;
+        ra.r = d_r;
+        ra.thr_scale = thr_scale;
+        ra.cross = w->cross[0];
+        ra.cnt = w->cnt;
+        ra.med = w->seg[0];
+        ra.big = w->seg[1];
+        hipLaunchKernelGGL(k_cheb_replay, dim3((unsigned int)((E + kCT - 1) / kCT)), dim3(kCT), 0, st, ra);
+        hipLaunchKernelGGL(k_cheb_replay_heavy, dim3(g->num_cus * 2), dim3(kHB), 0, st, ra);
         unsigned int c = 0;
         FLSG_HIP(hipMemcpyAsync(&c, w->cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         FLSG_HIP(hipStreamSynchronize(st));
