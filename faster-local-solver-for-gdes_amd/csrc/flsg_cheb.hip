@@ -12,7 +12,9 @@
 //   2. events: entry i owns positions P_i .. P_i + d_i of a flat event space
 //      (P = exclusive scan of d_i + 1): position P_i + j is "add val_i to row
 //      entry j", position P_i + d_i is "subtract dx_{u_i} from r_{u_i}";
-//      key = (destination << pbits) | position;
+//      key = (destination << pbits) | (2 position + [subtraction]) and the
+//      sort payload is the FP64 addend itself, so the replay reads every
+//      event's value in sorted order instead of gathering it;
 //   3. radix sort of the keys (hipCUB) groups every destination's events in
 //      the reference's order;
 //   4. every destination's events are replayed sequentially in FP64
@@ -34,13 +36,14 @@ constexpr int kCT = 256;
 struct ChebWork {
     int64_t cap = 0;               // events capacity (m + n)
     uint64_t *keys[2] = {nullptr, nullptr};
-    int32_t *vals[2] = {nullptr, nullptr};
+    double *vals[2] = {nullptr, nullptr};   // sorted payload: each event's addend
     int64_t *P = nullptr;          // n + 1
     int32_t *dplus = nullptr;      // n
-    uint64_t *cross[2] = {nullptr, nullptr};   // n
+    uint64_t *cross[2] = {nullptr, nullptr};   // n: compacted, sorted crossings
+    uint64_t *xsub = nullptr;      // crossing sub-lists (kSub x xcap)
     int32_t *queue = nullptr;      // n
     unsigned int *cnt = nullptr;   // crossings, medium and heavy destinations (3)
-    uint32_t *seg[2] = {nullptr, nullptr};   // n + 1 each: medium / heavy destinations
+    uint32_t *seg[2] = {nullptr, nullptr};   // medium / heavy sub-lists (kSub x lcap each)
     unsigned long long *oper = nullptr;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
@@ -97,7 +100,8 @@ __global__ __launch_bounds__(kCT) void k_cheb_update(ChebCoefArgs a) {
 // step 2: one thread per event position
 __global__ __launch_bounds__(kCT) void k_cheb_events(const int32_t *indptr, const int32_t *indices,
                                                      const int32_t *degree, const int32_t *queue, const int64_t *P,
-                                                     int k, int64_t E, int pbits, uint64_t *keys, int32_t *vals) {
+                                                     const double *val, const double *dx, int k, int64_t E, int pbits,
+                                                     uint64_t *keys, double *vals) {
     for (int64_t pos = blockIdx.x * (int64_t)kCT + threadIdx.x; pos < E; pos += (int64_t)gridDim.x * kCT) {
         // entry i with P[i] <= pos < P[i+1]
         int lo = 0, hi = k;
@@ -110,68 +114,81 @@ __global__ __launch_bounds__(kCT) void k_cheb_events(const int32_t *indptr, cons
         const int u = queue[i];
         const int64_t j = pos - P[i];
         const int d = degree[u];
-        const int v = (j < d) ? indices[indptr[u] + j] : u;
-        keys[pos] = ((uint64_t)(uint32_t)v << pbits) | (uint64_t)pos;
-        vals[pos] = i;
+        const bool sub = j >= d;
+        const int v = sub ? u : indices[indptr[u] + j];
+        keys[pos] = ((uint64_t)(uint32_t)v << pbits) | ((uint64_t)pos << 1) | (sub ? 1u : 0u);
+        vals[pos] = sub ? -dx[u] : val[i];      // ppr_solver.py:657 / :652
     }
 }
 
 // step 4: sequential replay per destination.  A destination's events must be
 // summed one after another in FP64 (bit-identical to the reference), but the
 // loads behind them need not be: destinations are split by event count into
-//   light  (<= kLight events): one thread, every load issued before the sum;
+//   light  (<= kLight events): the head's lane, from its wave's lanes;
 //   medium (<= kBig):          one wave, 64 events loaded per chunk and
 //                              staged in LDS, summed from there in order;
 //   heavy  (>  kBig):          one 1024-thread block, 15 waves load rounds of
 //                              960 events into LDS while wave 0 sums the last.
-// Positions are < 2^31, so bit 31 of a position word marks a checked add.
+// Positions are < 2^31, so bit 31 of a position word marks a checked add
+// (every add except the subtraction of dx).
 constexpr int kLight = 16;
 constexpr int kBig = 256;
 constexpr int kHB = 1024;
 constexpr int kRound = kHB - kWave;
 static_assert(kRound % kWave == 0, "a round is whole chunks");
 constexpr uint32_t kChk = 0x80000000u;
+// Appends go to kSub sub-lists per list, each with its own counter on its own
+// 64-byte line: returning atomics on one address serialise (about 11 ns each
+// on MI355X, one per appending wave), on kSub addresses they overlap.
+// Counters: crossings, medium, heavy (kSub each), then the crossing total.
+constexpr int kSub = 256;
+constexpr int kCs = 16;
+constexpr int kCntWords = (3 * kSub + 1) * kCs;
+__host__ __device__ __forceinline__ int sub_word(int list, int s) { return (list * kSub + s) * kCs; }
 
 struct ReplayArgs {
     const uint64_t *keys;
-    const int32_t *vals;
+    const double *vals;   // addend of each sorted event
     int64_t E;
-    int pbits;
+    int pbits;   // bits of 2 position + [subtraction] in a key
     int vbits;
-    const int64_t *P;
     const int32_t *degree;
-    const double *val;
-    const double *dx;
     double *r;
     double thr_scale;
-    uint64_t *cross;
-    unsigned int *cnt;   // [0] crossings [1] medium [2] heavy
-    uint32_t *med;       // event index of each medium destination's first event
+    uint64_t *cross;     // kSub sub-lists of xcap crossing keys
+    int64_t xcap;
+    unsigned int *cnt;   // kCntWords
+    uint32_t *med;       // kSub sub-lists of lcap first-event indices of medium destinations
     uint32_t *big;       // same, heavy destinations
+    int64_t lcap;
 };
 
-__device__ __forceinline__ void emit_cross(const ReplayArgs &a, int v, uint32_t first) {
-    const unsigned int slot = atomicAdd(a.cnt, 1u);
-    a.cross[slot] = ((uint64_t)first << a.vbits) | (uint64_t)(uint32_t)v;
+__device__ __forceinline__ void emit_cross(const ReplayArgs &a, int sub, int v, uint32_t first) {
+    const unsigned int slot = atomicAdd(a.cnt + sub_word(0, sub), 1u);
+    a.cross[sub * a.xcap + slot] = ((uint64_t)first << a.vbits) | (uint64_t)(uint32_t)v;
+}
+
+// event f exists and belongs to destination v
+__device__ __forceinline__ bool e_same(const ReplayArgs &a, int64_t f, int v) {
+    return f < a.E && (int)(a.keys[f] >> a.pbits) == v;
+}
+
+// position word of a key: the position, with kChk unless the event is the
+// subtraction of dx (not checked, ppr_solver.py:657)
+__device__ __forceinline__ uint32_t pos_word(const ReplayArgs &a, uint64_t kf) {
+    const uint32_t low = (uint32_t)(kf & ((1ull << a.pbits) - 1));
+    return (low >> 1) | ((low & 1u) ? 0u : kChk);
 }
 
 // event f of destination v: its addend and position word; false past the segment
-__device__ __forceinline__ bool load_event(const ReplayArgs &a, int v, int64_t f, double dxv, double &ad,
-                                           uint32_t &pc) {
+__device__ __forceinline__ bool load_event(const ReplayArgs &a, int v, int64_t f, double &ad, uint32_t &pc) {
     ad = -0.0;   // exact identity of FP64 addition, so padding never changes a sum
     pc = 0;
     if (f >= a.E) return false;
     const uint64_t kf = a.keys[f];
     if ((int)(kf >> a.pbits) != v) return false;
-    const uint32_t pos = (uint32_t)(kf & ((1ull << a.pbits) - 1));
-    const int i = a.vals[f];
-    if ((int64_t)pos == a.P[i + 1] - 1) {
-        ad = -dxv;                                              // ppr_solver.py:657
-        pc = pos;
-    } else {
-        ad = a.val[i];                                          // ppr_solver.py:652
-        pc = pos | kChk;                                        // checked at ppr_solver.py:653
-    }
+    ad = a.vals[f];
+    pc = pos_word(a, kf);
     return true;
 }
 
@@ -236,79 +253,109 @@ __device__ __forceinline__ void chunk_sum(const double *s_add, double ad, uint32
     }
 }
 
-// heads of segments: light ones replayed here, the rest queued by size
+// one thread per event; a wave covers 64 events and looks kLight events past
+// its end (lanes 0..kLight-1 hold those as a second event), so every segment
+// whose head is in the wave and that has at most kLight events lies in the
+// window.  Such a segment is summed by its head lane from the window's
+// addends through shuffles; longer ones go to the medium or heavy list, and
+// a segment whose head is in an earlier wave belongs to that wave.
 __global__ __launch_bounds__(kCT) void k_cheb_replay(ReplayArgs a) {
-    const int64_t e = blockIdx.x * (int64_t)kCT + threadIdx.x;
-    if (e >= a.E) return;
-    const uint64_t key = a.keys[e];
-    const int v = (int)(key >> a.pbits);
-    if (e > 0 && (int)(a.keys[e - 1] >> a.pbits) == v) return;   // not the head of v's events
-    uint64_t kk[kLight];
-    kk[0] = key;
-    int len = 1;
-#pragma unroll
-    for (int j = 1; j <= kLight; j++) {
-        const int64_t f = e + j;
-        uint64_t kf = 0;
-        if (f < a.E) kf = a.keys[f];
-        const bool same = f < a.E && (int)(kf >> a.pbits) == v;   // same-destination events form a prefix
-        len += same ? 1 : 0;
-        if (j < kLight) kk[j] = kf;
+    const int64_t f = blockIdx.x * (int64_t)kCT + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t f0 = f - lane;
+    const bool in = f < a.E;
+    const uint64_t key = in ? a.keys[f] : 0;
+    const int v = in ? (int)(key >> a.pbits) : -1;
+    // the look-ahead events f0 + 64 .. f0 + 64 + kLight - 1
+    const int64_t f2 = f0 + kWave + lane;
+    const bool in2 = lane < kLight && f2 < a.E;
+    const uint64_t key2 = in2 ? a.keys[f2] : 0;
+    const int v2 = in2 ? (int)(key2 >> a.pbits) : -3;
+    int vp = __shfl_up(v, 1);
+    int vn = __shfl_down(v, 1);
+    int v2n = __shfl_down(v2, 1);
+    const int v2_first = __shfl(v2, 0);   // event f0 + 64 (-3 past the end)
+    if (lane == 0) vp = (in && f > 0) ? (int)(a.keys[f - 1] >> a.pbits) : -2;
+    if (lane == kWave - 1) vn = v2_first;
+    if (lane == kLight - 1) v2n = (f2 + 1 < a.E) ? (int)(a.keys[f2 + 1] >> a.pbits) : -4;
+    const bool head = in && vp != v;
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long tm = __ballot(in && vn != v);
+    const unsigned long long tm2 = __ballot(in2 && v2n != v2);   // bits 0 .. kLight-1
+    // my segment: head lane h (or -1: it began in an earlier wave) and tail
+    // index t in the window (64 + j for look-ahead event j; 128 if beyond)
+    const unsigned long long below = (lane == kWave - 1) ? ~0ull : ((2ull << lane) - 1);
+    const unsigned long long hb = hm & below, tb = tm & ~(below >> 1);
+    const int h = hb ? 63 - __clzll((long long)hb) : -1;
+    const int t = tb ? __ffsll((long long)tb) - 1 : (tm2 ? kWave + __ffsll((long long)tm2) - 1 : 128);
+    const bool light = in && h >= 0 && t - h < kLight;
+    // the segment that runs into the look-ahead, if light, takes those events
+    const int t2 = tm2 ? __ffsll((long long)tm2) - 1 : -1;
+    const bool cross_light = __ballot(light && t >= kWave) != 0;
+    double ad = -0.0, ad2 = -0.0;
+    uint32_t pc = 0, pc2 = 0;
+    if (light) {
+        ad = a.vals[f];
+        pc = pos_word(a, key);
     }
-    if (len > kLight) {
-        const int64_t f = e + kBig;
-        const bool heavy = f < a.E && (int)(a.keys[f] >> a.pbits) == v;
-        const unsigned int slot = atomicAdd(a.cnt + (heavy ? 2 : 1), 1u);
-        (heavy ? a.big : a.med)[slot] = (uint32_t)e;
-        return;
+    if (cross_light && lane <= t2) {
+        ad2 = a.vals[f2];
+        pc2 = pos_word(a, key2);
     }
-    const uint64_t pmask = (1ull << a.pbits) - 1;
-    int iv[kLight];
-#pragma unroll
-    for (int j = 0; j < kLight; j++) iv[j] = (j < len) ? a.vals[e + j] : 0;
-    int64_t endp[kLight];
-    double vv[kLight];
-#pragma unroll
-    for (int j = 0; j < kLight; j++) {
-        endp[j] = (j < len) ? a.P[iv[j] + 1] - 1 : -1;
-        vv[j] = (j < len) ? a.val[iv[j]] : 0.0;
+    const bool lh = head && light;
+    const int len = lh ? t - lane + 1 : 0;
+    double rv = 0.0, thr = 0.0;
+    if (lh) {
+        rv = a.r[v];
+        thr = a.thr_scale * (double)a.degree[v];
     }
-    const double thr = a.thr_scale * (double)a.degree[v];
-    const double dxv = a.dx[v];
-    double rv = a.r[v];
     bool marked = false;
     uint32_t first = 0;
-#pragma unroll
-    for (int j = 0; j < kLight; j++) {
+    const int lo = __double2loint(ad), hi = __double2hiint(ad);
+    const int lo2 = __double2loint(ad2), hi2 = __double2hiint(ad2);
+    for (int j = 0; __ballot(j < len); j++) {
+        const int src = lane + j;                   // < 64 + kLight for the lanes that use it
+        const int s1 = src & (kWave - 1);
+        const bool second = src >= kWave;
+        const int l1 = __shfl(lo, s1), h1 = __shfl(hi, s1), p1 = __shfl((int)pc, s1);
+        const int l2 = __shfl(lo2, s1), h2 = __shfl(hi2, s1), p2 = __shfl((int)pc2, s1);
+        const double aj = second ? __hiloint2double(h2, l2) : __hiloint2double(h1, l1);
+        const uint32_t pj = (uint32_t)(second ? p2 : p1);
         if (j < len) {
-            const int64_t pos = (int64_t)(kk[j] & pmask);
-            if (pos == endp[j]) {
-                rv += -dxv;                                     // ppr_solver.py:657
-            } else {
-                rv += vv[j];                                    // ppr_solver.py:652
-                if (!marked && thr <= fabs(rv)) {               // ppr_solver.py:653
-                    marked = true;
-                    first = (uint32_t)pos;
-                }
+            rv += aj;
+            if ((pj & kChk) && !marked && thr <= fabs(rv)) {
+                marked = true;
+                first = pj & ~kChk;
             }
         }
     }
-    a.r[v] = rv;
-    if (marked) emit_cross(a, v, first);
+    if (lh) a.r[v] = rv;
+    // one aggregated append per wave and list, into the wave's sub-lists (a
+    // wave adds at most 64 entries to each)
+    const int sub = (int)((f0 / kWave) & (kSub - 1));
+    const long long cs = wave_append(lh && marked, a.cnt + sub_word(0, sub));
+    if (cs >= 0) a.cross[sub * a.xcap + cs] = ((uint64_t)first << a.vbits) | (uint64_t)(uint32_t)v;
+    // more than kLight events: at most 64 + kLight when the tail is in the
+    // window, else probe the event kBig places on
+    const bool longh = head && !light;
+    const bool heavy = longh && t == 128 && e_same(a, f + kBig, v);
+    const long long ms = wave_append(longh && !heavy, a.cnt + sub_word(1, sub));
+    if (ms >= 0) a.med[sub * a.lcap + ms] = (uint32_t)f;
+    const long long bs = wave_append(heavy, a.cnt + sub_word(2, sub));
+    if (bs >= 0) a.big[sub * a.lcap + bs] = (uint32_t)f;
 }
 
 // one medium destination on one wave; s_w is the wave's own 64 LDS slots
-__device__ void replay_wave(const ReplayArgs &a, uint32_t e, int lane, double *s_w) {
+__device__ void replay_wave(const ReplayArgs &a, uint32_t e, int lane, double *s_w, int sub) {
     const int v = (int)(a.keys[e] >> a.pbits);
     const double thr = a.thr_scale * (double)a.degree[v];
-    const double dxv = a.dx[v];
     double rv = a.r[v];
     bool marked = false;
     uint32_t first = 0;
     for (int64_t f0 = e;; f0 += kWave) {
         double ad;
         uint32_t pc;
-        const unsigned long long ok = __ballot(load_event(a, v, f0 + lane, dxv, ad, pc));
+        const unsigned long long ok = __ballot(load_event(a, v, f0 + lane, ad, pc));
         s_w[lane] = ad;
         __builtin_amdgcn_wave_barrier();
         chunk_sum(s_w, ad, pc, lane, __popcll(ok), thr, rv, marked, first);
@@ -317,8 +364,22 @@ __device__ void replay_wave(const ReplayArgs &a, uint32_t e, int lane, double *s
     }
     if (lane == 0) {
         a.r[v] = rv;
-        if (marked) emit_cross(a, v, first);
+        if (marked) emit_cross(a, sub, v, first);
     }
+}
+
+// entry `item` of kSub sub-lists of capacity cap, given their inclusive
+// prefix counts
+__device__ __forceinline__ uint32_t sub_item(const uint32_t *lists, int64_t cap, const unsigned int *pre,
+                                             unsigned int item) {
+    int lo = 0, hi = kSub - 1;   // first sub-list whose prefix exceeds item
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] > item) hi = mid;
+        else lo = mid + 1;
+    }
+    const unsigned int before = lo ? pre[lo - 1] : 0u;
+    return lists[lo * cap + (item - before)];
 }
 
 // heavy destinations (one per block), then medium ones (one per wave).  Both
@@ -330,14 +391,25 @@ __global__ __launch_bounds__(kHB) void k_cheb_replay_heavy(ReplayArgs a) {
     __shared__ double s_ad[2][kRound];
     __shared__ uint32_t s_pc[2][kRound];
     __shared__ double s_w[kHB / kWave][kWave];
+    __shared__ unsigned int s_pre[2][kSub];   // inclusive prefix sums: medium, heavy sub-list counts
     const int t = threadIdx.x, lane = t & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
-    const unsigned int nbig = a.cnt[2];
+    if (t < 2 * kSub) s_pre[t / kSub][t % kSub] = a.cnt[sub_word(1 + t / kSub, t % kSub)];
+    __syncthreads();
+    for (int d = 1; d < kSub; d <<= 1) {
+        unsigned int y = 0;
+        if (t < 2 * kSub && t % kSub >= d) y = s_pre[t / kSub][t % kSub - d];
+        __syncthreads();
+        if (t < 2 * kSub) s_pre[t / kSub][t % kSub] += y;
+        __syncthreads();
+    }
+    // uniform by construction: the item loops below hold barriers
+    const unsigned int nmed = (unsigned int)__builtin_amdgcn_readfirstlane((int)s_pre[0][kSub - 1]);
+    const unsigned int nbig = (unsigned int)__builtin_amdgcn_readfirstlane((int)s_pre[1][kSub - 1]);
     for (unsigned int item = blockIdx.x; item < nbig; item += gridDim.x) {
-        const uint32_t e = a.big[item];
+        const uint32_t e = sub_item(a.big, a.lcap, s_pre[1], item);
         const int v = (int)(a.keys[e] >> a.pbits);
         const double thr = a.thr_scale * (double)a.degree[v];
-        const double dxv = a.dx[v];
         double rv = (wid == 0) ? a.r[v] : 0.0;
         bool marked = false;
         uint32_t first = 0;
@@ -345,7 +417,7 @@ __global__ __launch_bounds__(kHB) void k_cheb_replay_heavy(ReplayArgs a) {
         if (wid > 0) {
             double ad;
             uint32_t pc;
-            ok = load_event(a, v, (int64_t)e + (t - kWave), dxv, ad, pc);
+            ok = load_event(a, v, (int64_t)e + (t - kWave), ad, pc);
             s_ad[0][t - kWave] = ad;   // -0.0 past the segment
             s_pc[0][t - kWave] = pc;
         }
@@ -358,7 +430,7 @@ __global__ __launch_bounds__(kHB) void k_cheb_replay_heavy(ReplayArgs a) {
                 if (c == kRound) {
                     double ad;
                     uint32_t pc;
-                    ok = load_event(a, v, f0 + kRound + (t - kWave), dxv, ad, pc);
+                    ok = load_event(a, v, f0 + kRound + (t - kWave), ad, pc);
                     s_ad[b ^ 1][t - kWave] = ad;
                     s_pc[b ^ 1][t - kWave] = pc;
                 }
@@ -382,13 +454,28 @@ __global__ __launch_bounds__(kHB) void k_cheb_replay_heavy(ReplayArgs a) {
         }
         if (t == 0) {
             a.r[v] = rv;
-            if (marked) emit_cross(a, v, first);
+            if (marked) emit_cross(a, (int)(item & (kSub - 1)), v, first);
         }
     }
-    const unsigned int nmed = a.cnt[1];
     const unsigned int waves = gridDim.x * (kHB / kWave);
     for (unsigned int item = blockIdx.x * (kHB / kWave) + wid; item < nmed; item += waves)
-        replay_wave(a, a.med[item], lane, s_w[wid]);
+        replay_wave(a, sub_item(a.med, a.lcap, s_pre[0], item), lane, s_w[wid], (int)(item & (kSub - 1)));
+}
+
+// crossing sub-lists -> one array (sub-list order), and the total
+__global__ __launch_bounds__(kSub) void k_cheb_xcompact(const uint64_t *xsub, int64_t xcap, const unsigned int *cnt,
+                                                         uint64_t *out, unsigned int *total) {
+    __shared__ unsigned int s_part[kSub / kWave];
+    const int t = threadIdx.x, sub = blockIdx.x;
+    const unsigned int c = cnt[sub_word(0, t)];
+    unsigned int below = wave_sum(t < sub ? c : 0u);
+    if (__lane_id() == 0) s_part[t / kWave] = below;
+    __syncthreads();
+    unsigned int off = 0;
+    for (int w = 0; w < kSub / kWave; w++) off += s_part[w];
+    const unsigned int mine = cnt[sub_word(0, sub)];
+    for (unsigned int j = t; j < mine; j += kSub) out[off + j] = xsub[sub * xcap + j];
+    if (sub == kSub - 1 && t == 0) *total = off + mine;
 }
 
 __global__ void k_cheb_extract(const uint64_t *sorted, unsigned int c, int vbits, int by_id, int32_t *queue) {
@@ -430,6 +517,16 @@ __global__ void k_cheb_init_state(const int32_t *indptr, const int32_t *indices,
     for (int j = 0; j < d; j++) r[indices[st + j]] += val;
 }
 
+// sub-list capacities for E events: the head kernel's wave w appends at most
+// 64 entries to each list's sub-list w mod kSub; the heavy kernel adds one
+// crossing per medium or heavy destination (at most n of each) to sub-list
+// item mod kSub
+static void sub_caps(int64_t E, int64_t n, int64_t *lcap, int64_t *xcap) {
+    const int64_t waves = (E + kWave - 1) / kWave;
+    *lcap = (int64_t)kWave * ((waves + kSub - 1) / kSub);
+    *xcap = *lcap + 2 * ((n + kSub - 1) / kSub);
+}
+
 static void free_cheb(ChebWork *w) {
     if (!w) return;
     (void)hipFree(w->keys[0]);
@@ -444,6 +541,7 @@ static void free_cheb(ChebWork *w) {
     (void)hipFree(w->cnt);
     (void)hipFree(w->seg[0]);
     (void)hipFree(w->seg[1]);
+    (void)hipFree(w->xsub);
     (void)hipFree(w->oper);
     (void)hipFree(w->tmp);
     (void)hipFree(w->dx);
@@ -467,16 +565,19 @@ static int ensure_cheb(flsg_graph g, hipStream_t st) {
     w->cap = cap;
     FLSG_HIP(hipMalloc((void **)&w->keys[0], sizeof(uint64_t) * cap));
     FLSG_HIP(hipMalloc((void **)&w->keys[1], sizeof(uint64_t) * cap));
-    FLSG_HIP(hipMalloc((void **)&w->vals[0], sizeof(int32_t) * cap));
-    FLSG_HIP(hipMalloc((void **)&w->vals[1], sizeof(int32_t) * cap));
+    FLSG_HIP(hipMalloc((void **)&w->vals[0], sizeof(double) * cap));
+    FLSG_HIP(hipMalloc((void **)&w->vals[1], sizeof(double) * cap));
     FLSG_HIP(hipMalloc((void **)&w->P, sizeof(int64_t) * (n + 2)));
     FLSG_HIP(hipMalloc((void **)&w->dplus, sizeof(int32_t) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->cross[0], sizeof(uint64_t) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->cross[1], sizeof(uint64_t) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->queue, sizeof(int32_t) * (n + 1)));
-    FLSG_HIP(hipMalloc((void **)&w->cnt, sizeof(unsigned int) * 3));
-    FLSG_HIP(hipMalloc((void **)&w->seg[0], sizeof(uint32_t) * (n + 1)));
-    FLSG_HIP(hipMalloc((void **)&w->seg[1], sizeof(uint32_t) * (n + 1)));
+    FLSG_HIP(hipMalloc((void **)&w->cnt, sizeof(unsigned int) * kCntWords));
+    int64_t lcap = 0, xcap = 0;
+    sub_caps(cap, n, &lcap, &xcap);
+    FLSG_HIP(hipMalloc((void **)&w->seg[0], sizeof(uint32_t) * kSub * lcap));
+    FLSG_HIP(hipMalloc((void **)&w->seg[1], sizeof(uint32_t) * kSub * lcap));
+    FLSG_HIP(hipMalloc((void **)&w->xsub, sizeof(uint64_t) * kSub * xcap));
     FLSG_HIP(hipMalloc((void **)&w->oper, sizeof(unsigned long long)));
     FLSG_HIP(hipMalloc((void **)&w->dx, sizeof(double) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&w->val, sizeof(double) * (n + 1)));
@@ -485,7 +586,7 @@ static int ensure_cheb(flsg_graph g, hipStream_t st) {
     FLSG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, (const int64_t *)nullptr, (int64_t *)nullptr, (int)(n + 1),
                                               st));
     FLSG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b2, (const uint64_t *)nullptr, (uint64_t *)nullptr,
-                                                (const int32_t *)nullptr, (int32_t *)nullptr, (int)cap, 0, 64, st));
+                                                (const double *)nullptr, (double *)nullptr, (int)cap, 0, 64, st));
     FLSG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, (const uint64_t *)nullptr, (uint64_t *)nullptr,
                                                (int)(n + 1), 0, 64, st));
     size_t b = b1 > b2 ? b1 : b2;
@@ -594,34 +695,37 @@ int solve_local_cheb(flsg_graph g, const flsg_params *p, int64_t s, double *d_x,
         FLSG_HIP(hipStreamSynchronize(st));
         if (E > w->cap) return fail(FLSG_ECAP, "event buffer too small");
         if (E >= (int64_t)kChk) return fail(FLSG_ECAP, "more than 2^31 events in one iteration");
-        const int pbits = bits_for((uint64_t)E);
+        const int pbits = bits_for((uint64_t)(2 * E));   // 2 position + [subtraction]
         if (pbits + vbits > 64) return fail(FLSG_ECAP, "event keys exceed 64 bits");
         hipLaunchKernelGGL(k_cheb_events, dim3(grid), dim3(kCT), 0, st, g->indptr, g->indices, g->degree, w->queue,
-                           w->P, (int)k, E, pbits, w->keys[0], w->vals[0]);
+                           w->P, w->val, w->dx, (int)k, E, pbits, w->keys[0], w->vals[0]);
         tb = w->tmp_bytes;
         FLSG_HIP(hipcub::DeviceRadixSort::SortPairs(w->tmp, tb, w->keys[0], w->keys[1], w->vals[0], w->vals[1],
                                                     (int)E, 0, pbits + vbits, st));
-        FLSG_HIP(hipMemsetAsync(w->cnt, 0, sizeof(unsigned int) * 3, st));
+        FLSG_HIP(hipMemsetAsync(w->cnt, 0, sizeof(unsigned int) * kCntWords, st));
+        int64_t lcap = 0, xcap = 0;
+        sub_caps(E, n, &lcap, &xcap);
         ReplayArgs ra{};
         ra.keys = w->keys[1];
         ra.vals = w->vals[1];
         ra.E = E;
         ra.pbits = pbits;
         ra.vbits = vbits;
-        ra.P = w->P;
         ra.degree = g->degree;
-        ra.val = w->val;
-        ra.dx = w->dx;
         ra.r = d_r;
         ra.thr_scale = thr_scale;
-        ra.cross = w->cross[0];
+        ra.cross = w->xsub;
+        ra.xcap = xcap;
         ra.cnt = w->cnt;
         ra.med = w->seg[0];
         ra.big = w->seg[1];
+        ra.lcap = lcap;
         hipLaunchKernelGGL(k_cheb_replay, dim3((unsigned int)((E + kCT - 1) / kCT)), dim3(kCT), 0, st, ra);
         hipLaunchKernelGGL(k_cheb_replay_heavy, dim3(g->num_cus * 2), dim3(kHB), 0, st, ra);
+        unsigned int *d_total = w->cnt + 3 * kSub * kCs;
+        hipLaunchKernelGGL(k_cheb_xcompact, dim3(kSub), dim3(kSub), 0, st, w->xsub, xcap, w->cnt, w->cross[0], d_total);
         unsigned int c = 0;
-        FLSG_HIP(hipMemcpyAsync(&c, w->cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        FLSG_HIP(hipMemcpyAsync(&c, d_total, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         FLSG_HIP(hipStreamSynchronize(st));
         iters++;
         if (c > 0) {
