@@ -64,6 +64,12 @@ int ensure_single_workspace(flsg_graph g) {
     FLSG_HIP(hipMalloc((void **)&g->work_c, sizeof(double) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&g->ctr, sizeof(SolveCounters)));
     FLSG_HIP(hipHostMalloc((void **)&g->h_ctr, sizeof(SolveCounters), hipHostMallocDefault));
+    // next-frontier sub-lists: node u goes to sub-list u mod kFrontSub, so each
+    // holds at most ceil(n / kFrontSub) entries per iteration
+    g->fsub_cap = (n + kFrontSub - 1) / kFrontSub;
+    FLSG_HIP(hipMalloc((void **)&g->fsub, sizeof(int32_t) * kFrontSub * g->fsub_cap));
+    FLSG_HIP(hipMalloc((void **)&g->fsubcnt, sizeof(unsigned int) * kFrontCntWords));
+    FLSG_HIP(hipMemset(g->fsubcnt, 0, sizeof(unsigned int) * kFrontCntWords));
     return FLSG_OK;
 }
 
@@ -131,6 +137,8 @@ int flsg_graph_destroy(flsg_graph g) {
     (void)hipDeviceSynchronize();
     (void)hipFree(g->front[0]);
     (void)hipFree(g->front[1]);
+    (void)hipFree(g->fsub);
+    (void)hipFree(g->fsubcnt);
     (void)hipFree(g->val);
     (void)hipFree(g->pieces);
     (void)hipFree(g->work_a);

@@ -29,6 +29,13 @@ int fail(int code, const std::string &msg);
 
 // ---------------------------------------------------------------- device counters
 // One block of per-solve counters living in device memory.
+// next-frontier sub-lists of the single-source push: kFrontSub counters per
+// parity, each on its own 64-byte line (returning atomics on one address
+// serialise)
+constexpr int kFrontSub = 256;
+constexpr int kFrontCs = 16;
+constexpr int kFrontCntWords = 2 * kFrontSub * kFrontCs;
+
 struct SolveCounters {
     unsigned int cnt[2];        // frontier size by parity
     unsigned int npieces[2];    // hub pieces by parity
@@ -59,6 +66,9 @@ struct flsg_graph_s {
     int num_cus = 0;
     // single-query workspace (lazily allocated)
     int32_t *front[2] = {nullptr, nullptr};
+    int32_t *fsub = nullptr;                // next-frontier sub-lists (flsg_single.hip)
+    int64_t fsub_cap = 0;
+    unsigned int *fsubcnt = nullptr;        // their counters, by parity
     double *val = nullptr;          // per frontier entry (float or double)
     int64_t *pieces = nullptr;      // hub pieces: (frontier index << 32) | piece no.
     int64_t pieces_cap = 0;

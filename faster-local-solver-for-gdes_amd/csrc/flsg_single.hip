@@ -1,7 +1,7 @@
 // flsg_single.hip -- single-source solves spread over the whole GPU.
 //
 // Local push (PPR / Katz gradient descent, heat-kernel levels).  One
-// iteration = two kernels:
+// iteration = three kernels:
 //   k_node  : for every frontier node u (frontier = list of ids):
 //               tmp = r_u; r_u = 0; x_u += tmp; val_u = push value;
 //             nodes with degree > kHub are also split into edge "pieces";
@@ -16,8 +16,9 @@
 //             pushes are >= 0, so r_v only grows and exactly one atomic sees
 //             the crossing: the frontier is built without a mark array and
 //             without an O(n) scan (the reference GPU path scans all n nodes
-//             every iteration, ppr_solver.py:139).  Appends are
-//             wave-aggregated (one atomic per wave).
+//             every iteration, ppr_solver.py:139).  A crossing node u is
+//             appended to sub-list u mod 256, each with its own counter;
+//   k_front_compact : joins the sub-lists into the next frontier array.
 // The host launches iterations in blocks and reads the frontier size back
 // once per block; launches past the end see an empty frontier and exit.
 //
@@ -51,6 +52,9 @@ struct PushArgs {
     int64_t *pieces;
     int64_t pieces_cap;
     SolveCounters *ctr;
+    int32_t *fsub;          // next-frontier sub-lists (kFrontSub x fsub_cap)
+    int64_t fsub_cap;
+    unsigned int *fsubcnt;  // kFrontCntWords
     double *x;
     double *r;          // residual read by the node phase (HK: level j)
     double *r_push;     // residual pushed into (HK: level j+1; else == r)
@@ -104,6 +108,10 @@ __global__ __launch_bounds__(kNT) void k_node(PushArgs a, int p) {
     SolveCounters *c = a.ctr;
     const unsigned int cnt = c->cnt[p];
     const unsigned long long iters = c->iters;
+    // the sub-list counters this iteration's push appends to (last read by
+    // the compaction two launches back)
+    if (blockIdx.x == 0)
+        for (int t = threadIdx.x; t < kFrontSub; t += kNT) a.fsubcnt[((p ^ 1) * kFrontSub + t) * kFrontCs] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->cnt[p ^ 1] = 0;
         c->npieces[p ^ 1] = 0;
@@ -197,6 +205,18 @@ __device__ __forceinline__ bool push_edge(const PushArgs &a, int v, double val) 
     return old < thr && thr <= nw;
 }
 
+// a node that crossed its threshold joins the next frontier through sub-list
+// (u mod kFrontSub): one counter per sub-list instead of one for all, since
+// returning atomics on a single address serialise (~11 ns each on MI355X)
+__device__ __forceinline__ void append_front(const PushArgs &a, unsigned int *subcnt, int u) {
+    const int sub = u & (kFrontSub - 1);
+    const unsigned int slot = atomicAdd(subcnt + sub * kFrontCs, 1u);
+    // a node crosses at most once per iteration (pushes are >= 0), so the
+    // sub-list cannot fill; the guard keeps a broken input from writing past it
+    if ((int64_t)slot < a.fsub_cap) a.fsub[sub * a.fsub_cap + slot] = u;
+    else atomicOr(&a.ctr->overflow, 4u);
+}
+
 // ------------------------------------------------------------------ push phase
 template <int KIND>
 __global__ __launch_bounds__(kNT) void k_push(PushArgs a, int p) {
@@ -210,8 +230,7 @@ __global__ __launch_bounds__(kNT) void k_push(PushArgs a, int p) {
     if (cnt == 0) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) c->trace_off = c->trace_off + cnt;
     const int32_t *F = p ? a.front1 : a.front0;
-    int32_t *Fn = p ? a.front0 : a.front1;
-    unsigned int *ncnt = &c->cnt[p ^ 1];
+    unsigned int *subcnt = a.fsubcnt + (p ^ 1) * kFrontSub * kFrontCs;
     const unsigned int nchunks = (cnt + kNT - 1) / kNT;
     unsigned int npc = c->npieces[p];
     if ((int64_t)npc > a.pieces_cap) npc = (unsigned int)a.pieces_cap;
@@ -250,8 +269,7 @@ __global__ __launch_bounds__(kNT) void k_push(PushArgs a, int p) {
                     vtx = a.indices[s_start[k] + off];
                     cross = push_edge<KIND>(a, vtx, s_val[k]);
                 }
-                const long long slot = wave_append(cross, ncnt);
-                if (slot >= 0) Fn[slot] = vtx;
+                if (cross) append_front(a, subcnt, vtx);
             }
             __syncthreads();
         } else {
@@ -272,11 +290,27 @@ __global__ __launch_bounds__(kNT) void k_push(PushArgs a, int p) {
                     vtx = a.indices[st + e];
                     cross = push_edge<KIND>(a, vtx, v);
                 }
-                const long long slot = wave_append(cross, ncnt);
-                if (slot >= 0) Fn[slot] = vtx;
+                if (cross) append_front(a, subcnt, vtx);
             }
         }
     }
+}
+
+// next-frontier sub-lists -> the frontier array of parity p ^ 1 and its size
+__global__ __launch_bounds__(kFrontSub) void k_front_compact(PushArgs a, int p) {
+    __shared__ unsigned int s_part[kFrontSub / kWave];
+    const unsigned int *subcnt = a.fsubcnt + (p ^ 1) * kFrontSub * kFrontCs;
+    int32_t *Fn = p ? a.front0 : a.front1;
+    const int t = threadIdx.x, sub = blockIdx.x;
+    const unsigned int c = min((int64_t)subcnt[t * kFrontCs], a.fsub_cap);
+    const unsigned int below = wave_sum(t < sub ? c : 0u);
+    if (__lane_id() == 0) s_part[t / kWave] = below;
+    __syncthreads();
+    unsigned int off = 0;
+    for (int w = 0; w < kFrontSub / kWave; w++) off += s_part[w];
+    const unsigned int mine = min((int64_t)subcnt[sub * kFrontCs], a.fsub_cap);
+    for (unsigned int j = t; j < mine; j += kFrontSub) Fn[off + j] = a.fsub[sub * a.fsub_cap + j];
+    if (sub == kFrontSub - 1 && t == 0) a.ctr->cnt[p ^ 1] = off + mine;
 }
 
 __global__ void k_init_local(SolveCounters *c, int32_t *front0, double *r, int64_t s, double r0) {
@@ -415,6 +449,9 @@ static PushArgs base_args(flsg_graph g, double *x, double *r) {
     a.pieces = g->pieces;
     a.pieces_cap = g->pieces_cap;
     a.ctr = g->ctr;
+    a.fsub = g->fsub;
+    a.fsub_cap = g->fsub_cap;
+    a.fsubcnt = g->fsubcnt;
     a.x = x;
     a.r = r;
     a.r_push = r;
@@ -466,6 +503,7 @@ static int run_local_push(flsg_graph g, PushArgs a, int64_t max_iters, hipStream
             const int p = (int)(t & 1);
             hipLaunchKernelGGL(k_node<KIND>, dim3(grid), dim3(kNT), 0, st, a, p);
             hipLaunchKernelGGL(k_push<KIND>, dim3(grid), dim3(kNT), 0, st, a, p);
+            hipLaunchKernelGGL(k_front_compact, dim3(kFrontSub), dim3(kFrontSub), 0, st, a, p);
         }
         FLSG_HIP(hipGetLastError());
         FLSG_HIP(hipMemcpyAsync(g->h_ctr, g->ctr, sizeof(SolveCounters), hipMemcpyDeviceToHost, st));
@@ -477,6 +515,7 @@ static int run_local_push(flsg_graph g, PushArgs a, int64_t max_iters, hipStream
         }
     }
     if (g->h_ctr->overflow & 2u) return fail(FLSG_ECAP, "hub piece list overflow");
+    if (g->h_ctr->overflow & 4u) return fail(FLSG_ECAP, "next-frontier sub-list overflow (a node crossed twice in one iteration)");
     *iters_out = (int64_t)g->h_ctr->iters;
     return FLSG_OK;
 }
@@ -566,6 +605,7 @@ static int solve_hk(flsg_graph g, const flsg_params *p, int64_t s, double *d_x, 
             a.last_level = (j + 1 == N);
             hipLaunchKernelGGL(k_node<K_HK>, dim3(grid), dim3(kNT), 0, st, a, par);
             hipLaunchKernelGGL(k_push<K_HK>, dim3(grid), dim3(kNT), 0, st, a, par);
+            hipLaunchKernelGGL(k_front_compact, dim3(kFrontSub), dim3(kFrontSub), 0, st, a, par);
             hipLaunchKernelGGL(k_clear_level, dim3(grid), dim3(kNT), 0, st, (const SolveCounters *)g->ctr, par,
                                a.r, n);
         }
@@ -575,6 +615,7 @@ static int solve_hk(flsg_graph g, const flsg_params *p, int64_t s, double *d_x, 
         if (g->h_ctr->cnt[j & 1] == 0) done = true;
     }
     if (g->h_ctr->overflow & 2u) return fail(FLSG_ECAP, "hub piece list overflow");
+    if (g->h_ctr->overflow & 4u) return fail(FLSG_ECAP, "next-frontier sub-list overflow (a node crossed twice in one iteration)");
     // x /= e^t  (hk_solver.py:291)
     hipLaunchKernelGGL(k_scale_div, dim3(grid), dim3(kNT), 0, st, d_x, n, exp(p->tau));
     res->oper = (int64_t)g->h_ctr->oper;
