@@ -139,6 +139,9 @@ int flsg_graph_destroy(flsg_graph g) {
     (void)hipFree(g->front[1]);
     (void)hipFree(g->fsub);
     (void)hipFree(g->fsubcnt);
+    (void)hipFree(g->rows_wave);
+    (void)hipFree(g->rows_block);
+    (void)hipFree(g->rows_cnt);
     (void)hipFree(g->val);
     (void)hipFree(g->pieces);
     (void)hipFree(g->work_a);

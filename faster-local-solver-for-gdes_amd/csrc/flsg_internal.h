@@ -66,6 +66,11 @@ struct flsg_graph_s {
     int num_cus = 0;
     // single-query workspace (lazily allocated)
     int32_t *front[2] = {nullptr, nullptr};
+    int32_t *rows_wave = nullptr;           // global sweeps: rows pulled by a wave / a block
+    int32_t *rows_block = nullptr;
+    unsigned int *rows_cnt = nullptr;
+    int64_t n_rows_wave = 0, n_rows_block = 0;
+    int rows_valid = 0;                     // cleared when degrees change
     int32_t *fsub = nullptr;                // next-frontier sub-lists (flsg_single.hip)
     int64_t fsub_cap = 0;
     unsigned int *fsubcnt = nullptr;        // their counters, by parity

@@ -39,6 +39,10 @@ constexpr int kNT = 256;          // threads per block
 constexpr int kHub = 4096;        // degree above which a node is split into pieces
 constexpr int kPiece = 4096;      // edges per hub piece
 constexpr int kIterBlock = 8;     // iterations launched between host checks
+constexpr int kRowLight = 64;     // global sweeps: rows up to this length are pulled by one thread,
+constexpr int kRowWave = 1024;    // up to this by one wave, longer ones by one block
+constexpr int kPB = 1024;         // threads of the long-row block
+constexpr int kPRound = kPB - kWave;   // entries per block round
 
 enum { K_PPR = 0, K_KATZ = 1, K_HK = 2 };
 
@@ -375,6 +379,15 @@ __global__ __launch_bounds__(kNT) void k_gsweep_node(const int32_t *degree, doub
 // mode 1 (PPR Chebyshev): r_new = r + (sum - dx); mode 2 (Katz Chebyshev):
 // r_new = r + sum - dx.
 template <int MODE>
+__device__ __forceinline__ double sweep_result(double s, const double *r_old, const double *dx, int64_t v) {
+    if constexpr (MODE == 0) return s;
+    else if constexpr (MODE == 1) return r_old[v] + (s - dx[v]);
+    else return r_old[v] + s - dx[v];
+}
+
+// rows of at most kRowLight entries: one thread each (longer rows are pulled
+// by k_gsweep_pull_long, from the lists built by k_row_classes)
+template <int MODE>
 __global__ __launch_bounds__(kNT) void k_gsweep_pull(const int32_t *indptr, const int32_t *indices,
                                                      const int32_t *degree, const double *val,
                                                      const double *r_old, const double *dx, double *r_new,
@@ -384,13 +397,12 @@ __global__ __launch_bounds__(kNT) void k_gsweep_pull(const int32_t *indptr, cons
     unsigned int act = 0;
     for (int64_t v = blockIdx.x * (int64_t)kNT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kNT) {
         const int d = degree[v];
+        if (d > kRowLight) continue;
         const int st = indptr[v];
         double s = 0.0;
+#pragma unroll 8
         for (int j = 0; j < d; j++) s += val[indices[st + j]];
-        double rn;
-        if constexpr (MODE == 0) rn = s;
-        else if constexpr (MODE == 1) rn = r_old[v] + (s - dx[v]);
-        else rn = r_old[v] + s - dx[v];
+        const double rn = sweep_result<MODE>(s, r_old, dx, v);
         r_new[v] = rn;
         if (d > 0 && thr_scale * (double)d <= fabs(rn)) act++;
     }
@@ -401,6 +413,122 @@ __global__ __launch_bounds__(kNT) void k_gsweep_pull(const int32_t *indptr, cons
         unsigned int t = 0;
         for (int w = 0; w < kNT / kWave; w++) t += s_cnt[w];
         if (t) atomicAdd(&c->act[p ^ 1], t);
+    }
+}
+
+// adds s_add[0 .. cnt) to s in order; s_add holds -0.0 up to the next multiple
+// of 8 (the identity of FP64 addition), and two register sets alternate so
+// each group's LDS reads are in flight while the previous group is added
+__device__ __forceinline__ void lds_chain(const double *s_add, int cnt, double &s) {
+    const int ng = (cnt + 7) >> 3;
+    double ga[8], gb[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) ga[k] = s_add[k];
+    for (int g = 0; g < ng; g += 2) {
+        const int ob = (g + 1 < ng ? g + 1 : g) * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k++) gb[k] = s_add[ob + k];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += ga[k];
+        if (g + 1 >= ng) break;
+        const int oa = (g + 2 < ng ? g + 2 : g) * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k++) ga[k] = s_add[oa + k];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += gb[k];
+    }
+}
+
+// rows longer than kRowLight: each sum is still one FP64 add after another in
+// row order, but its loads are spread over a wave (rows up to kRowWave: 64
+// entries per chunk staged in LDS) or a block (longer rows: 15 waves load
+// 960-entry rounds into LDS, double-buffered, while wave 0 adds the last).
+// Rows are assigned by static strides of uniform values, and the count of
+// active rows is reduced per block (one atomic per block).
+template <int MODE>
+__global__ __launch_bounds__(kPB) void k_gsweep_pull_long(const int32_t *indptr, const int32_t *indices,
+                                                          const int32_t *degree, const double *val,
+                                                          const double *r_old, const double *dx, double *r_new,
+                                                          const int32_t *rows_wave, int64_t n_wave,
+                                                          const int32_t *rows_block, int64_t n_block,
+                                                          double thr_scale, SolveCounters *c, int p) {
+    __shared__ double s_ad[2][kPRound];
+    __shared__ double s_w[kPB / kWave][kWave];
+    __shared__ unsigned int s_act[kPB / kWave];
+    if (__builtin_amdgcn_readfirstlane((int)c->act[p]) == 0) return;
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
+    unsigned int act = 0;
+    for (int64_t item = blockIdx.x; item < n_block; item += gridDim.x) {
+        const int v = __builtin_amdgcn_readfirstlane(rows_block[item]);
+        const int d = __builtin_amdgcn_readfirstlane(degree[v]);
+        const int st = __builtin_amdgcn_readfirstlane(indptr[v]);
+        double s = 0.0;
+        if (wid > 0) {
+            const int j = t - kWave;
+            s_ad[0][j] = j < d ? val[indices[st + j]] : -0.0;
+        }
+        __syncthreads();
+        int b = 0;
+        for (int j0 = 0; j0 < d; j0 += kPRound) {
+            if (wid > 0) {
+                if (j0 + kPRound < d) {
+                    const int j = j0 + kPRound + (t - kWave);
+                    s_ad[b ^ 1][t - kWave] = j < d ? val[indices[st + j]] : -0.0;
+                }
+            } else {
+                lds_chain(s_ad[b], d - j0 < kPRound ? d - j0 : kPRound, s);
+            }
+            __syncthreads();
+            b ^= 1;
+        }
+        if (t == 0) {
+            const double rn = sweep_result<MODE>(s, r_old, dx, v);
+            r_new[v] = rn;
+            if (thr_scale * (double)d <= fabs(rn)) act++;
+        }
+    }
+    const int64_t nw = (int64_t)gridDim.x * (kPB / kWave);
+    for (int64_t item = (int64_t)blockIdx.x * (kPB / kWave) + wid; item < n_wave; item += nw) {
+        const int v = __builtin_amdgcn_readfirstlane(rows_wave[item]);
+        const int d = __builtin_amdgcn_readfirstlane(degree[v]);
+        const int st = __builtin_amdgcn_readfirstlane(indptr[v]);
+        double s = 0.0;
+        for (int j0 = 0; j0 < d; j0 += kWave) {
+            const int j = j0 + lane;
+            s_w[wid][lane] = j < d ? val[indices[st + j]] : -0.0;
+            __builtin_amdgcn_wave_barrier();
+            lds_chain(s_w[wid], d - j0 < kWave ? d - j0 : kWave, s);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) {
+            const double rn = sweep_result<MODE>(s, r_old, dx, v);
+            r_new[v] = rn;
+            if (thr_scale * (double)d <= fabs(rn)) act++;
+        }
+    }
+    act = wave_sum(act);
+    if (lane == 0) s_act[wid] = act;
+    __syncthreads();
+    if (t == 0) {
+        unsigned int tot = 0;
+        for (int w = 0; w < kPB / kWave; w++) tot += s_act[w];
+        if (tot) atomicAdd(&c->act[p ^ 1], tot);
+    }
+}
+
+// row classes of the global sweeps (they depend on the degrees only)
+__global__ __launch_bounds__(kNT) void k_row_classes(const int32_t *degree, int64_t n, int32_t *rows_wave,
+                                                     int32_t *rows_block, unsigned int *cnt) {
+    for (int64_t base = blockIdx.x * (int64_t)kNT; base < n; base += (int64_t)gridDim.x * kNT) {
+        const int64_t v = base + threadIdx.x;
+        const int d = v < n ? degree[v] : 0;
+        const long long sw = wave_append(d > kRowLight && d <= kRowWave, cnt);
+        if (sw >= 0) rows_wave[sw] = (int32_t)v;
+        const long long sb = wave_append(d > kRowWave, cnt + 1);
+        if (sb >= 0) rows_block[sb] = (int32_t)v;
     }
 }
 
@@ -699,6 +827,28 @@ static void cheb_coefs(const flsg_params *p, int64_t k, std::vector<double> &c) 
     }
 }
 
+// rows pulled by a wave / a block in the global sweeps; rebuilt after the
+// degrees change
+static int ensure_row_lists(flsg_graph g, hipStream_t st) {
+    if (g->rows_valid) return FLSG_OK;
+    const int64_t n = g->n;
+    if (!g->rows_wave) {
+        FLSG_HIP(hipMalloc((void **)&g->rows_wave, sizeof(int32_t) * (n + 1)));
+        FLSG_HIP(hipMalloc((void **)&g->rows_block, sizeof(int32_t) * (n + 1)));
+        FLSG_HIP(hipMalloc((void **)&g->rows_cnt, sizeof(unsigned int) * 2));
+    }
+    FLSG_HIP(hipMemsetAsync(g->rows_cnt, 0, sizeof(unsigned int) * 2, st));
+    hipLaunchKernelGGL(k_row_classes, dim3(grid_for(g, 8)), dim3(kNT), 0, st, g->degree, n, g->rows_wave,
+                       g->rows_block, g->rows_cnt);
+    unsigned int h[2] = {0, 0};
+    FLSG_HIP(hipMemcpyAsync(h, g->rows_cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    FLSG_HIP(hipStreamSynchronize(st));
+    g->n_rows_wave = h[0];
+    g->n_rows_block = h[1];
+    g->rows_valid = 1;
+    return FLSG_OK;
+}
+
 static int solve_global(flsg_graph g, const flsg_params *p, int64_t s, double *d_x, double *d_r,
                         flsg_result *res, hipStream_t st) {
     const int64_t n = g->n;
@@ -709,6 +859,9 @@ static int solve_global(flsg_graph g, const flsg_params *p, int64_t s, double *d
     const int grid = grid_for(g, 8);
     const int64_t max_sweeps = p->max_iters > 0 ? p->max_iters : 100000;
     double *val = g->work_b, *dx = g->work_c;
+    int rc0 = ensure_row_lists(g, st);
+    if (rc0) return rc0;
+    const int lgrid = grid_for(g, 2);
     FLSG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
     FLSG_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * n, st));
     hipLaunchKernelGGL(k_init_global, dim3(1), dim3(1), 0, st, g->ctr);
@@ -757,16 +910,25 @@ static int solve_global(flsg_graph g, const flsg_params *p, int64_t s, double *d
                                        n, alpha, om, g->ctr, par);
                 hipLaunchKernelGGL(k_gsweep_pull<0>, dim3(grid), dim3(kNT), 0, st, g->indptr, g->indices, g->degree,
                                    val, d_r, dx, d_r, n, thr_scale, g->ctr, par);
+                hipLaunchKernelGGL(k_gsweep_pull_long<0>, dim3(lgrid), dim3(kPB), 0, st, g->indptr, g->indices,
+                                   g->degree, val, d_r, dx, d_r, g->rows_wave, g->n_rows_wave, g->rows_block,
+                                   g->n_rows_block, thr_scale, g->ctr, par);
             } else if (ppr) {
                 hipLaunchKernelGGL((k_csweep_node<K_PPR, 1>), dim3(grid), dim3(kNT), 0, st, g->degree, d_x, d_r, dx,
                                    val, n, g->coef, alpha, om, g->ctr, par, k);
                 hipLaunchKernelGGL(k_gsweep_pull<1>, dim3(grid), dim3(kNT), 0, st, g->indptr, g->indices, g->degree,
                                    val, d_r, dx, d_r, n, thr_scale, g->ctr, par);
+                hipLaunchKernelGGL(k_gsweep_pull_long<1>, dim3(lgrid), dim3(kPB), 0, st, g->indptr, g->indices,
+                                   g->degree, val, d_r, dx, d_r, g->rows_wave, g->n_rows_wave, g->rows_block,
+                                   g->n_rows_block, thr_scale, g->ctr, par);
             } else {
                 hipLaunchKernelGGL((k_csweep_node<K_KATZ, 2>), dim3(grid), dim3(kNT), 0, st, g->degree, d_x, d_r, dx,
                                    val, n, g->coef, alpha, om, g->ctr, par, k);
                 hipLaunchKernelGGL(k_gsweep_pull<2>, dim3(grid), dim3(kNT), 0, st, g->indptr, g->indices, g->degree,
                                    val, d_r, dx, d_r, n, thr_scale, g->ctr, par);
+                hipLaunchKernelGGL(k_gsweep_pull_long<2>, dim3(lgrid), dim3(kPB), 0, st, g->indptr, g->indices,
+                                   g->degree, val, d_r, dx, d_r, g->rows_wave, g->n_rows_wave, g->rows_block,
+                                   g->n_rows_block, thr_scale, g->ctr, par);
             }
         }
         FLSG_HIP(hipGetLastError());

@@ -7,7 +7,6 @@ the same size)."""
 from __future__ import annotations
 
 import json
-import sys
 import time
 
 import numpy as np
@@ -15,15 +14,20 @@ import numpy as np
 from . import _abi, engine, graph
 
 
-def plain_bench(seeds: int = 50, eps: float = 1e-7, alpha: float = 0.1, log=print) -> list:
+METHODS = {"local_gd": ("local GD", _abi.FLSG_GD, True),
+           "local_cheb": ("local Chebyshev", _abi.FLSG_CHEB, True),
+           "global_gd": ("global GD", _abi.FLSG_GD, False),
+           "global_cheb": ("global Chebyshev", _abi.FLSG_CHEB, False)}
+
+
+def plain_bench(seeds: int = 50, eps: float = 1e-7, alpha: float = 0.1, log=print,
+                methods=("local_gd", "local_cheb", "global_gd", "global_cheb")) -> list:
     g = graph.powerlaw_graph_large()
     dg = engine.DeviceGraph(g)
     try:
         ss = graph.uniform_seeds(g, seeds, seed=0)
         out = []
-        for name, method, local in (("local GD", _abi.FLSG_GD, True), ("local Chebyshev", _abi.FLSG_CHEB, True),
-                                    ("global GD", _abi.FLSG_GD, False),
-                                    ("global Chebyshev", _abi.FLSG_CHEB, False)):
+        for name, method, local in (METHODS[m] for m in methods):
             p = engine.make_params(_abi.FLSG_PPR, method, local, _abi.FLSG_FP64, alpha=alpha, eps=eps)
             engine.solve(dg, p, int(ss[0]))      # warm-up
             opers, ms, iters = [], [], []
@@ -46,6 +50,11 @@ def plain_bench(seeds: int = 50, eps: float = 1e-7, alpha: float = 0.1, log=prin
 
 
 if __name__ == "__main__":
-    for r in plain_bench(seeds=int(sys.argv[1]) if len(sys.argv) > 1 else 50,
-                         log=lambda d: print(json.dumps(d), flush=True)):
-        pass
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--seeds", type=int, default=50)
+    ap.add_argument("--methods", default="local_gd,local_cheb,global_gd,global_cheb",
+                    help="comma list of " + ", ".join(METHODS))
+    args = ap.parse_args()
+    plain_bench(seeds=args.seeds, methods=tuple(args.methods.split(",")),
+                log=lambda d: print(json.dumps(d), flush=True))

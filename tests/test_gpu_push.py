@@ -100,6 +100,31 @@ def test_ppr_global_parity_small(small_graph, dev_small, method):
         assert np.array_equal(out.r.cpu().numpy(), o["r"])
 
 
+@pytest.fixture(scope="module")
+def hub_graph():
+    """20,000 nodes with five rows over 1024 entries and 193 of 65-1024: every
+    row class of the global sweeps' pull (thread, wave, block)."""
+    return graph.powerlaw_graph_small(n=20000, wmax=5000.0, seed=5)
+
+
+@pytest.mark.parametrize("method", ["gd", "cheb"])
+def test_ppr_global_parity_hub_rows(hub_graph, method):
+    g = hub_graph
+    assert int(g.degree.max()) > 1024
+    dg = engine.DeviceGraph(g)
+    m = _abi.FLSG_GD if method == "gd" else _abi.FLSG_CHEB
+    p = engine.make_params(_abi.FLSG_PPR, m, False, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-6)
+    fn = O.ppr_gd if method == "gd" else O.ppr_cheby
+    hubs = np.argsort(-np.asarray(g.degree))[:2]
+    for s in list(graph.uniform_seeds(g, 2, seed=3)) + [int(h) for h in hubs]:
+        o = fn(g, s, ALPHA, 1e-6, local=False)
+        out = engine.solve(dg, p, int(s))
+        assert out.result["oper"] == o["oper"], f"seed {s}"
+        assert np.array_equal(out.x.cpu().numpy(), o["x"]), f"seed {s}"
+        assert np.array_equal(out.r.cpu().numpy(), o["r"]), f"seed {s}"
+    dg.close()
+
+
 @pytest.mark.parametrize("local", [True, False])
 def test_katz_gd_parity_small(small_graph, dev_small, local):
     g = small_graph
