@@ -9,6 +9,8 @@ device propagation (FP32 rounds + FP64 certificate, so on return
 from the reference's per-column FIFO, so P differs from it by at most what
 the two residuals allow (both satisfy the same invariant and certificate).
 ``runtimes`` are the device time split evenly over the columns.
+``instantGNNPush`` / ``instantGNNUpdateBatch`` (one column, the reference's
+queue arguments) run the same device propagation in FP64.
 SOR pushes and edge deletion stay CPU reference baselines (NotImplementedError).
 """
 from __future__ import annotations
@@ -99,12 +101,102 @@ def updetaGraph(indptr, indices, degree, us, vs, onlyinsert=True):
     return ins, {}
 
 
-def instantGNNPush(*args, **kwargs):
-    raise NotImplementedError("the per-column FIFO push is replaced by instantGNNParallel's device propagation")
+def _eps_from_vec(degree, eps_vec, beta) -> float:
+    """eps of eps_vec = eps * degree**(1-beta), the thresholds InstantGNN.py:57
+    builds; other threshold vectors are rejected."""
+    d = np.asarray(degree, dtype=np.float64)
+    ev = np.asarray(eps_vec, dtype=np.float64)
+    pos = d > 0
+    if not pos.any():
+        return float(ev.max()) if ev.size else 0.0
+    w = d[pos] ** (1.0 - beta)
+    eps = float(np.median(ev[pos] / w))
+    if not np.allclose(ev[pos], eps * w, rtol=1e-6, atol=0.0):
+        raise ValueError("eps_vec must be eps * degree**(1-beta) (InstantGNN.py:57)")
+    return eps
 
 
-def instantGNNUpdateBatch(*args, **kwargs):
-    raise NotImplementedError("use instantGNNUpdateBatchParallel (all columns on the device)")
+def _queue_nodes(queue, front, rear, qlen):
+    """Entries front+1 .. rear of the circular queue (InstantGNN.py:14-16)."""
+    out = []
+    f = int(front)
+    while f != int(rear):
+        f = (f + 1) % int(qlen)
+        out.append(int(queue[f]))
+    return np.unique(np.asarray(out, dtype=np.int32))
+
+
+def _column_prop(indptr, indices, degree, x, p, r, alpha, eps, beta):
+    cols = lambda a: np.asarray(a, dtype=np.float64).reshape(-1, 1)
+    return DynamicPropagation(_slack(indptr, indices, degree), X=cols(x), alpha=float(alpha), eps=float(eps),
+                              beta=float(beta), precision=_abi.FLSG_FP64, R0=cols(r), P0=cols(p))
+
+
+def _push_from(prop, nodes):
+    torch = prop.torch
+    d_nodes = torch.from_numpy(np.ascontiguousarray(nodes, dtype=np.int32)).to(prop.dg.device)
+    return prop._propagate(d_nodes, int(nodes.size)) if nodes.size else None
+
+
+def instantGNNPush(n, indptr, indices, degree, alpha, eps_vec, p, r, queue, front, rear, q_mark, beta):
+    """InstantGNN.py:12-43 for one feature column: pushes from the nodes in
+    queue positions front+1 .. rear (the queue is circular with length n,
+    which the reference's drivers pass as node count + 2) until every
+    |r_u| < eps_vec_u; p and r are updated in place and q_mark cleared.  The
+    device pushes in synchronous FP64 rounds, not in FIFO order, so p matches
+    the reference's through the propagation invariant and the certificate
+    rather than bitwise.  Returns the runtime in seconds."""
+    import time
+    t0 = time.perf_counter()
+    eps = _eps_from_vec(degree, eps_vec, beta)
+    nodes = _queue_nodes(queue, front, rear, n)
+    prop = _column_prop(indptr, indices, degree, r, p, r, alpha, eps, beta)
+    try:
+        _push_from(prop, nodes)
+        p[:] = prop.P.cpu().numpy()[:, 0].astype(p.dtype)
+        r[:] = prop.R.cpu().numpy()[:, 0].astype(r.dtype)
+    finally:
+        prop.close()
+    q_mark[:] = False
+    return time.perf_counter() - t0
+
+
+def instantGNNUpdateBatch(n, indptr, indices, degree, old_degree, p, r, x, alpha, queue, front, rear, q_mark, eps_vec,
+                          insert_neighbor, delete_neighbor, algo='fwd', omega=1, beta=0):
+    """InstantGNN.py:298-329 for one column, after updetaGraph has appended the
+    inserted neighbours to the rows: rescales / corrects p and r of every
+    endpoint in insert_neighbor for the new degrees (the sum over a row's new
+    neighbours is taken over the appended row tail, the same set), then pushes
+    from them and from any nodes already queued.  Edge deletion and SOR pushes
+    are not built.  Returns the runtime in seconds."""
+    import time
+    import ctypes as C
+    from flsg.engine import _check, _ptr, _stream_ptr
+    if "sor" in algo:
+        _d.sor_not_built("instantGNNUpdateBatch(algo=%r)" % algo)
+    if "insert" not in algo and any(len(v) for v in dict(delete_neighbor).values()):
+        raise NotImplementedError("edge deletion is not built on the GPU")
+    t0 = time.perf_counter()
+    eps = _eps_from_vec(degree, eps_vec, beta)
+    keys = np.asarray(sorted(int(u) for u in dict(insert_neighbor).keys()), dtype=np.int32)
+    prop = _column_prop(indptr, indices, degree, x, p, r, alpha, eps, beta)
+    try:
+        torch = prop.torch
+        dev = prop.dg.device
+        d_keys = torch.from_numpy(keys if keys.size else np.zeros(1, np.int32)).to(dev)
+        d_old = torch.from_numpy(np.ascontiguousarray(old_degree, dtype=np.int32)).to(dev)
+        prop.params.degrees_changed = 1
+        _check(prop.dg.lib.flsg_ignn_update(prop.dg.handle, C.byref(prop.params), _ptr(prop.X), _ptr(prop.P),
+                                            _ptr(prop.R), _ptr(d_keys), int(keys.size), _ptr(d_old),
+                                            _stream_ptr(torch)))
+        nodes = np.union1d(keys, _queue_nodes(queue, front, rear, n)).astype(np.int32)
+        _push_from(prop, nodes)
+        p[:] = prop.P.cpu().numpy()[:, 0].astype(p.dtype)
+        r[:] = prop.R.cpu().numpy()[:, 0].astype(r.dtype)
+    finally:
+        prop.close()
+    q_mark[:] = False
+    return time.perf_counter() - t0
 
 
 def instantGNNSorPush(*args, **kwargs):

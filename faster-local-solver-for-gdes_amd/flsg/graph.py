@@ -245,6 +245,45 @@ def alpha_roundtrip(alpha: float):
     return 2 * a / (1 + a), float(omega)
 
 
+def save_exp_npz(path: str, g: SlackGraph, node_feature: np.ndarray, labels: np.ndarray | None = None,
+                 split: dict | None = None) -> None:
+    """Write a dynamic-graph workload in the file layout the reference's
+    InstantGNN experiment reads (``data_process_GNN.py:86-87``, read at
+    ``instantGNN-arxiv-exp.py:174-190``): slack CSR ``indptr``/``indices``
+    (free slots hold n) /``degree`` (int64 live lengths), the insert batches
+    ``Us``/``Vs`` (int32 [snapshots, batch]), ``node_feature`` (float32),
+    ``labels`` (int64 [n, 1]) and ``split_idx`` (a dict of train/valid/test
+    index arrays, stored as an object array as the reference stores it)."""
+    n = g.n
+    if labels is None:
+        labels = np.zeros((n, 1), dtype=np.int64)
+    if split is None:
+        idx = np.arange(n, dtype=np.int64)
+        a, b = int(0.54 * n), int(0.72 * n)
+        split = {"train": idx[:a], "valid": idx[a:b], "test": idx[b:]}
+    np.savez(path, indices=np.asarray(g.indices, dtype=np.int32), indptr=np.asarray(g.indptr, dtype=np.int64),
+             degree=np.asarray(g.degree, dtype=np.int64), Us=np.asarray(g.us, dtype=np.int32),
+             Vs=np.asarray(g.vs, dtype=np.int32), node_feature=np.asarray(node_feature, dtype=np.float32),
+             labels=np.asarray(labels, dtype=np.int64), split_idx=np.array([split], dtype=object))
+
+
+def load_exp_npz(path: str, name: str | None = None) -> tuple[SlackGraph, np.ndarray]:
+    """Read ``save_exp_npz`` / ``data_process_GNN.py`` output: (slack graph with
+    its insert batches, node features).  Only the numeric members are read, so
+    the pickled ``split_idx`` is never loaded."""
+    with np.load(path, allow_pickle=False) as d:
+        indptr = d["indptr"].astype(np.int64)
+        indices = d["indices"].astype(np.int32)
+        degree = d["degree"].astype(np.int32)
+        us = d["Us"].astype(np.int32)
+        vs = d["Vs"].astype(np.int32)
+        feat = d["node_feature"].astype(np.float32)
+    n = degree.size
+    g = SlackGraph(n=n, indptr=indptr, indices=indices, degree=degree, us=us, vs=vs,
+                   name=name or os.path.basename(path))
+    return g, feat
+
+
 def save_npz(path: str, g: CSRGraph) -> None:
     """Write a file ``scipy.sparse.load_npz`` reads (the reference's
     ``demo.py:39`` input format)."""

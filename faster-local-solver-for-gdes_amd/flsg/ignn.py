@@ -129,8 +129,20 @@ def ignn_residual(indptr, indices, degree, X, P, alpha, beta):
     return R
 
 
+def synthetic_workload(snapshots: int = 16, cols: int = 128):
+    """The 169,343-node citation graph with 950,000 edges held out in
+    `snapshots` insert batches, and X = `cols` row-normalised N(0,1) FP32
+    feature columns (held as FP64)."""
+    from . import graph as _graph
+    sg = _graph.citation_dynamic_graph(snapshots=snapshots)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((sg.n, cols))
+    X /= np.linalg.norm(X, axis=1, keepdims=True)
+    return sg, X.astype(np.float32).astype(np.float64)
+
+
 def snapshot_bench(snapshots: int = 16, cols: int = 128, alpha: float = 0.1, eps: float = 1e-7, beta: float = 0.5,
-                   plain_cols: int = 8, log=print) -> list:
+                   plain_cols: int = 8, log=print, dataset: str | None = None) -> list:
     """The dynamic feature-propagation workload: the 169,343-node citation
     graph with 950,000 edges held out and re-inserted in `snapshots` batches,
     X = 128 row-normalised N(0,1) FP32 feature columns.  After the initial
@@ -139,13 +151,17 @@ def snapshot_bench(snapshots: int = 16, cols: int = 128, alpha: float = 0.1, eps
     columns at once in FP64, and -- the per-column form of the reference,
     InstantGNN.py:332-359 -- one FP64 propagation per column for `plain_cols`
     columns (its per-snapshot time scaled to all columns is reported as such).
+    With `dataset`, the graph, insert batches and features come from a file
+    in the reference's experiment layout (graph.save_exp_npz).
     Returns one dict per (path, snapshot)."""
     from . import graph as _graph
-    sg = _graph.citation_dynamic_graph(snapshots=snapshots)
-    rng = np.random.default_rng(3)
-    X = rng.standard_normal((sg.n, cols))
-    X /= np.linalg.norm(X, axis=1, keepdims=True)
-    X = X.astype(np.float32).astype(np.float64)
+    if dataset:
+        sg, feat = _graph.load_exp_npz(dataset)
+        X = feat.astype(np.float64)
+        cols = X.shape[1]
+        snapshots = sg.us.shape[0]
+    else:
+        sg, X = synthetic_workload(snapshots, cols)
     out = []
 
     def rec(path, k, st, scale=1.0):
@@ -186,7 +202,14 @@ if __name__ == "__main__":
     ap.add_argument("--snapshots", type=int, default=16)
     ap.add_argument("--plain-cols", type=int, default=8)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--dataset", default=None, help="workload file in the reference's experiment layout")
+    ap.add_argument("--write-dataset", default=None, help="write the synthetic workload to this file and exit")
     args = ap.parse_args()
+    if args.write_dataset:
+        from . import graph as _graph
+        sg, X = synthetic_workload(args.snapshots)
+        _graph.save_exp_npz(args.write_dataset, sg, X.astype(np.float32))
+        sys.exit(0)
     fh = open(args.out, "w") if args.out else None
 
     def _log(d):
@@ -196,7 +219,7 @@ if __name__ == "__main__":
             fh.write(line + "\n")
             fh.flush()
 
-    snapshot_bench(snapshots=args.snapshots, plain_cols=args.plain_cols, log=_log)
+    snapshot_bench(snapshots=args.snapshots, plain_cols=args.plain_cols, log=_log, dataset=args.dataset)
     if fh:
         fh.close()
     sys.exit(0)

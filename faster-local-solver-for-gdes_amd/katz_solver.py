@@ -83,7 +83,8 @@ def solve_a_graph_all(n, indptr, indices, degree, alpha, eps, omega, graph_name,
             lo, lt, go, gt)
 
 
-def solve_a_graph_cheby(n, indptr, indices, degree, alpha, eps, l_max, mu_min, graph_name, s_nodes):
+def solve_a_graph_cheby(n, indptr, indices, degree, alpha, l_max, mu_min, eps, graph_name, s_nodes):
+    """katz_solver.py:670-693 (argument order as there: alpha, l_max, mu_min, eps)."""
     lo, lt = _d.per_seed(n, indptr, indices, degree, s_nodes,
                          _params(_abi.FLSG_CHEB, True, alpha, eps, l_max, mu_min))
     go, gt = _d.per_seed(n, indptr, indices, degree, s_nodes,

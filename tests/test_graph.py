@@ -88,3 +88,33 @@ def test_npz_roundtrip(tmp_path, small_graph):
     graph.save_npz(p, small_graph)
     h = graph.load_npz(p)
     assert np.array_equal(h.indices, small_graph.indices) and np.array_equal(h.degree, small_graph.degree)
+
+
+def test_exp_npz_layout_and_roundtrip(tmp_path):
+    """The dynamic workload file has the members, dtypes and free-slot sentinel
+    the reference's InstantGNN experiment reads (data_process_GNN.py:86-87,
+    instantGNN-arxiv-exp.py:174-190), and reads back unchanged."""
+    sg = graph.citation_dynamic_graph(n=400, m_directed=2400, delnum=300, snapshots=4, seed=5)
+    feat = np.random.default_rng(0).standard_normal((sg.n, 8)).astype(np.float32)
+    path = str(tmp_path / "dyn-exp.npz")
+    graph.save_exp_npz(path, sg, feat)
+    with np.load(path, allow_pickle=False) as d:
+        assert set(d.files) == {"indices", "indptr", "degree", "Us", "Vs", "node_feature", "labels", "split_idx"}
+        assert d["indices"].dtype == np.int32 and d["degree"].dtype == np.int64
+        assert d["Us"].shape == d["Vs"].shape == (4, 75) and d["Us"].dtype == np.int32
+        assert d["labels"].shape == (sg.n, 1)
+        ip, idx, deg = d["indptr"], d["indices"], d["degree"]
+        # slots past the live length hold n, as data_process_GNN.py:117-119 writes them
+        for u in range(sg.n):
+            assert np.all(idx[ip[u] + deg[u]:ip[u + 1]] == sg.n)
+    # the reference reads split_idx through pickle; it is a dict of index arrays
+    with np.load(path, allow_pickle=True) as d:
+        split = d["split_idx"].reshape(1, -1)[0][0]
+        assert sorted(split) == ["test", "train", "valid"]
+        assert sum(v.size for v in split.values()) == sg.n
+    g2, f2 = graph.load_exp_npz(path)
+    assert g2.n == sg.n
+    assert np.array_equal(g2.indptr, sg.indptr) and np.array_equal(g2.indices, sg.indices)
+    assert np.array_equal(g2.degree, sg.degree)
+    assert np.array_equal(g2.us, sg.us) and np.array_equal(g2.vs, sg.vs)
+    assert np.array_equal(f2, feat)
