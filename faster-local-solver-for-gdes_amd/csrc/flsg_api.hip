@@ -62,6 +62,8 @@ int ensure_single_workspace(flsg_graph g) {
     FLSG_HIP(hipMalloc((void **)&g->work_a, sizeof(double) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&g->work_b, sizeof(double) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&g->work_c, sizeof(double) * (n + 1)));
+    FLSG_HIP(hipMalloc((void **)&g->local_x, sizeof(double) * (n + 1)));
+    FLSG_HIP(hipMalloc((void **)&g->local_r, sizeof(double) * (n + 1)));
     FLSG_HIP(hipMalloc((void **)&g->ctr, sizeof(SolveCounters)));
     FLSG_HIP(hipHostMalloc((void **)&g->h_ctr, sizeof(SolveCounters), hipHostMallocDefault));
     // next-frontier sub-lists: node u goes to sub-list u mod kFrontSub, so each
@@ -147,6 +149,8 @@ int flsg_graph_destroy(flsg_graph g) {
     (void)hipFree(g->work_a);
     (void)hipFree(g->work_b);
     (void)hipFree(g->work_c);
+    (void)hipFree(g->local_x);
+    (void)hipFree(g->local_r);
     (void)hipFree(g->coef);
     (void)hipFree(g->ctr);
     (void)hipFree(g->trace_dev_off);
@@ -155,6 +159,7 @@ int flsg_graph_destroy(flsg_graph g) {
     cheb_free(g);
     ignn_free(g);
     dyn_free(g);
+    single_graphs_free(g);
     delete g;
     return FLSG_OK;
 }

@@ -92,6 +92,12 @@ struct flsg_graph_s {
     void *ignn = nullptr;
     // edge-insertion workspace (flsg_dyn.hip)
     void *dyn = nullptr;
+    // captured iteration blocks of the single-source push (flsg_single.hip)
+    // and the handle-owned x / r they run on (fixed addresses, so the
+    // captured graph is reused across solves)
+    void *graphs = nullptr;
+    double *local_x = nullptr;
+    double *local_r = nullptr;
     // batched workspace
     int2 *colx = nullptr;                   // nnz_cap column entries {v, degree[v]} (batched edge phases)
     flsg::BatchArena *arena = nullptr;
@@ -114,6 +120,7 @@ int colx_refresh(flsg_graph g, hipStream_t st);
 void cheb_free(flsg_graph g);
 void ignn_free(flsg_graph g);
 void dyn_free(flsg_graph g);
+void single_graphs_free(flsg_graph g);
 
 // ---------------------------------------------------------------- device helpers
 __device__ __forceinline__ unsigned int lane_id() { return __lane_id(); }

@@ -29,6 +29,8 @@
 // sweeps round exactly like the reference.
 #include <math.h>
 
+#include <string.h>
+
 #include <vector>
 
 #include "flsg_internal.h"
@@ -38,7 +40,8 @@ namespace flsg {
 constexpr int kNT = 256;          // threads per block
 constexpr int kHub = 4096;        // degree above which a node is split into pieces
 constexpr int kPiece = 4096;      // edges per hub piece
-constexpr int kIterBlock = 8;     // iterations launched between host checks
+constexpr int kIterBlock = 8;     // iterations launched between host checks (even: parity restarts at 0)
+static_assert(kIterBlock % 2 == 0, "a block must end on parity 0");
 constexpr int kRowLight = 64;     // global sweeps: rows up to this length are pulled by one thread,
 constexpr int kRowWave = 1024;    // up to this by one wave, longer ones by one block
 constexpr int kPB = 1024;         // threads of the long-row block
@@ -621,22 +624,80 @@ static int finish_trace(flsg_graph g, flsg_trace *tr, int64_t iters, hipStream_t
     return FLSG_OK;
 }
 
+// ------------------------------------------------------------------ captured iteration blocks
+// One block of kIterBlock iterations (3 kernels each, parity alternating, the
+// same arguments throughout a solve) is captured once into a hipGraph and
+// replayed until the frontier is empty.  The graph holds kernel nodes only
+// (counters are cleared by k_node, not by memsets).  The executable stays
+// cached on the handle while the arguments are unchanged (callers that reuse
+// their x / r buffers skip the capture).
+struct BlockGraph {
+    hipStream_t cs = nullptr;       // capture stream (nothing runs on it)
+    hipGraphExec_t exec = nullptr;
+    int kind = -1;
+    int grid = 0;
+    PushArgs args{};
+};
+
+void single_graphs_free(flsg_graph g) {
+    BlockGraph *c = (BlockGraph *)g->graphs;
+    if (!c) return;
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->cs) (void)hipStreamDestroy(c->cs);
+    delete c;
+    g->graphs = nullptr;
+}
+
+template <int KIND>
+static int push_block_exec(flsg_graph g, const PushArgs &a, int grid, hipGraphExec_t *out) {
+    BlockGraph *c = (BlockGraph *)g->graphs;
+    if (!c) {
+        c = new BlockGraph();
+        g->graphs = c;
+        FLSG_HIP(hipStreamCreateWithFlags(&c->cs, hipStreamNonBlocking));
+    }
+    if (c->exec && c->kind == KIND && c->grid == grid && memcmp(&c->args, &a, sizeof(PushArgs)) == 0) {
+        *out = c->exec;
+        return FLSG_OK;
+    }
+    if (c->exec) {
+        FLSG_HIP(hipGraphExecDestroy(c->exec));
+        c->exec = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    FLSG_HIP(hipStreamBeginCapture(c->cs, hipStreamCaptureModeThreadLocal));
+    for (int b = 0; b < kIterBlock; b++) {
+        const int p = b & 1;   // blocks start at even iterations
+        hipLaunchKernelGGL(k_node<KIND>, dim3(grid), dim3(kNT), 0, c->cs, a, p);
+        hipLaunchKernelGGL(k_push<KIND>, dim3(grid), dim3(kNT), 0, c->cs, a, p);
+        hipLaunchKernelGGL(k_front_compact, dim3(kFrontSub), dim3(kFrontSub), 0, c->cs, a, p);
+    }
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(c->cs, &gr);
+    FLSG_HIP(le);
+    FLSG_HIP(ce);
+    const hipError_t ie = hipGraphInstantiate(&c->exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    FLSG_HIP(ie);
+    c->kind = KIND;
+    c->grid = grid;
+    memcpy(&c->args, &a, sizeof(PushArgs));
+    *out = c->exec;
+    return FLSG_OK;
+}
+
 // ------------------------------------------------------------------ local GD / Katz GD
 template <int KIND>
 static int run_local_push(flsg_graph g, PushArgs a, int64_t max_iters, hipStream_t st, int64_t *iters_out) {
     const int grid = grid_for(g, 8);
-    int64_t t = 0;
+    hipGraphExec_t exec = nullptr;
+    int rc = push_block_exec<KIND>(g, a, grid, &exec);
+    if (rc) return rc;
     for (;;) {
-        for (int b = 0; b < kIterBlock; b++, t++) {
-            const int p = (int)(t & 1);
-            hipLaunchKernelGGL(k_node<KIND>, dim3(grid), dim3(kNT), 0, st, a, p);
-            hipLaunchKernelGGL(k_push<KIND>, dim3(grid), dim3(kNT), 0, st, a, p);
-            hipLaunchKernelGGL(k_front_compact, dim3(kFrontSub), dim3(kFrontSub), 0, st, a, p);
-        }
-        FLSG_HIP(hipGetLastError());
+        FLSG_HIP(hipGraphLaunch(exec, st));
         FLSG_HIP(hipMemcpyAsync(g->h_ctr, g->ctr, sizeof(SolveCounters), hipMemcpyDeviceToHost, st));
         FLSG_HIP(hipStreamSynchronize(st));
-        if (g->h_ctr->cnt[t & 1] == 0) break;
+        if (g->h_ctr->cnt[0] == 0) break;   // the next block's first frontier (kIterBlock is even)
         if ((int64_t)g->h_ctr->iters >= max_iters) {
             *iters_out = (int64_t)g->h_ctr->iters;
             return fail(FLSG_ENOCONV, "iteration cap reached");
@@ -651,21 +712,26 @@ static int run_local_push(flsg_graph g, PushArgs a, int64_t max_iters, hipStream
 static int solve_local_gd(flsg_graph g, const flsg_params *p, int64_t s, double *d_x, double *d_r,
                           flsg_result *res, flsg_trace *tr, hipStream_t st) {
     const int64_t n = g->n;
-    FLSG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
-    FLSG_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * n, st));
-    PushArgs a = base_args(g, d_x, d_r);
+    // the push runs on the handle's x / r (fixed addresses: the captured
+    // iteration block is reused across solves) and copies them out at the end
+    double *lx = g->local_x, *lr = g->local_r;
+    FLSG_HIP(hipMemsetAsync(lx, 0, sizeof(double) * n, st));
+    FLSG_HIP(hipMemsetAsync(lr, 0, sizeof(double) * n, st));
+    PushArgs a = base_args(g, lx, lr);
     a.alpha = p->alpha;
     a.om_alpha = 1.0 - p->alpha;
     const bool ppr = p->kind == FLSG_PPR;
     a.thr_scale = ppr ? p->eps * p->alpha : p->eps;
     int rc = setup_trace(g, tr, a, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_init_local, dim3(1), dim3(1), 0, st, g->ctr, g->front[0], d_r, s, ppr ? p->alpha : 1.0);
+    hipLaunchKernelGGL(k_init_local, dim3(1), dim3(1), 0, st, g->ctr, g->front[0], lr, s, ppr ? p->alpha : 1.0);
     int64_t iters = 0;
     rc = ppr ? run_local_push<K_PPR>(g, a, p->max_iters, st, &iters)
              : run_local_push<K_KATZ>(g, a, p->max_iters, st, &iters);
     if (rc) return rc;
-    if (!ppr) hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(1), 0, st, d_x, s, -1.0);   // katz_solver.py:243
+    if (!ppr) hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(1), 0, st, lx, s, -1.0);   // katz_solver.py:243
+    FLSG_HIP(hipMemcpyAsync(d_x, lx, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    FLSG_HIP(hipMemcpyAsync(d_r, lr, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     res->oper = (int64_t)g->h_ctr->oper;
     res->iters = iters;
     res->nodes = (int64_t)g->h_ctr->trace_off;

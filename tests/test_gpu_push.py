@@ -69,6 +69,32 @@ def test_ppr_local_gd_parity_small(small_graph, dev_small):
         assert (np.abs(rr) / thr).max() < 1.0 + 1e-9
 
 
+def test_local_push_graph_replays(small_graph, dev_small):
+    """The local push runs as a captured 8-iteration hipGraph, replayed until
+    the frontier empties and kept across solves while the arguments match.
+    Seeds A, A, B, A, Katz A, A with eager host work and synchronisations in
+    between: every solve matches the oracle (frontier sets, oper, x)."""
+    import torch
+    g = small_graph
+    sa, sb = (int(s) for s in graph.uniform_seeds(g, 2, seed=41))
+    beta = 1.0 / (g.d_max + 1)
+    runs = [(_abi.FLSG_PPR, sa), (_abi.FLSG_PPR, sa), (_abi.FLSG_PPR, sb), (_abi.FLSG_PPR, sa),
+            (_abi.FLSG_KATZ, sa), (_abi.FLSG_PPR, sa)]
+    for kind, s in runs:
+        if kind == _abi.FLSG_PPR:
+            p = engine.make_params(kind, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-6)
+            o = O.ppr_gd(g, s, ALPHA, 1e-6, local=True, trace=True)
+        else:
+            p = engine.make_params(kind, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=beta, eps=1e-7)
+            o = O.katz_gd(g, s, beta, 1e-7, local=True, trace=True)
+        out = engine.solve(dev_small, p, s, trace=True)
+        assert _compare_traces(out.trace, o["trace"]) == []
+        assert out.result["oper"] == o["oper"]
+        assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
+        torch.ones(1 << 20, device="cuda").sum().item()   # unrelated eager work between solves
+        torch.cuda.synchronize()
+
+
 def test_ppr_local_gd_alpha_roundtrip(small_graph, dev_small):
     """The harness alpha (demo.py:31-35): 0.1 -> 0.10000000000000002."""
     a, _ = graph.alpha_roundtrip(0.1)
