@@ -81,6 +81,19 @@ static int64_t nonzero_marks(int64_t n, const uint8_t *mark, int64_t *out) {
     return c;
 }
 
+/* Enqueue tests of the last local GD push on this thread whose |r_v| lay
+ * within one ulp of the threshold eps_vec[v] (either side): the nodes whose
+ * frontier membership a different but equally rounded summation order could
+ * flip.  Read with ora_near_threshold() after the call. */
+static _Thread_local int64_t g_near;
+
+static inline int64_t near_ulp(double thr, double r) {
+    const double a = fabs(r);
+    return fabs(a - thr) <= nextafter(thr, INFINITY) - thr ? 1 : 0;
+}
+
+int64_t ora_near_threshold(void) { return g_near; }
+
 /* ------------------------------------------------------------------------
  * PPR local / global gradient descent.
  * Restates ppr_solver.py:208-328 (ppr_gd, debug=False).
@@ -88,6 +101,7 @@ static int64_t nonzero_marks(int64_t n, const uint8_t *mark, int64_t *out) {
 int ora_ppr_gd(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
                int64_t s, double alpha, double eps, int local, double *xt, double *rt,
                int64_t *oper_out, double *time_out, ora_trace *tr) {
+    g_near = 0;
     /* ppr_solver.py:213 */
     if (!(eps * (double)degree[s] <= 1.)) return ORA_EASSERT;
     if (degree[s] == 0) return ORA_EARG;
@@ -135,6 +149,7 @@ int ora_ppr_gd(int64_t n, const int64_t *indptr, const int32_t *indices, const i
                 for (int64_t j = indptr[u]; j < indptr[u + 1]; j++) {
                     int64_t v = indices[j];
                     rt[v] += val;
+                    if (!q_mark[v]) g_near += near_ulp(eps_vec[v], rt[v]);
                     if (!q_mark[v] && eps_vec[v] <= fabs(rt[v])) {
                         queue[rear] = v;
                         q_mark[v] = 1;
@@ -180,6 +195,7 @@ int ora_ppr_gd(int64_t n, const int64_t *indptr, const int32_t *indices, const i
 int ora_ppr_cheby(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
                   int64_t s, double alpha, double eps, int local, double *xt, double *rt,
                   int64_t *oper_out, double *time_out, ora_trace *tr) {
+    g_near = 0;
     if (degree[s] == 0) return ORA_EARG;
     double t0 = now_s();
     int64_t m = 0;
@@ -264,6 +280,7 @@ int ora_ppr_cheby(int64_t n, const int64_t *indptr, const int32_t *indices, cons
                 for (int64_t j = indptr[u]; j < indptr[u + 1]; j++) {
                     int64_t v = indices[j];
                     rt[v] += val;
+                    if (!q_mark[v]) g_near += near_ulp(eps_vec[v], rt[v]);
                     if (!q_mark[v] && eps_vec[v] <= fabs(rt[v])) {
                         queue[rear] = v;
                         q_mark[v] = 1;
@@ -287,6 +304,7 @@ int ora_ppr_cheby(int64_t n, const int64_t *indptr, const int32_t *indices, cons
 int ora_katz_gd(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
                 int64_t s, double alpha, double eps, int local, double *xt, double *rt,
                 int64_t *oper_out, double *time_out, ora_trace *tr) {
+    g_near = 0;
     double t0 = now_s();
     int64_t m = 0;
     for (int64_t i = 0; i < n; i++) m += degree[i];
@@ -326,6 +344,7 @@ int ora_katz_gd(int64_t n, const int64_t *indptr, const int32_t *indices, const 
                 for (int64_t j = indptr[u]; j < indptr[u + 1]; j++) {
                     int64_t v = indices[j];
                     rt[v] += val;
+                    if (!q_mark[v]) g_near += near_ulp(eps_vec[v], rt[v]);
                     if (!q_mark[v] && eps_vec[v] <= fabs(rt[v])) {
                         queue[rear] = v;
                         q_mark[v] = 1;
@@ -363,6 +382,7 @@ int ora_katz_gd(int64_t n, const int64_t *indptr, const int32_t *indices, const 
 int ora_katz_cheby(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
                    int64_t s, double alpha, double l_max, double mu_min, double eps, int local,
                    double *xt, double *rt, int64_t *oper_out, double *time_out, ora_trace *tr) {
+    g_near = 0;
     double t0 = now_s();
     int64_t m = 0;
     for (int64_t i = 0; i < n; i++) m += degree[i];
@@ -444,6 +464,7 @@ int ora_katz_cheby(int64_t n, const int64_t *indptr, const int32_t *indices, con
                 for (int64_t j = indptr[u]; j < indptr[u + 1]; j++) {
                     int64_t v = indices[j];
                     rt[v] += val;
+                    if (!q_mark[v]) g_near += near_ulp(eps_vec[v], rt[v]);
                     if (!q_mark[v] && eps_vec[v] <= fabs(rt[v])) {
                         queue[rear] = v;
                         q_mark[v] = 1;

@@ -77,6 +77,8 @@ def lib():
         L.ora_ignn_update_column.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, vp, vp, vp, i64, vp]
         L.ora_ppr_batch.argtypes = [i64, vp, vp, vp, vp, i64, dbl, dbl, i32, i32, vp, vp]
         L.ora_max_threads.restype = i32
+        L.ora_near_threshold.argtypes = []
+        L.ora_near_threshold.restype = i64
         _lib = L
     return _lib
 
@@ -132,7 +134,8 @@ def _run_local(fn, g, s, *args, trace=False, trace_cap=None):
         raise OverflowError("trace capacity")
     if rc not in (ORA_OK,):
         raise RuntimeError(f"oracle error {rc}")
-    out = {"x": x, "r": r, "oper": int(oper[0]), "time": float(t[0])}
+    # enqueue tests within one ulp of the threshold (local methods; same thread)
+    out = {"x": x, "r": r, "oper": int(oper[0]), "time": float(t[0]), "near_thr": int(lib().ora_near_threshold())}
     if trace:
         out["trace"] = _unpack_trace(tr, off, nodes)
     return out

@@ -348,15 +348,21 @@ def dev_large(large_graph):
     return engine.DeviceGraph(large_graph)
 
 
-def test_ppr_local_gd_parity_large(large_graph, dev_large):
-    """Full-size graph (2.39M nodes), eps = 1e-7: oracle parity on 3 seeds."""
+def test_ppr_local_gd_parity_large(large_graph, dev_large, capsys):
+    """Full-size graph (2.39M nodes), eps = 1e-7: oracle parity on 3 seeds.
+    Frontier sets must be equal in every iteration; the oracle's enqueue tests
+    that fell within one ulp of the threshold (where another summation order
+    could flip membership) are counted and reported beside the result."""
     g = large_graph
     p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-7)
     for s in graph.uniform_seeds(g, 3, seed=21):
         o = O.ppr_gd(g, s, ALPHA, 1e-7, local=True, trace=True)
         out = engine.solve(dev_large, p, int(s), trace=True, trace_cap_nodes=64 * g.n)
         diffs = _compare_traces(out.trace, o["trace"])
-        assert diffs == [], f"seed {s}: {diffs}"
+        with capsys.disabled():
+            print(f"\n  seed {s}: {len(o['trace'])} iterations, frontier set differences {len(diffs)}, "
+                  f"enqueue tests within 1 ulp of the threshold {o['near_thr']}")
+        assert diffs == [], f"seed {s}: {diffs} (near-threshold tests {o['near_thr']})"
         assert out.result["oper"] == o["oper"]
         assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
 

@@ -343,3 +343,19 @@ def test_update_graph_skips_existing_and_orders_keys():
     assert keys.tolist() == [5, 7, 3, 9]
     assert [nbr[off[k]:off[k + 1]].tolist() for k in range(4)] == [[7], [5], [9], [3]]
     assert degree[5] == 3 and degree[7] == 3 and degree[0] == 1
+
+
+def test_near_threshold_count_star():
+    """Star with centre 0 and 4 leaves, alpha = 1/2, eps = 1/8: the centre's
+    push gives every leaf r = (1-alpha) alpha / 4 = 1/16, exactly its
+    threshold eps alpha d = 1/16 (dyadic, no rounding), so the first
+    iteration's four enqueue tests sit on the threshold and are counted; the
+    leaves still join the frontier (the inclusive test eps_vec <= |r|)."""
+    k = 4
+    g = CSRGraph.from_edges(k + 1, np.zeros(k, dtype=np.int64), np.arange(1, k + 1))
+    o = O.ppr_gd(g, 0, 0.5, 0.125, local=True, trace=True)
+    assert o["near_thr"] >= k
+    assert sorted(o["trace"][1].tolist()) == [1, 2, 3, 4]
+    # a threshold well away from every residual counts nothing
+    o2 = O.ppr_gd(g, 0, 0.5, 0.1, local=True)
+    assert o2["near_thr"] == 0
