@@ -136,11 +136,15 @@ def main():
     kern_ms = float(np.mean([b.ms for b in runs]))
     achieved = bytes_step / (kern_ms * 1e-3) / 1e9
 
+    # measured HBM bytes per launch, from the committed rocprofv3 counter runs;
+    # only for the workload those runs profiled
     traffic = None
     tpath = os.path.join(REPO, "profiles", "k_batch_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("n") == g.n and tj.get("seeds") == len(seeds) and tj.get("eps") == args.eps:
+            traffic = tj.get("bytes_per_launch")
 
     # ------------------------------------------------------------ batched FP64 push (same launch, FP64 records)
     b64_qps = None
@@ -195,7 +199,7 @@ def main():
         log(f"batched FP32 vs FP64 oracle: {l1}")
 
     line = {
-        "metric": "certified PPR queries/s, batched FP32 local GD (FP64 threshold + FP64 certificate), eps=1e-7",
+        "metric": f"certified PPR queries/s, batched FP32 local GD (FP64 threshold + FP64 certificate), eps={args.eps:g}",
         "value": qps,
         "unit": "queries/s",
         "n_gpus": 1,

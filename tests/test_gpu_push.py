@@ -367,11 +367,12 @@ def test_ppr_local_gd_parity_large(large_graph, dev_large, capsys):
         assert _rel_l1(out.x.cpu().numpy(), o["x"]) <= REL_L1
 
 
-def test_batch_certified_large(large_graph, dev_large):
-    """1024 mixed seeds at full size: every query certified on the device;
-    16 of them re-certified by an independent host FP64 recompute."""
+@pytest.mark.parametrize("eps", [1e-7, 1e-6])
+def test_batch_certified_large(large_graph, dev_large, eps):
+    """1024 mixed seeds at full size, eps in {1e-7, 1e-6}: every query
+    certified on the device; 16 of them re-certified by an independent host
+    FP64 recompute."""
     g = large_graph
-    eps = 1e-7
     seeds = graph.mixed_seeds(g, 1024, seed=0)
     p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
     bo = engine.batch_solve(dev_large, p, seeds)
