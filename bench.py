@@ -79,8 +79,8 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (queries would shard with no "
-              "exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (queries would shard across GPUs "
+              "with no exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -75,7 +75,7 @@ def per_seed(n, indptr, indices, degree, s_nodes, params):
 
 def sor_not_built(name: str):
     raise NotImplementedError(
-        f"{name}: SOR / Gauss-Southwell stay CPU reference baselines in this build (they are "
+        f"{name}: SOR / Gauss-Southwell stay CPU reference baselines (they are "
         "inherently sequential; a graph-coloured GPU variant would have to meet the same residual "
         "certificate before replacing them).")
 

@@ -74,7 +74,7 @@ def solve_a_graph_gpu(n, indptr, indices, degree, alpha, eps, s_nodes):
 
 
 def solve_a_graph_all(n, indptr, indices, degree, alpha, eps, omega, graph_name, s_nodes):
-    """katz_solver.py:605-666; SOR slots are CPU baselines outside this build (NaN)."""
+    """katz_solver.py:605-666; SOR slots are CPU baselines not run here (NaN)."""
     k = len(s_nodes)
     nan = np.full(k, np.nan)
     lo, lt = _d.per_seed(n, indptr, indices, degree, s_nodes, _params(_abi.FLSG_GD, True, alpha, eps))

@@ -102,7 +102,7 @@ def solve_a_graph_cheby(n, indptr, indices, degree, alpha, eps, omega, graph_nam
 
 def solve_a_graph_all(n, indptr, indices, degree, alpha, eps, omega, graph_name, s_nodes):
     """ppr_solver.py:511-579.  The GD slots are computed on the GPU; the
-    SOR / GS slots are CPU reference baselines outside this build and are
+    SOR / GS slots are CPU reference baselines not run here and are
     returned as NaN."""
     k = len(s_nodes)
     nan_o = np.full(k, np.nan)
