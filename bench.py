@@ -42,6 +42,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _cpu_model() -> str:
+    """The host CPU's model name (the lscpu 'Model name' line), if readable."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,8 +197,9 @@ def main():
         opers, times = O.ppr_batch(g, seeds[:sample], args.alpha, args.eps, kind=0, nthreads=cores)
         wall_c = time.perf_counter() - tc
         cpu = {"value": sample / wall_c, "unit": "queries/s", "cores": cores, "kind": "port",
-               "sample": f"first {sample} of the 1024 seeds, PPR local GD FP64 (C restatement, OpenMP over "
-                         f"seeds), median {float(np.median(times)):.3f} s/query single-thread"}
+               "sample": f"first {sample} of the {args.nq} seeds, PPR local GD FP64 (C restatement, OpenMP over "
+                         f"seeds), median {float(np.median(times)):.3f} s/query single-thread",
+               "host": {"nproc": os.cpu_count(), "model": _cpu_model()}}
         log(f"cpu: {cpu['value']:.2f} q/s on {cores} threads")
         # L1 error of batched FP32 results against the FP64 oracle (a few seeds)
         errs = []
