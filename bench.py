@@ -212,7 +212,8 @@ def main():
         log(f"batched FP32 vs FP64 oracle: {l1}")
 
     line = {
-        "metric": f"certified PPR queries/s, batched FP32 local GD (FP64 threshold + FP64 certificate), eps={args.eps:g}",
+        "metric": "certified PPR queries/s, batched FP32 local GD (FP64 threshold + FP64 certificate), eps="
+                  + f"{args.eps:g}".replace("e-0", "e-"),
         "value": qps,
         "unit": "queries/s",
         "n_gpus": 1,
