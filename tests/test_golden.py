@@ -49,3 +49,32 @@ def test_oracle_matches_golden(gold, small_graph, name):
             x = np.zeros(small_graph.n)
             x[gold[f"{name}_x{i}_ids"]] = gold[f"{name}_x{i}_vals"]
             assert np.array_equal(o["x"], x)
+
+
+@pytest.fixture(scope="module")
+def gold_kh():
+    with np.load(os.path.join(HERE, "golden", "config1_katz_hk.npz"), allow_pickle=False) as d:
+        return {k: d[k] for k in d.files}
+
+
+def _kh_runs(gk, g):
+    beta, l_max, mu_min = float(gk["beta"]), float(gk["l_max"]), float(gk["mu_min"])
+    ke, tau, he = float(gk["katz_eps"]), float(gk["hk_tau"]), float(gk["hk_eps"])
+    return {"katz_gd": lambda s: O.katz_gd(g, s, beta, ke, local=True, trace=True),
+            "katz_cheb": lambda s: O.katz_cheby(g, s, beta, l_max, mu_min, ke, local=True, trace=True),
+            "hk0": lambda s: O.hk_push(g, s, tau, he, local=True, n_terms=0, trace=True),
+            "hk40": lambda s: O.hk_push(g, s, tau, he, local=True, n_terms=40, trace=True)}
+
+
+@pytest.mark.parametrize("name", ["katz_gd", "katz_cheb", "hk0", "hk40"])
+def test_oracle_matches_golden_katz_hk(gold_kh, small_graph, name):
+    fn = _kh_runs(gold_kh, small_graph)[name]
+    off = gold_kh[f"{name}_front_off"]
+    assert np.array_equal(graph.uniform_seeds(small_graph, 5, seed=5), gold_kh["seeds"])
+    for i, s in enumerate(gold_kh["seeds"]):
+        o = fn(int(s))
+        assert o["oper"] == gold_kh[f"{name}_oper"][i]
+        assert len(o["trace"]) == gold_kh[f"{name}_iters"][i]
+        assert o.get("est_n", 0) == gold_kh[f"{name}_est_n"][i]
+        assert np.array_equal(MG.frontier_checks(o["trace"]), gold_kh[f"{name}_front"][off[i]:off[i + 1]])
+        assert np.array_equal(MG.digest(o["x"]), gold_kh[f"{name}_x_sha256"][i])

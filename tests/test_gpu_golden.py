@@ -6,7 +6,9 @@ eps 1e-6, 20 seeds), read from the file alone (no oracle call):
     bit-identical;
   * local GD: operation count, iteration count and every frontier as a set
     (size and id sum) equal; the L1 norm of x within relative 1e-12, and the
-    full x of the stored seeds within relative L1 1e-12."""
+    full x of the stored seeds within relative L1 1e-12;
+  * Katz GD / Katz Chebyshev / the heat-kernel level push on 5 seeds of the
+    same graph (config1_katz_hk.npz) to the same bars."""
 import importlib.util
 import os
 
@@ -73,4 +75,46 @@ def test_local_gd_matches_golden(gold, dev):
         if f"gd_x{i}_ids" in gold:
             xg = np.zeros(n)
             xg[gold[f"gd_x{i}_ids"]] = gold[f"gd_x{i}_vals"]
+            assert np.abs(x - xg).sum() <= 1e-12 * np.abs(xg).sum()
+
+
+@pytest.fixture(scope="module")
+def gold_kh():
+    with np.load(os.path.join(HERE, "golden", "config1_katz_hk.npz"), allow_pickle=False) as d:
+        return {k: d[k] for k in d.files}
+
+
+@pytest.mark.parametrize("name", ["katz_gd", "katz_cheb", "hk0", "hk40"])
+def test_katz_hk_match_golden(gold_kh, dev, name):
+    """Katz Chebyshev bit-identical (digest, queues in order); Katz GD and the
+    heat kernel: counts and frontier sets equal, x within relative 1e-12."""
+    g = gold_kh
+    if name == "katz_gd":
+        p = engine.make_params(_abi.FLSG_KATZ, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=float(g["beta"]),
+                               eps=float(g["katz_eps"]))
+    elif name == "katz_cheb":
+        p = engine.make_params(_abi.FLSG_KATZ, _abi.FLSG_CHEB, True, _abi.FLSG_FP64, alpha=float(g["beta"]),
+                               eps=float(g["katz_eps"]), l_max=float(g["l_max"]), mu_min=float(g["mu_min"]))
+    else:
+        p = engine.make_params(_abi.FLSG_HK, _abi.FLSG_GD, True, _abi.FLSG_FP64, eps=float(g["hk_eps"]),
+                               tau=float(g["hk_tau"]), n_terms=40 if name == "hk40" else 0)
+    off = g[f"{name}_front_off"]
+    n = dev.n
+    for i, s in enumerate(g["seeds"]):
+        out = engine.solve(dev, p, int(s), trace=True)
+        assert out.result["oper"] == int(g[f"{name}_oper"][i]), f"{name} seed {s}"
+        assert len(out.trace) == g[f"{name}_iters"][i]
+        if name.startswith("hk"):
+            assert out.extra["est_n"] == g[f"{name}_est_n"][i]
+        x = out.x.cpu().numpy()
+        if name == "katz_cheb":
+            assert np.array_equal(MG.frontier_checks(out.trace), g[f"{name}_front"][off[i]:off[i + 1]])
+            assert np.array_equal(MG.digest(x), g[f"{name}_x_sha256"][i])
+        else:
+            checks = MG.frontier_checks([np.sort(t) for t in out.trace])
+            assert np.array_equal(checks[:, :2], g[f"{name}_front"][off[i]:off[i + 1], :2])
+            assert abs(np.abs(x).sum() - g[f"{name}_x_l1"][i]) <= 1e-12 * g[f"{name}_x_l1"][i]
+        if i == 0:
+            xg = np.zeros(n)
+            xg[g[f"{name}_x0_ids"]] = g[f"{name}_x0_vals"]
             assert np.abs(x - xg).sum() <= 1e-12 * np.abs(xg).sum()
