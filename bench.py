@@ -151,12 +151,21 @@ def main():
     # measured HBM bytes per launch, from the committed rocprofv3 counter runs;
     # only for the workload those runs profiled
     traffic = None
+    atomics = None
     tpath = os.path.join(REPO, "profiles", "k_batch_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
         if tj.get("n") == g.n and tj.get("seeds") == len(seeds) and tj.get("eps") == args.eps:
             traffic = tj.get("bytes_per_launch")
+            # the limit that binds: memory-side atomics (counted by rocprofv3 on
+            # this workload) per second of this run's kernel time, against the
+            # random-atomic probe's rate on the same part
+            na = tj.get("memory_atomics_per_launch")
+            ceil = tj.get("atomic_probe_ceiling_per_s")
+            if na and ceil:
+                rate = na / (kern_ms * 1e-3)
+                atomics = {"per_launch": na, "per_s": rate, "probe_ceiling_per_s": ceil, "frac": rate / ceil}
 
     # ------------------------------------------------------------ batched FP64 push (same launch, FP64 records)
     b64_qps = None
@@ -232,7 +241,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "memory_atomics": atomics,
             "kernel": "k_batch", "kernel_ms": kern_ms, "algorithmic_bytes": bytes_step,
         },
         "cpu_baseline": cpu,
