@@ -130,6 +130,8 @@ def main():
         bo = engine.batch_solve(dg, p32, seeds, slots=args.slots, block=args.block, hub_bytes=args.hub_bytes)
         runs.append(bo)
         log(f"step {k}: {bo.ms:.1f} ms (per-query push {np.mean([r['ms_push'] for r in bo.results]):.2f} ms, "
+            f"recompute {np.mean([r['ms_recompute'] for r in bo.results]):.2f}, "
+            f"correct {np.mean([r['ms_correct'] for r in bo.results]):.2f}, "
             f"total {np.mean([r['ms'] for r in bo.results]):.2f} ms)")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start

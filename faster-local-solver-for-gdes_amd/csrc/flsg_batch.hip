@@ -569,6 +569,8 @@ __device__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int 
         rr.nodes = (int64_t)*s_tn;
         rr.ms = (double)(t_end - t_start) * 1e-5;
         rr.ms_push = rr.ms;
+        rr.ms_recompute = rr.ms;
+        rr.ms_correct = rr.ms;
         rr.status = fits ? FLSG_OK : FLSG_ECAP;
         a.res[q] = rr;
         a.out_off[q] = fits ? (int64_t)obase : -1;
@@ -829,6 +831,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const uint64_t t_push = now_ticks();
         unsigned long long cert_ops = 0;
         unsigned long long n_corr = 0;
+        uint64_t t_rec = t_push, t_corr = t_push;
         if (!P.fp64) {
             // ---------------- 2. FP64 certificate of the FP32 result
             const unsigned int S32 = s_sup;
@@ -945,6 +948,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 }
             }
             __syncthreads();
+            t_rec = now_ticks();
             // 2d. FP64 correction push (mark-based queue: the recomputed residual has both signs)
             int64_t corr_iters = 0;
             for (;;) {
@@ -1018,6 +1022,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 __syncthreads();
             }
             __syncthreads();
+            t_corr = now_ticks();
         }
         const unsigned int S = s_sup;
         const unsigned int T = s_tl;
@@ -1026,21 +1031,26 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         __syncthreads();
         const unsigned long long obase = s_base;
         const bool fits = obase + S <= (unsigned long long)a.out_cap;
-        // one pass over the touched list: certificate value, output of the
+        // one pass over the touched records: certificate value, output of the
         // support (listed by flag in the FP32 path, by x != 0 in the FP64 path,
         // whose x only grows; in any order), record reset; the loads of kUnroll
-        // entries are issued before any is used
+        // entries are issued before any is used.  When the query touched more
+        // than a quarter of the graph, the pass streams over all n records of
+        // the slot (coalesced, untouched ones -- all bits zero but d -- are
+        // skipped) instead of gathering the touched list's records one by one.
         if (tid == 0) s_ncnt = 0;
         __syncthreads();
         double cmax = 0.0;
-        for (unsigned int i0 = 0; i0 < T; i0 += NT * kUnroll) {
+        const bool stream = (int64_t)T * 4 > a.n;
+        const unsigned int lim = stream ? (unsigned int)a.n : T;
+        for (unsigned int i0 = 0; i0 < lim; i0 += NT * kUnroll) {
             int vv[kUnroll];
             float4 lo[kUnroll];
             double2 hi[kUnroll];
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) {
                 const unsigned int i = i0 + j * NT + tid;
-                vv[j] = i < T ? tl[i] : -1;
+                vv[j] = i < lim ? (stream ? (int)i : tl[i]) : -1;
             }
             // (the FP64 push never writes the FP32 half of a record, so there only
             // the FP64 half is cleared; the degree is read from the record's own
@@ -1062,6 +1072,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 is_sup[j] = false;
                 id[j] = vv[j];
                 if (vv[j] < 0) continue;
+                if (stream && (__float_as_uint(lo[j].x) | __float_as_uint(lo[j].y) | __float_as_uint(lo[j].w)) == 0u &&
+                    __double_as_longlong(hi[j].x) == 0ll && __double_as_longlong(hi[j].y) == 0ll) {
+                    id[j] = -1;   // untouched record: nothing to certify, output or reset
+                    continue;
+                }
                 const int d = __float_as_int(lo[j].z);
                 const double r64 = hi[j].x;
                 if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)d));
@@ -1098,6 +1113,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             rr.cert = cmax;
             rr.ms = (double)(t_end - t_start) * 1e-5;       // 100 MHz constant clock
             rr.ms_push = (double)(t_push - t_start) * 1e-5;
+            rr.ms_recompute = (double)(t_rec - t_start) * 1e-5;
+            rr.ms_correct = (double)(t_corr - t_start) * 1e-5;
             rr.status = fits ? status : FLSG_ECAP;
             rr.touched = (int32_t)T;
             a.res[q] = rr;
@@ -1128,6 +1145,27 @@ __global__ void k_slot_degrees(NodeRec *rec, const int32_t *degree, int64_t n, i
         const int u = keys[k];
         rec[sl * (n + kSink) + u].d = degree[u];
     }
+}
+
+// records of every slot not in the between-queries state (all fields zero but
+// d; d = degree, 0 for sink records), counted into *cnt
+__global__ void k_arena_dirty(const NodeRec *rec, const int32_t *degree, int64_t n, int64_t total,
+                              unsigned long long *cnt) {
+    const int64_t rn = n + kSink;
+    unsigned long long c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = i % rn;
+        float4 lo;
+        double2 hi;
+        __builtin_memcpy(&lo, rec + i, 16);
+        __builtin_memcpy(&hi, reinterpret_cast<const char *>(rec + i) + 16, 16);
+        const int dd = u < n ? degree[u] : 0;
+        const bool bad = (__float_as_uint(lo.x) | __float_as_uint(lo.y) | __float_as_uint(lo.w)) != 0u ||
+                         __float_as_int(lo.z) != dd || __double_as_longlong(hi.x) != 0ll ||
+                         __double_as_longlong(hi.y) != 0ll;
+        c += bad ? 1ull : 0ull;
+    }
+    if (c) atomicAdd(cnt, c);
 }
 
 __global__ void k_fill_queries(flsg_query *q, const int64_t *seeds, int64_t nq, int32_t kind, double alpha, double eps,
@@ -1382,6 +1420,31 @@ extern "C" int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, i
     hipLaunchKernelGGL(k_slot_degrees, dim3(g->num_cus * 8), dim3(256), 0, (hipStream_t)stream, A->rec,
                        (const int32_t *)g->degree, g->n, A->slots, d_keys, n_keys);
     FLSG_HIP(hipGetLastError());
+    return FLSG_OK;
+}
+
+extern "C" int flsg_batch_arena_dirty(flsg_graph g, int64_t *h_count, void *stream) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (!h_count) return fail(FLSG_EARG, "NULL argument");
+    *h_count = 0;
+    BatchArena *A = g->arena;
+    if (!A) return FLSG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long *d_cnt = nullptr;
+    FLSG_HIP(hipMalloc((void **)&d_cnt, sizeof(unsigned long long)));
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpyAsync(d_cnt, &h, sizeof(h), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_arena_dirty, dim3(g->num_cus * 8), dim3(256), 0, st, (const NodeRec *)A->rec,
+                           (const int32_t *)g->degree, g->n, (int64_t)(A->n + kSink) * A->slots, d_cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_cnt);
+    FLSG_HIP(e);
+    *h_count = (int64_t)h;
     return FLSG_OK;
 }
 

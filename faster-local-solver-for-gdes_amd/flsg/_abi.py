@@ -28,6 +28,7 @@ class Result(C.Structure):
         ("oper", C.c_int64), ("iters", C.c_int64), ("oper_cert", C.c_int64), ("n_corr", C.c_int64),
         ("supp", C.c_int64), ("nodes", C.c_int64), ("cert", C.c_double), ("ms", C.c_double),
         ("ms_push", C.c_double), ("status", C.c_int32), ("touched", C.c_int32),
+        ("ms_recompute", C.c_double), ("ms_correct", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -76,6 +77,7 @@ SIGNATURES = {
                                     C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
                                     C.c_void_p, C.c_void_p]),
     "flsg_batch_refresh_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "flsg_batch_arena_dirty": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "flsg_graph_insert_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                           C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "flsg_ignn_propagate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -183,6 +183,15 @@ def solve(dg: DeviceGraph, params: _abi.Params, seed: int, trace: bool = False,
     return SolveOut(x=x, r=r, result=res.as_dict(), trace=frontiers, extra=extra)
 
 
+def arena_dirty(dg: DeviceGraph) -> int:
+    """Slot-arena records not back in the between-queries state (0 after
+    every completed batched launch)."""
+    torch = dg.torch
+    cnt = C.c_int64(0)
+    _check(dg.lib.flsg_batch_arena_dirty(dg.handle, C.byref(cnt), _stream_ptr(torch)))
+    return int(cnt.value)
+
+
 @dataclass
 class BatchOut:
     ids: object          # torch int32 [total] (device)

@@ -84,6 +84,10 @@ typedef struct {
     double ms_push;      /* batched: time of the FP32 push (before the FP64 certificate) */
     int32_t status;      /* FLSG_OK or error code for this query              */
     int32_t touched;     /* batched: records the query touched (and reset)    */
+    double ms_recompute; /* batched FP32: time from the start to the end of the FP64
+                            residual recompute                                */
+    double ms_correct;   /* batched FP32: time from the start to the end of the FP64
+                            correction push (then: output and record reset)   */
 } flsg_result;
 
 /* Optional per-iteration frontier trace (device buffers): iteration t
@@ -216,6 +220,12 @@ int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queri
                       int32_t n_slots, int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off, int64_t out_cap,
                       uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters, const flsg_hk_config *hk,
                       void *stream);
+
+/* Number of slot-arena records not back in the between-queries state (every
+ * field zero except the degree copy) -- 0 after any completed launch.
+ * Synchronises the stream.  No counterpart in the reference (a check of the
+ * arena's reset invariant, for tests). */
+int flsg_batch_arena_dirty(flsg_graph g, int64_t *h_count, void *stream);
 
 /* After flsg_graph_insert_edges on a handle with a slot arena: refresh the
  * degree copy the slots keep for the listed nodes. */

@@ -28,9 +28,37 @@ def test_library_exports_every_symbol():
 
 def test_struct_layouts():
     assert ctypes.sizeof(_abi.Params) == 4 * 4 + 5 * 8 + 2 * 8
-    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 3 * 8 + 2 * 4
+    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 3 * 8 + 2 * 4 + 2 * 8
     assert ctypes.sizeof(_abi.Query) == 8 + 2 * 4 + 2 * 8
     assert ctypes.sizeof(_abi.Trace) == 5 * 8 + 2 * 4
+
+
+def test_struct_offsets_match_the_header(tmp_path):
+    """Every field of the ctypes mirror sits at the offset a C compiler gives
+    it from include/flsg.h (gcc on a generated probe)."""
+    import subprocess
+    structs = {"flsg_params": _abi.Params, "flsg_result": _abi.Result, "flsg_query": _abi.Query,
+               "flsg_trace": _abi.Trace, "flsg_hk_config": _abi.HkConfig}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "flsg.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", _build.INCLUDE, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln.strip():
+            c, f, v = ln.split()
+            got[(c, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
 
 
 def test_version_without_gpu():

@@ -257,6 +257,24 @@ def test_batch_degree_bins(small_graph, split, piece):
         dg.close()
 
 
+def test_batch_arena_clean_after_launches(small_graph, dev_small):
+    """Every launch resets each record it touched (through the touched list or
+    by streaming the slot): FP32 and FP64 pushes, PPR and Katz, small and
+    large supports, leave the whole arena in its between-queries state."""
+    g = small_graph
+    seeds = graph.mixed_seeds(g, 96, seed=5)
+    modes = set()
+    for kind, a, eps in ((_abi.FLSG_PPR, ALPHA, 1e-6), (_abi.FLSG_PPR, ALPHA, 1e-3),
+                         (_abi.FLSG_KATZ, 1.0 / (g.d_max + 1), 1e-7)):
+        for prec in (_abi.FLSG_FP32, _abi.FLSG_FP64):
+            p = engine.make_params(kind, _abi.FLSG_GD, True, prec, alpha=a, eps=eps)
+            bo = engine.batch_solve(dev_small, p, seeds)
+            assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo.results)
+            assert engine.arena_dirty(dev_small) == 0, (kind, eps, prec)
+            modes |= {r["touched"] * 4 > g.n for r in bo.results}
+    assert modes == {True, False}   # both reset modes ran
+
+
 def test_batch_bad_seed_fails_alone(small_graph, dev_small):
     """A query whose seed lies outside the graph fails with FLSG_EARG on its
     own record (the kernel checks before touching a slot); the rest of the
@@ -377,6 +395,11 @@ def test_batch_certified_large(large_graph, dev_large, eps):
     p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
     bo = engine.batch_solve(dev_large, p, seeds)
     assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo.results)
+    # queries touching more than n/4 records reset by streaming the slot,
+    # the others through their touched list: both leave the arena clean
+    tn = np.array([r["touched"] for r in bo.results])
+    print(f"eps {eps}: touched > n/4 in {(tn * 4 > g.n).sum()} of {len(tn)} queries")
+    assert engine.arena_dirty(dev_large) == 0
     _check_batch(g, seeds, bo, eps, n_check=16)
 
 
@@ -389,6 +412,7 @@ def test_batch_fp64_large(large_graph, dev_large):
     p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=eps)
     bo = engine.batch_solve(dev_large, p, seeds)
     assert all(r["status"] == 0 and r["cert"] < 1.0 for r in bo.results)
+    assert engine.arena_dirty(dev_large) == 0
     for q in range(3):
         o = O.ppr_gd(g, int(seeds[q]), ALPHA, eps, local=True)
         assert bo.results[q]["oper"] == o["oper"]
