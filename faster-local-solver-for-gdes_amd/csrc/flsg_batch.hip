@@ -50,6 +50,40 @@ constexpr unsigned kSeen = 2u;       // flags: listed in the support
 constexpr unsigned kTouched = 4u;    // flags: in the query's touched list (reset at the end)
 constexpr int kSink = 64;            // per-slot sink records (d = 0) for the idle lanes of an edge batch
 
+// Boundary certificate of the FP32 push.  A node the push reached but never
+// pushed (a boundary node) holds r32, the FP32 sum of the m <= d * iters push
+// values it received (undirected graph: a node's in-degree is its degree d;
+// each neighbour pushes at most once per iteration).  Against the FP64 residual
+// of the returned x -- the pushed nodes' x32 -- r32 is off by at most
+//     g(d, iters) * r32 + (d * iters + 1) * 2^-125,
+//     g(d, iters) = 1.03 * (2 d iters + iters + 4) * 2^-24,
+// because (u = 2^-24):
+//   - every FP32 add (memory-side or LDS atomic) lands within 2u of its exact
+//     result, so the sum of m >= 0 terms is within 2mu / (1 - 2mu) of theirs;
+//   - every push value fl32(push(r)) is within u (+ 2^-52) of push(r);
+//   - every x32_u is an FP32 sum of at most iters residuals r >= 0, within
+//     iters * u / (1 - iters * u) of their sum;
+//   - a flushed denormal result loses at most 2^-126 per add;
+// and (2 d iters + iters + 4) u <= 1e-2 keeps the product of these factors
+// below the 1.03 margin.  Such a node keeps its FP32 residual (plus the bound)
+// when that certifies it; the rest are recomputed in FP64 like the support.
+constexpr int kLazyDeg = 64;                 // boundary nodes of higher degree are recomputed
+constexpr double kNearFrac = 0.99609375;     // 1 - 2^-8: boundary residuals above this fraction of
+                                             // the threshold are recomputed
+constexpr double kLazyGmax = 0.0039;         // g(kLazyDeg, iters) limit: kNearFrac * (1 + g) < 1 - 1e-6
+
+__device__ __forceinline__ double lazy_g(int d, int64_t it) {
+    return 1.03 * (2.0 * (double)d * (double)it + (double)it + 4.0) * 0x1p-24;
+}
+
+// error bound of a record whose FP32 residual field holds rl: > 0 a boundary
+// residual still pending, < 0 one the correction push has moved into x (its
+// error stays), +-0 an exact (FP64) record
+__device__ __forceinline__ double lazy_bound(float rl, int d, int64_t it) {
+    if (rl == 0.0f) return 0.0;
+    return lazy_g(d, it) * fabs((double)rl) + ((double)d * (double)it + 1.0) * 0x1p-125;
+}
+
 struct __attribute__((aligned(32))) NodeRec {
     union {
         struct {
@@ -87,6 +121,7 @@ struct BatchArena {
     uint32_t *pcs = nullptr;     // slots * pcs_stride: hub pieces of the current list
     int64_t pcs_stride = 0;
     int32_t *tl = nullptr;       // slots * n: records touched by the current query
+    int32_t *nl = nullptr;       // slots * n: FP32 push: boundary candidates for an exact recompute
     uint64_t *ctl = nullptr;     // launch control for the synchronous entry point
     flsg_result *d_res = nullptr;
     int64_t d_res_cap = 0;
@@ -109,6 +144,7 @@ struct BatchArgs {
     uint32_t *pcs;
     int64_t pcs_stride;
     int32_t *tl;
+    int32_t *nl;
     int32_t piece_w;             // edges per hub piece
     int32_t hub_w;               // entries with more edges go through pieces
     int32_t hub_bytes;           // dynamic LDS of the hub cache (0 = none)
@@ -209,8 +245,10 @@ __device__ __forceinline__ void wave_sync() {
 //   pre(vtx, v, S &, stage): issue one edge's LDS (stage 0) or memory-side
 //                           (stage 1) operations, results into S
 //   post(vtx, dv, v, S, act) -> int: bit 0 appends vtx to next[] (counter *s_next),
-//                           bit 1 appends it to tl[] (counter *s_tl); the
-//                           result is masked by act, side effects must test it
+//                           bit 1 appends it to tl[] (counter *s_tl), bit 2 to
+//                           nl[] (counter *s_nl, entries past nl_cap dropped but
+//                           counted); the result is masked by act, side effects
+//                           must test it
 // A lane issues pre() for all its kUnroll edges before post() consumes any
 // result, so kUnroll atomics per lane are in flight at once.  Both run
 // unpredicated (a predicated atomic whose result is used makes the compiler
@@ -221,7 +259,8 @@ template <typename S, typename Load, typename Pre, typename Post>
 __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w, int hub_w,
                                            unsigned int *s_work, WaveLds &W, const int2 *colx, Load load,
                                            Pre pre, Post post, int32_t *next, unsigned int *s_next, int32_t *tl,
-                                           unsigned int *s_tl, int sink) {
+                                           unsigned int *s_tl, int sink, int32_t *nl = nullptr,
+                                           unsigned int *s_nl = nullptr, unsigned int nl_cap = 0) {
     const int lane = __lane_id();
     const unsigned int ngroups = (cnt + kWave - 1) / kWave;
     const unsigned int nitems = ngroups + npc;
@@ -292,13 +331,17 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
             }
             int cr[kUnroll];
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 3 : 0);
+            for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 7 : 0);
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
                 const long long slot = wave_append((cr[u] & 1) != 0, s_next);
                 if (slot >= 0) next[slot] = vtx[u];
                 const long long ts = wave_append((cr[u] & 2) != 0, s_tl);
                 if (ts >= 0) tl[ts] = vtx[u];
+                if (s_nl) {
+                    const long long ns = wave_append((cr[u] & 4) != 0, s_nl);
+                    if (ns >= 0 && ns < (long long)nl_cap) nl[ns] = vtx[u];
+                }
             }
         }
         if (group) wave_sync();
@@ -312,6 +355,124 @@ __device__ __forceinline__ void add_pieces(unsigned int i, int d, int pw, int hw
         const unsigned int np = (unsigned int)((d + pw - 1) / pw);
         const unsigned int b = atomicAdd(s_npc, np);
         for (unsigned int k = 0; k < np; k++) pcs[b + k] = (i << 8) | k;
+    }
+}
+
+// FP64 pull of the certificate: for the listed rows v, sum over the row's
+// column entries u of push(x32_u) -- a gather per column entry where a
+// scatter would take a memory-side atomic.  Rows of at most hub_w edges go 64
+// at a time per wavefront: the row's lane loads row start and record, the
+// edges are flattened over the wave, their terms summed in LDS, and the row's
+// lane then finishes the record (finish(ok, v, record lo half, sum), called by
+// every lane).  Longer rows are listed in ent[j] = {start, degree, v} for
+// j < *s_nlong, with their pieces in pcs[] for pull_pieces().  x32 of the
+// nodes below xn comes from the LDS copy xc[].  ops counts the rows' edges.
+template <typename Row, typename PushOf, typename Finish>
+__device__ __forceinline__ void pull_rows(unsigned int cnt, Row row_of, const int32_t *indptr, NodeRec *rec,
+                                          const int2 *colx, const float *xc, int xn, int hub_w, int piece_w,
+                                          Ent *ent, uint32_t *pcs, unsigned int *s_npc, unsigned int *s_nlong,
+                                          unsigned int *s_work, WaveLds &W, PushOf push_of, Finish finish,
+                                          unsigned long long &ops) {
+    const int lane = __lane_id();
+    const unsigned int ngroups = (cnt + kWave - 1) / kWave;
+    for (;;) {
+        unsigned int item = 0;
+        if (lane == 0) item = atomicAdd(s_work, 1u);
+        item = __shfl(item, 0);
+        if (item >= ngroups) break;
+        const unsigned int i = item * kWave + lane;
+        int v = -1, st = 0, d = 0;
+        float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < cnt) {
+            v = row_of(i);
+            st = indptr[v];
+            __builtin_memcpy(&lo, rec + v, 16);
+            d = __float_as_int(lo.z);
+        }
+        ops += (unsigned long long)d;
+        const bool lng = d > hub_w;
+        if (lng) {
+            const unsigned int j = atomicAdd(s_nlong, 1u);
+            Ent e;
+            e.st = st;
+            e.d = d;
+            e.v = (double)v;
+            ent[j] = e;
+            add_pieces(j, d, piece_w, hub_w, pcs, s_npc);
+        }
+        int incl = lng ? 0 : d;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const int tot = __shfl(incl, kWave - 1);
+        W.incl[lane] = incl;
+        W.start[lane] = st;
+        W.val[lane] = 0.0;
+        wave_sync();
+        for (int e0 = 0; e0 < tot; e0 += kWave * kUnroll) {
+            int k[kUnroll];
+            int2 c[kUnroll];
+            float xv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int e = e0 + u * kWave + lane;
+                k[u] = -1;
+                c[u] = make_int2(0, 0);
+                if (e < tot) {
+                    k[u] = upper_bound_lds(W.incl, 0, kWave, e);
+                    const int o = e - (k[u] ? W.incl[k[u] - 1] : 0);
+                    c[u] = colx[W.start[k[u]] + o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                xv[u] = k[u] < 0 ? 0.0f : c[u].x < xn ? xc[c[u].x] : rec[c[u].x].x;
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                if (xv[u] != 0.0f) atomicAdd(&W.val[k[u]], push_of(xv[u], c[u].y));
+        }
+        wave_sync();
+        finish(v >= 0 && !lng, v, lo, W.val[lane]);
+        wave_sync();
+    }
+}
+
+// The pieces of the long rows pull_rows() listed: one wavefront per piece, one
+// atomic per piece into the row's r64 (zero before).
+template <typename PushOf>
+__device__ __forceinline__ void pull_pieces(unsigned int npc, const uint32_t *pcs, const Ent *ent, int piece_w,
+                                            NodeRec *rec, const int2 *colx, const float *xc, int xn,
+                                            unsigned int *s_work, PushOf push_of) {
+    const int lane = __lane_id();
+    for (;;) {
+        unsigned int item = 0;
+        if (lane == 0) item = atomicAdd(s_work, 1u);
+        item = __shfl(item, 0);
+        if (item >= npc) break;
+        const uint32_t pc = pcs[item];
+        const Ent e = ent[pc >> 8];
+        const int beg = (int)(pc & 255u) * piece_w;
+        const int end = min(e.d, beg + piece_w);
+        double acc = 0.0;
+        for (int e0 = beg; e0 < end; e0 += kWave * kUnroll) {
+            int2 c[kUnroll];
+            float xv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int ed = e0 + u * kWave + lane;
+                c[u] = ed < end ? colx[e.st + ed] : make_int2(-1, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                xv[u] = c[u].x < 0 ? 0.0f : c[u].x < xn ? xc[c[u].x] : rec[c[u].x].x;
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                if (xv[u] != 0.0f) acc += push_of(xv[u], c[u].y);
+        }
+        acc = wave_sum(acc);
+        if (lane == 0 && acc != 0.0) atomic_add_wg(&rec[(int)e.v].r64, acc);
     }
 }
 
@@ -583,7 +744,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __shared__ WaveLds s_w[NT / kWave];
     __shared__ unsigned long long s_red[NT / kWave];
     __shared__ double s_redd[NT / kWave];
-    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_tnn, s_work, s_npc, s_tl;
+    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl;
     __shared__ unsigned long long s_base, s_vol;
     // hub cache: the residuals of nodes [0, H) live here for the whole query
     // (ids are degree-ranked on relabelled graphs, so these take the largest
@@ -603,6 +764,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     int32_t *sup = a.sup + off;
     uint32_t *pcs = a.pcs + (int64_t)(a.slot_base + (int)blockIdx.x) * a.pcs_stride;
     int32_t *tl = a.tl + off;
+    int32_t *nl = a.nl + off;
     const int32_t *indptr = a.indptr;
     const int2 *colx = a.colx;
     const int pw = a.piece_w;
@@ -667,6 +829,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             s_cnt = 1;
             s_sup = 0;
             s_tl = 1;
+            s_nl = 0;
             s_vol = 0;
         }
         __syncthreads();
@@ -803,10 +966,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         const float old = r.v;
                         const float nw = old + (float)val;
                         const double thr = P.thr_scale * (double)dv;
-                        return (((double)old < thr && thr <= (double)nw) ? 1 : 0) |
-                               ((pristine32(old) && nw != 0.0f) ? 2 : 0);
+                        const bool cross = (double)old < thr && thr <= (double)nw;
+                        const bool first = pristine32(old) && nw != 0.0f;
+                        // boundary candidates for an exact recompute: first touch
+                        // of a high-degree node, or a residual that entered
+                        // [kNearFrac * thr, thr) with this add
+                        const double cn = kNearFrac * thr;
+                        const bool near = dv > kLazyDeg ? first : (!cross && (double)old < cn && cn <= (double)nw);
+                        return (cross ? 1 : 0) | (first ? 2 : 0) | (near ? 4 : 0);
                     },
-                    qn, &s_ncnt, tl, &s_tl, sink);
+                    qn, &s_ncnt, tl, &s_tl, sink, nl, &s_nl, (unsigned int)a.n);
             }
             __syncthreads();
             if (tid == 0) s_cnt = s_ncnt;
@@ -834,117 +1003,97 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         uint64_t t_rec = t_push, t_corr = t_push;
         if (!P.fp64) {
             // ---------------- 2. FP64 certificate of the FP32 result
+            // Exact FP64 residual r64 = b - x + push(x) of the support (from the
+            // FP32 x) and of the boundary nodes the error bound (lazy_bound)
+            // cannot certify; every other boundary node keeps its FP32 residual
+            // and the bound.  The exact set: the support, then the boundary
+            // candidates nl[] the push listed (degree > kLazyDeg, or residual
+            // within 2^-8 of the threshold) -- or every touched boundary node
+            // when the bound does not hold at this iteration count or nl[]
+            // overflowed.
             const unsigned int S32 = s_sup;
+            const unsigned int NL = s_nl;
+            const unsigned int T32 = s_tl;
+            const bool lazy = (int64_t)NL <= a.n && lazy_g(kLazyDeg, iters) <= kLazyGmax;
+            // the hub cache's LDS now holds x32 of the same nodes for the pull
+            const int xn = hub_n;
+            float *xc = hub32;
+            for (int v = tid; v < xn; v += NT) xc[v] = rec[v].x;
             if (tid == 0) {
+                s_cnt = 0;
+                s_ncnt = 0;
                 s_npc = 0;
                 s_work = 0;
+                s_nlong = 0;
             }
             __syncthreads();
-            // entries of the support sweep: row, degree and push value of x32_u
-            for (unsigned int i0 = tid; i0 < S32; i0 += NT * kUnroll) {
-                int uu[kUnroll], ss[kUnroll];
-                float4 lo[kUnroll];
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    const unsigned int i = i0 + j * NT;
-                    uu[j] = i < S32 ? sup[i] : -1;
-                }
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    ss[j] = 0;
-                    if (uu[j] >= 0) {
-                        __builtin_memcpy(&lo[j], rec + uu[j], 16);
-                        ss[j] = indptr[uu[j]];
+            {
+                const unsigned int lim = lazy ? NL : T32;
+                const int32_t *src = lazy ? nl : tl;
+                for (unsigned int i0 = 0; i0 < lim; i0 += NT) {
+                    const unsigned int i = i0 + tid;
+                    int v = -1;
+                    if (i < lim) {
+                        v = src[i];
+                        if (!pristine32(rec[v].x)) v = -1;   // pushed: already in the support list
                     }
+                    const long long k = wave_append(v >= 0, &s_ncnt);
+                    if (k >= 0) qn[k] = v;
                 }
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    if (uu[j] < 0) continue;
-                    const unsigned int i = i0 + j * NT;
-                    Ent e;
-                    e.st = ss[j];
-                    e.d = __float_as_int(lo[j].z);
-                    e.v = bpush_value(P, (double)lo[j].y, e.d);
-                    ent[i] = e;
-                    add_pieces(i, e.d, pw, hw, pcs, &s_npc);
+                // the seed's residual was set, not added: never listed by the push
+                if (lazy && tid == 0) {
+                    const NodeRec &rs = rec[s];
+                    if (pristine32(rs.x) && (rs.d > kLazyDeg || P.r0 >= kNearFrac * P.thr_scale * (double)rs.d))
+                        qn[atomicAdd(&s_ncnt, 1u)] = s;
                 }
             }
             __syncthreads();
-            const unsigned int npc_sup = s_npc;
-            // 2a. sweep A: r64 = pushes of x32 over supp (every target is already
-            // listed; r64 is still zero everywhere); hub targets sum in LDS.
-            // Outside the support r64 is the whole recomputed residual and only
-            // grows here, so the atomic that carries it across thr_v queues v for
-            // the correction push -- exactly one does, as in the push itself.
-            const int hub64_cap = a.hub_bytes / 8;
-            const int hub64_n = hub64_cap > kWave ? (int)min((int64_t)(hub64_cap - kWave), a.n) : 0;
-            const int hub64_sink = hub64_cap > kWave ? hub64_cap - kWave - sink : INT32_MIN;
-            for (int i = tid; i < hub64_cap; i += NT) hub64[i] = 0.0;
-            if (tid == 0) s_cnt = 0;
-            __syncthreads();
-            wave_edges<Res64>(
-                S32, pcs, npc_sup, pw, hw, &s_work, W, colx, load_ent,
-                [&](int vtx, double v, Res64 &r, int stage) {
-                    const int li = vtx < hub64_n ? vtx : vtx >= sink ? vtx + hub64_sink : -1;
-                    if (stage == 0) {
-                        r.lds = li >= 0;
-                        if (li >= 0) r.v = atomicAdd(&hub64[li], v);
-                    } else if (li < 0) {
-                        r.v = atomic_add_wg(&rec[vtx].r64, v);
-                    }
-                },
-                [&](int vtx, int dv, double v, const Res64 &r, bool act) -> int {
-                    const double thr = P.thr_scale * (double)dv;
-                    if (!act || !(r.v < thr && thr <= r.v + v)) return 0;
-                    NodeRec &rv = rec[vtx];
-                    if (!pristine32(rv.x)) return 0;      // support: finished in 2b
-                    fl_or(&rv.fl, kInQ);
-                    return 1;
-                },
-                qc, &s_cnt, tl, &s_tl, sink);
-            __syncthreads();
-            for (int v = tid; v < hub64_n; v += NT) {
-                const double h = hub64[v];
-                if (!pristine64(h)) rec[v].r64 = h;
-            }
-            __syncthreads();
-            // 2b. the support: r64 = b - x + (pushes of x) in FP64, x64 = x32,
-            // queued for the correction push if |r64| >= thr
-            for (unsigned int i0 = 0; i0 < S32; i0 += NT * kUnroll) {
-                int vv[kUnroll];
-                float4 lo[kUnroll];
-                double rs[kUnroll];
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    const unsigned int i = i0 + j * NT + tid;
-                    vv[j] = i < S32 ? sup[i] : -1;
+            const unsigned int NLIST = S32 + s_ncnt;
+            // a listed record, finished: r64 = b - x + (pulled pushes) in FP64,
+            // x64 = x32, FP32 residual field -0.0 (exact record), support flag
+            // for the pushed nodes, queued for the correction push if |r64| >= thr
+            auto finish = [&](bool ok, int v, const float4 &lo, double acc) {
+                bool enq = false;
+                if (ok) {
+                    const double xd = (double)lo.y;
+                    const int d = __float_as_int(lo.z);
+                    double rt = acc - xd;
+                    if (v == s) rt = rt + P.r0;
+                    enq = fabs(rt) >= P.thr_scale * (double)d;
+                    char *pr = reinterpret_cast<char *>(rec + v);
+                    const double2 hv = make_double2(rt, xd);
+                    __builtin_memcpy(pr + 16, &hv, 16);
+                    const uint32_t f = (pristine32(lo.y) ? 0u : kSeen) | (enq ? kInQ : 0u);
+                    const float4 lv = make_float4(-0.0f, lo.y, lo.z, __uint_as_float(f));
+                    __builtin_memcpy(pr, &lv, 16);
                 }
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    rs[j] = 0.0;
-                    if (vv[j] >= 0) {
-                        __builtin_memcpy(&lo[j], rec + vv[j], 16);
-                        rs[j] = rec[vv[j]].r64;
+                const long long slot = wave_append(enq, &s_cnt);
+                if (slot >= 0) qc[slot] = v;
+            };
+            auto push_of = [&](float xu, int du) { return bpush_value(P, (double)xu, du); };
+            // 2a. rows of the support and of the listed boundary nodes
+            pull_rows(
+                NLIST, [&](unsigned int i) { return i < S32 ? sup[i] : qn[i - S32]; }, indptr, rec, colx, xc, xn,
+                hw, pw, ent, pcs, &s_npc, &s_nlong, &s_work, W, push_of, finish, cert_ops);
+            __syncthreads();
+            if (tid == 0) s_work = 0;
+            __syncthreads();
+            // 2b. pieces of the long rows, then their records
+            pull_pieces(s_npc, pcs, ent, pw, rec, colx, xc, xn, &s_work, push_of);
+            __syncthreads();
+            {
+                const unsigned int nlong = s_nlong;
+                for (unsigned int j0 = 0; j0 < nlong; j0 += NT) {
+                    const unsigned int j = j0 + tid;
+                    int v = -1;
+                    float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
+                    double acc = 0.0;
+                    if (j < nlong) {
+                        v = (int)ent[j].v;
+                        __builtin_memcpy(&lo, rec + v, 16);
+                        acc = rec[v].r64;
                     }
-                }
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    bool enq = false;
-                    if (vv[j] >= 0) {
-                        const double xd = (double)lo[j].y;
-                        const int d = __float_as_int(lo[j].z);
-                        double rt = rs[j] - xd;
-                        if (vv[j] == s) rt = rt + P.r0;
-                        enq = fabs(rt) >= P.thr_scale * (double)d;
-                        const double2 hv = make_double2(rt, xd);
-                        __builtin_memcpy(reinterpret_cast<char *>(rec + vv[j]) + 16, &hv, 16);
-                        rec[vv[j]].fl = kSeen | (enq ? kInQ : 0u);
-                        cert_ops += (unsigned long long)d;
-                    }
-                    const long long slot = wave_append(enq, &s_cnt);
-                    if (slot >= 0) qc[slot] = vv[j];
+                    finish(j < nlong, v, lo, acc);
                 }
             }
             __syncthreads();
@@ -973,13 +1122,17 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     NodeRec &ru = rec[u];
                     const int d = ru.d;
                     const uint32_t fl = fl_and(&ru.fl, ~kInQ);
-                    const double tmp = ru.r64;
+                    // a pending boundary residual (rl > 0) joins the FP64 one;
+                    // once pushed its error bound stays (rl < 0)
+                    const float rl = ru.r;
+                    const double tmp = ru.r64 + (rl > 0.0f ? (double)rl : 0.0);
                     Ent e;
                     e.st = 0;
                     e.d = 0;
                     e.v = 0.0;
-                    if (fabs(tmp) >= P.thr_scale * (double)d) {
+                    if (fabs(tmp) + lazy_bound(rl, d, iters) >= P.thr_scale * (double)d) {
                         ru.r64 = 0.0;
+                        if (rl > 0.0f) ru.r = -rl;
                         ru.x64 = ru.x64 + tmp;
                         if ((fl & kSeen) == 0) {
                             fl_or(&ru.fl, kSeen);
@@ -1007,10 +1160,14 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         if (!act) return 0;
                         NodeRec &rv = rec[vtx];
                         const double nw = old + val;
+                        const float rl = rv.r;
                         int res = 0;
                         // records the main push never reached still hold r == +0.0
-                        if (old == 0.0 && pristine32(rv.r) && (fl_or(&rv.fl, kTouched) & kTouched) == 0) res = 2;
-                        if (fabs(nw) >= P.thr_scale * (double)dv && (fl_or(&rv.fl, kInQ) & kInQ) == 0) res |= 1;
+                        if (old == 0.0 && pristine32(rl) && (fl_or(&rv.fl, kTouched) & kTouched) == 0) res = 2;
+                        const double eff = nw + (rl > 0.0f ? (double)rl : 0.0);
+                        if (fabs(eff) + lazy_bound(rl, dv, iters) >= P.thr_scale * (double)dv &&
+                            (fl_or(&rv.fl, kInQ) & kInQ) == 0)
+                            res |= 1;
                         return res;
                     },
                     qn, &s_ncnt, tl, &s_tl, sink);
@@ -1078,8 +1235,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     continue;
                 }
                 const int d = __float_as_int(lo[j].z);
-                const double r64 = hi[j].x;
-                if (r64 != 0.0) cmax = fmax(cmax, fabs(r64) / (P.thr_scale * (double)d));
+                // FP32 push: a boundary record may hold its residual in FP32
+                // (pending, rl > 0) and carries the error bound of that sum
+                const float rl = P.fp64 ? 0.0f : lo[j].x;
+                const double res = hi[j].x + (rl > 0.0f ? (double)rl : 0.0);
+                const double bnd = lazy_bound(rl, d, iters);
+                if (res != 0.0 || bnd != 0.0) cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
                 is_sup[j] = P.fp64 ? hi[j].y != 0.0 : (__float_as_uint(lo[j].w) & kSeen) != 0u;
                 if (is_sup[j] && a.out_map) id[j] = a.out_map[vv[j]];
                 char *pr = reinterpret_cast<char *>(rec + vv[j]);
@@ -1221,6 +1382,7 @@ void batch_free(flsg_graph g) {
     (void)hipFree(A->sup);
     (void)hipFree(A->pcs);
     (void)hipFree(A->tl);
+    (void)hipFree(A->nl);
     (void)hipFree(A->ctl);
     (void)hipFree(A->d_res);
     (void)hipFree(A->d_q);
@@ -1229,9 +1391,9 @@ void batch_free(flsg_graph g) {
 }
 
 int64_t slot_bytes(int64_t n) {
-    // 32 B node record + q0 q1 sup pcs tl (4 B) + frontier entry (16 B), kSink sink records
+    // 32 B node record + q0 q1 sup pcs tl nl (4 B) + frontier entry (16 B), kSink sink records
     // (the hub-piece list adds 4 B per 64 column slots of the graph)
-    return n * (32 + 5 * 4 + 16) + (int64_t)kSink * 32;
+    return n * (32 + 6 * 4 + 16) + (int64_t)kSink * 32;
 }
 
 static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
@@ -1254,6 +1416,7 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     A->pcs_stride = g->n + g->nnz_cap / kWave + 1;
     FLSG_HIP(hipMalloc((void **)&A->pcs, sizeof(uint32_t) * (size_t)A->pcs_stride * (size_t)slots));
     FLSG_HIP(hipMalloc((void **)&A->tl, sizeof(int32_t) * ns));
+    FLSG_HIP(hipMalloc((void **)&A->nl, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
     hipLaunchKernelGGL(k_slot_init, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree, g->n,
                        (int64_t)nrec);
@@ -1298,6 +1461,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.pcs = A->pcs;
     a.pcs_stride = A->pcs_stride;
     a.tl = A->tl;
+    a.nl = A->nl;
     // hub pieces are encoded (entry << 8) | piece: entries < 2^24, < 256 pieces per node
     if (g->n >= (1 << 24)) return fail(FLSG_EARG, "batched solves need n < 2^24");
     // degree bins: entries with more than hub_w edges are split into pieces of
