@@ -197,7 +197,7 @@ class BatchOut:
     ids: object          # torch int32 [total] (device)
     vals: object         # torch float64 [total] (device)
     offsets: np.ndarray  # int64 [nq] start of query q in ids/vals
-    results: list        # per-query result dicts
+    results: np.ndarray  # per-query result records (structured array, fields of flsg_result)
     ms: float            # wall-clock of the launch (events), milliseconds
 
     def query(self, q: int):
@@ -247,5 +247,6 @@ def batch_solve(dg: DeviceGraph, params: _abi.Params, seeds, out_cap: int | None
     if not (allow_failed and per_query):
         _check(rc)
     torch.cuda.synchronize()
-    return BatchOut(ids=ids, vals=vals, offsets=offs.cpu().numpy()[:nq], results=[res[i].as_dict() for i in range(nq)],
+    # the result records as a structured array over the ctypes buffer (no copy)
+    return BatchOut(ids=ids, vals=vals, offsets=offs.cpu().numpy()[:nq], results=np.ctypeslib.as_array(res)[:nq],
                     ms=e0.elapsed_time(e1))

@@ -64,4 +64,4 @@ def test_struct_offsets_match_the_header(tmp_path):
 def test_version_without_gpu():
     lib = _build.load_lib()
     assert lib.flsg_version() >= 1
-    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 72 + 64 * 32
+    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 76 + 64 * 32
