@@ -74,14 +74,14 @@ constexpr double kNearFrac = 0.99609375;     // 1 - 2^-8: boundary residuals abo
                                              // the threshold are recomputed
 constexpr double kLazyGmax = 0.0039;         // g(kLazyDeg, iters) limit: kNearFrac * (1 + g) < 1 - 1e-6
 
-__device__ __forceinline__ double lazy_g(int d, int64_t it) {
+__host__ __device__ __forceinline__ double lazy_g(int d, int64_t it) {
     return 1.03 * (2.0 * (double)d * (double)it + (double)it + 4.0) * 0x1p-24;
 }
 
 // error bound of a record whose FP32 residual field holds rl: > 0 a boundary
 // residual still pending, < 0 one the correction push has moved into x (its
 // error stays), +-0 an exact (FP64) record
-__device__ __forceinline__ double lazy_bound(float rl, int d, int64_t it) {
+__host__ __device__ __forceinline__ double lazy_bound(float rl, int d, int64_t it) {
     if (rl == 0.0f) return 0.0;
     return lazy_g(d, it) * fabs((double)rl) + ((double)d * (double)it + 1.0) * 0x1p-125;
 }
@@ -1570,6 +1570,8 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
 using namespace flsg;
 
 extern "C" int64_t flsg_batch_slot_bytes(int64_t n) { return slot_bytes(n); }
+
+extern "C" double flsg_boundary_bound(float r32, int32_t d, int64_t iters) { return lazy_bound(r32, d, iters); }
 
 extern "C" int flsg_batch_set_slots(flsg_graph g, int32_t slots) {
     int rc = check_graph(g);

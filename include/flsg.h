@@ -221,6 +221,16 @@ int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queri
                       uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters, const flsg_hk_config *hk,
                       void *stream);
 
+/* Rounding-error bound the batched FP32 certificate applies to a boundary
+ * node (reached by the push, never pushed) of degree d whose FP32 residual is
+ * r32 after `iters` push iterations: |r_FP64 - r32| <= bound, r_FP64 being the
+ * FP64 residual of the returned x (0 for r32 == 0; |r32| is used, the sign
+ * marks a residual the correction push has moved into x).  Valid while
+ * (2 d iters + iters + 4) * 2^-24 <= 1e-2; the kernels recompute every other
+ * node in FP64.  Host function, no GPU needed.  No counterpart in the
+ * reference (the certificate's error bound, exposed for tests). */
+double flsg_boundary_bound(float r32, int32_t d, int64_t iters);
+
 /* Number of slot-arena records not back in the between-queries state (every
  * field zero except the degree copy) -- 0 after any completed launch.
  * Synchronises the stream.  No counterpart in the reference (a check of the

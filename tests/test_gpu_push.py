@@ -275,6 +275,27 @@ def test_batch_arena_clean_after_launches(small_graph, dev_small):
     assert modes == {True, False}   # both reset modes ran
 
 
+@pytest.mark.parametrize("alpha,eps", [(0.01, 1e-6), (0.005, 1e-7)])
+def test_batch_fp32_boundary_regimes(small_graph, dev_small, alpha, eps):
+    """FP32 certificate in both regimes of the boundary bound: at alpha 0.01
+    (about 295 push iterations) boundary nodes well below their threshold keep
+    their FP32 residual plus the rounding-error bound; at alpha 0.005 (about
+    1050 iterations, past the 494 up to which the bound is used at degree 64)
+    every boundary node is recomputed in FP64.  Certified on the device and
+    re-certified by a host FP64 recompute in both."""
+    g = small_graph
+    seeds = graph.mixed_seeds(g, 24, seed=17)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=alpha, eps=eps)
+    bo = engine.batch_solve(dev_small, p, seeds)
+    its = bo.results["iters"]
+    if alpha == 0.01:
+        assert its.max() < 494
+    else:
+        assert its.min() > 494
+    _check_batch(g, seeds, bo, eps, alpha=alpha, n_check=8)
+    assert engine.arena_dirty(dev_small) == 0
+
+
 def test_batch_bad_seed_fails_alone(small_graph, dev_small):
     """A query whose seed lies outside the graph fails with FLSG_EARG on its
     own record (the kernel checks before touching a slot); the rest of the
