@@ -31,10 +31,13 @@ for _p in (PKG, REPO):
 #   push, per frontier entry: 4 queue id + 16 record read (r, x, d) + 8 record
 #     write (r, x) + 4 row start + 4 next-frontier id                     = 36
 #   push, per edge: 8 column entry {v, d_v} + 4 r_v read + 4 r_v write    = 16
-#   certificate, per support / correction edge: 8 column entry + 16 r64 RMW = 24
+#   certificate, per pulled column entry: 8 column entry + 4 x32_u read  = 12
+#     (the few correction-push edges, 24 B each, are counted at 12)
+#   certificate, per support record: 8 list entry (id, row start) + 16
+#     record read + 32 record write (r64, x64, flags)                     = 56
 #   per touched record: 16 read (r64, x) + 16 reset write                 = 32
 #   per output entry: 4 id + 8 value                                       = 12
-B_NODE_FP32, B_EDGE_FP32, B_CERT_EDGE, B_TOUCHED, B_OUT = 36, 16, 24, 32, 12
+B_NODE_FP32, B_EDGE_FP32, B_CERT_EDGE, B_CERT_ROW, B_TOUCHED, B_OUT = 36, 16, 12, 56, 32, 12
 HBM_PEAK_GBS = 8000.0
 
 
@@ -150,8 +153,8 @@ def main():
     cert_edges = int(res["oper_cert"].sum())
     supp = int(res["supp"].sum())
     touched = int(res["touched"].astype(np.int64).sum())
-    bytes_step = (B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * cert_edges + B_TOUCHED * touched
-                  + B_OUT * supp)
+    bytes_step = (B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * cert_edges + B_CERT_ROW * supp
+                  + B_TOUCHED * touched + B_OUT * supp)
     kern_ms = float(np.mean(kern))
     achieved = bytes_step / (kern_ms * 1e-3) / 1e9
 
