@@ -159,6 +159,7 @@ int flsg_graph_destroy(flsg_graph g) {
     cheb_free(g);
     ignn_free(g);
     dyn_free(g);
+    sor_free(g);
     single_graphs_free(g);
     delete g;
     return FLSG_OK;

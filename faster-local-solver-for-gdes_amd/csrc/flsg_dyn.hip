@@ -177,6 +177,7 @@ extern "C" int flsg_graph_insert_edges(flsg_graph g, const int32_t *d_us, const 
     hipLaunchKernelGGL(k_dyn_keys, dim3(grid), dim3(kDT), 0, st, w->cnt, n, d_keys_out, w->flags,
                        d_keys_out ? 0 : 1);
     g->rows_valid = 0;          // the global sweeps' row classes follow the degrees
+    sor_invalidate(g);          // so does the SOR colouring
     rc = colx_refresh(g, st);   // degree-carrying column copy of the batched path, if built
     if (rc) return rc;
     FLSG_HIP(hipMemcpyAsync(hf, w->flags, sizeof(hf), hipMemcpyDeviceToHost, st));

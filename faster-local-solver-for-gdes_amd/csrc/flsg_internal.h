@@ -92,6 +92,8 @@ struct flsg_graph_s {
     void *ignn = nullptr;
     // edge-insertion workspace (flsg_dyn.hip)
     void *dyn = nullptr;
+    // coloured SOR: colouring and frontier workspace (flsg_sor.hip)
+    void *sor = nullptr;
     // captured iteration blocks of the single-source push (flsg_single.hip)
     // and the handle-owned x / r they run on (fixed addresses, so the
     // captured graph is reused across solves)
@@ -120,6 +122,8 @@ int colx_refresh(flsg_graph g, hipStream_t st);
 void cheb_free(flsg_graph g);
 void ignn_free(flsg_graph g);
 void dyn_free(flsg_graph g);
+void sor_free(flsg_graph g);
+void sor_invalidate(flsg_graph g);
 void single_graphs_free(flsg_graph g);
 
 // ---------------------------------------------------------------- device helpers

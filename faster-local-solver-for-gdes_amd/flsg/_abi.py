@@ -79,6 +79,10 @@ SIGNATURES = {
     "flsg_batch_refresh_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "flsg_batch_arena_dirty": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "flsg_boundary_bound": (C.c_double, [C.c_float, C.c_int32, C.c_int64]),
+    "flsg_sor_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_double, C.c_int64, C.c_void_p, C.c_void_p,
+                                 C.POINTER(Result), C.c_void_p]),
+    "flsg_graph_colouring": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                       C.c_void_p]),
     "flsg_graph_insert_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                           C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "flsg_ignn_propagate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -183,6 +183,38 @@ def solve(dg: DeviceGraph, params: _abi.Params, seed: int, trace: bool = False,
     return SolveOut(x=x, r=r, result=res.as_dict(), trace=frontiers, extra=extra)
 
 
+def sor_solve(dg: DeviceGraph, alpha: float, eps: float, omega: float, seed: int, local: bool = True,
+              max_iters: int = 0) -> SolveOut:
+    """Graph-coloured SOR push for PPR (ppr_sor, ppr_solver.py:331-461) on the
+    whole GPU: FP64, local (frontier) or global (sweeps)."""
+    torch = dg.torch
+    n = dg.n
+    x = torch.empty(n, dtype=torch.float64, device=dg.device)
+    r = torch.empty(n, dtype=torch.float64, device=dg.device)
+    res = _abi.Result()
+    p = make_params(_abi.FLSG_PPR, _abi.FLSG_GD, local, _abi.FLSG_FP64, alpha=alpha, eps=eps, max_iters=max_iters)
+    seed = int(dg.to_internal([seed])[0])
+    _check(dg.lib.flsg_sor_solve(dg.handle, C.byref(p), float(omega), seed, _ptr(x), _ptr(r), C.byref(res),
+                                 _stream_ptr(torch)))
+    if dg.perm is not None:
+        x = x[dg.iperm]
+        r = r[dg.iperm]
+    return SolveOut(x=x, r=r, result=res.as_dict(), trace=None, extra={})
+
+
+def colouring(dg: DeviceGraph):
+    """The SOR colouring of the device graph: (colour[n] numpy int32 in the
+    caller's node ids, number of colours, rounds)."""
+    torch = dg.torch
+    col = torch.empty(dg.n, dtype=torch.int32, device=dg.device)
+    k = C.c_int32(0)
+    rounds = C.c_int64(0)
+    _check(dg.lib.flsg_graph_colouring(dg.handle, _ptr(col), C.byref(k), C.byref(rounds), _stream_ptr(torch)))
+    if dg.perm is not None:
+        col = col[dg.iperm]
+    return col.cpu().numpy(), int(k.value), int(rounds.value)
+
+
 def arena_dirty(dg: DeviceGraph) -> int:
     """Slot-arena records not back in the between-queries state (0 after
     every completed batched launch)."""

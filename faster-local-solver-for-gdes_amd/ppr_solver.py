@@ -10,8 +10,11 @@ libflsg.so's HIP kernels on the GPU (no CPU fallback).  Differences:
     operation counts of the reference's numba code);
   * ``ppr_gd_local_gpu`` runs the certified FP32 path (the reference's GPU
     path is FP32 end-to-end); it returns FP64 arrays;
-  * SOR / Gauss-Southwell (``ppr_sor``) remain CPU reference baselines and
-    raise NotImplementedError here;
+  * ``ppr_sor`` (SOR, and Gauss-Southwell / APPR at omega = 1) runs a
+    graph-coloured variant: the nodes of one colour share no edge and push at
+    once, colours in order.  It stops on the reference's certificate; its
+    push order (colour order, not FIFO) and so its x, oper and iteration
+    counts differ from the reference's within that certificate;
   * isolated nodes (degree 0) are never active (the reference's global loops
     divide by zero / never stop on them).
 """
@@ -101,20 +104,34 @@ def solve_a_graph_cheby(n, indptr, indices, degree, alpha, eps, omega, graph_nam
 
 
 def solve_a_graph_all(n, indptr, indices, degree, alpha, eps, omega, graph_name, s_nodes):
-    """ppr_solver.py:511-579.  The GD slots are computed on the GPU; the
-    SOR / GS slots are CPU reference baselines not run here and are
-    returned as NaN."""
-    k = len(s_nodes)
-    nan_o = np.full(k, np.nan)
+    """ppr_solver.py:511-579: (s_nodes, local_sor_opers, local_sor_algo_times,
+    global_sor_*, local_appr_* (omega = 1), global_appr_*, local_gd_*,
+    global_gd_*), every slot computed on the GPU (SOR / APPR by the coloured
+    variant)."""
     lo, lt = _d.per_seed(n, indptr, indices, degree, s_nodes, _params(_abi.FLSG_GD, True, alpha, eps))
     go, gt = _d.per_seed(n, indptr, indices, degree, s_nodes, _params(_abi.FLSG_GD, False, alpha, eps))
-    return (s_nodes, nan_o, nan_o.copy(), nan_o.copy(), nan_o.copy(), nan_o.copy(), nan_o.copy(), nan_o.copy(),
-            nan_o.copy(), lo, lt, go, gt)
+    lso, lst = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, True)
+    gso, gst = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, False)
+    lao, lat = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, 1.0, True)
+    gao, gat = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, 1.0, False)
+    return (s_nodes, lso, lst, gso, gst, lao, lat, gao, gat, lo, lt, go, gt)
 
 
-def ppr_sor(*args, **kwargs):
-    _d.sor_not_built("ppr_sor")
+def ppr_sor(n, indptr, indices, degree, s_node, alpha, eps, omega, local, opt_x=None, debug=False):
+    """SOR / Gauss-Southwell push (ppr_solver.py:331-461), graph-coloured on
+    the GPU.  Returns (xt, rt, oper, algo_time) with oper a float as the
+    reference's."""
+    return _d.single_sor(n, indptr, indices, degree, s_node, alpha, eps, omega, local)
 
 
-def solve_a_graph_omega(*args, **kwargs):
-    _d.sor_not_built("solve_a_graph_omega")
+def solve_a_graph_omega(n, indptr, indices, degree, alpha, eps, omega, graph_name, s_nodes):
+    """ppr_solver.py:463-508: the 13-tuple of solve_a_graph_all with only the
+    local SOR and local APPR (omega = 1) slots filled; the others stay 0 as
+    the reference leaves them."""
+    k = len(s_nodes)
+    z_o = np.zeros(k, dtype=np.int64)
+    z_t = np.zeros(k)
+    lso, lst = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, True)
+    lao, lat = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, 1.0, True)
+    return (s_nodes, lso, lst, z_o, z_t.copy(), lao, lat, z_o.copy(), z_t.copy(), z_o.copy(), z_t.copy(),
+            z_o.copy(), z_t.copy())

@@ -241,6 +241,30 @@ int flsg_batch_arena_dirty(flsg_graph g, int64_t *h_count, void *stream);
  * degree copy the slots keep for the listed nodes. */
 int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, int64_t n_keys, void *stream);
 
+/* ---------------------------------------------------------------- coloured SOR */
+
+/* Graph-coloured SOR / Gauss-Southwell push for PPR.  Replaces
+ *   ppr_sor(..., omega, local=True/False)     ppr_solver.py:331-461
+ * (omega = 1 is the APPR push of solve_a_graph_all, ppr_solver.py:559-570).
+ * Colours are visited in order and the nodes of one colour, which share no
+ * edge, push at once; the local variant sweeps its frontier this way, the
+ * global variant every live node.  Both stop on the reference's certificate
+ * |r_u| < eps*alpha*d_u for every live u.  p->kind must be FLSG_PPR,
+ * p->precision FLSG_FP64, p->local selects the variant; 0 < omega < 2.
+ * d_x, d_r: dense FP64 outputs of n entries.  h_res->oper is the reference's
+ * operation count (local: sum of degree over pushed nodes; global: n + m per
+ * sweep), iters the frontier passes / sweeps, nodes the pushed nodes.
+ * The colouring is computed on first use (and after edge insertions) and is
+ * not part of h_res->ms.  Graphs with self-loops are rejected (FLSG_EARG). */
+int flsg_sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t seed, double *d_x, double *d_r,
+                   flsg_result *h_res, void *stream);
+
+/* The colouring the SOR solve uses (Jones-Plassmann rounds, priority
+ * (degree, hash), first-fit colours): d_colour_out (device, n int32, may be
+ * NULL), *h_num_colours, *h_rounds.  No counterpart in the reference. */
+int flsg_graph_colouring(flsg_graph g, int32_t *d_colour_out, int32_t *h_num_colours, int64_t *h_rounds,
+                         void *stream);
+
 /* ---------------------------------------------------------------- dynamic graphs */
 
 /* In-place insertion of k undirected edges (d_us[i], d_vs[i]) into a slack-CSR

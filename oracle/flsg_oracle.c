@@ -877,3 +877,266 @@ int ora_max_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------
+ * SOR / Gauss-Southwell push (ppr_solver.py:331-461, ppr_sor).
+ *
+ * local = 1: FIFO queue of capacity n+2 used circularly, with the iteration
+ * flag node n (:356-366); a popped node is skipped while eps_vec[u] > |r_u|
+ * (:398-399); an active node adds degree[u] to oper (a float, :401), moves
+ * res = omega*r_u into x_u, pushes (1-alpha)*res/degree[u] to its live
+ * neighbours (:402-413) and is re-queued while still active (:415-419).
+ * local = 0: natural-order sweeps u = 0..n-1 updating in place (:421-429),
+ * oper += n + m per sweep, stop once no node is active (:449-452).
+ * Deviation (as for the global GD sweeps above): isolated nodes are not
+ * counted as active -- their eps_vec is 0, so the reference's count never
+ * reaches 0 on a graph with one; they are skipped in the sweep, which is exact
+ * since their residual stays 0.
+ * iters_out: local = non-empty flag passes (iterations), global = sweeps.
+ * -------------------------------------------------------------------- */
+int ora_ppr_sor(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                int64_t s, double alpha, double eps, double omega, int local, double *xt, double *rt,
+                double *oper_out, int64_t *iters_out, double *time_out) {
+    if (!(eps * (double)degree[s] <= 1.)) return ORA_EASSERT;   /* :333 */
+    if (degree[s] == 0) return ORA_EARG;
+    double t0 = now_s();
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++) m += degree[i];            /* :336 */
+    memset(xt, 0, sizeof(double) * n);
+    memset(rt, 0, sizeof(double) * n);
+    rt[s] = alpha;                                               /* :339 */
+    double *eps_vec = (double *)malloc(sizeof(double) * n);
+    if (!eps_vec) return ORA_ENOMEM;
+    const double ea = eps * alpha;                               /* :340 */
+    for (int64_t i = 0; i < n; i++) eps_vec[i] = ea * (double)degree[i];
+    double oper = 0.;
+    int64_t iters = 0;
+    if (local) {
+        const int64_t q_max = n + 2;                             /* :354-356 */
+        int64_t *queue = (int64_t *)malloc(sizeof(int64_t) * q_max);
+        uint8_t *q_mark = (uint8_t *)calloc(q_max, 1);
+        if (!queue || !q_mark) { free(queue); free(q_mark); free(eps_vec); return ORA_ENOMEM; }
+        int64_t front = 0, rear = 0;
+        queue[rear] = n; rear = (rear + 1) % q_max; q_mark[n] = 1;            /* :358-361 */
+        queue[rear] = s; rear = (rear + 1) % q_max; q_mark[s] = 1;            /* :363-365 */
+        for (;;) {
+            const int64_t u = queue[front];                                    /* :371-373 */
+            front = (front + 1) % q_max;
+            q_mark[u] = 0;
+            if (u == n) {                                                      /* :375-396 */
+                if (rear - front == 0) break;
+                queue[rear] = n; rear = (rear + 1) % q_max; q_mark[u] = 1;
+                iters++;
+                continue;
+            }
+            if (eps_vec[u] > fabs(rt[u])) continue;                            /* :398-399 */
+            oper += (double)degree[u];                                         /* :401 */
+            const double res = omega * rt[u];                                  /* :404-406 */
+            xt[u] += res;
+            rt[u] -= res;
+            const double push_res = (1. - alpha) * res / (double)degree[u];   /* :407 */
+            for (int64_t j = indptr[u]; j < indptr[u] + degree[u]; j++) {     /* :408-413 */
+                const int64_t v = indices[j];
+                rt[v] += push_res;
+                if (!q_mark[v] && eps_vec[v] <= fabs(rt[v])) {
+                    queue[rear] = v; rear = (rear + 1) % q_max; q_mark[v] = 1;
+                }
+            }
+            if (!q_mark[u] && eps_vec[u] <= fabs(rt[u])) {                    /* :415-419 */
+                queue[rear] = u; rear = (rear + 1) % q_max; q_mark[u] = 1;
+            }
+        }
+        free(queue); free(q_mark);
+    } else {
+        for (;;) {
+            for (int64_t u = 0; u < n; u++) {                                  /* :421-429 */
+                if (degree[u] == 0) continue;
+                const double res = omega * rt[u];
+                xt[u] += res;
+                rt[u] -= res;
+                const double push_res = (1. - alpha) * res / (double)degree[u];
+                for (int64_t j = indptr[u]; j < indptr[u] + degree[u]; j++) rt[indices[j]] += push_res;
+            }
+            oper += (double)(n + m);                                           /* :447-448 */
+            iters++;
+            int64_t act = 0;                                                   /* :449-452 */
+            for (int64_t i = 0; i < n; i++)
+                if (degree[i] > 0 && eps_vec[i] <= fabs(rt[i])) act++;
+            if (act <= 0) break;
+        }
+    }
+    free(eps_vec);
+    if (oper_out) *oper_out = oper;
+    if (iters_out) *iters_out = iters;
+    if (time_out) *time_out = now_s() - t0;
+    return ORA_OK;
+}
+
+/* ------------------------------------------------------------------------
+ * Graph colouring and the coloured SOR push: NOT the reference's algorithm.
+ * They restate, one step after another, what libflsg's GPU SOR variant does,
+ * so its results can be checked exactly (the colouring) or to rounding (the
+ * push).  The reference's ppr_sor above is the semantic anchor: both stop on
+ * the same certificate (every live |r_u| < eps*alpha*d_u).
+ *
+ * Colouring: Jones-Plassmann rounds.  Priority of u = (degree[u], mix32(u))
+ * compared as one 64-bit key (mix32 is a bijection, so keys are distinct).
+ * Isolated nodes get colour 0 up front.  A round selects every uncoloured u
+ * whose live neighbours (self-loops ignored) include no uncoloured node of
+ * higher key -- an independent set -- and gives each the smallest colour no
+ * coloured neighbour holds.  Returns the number of colours; *rounds_out the
+ * number of rounds.
+ * -------------------------------------------------------------------- */
+static inline uint32_t mix32(uint32_t h) {   /* MurmurHash3 fmix32 (bijective) */
+    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+    return h;
+}
+static inline uint64_t colour_key(const int32_t *degree, int64_t u) {
+    return ((uint64_t)(uint32_t)degree[u] << 32) | (uint64_t)mix32((uint32_t)u);
+}
+
+int64_t ora_colour(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                   int32_t *colour, int64_t *rounds_out) {
+    uint8_t *sel = (uint8_t *)calloc(n, 1);
+    if (!sel) return -1;
+    int64_t left = 0, ncol = 1, rounds = 0;
+    for (int64_t u = 0; u < n; u++) {
+        colour[u] = degree[u] == 0 ? 0 : -1;
+        left += degree[u] != 0;
+    }
+    while (left > 0) {
+        rounds++;
+        for (int64_t u = 0; u < n; u++) {
+            sel[u] = 0;
+            if (colour[u] >= 0) continue;
+            const uint64_t ku = colour_key(degree, u);
+            int ok = 1;
+            for (int64_t j = indptr[u]; j < indptr[u] + degree[u] && ok; j++) {
+                const int64_t v = indices[j];
+                if (v != u && colour[v] < 0 && colour_key(degree, v) > ku) ok = 0;
+            }
+            sel[u] = (uint8_t)ok;
+        }
+        for (int64_t u = 0; u < n; u++) {
+            if (!sel[u]) continue;
+            /* smallest colour not held by a coloured neighbour, 64 at a time */
+            for (int64_t base = 0;; base += 64) {
+                uint64_t used = 0;
+                for (int64_t j = indptr[u]; j < indptr[u] + degree[u]; j++) {
+                    const int64_t v = indices[j];
+                    const int64_t c = colour[v];
+                    if (v != u && !sel[v] && c >= base && c < base + 64) used |= 1ull << (c - base);
+                }
+                if (~used) {
+                    colour[u] = (int32_t)(base + __builtin_ctzll(~used));
+                    break;
+                }
+            }
+            if (colour[u] + 1 > ncol) ncol = colour[u] + 1;
+            left--;
+        }
+    }
+    free(sel);
+    if (rounds_out) *rounds_out = rounds;
+    return ncol;
+}
+
+/* Coloured SOR push (local or global), sequential restatement of the GPU
+ * variant.  Per iteration (local: the current frontier F; global: every live
+ * node), colours are visited in increasing order; within colour c every
+ * listed node u with eps_vec[u] <= |r_u| (global: every live node) takes
+ * res = omega*r_u and pushes (1-alpha)*res/degree[u] to its neighbours --
+ * none of which has colour c, so the nodes of one colour are independent.
+ * After the colour, each such u applies x_u += res, r_u -= res (the GPU
+ * applies it in the next colour's launch).  Local: the next frontier is every
+ * node whose residual changed in the iteration (listed at its first change)
+ * and satisfies eps_vec[v] <= |r_v| when the iteration ends -- the active set
+ * after the sweep, whatever the order of the adds within a colour.  Nodes of
+ * F are processed in F's order within a colour. */
+int ora_ppr_sor_coloured(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                         const int32_t *colour, int64_t ncol, int64_t s, double alpha, double eps,
+                         double omega, int local, int64_t max_iters, double *xt, double *rt,
+                         int64_t *oper_out, int64_t *iters_out, int64_t *nodes_out) {
+    if (!(eps * (double)degree[s] <= 1.)) return ORA_EASSERT;
+    if (degree[s] == 0) return ORA_EARG;
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++) m += degree[i];
+    memset(xt, 0, sizeof(double) * n);
+    memset(rt, 0, sizeof(double) * n);
+    rt[s] = alpha;
+    const double ea = eps * alpha;
+    double *eps_vec = (double *)malloc(sizeof(double) * n);
+    int64_t *F = (int64_t *)malloc(sizeof(int64_t) * n);
+    int64_t *G = (int64_t *)malloc(sizeof(int64_t) * n);
+    int64_t *B = (int64_t *)malloc(sizeof(int64_t) * n);
+    double *res = (double *)malloc(sizeof(double) * n);
+    int64_t *coff = (int64_t *)calloc(ncol + 1, sizeof(int64_t));
+    uint8_t *mark = (uint8_t *)calloc(n, 1);
+    if (!eps_vec || !F || !G || !B || !res || !coff || !mark) {
+        free(eps_vec); free(F); free(G); free(B); free(res); free(coff); free(mark);
+        return ORA_ENOMEM;
+    }
+    for (int64_t i = 0; i < n; i++) eps_vec[i] = ea * (double)degree[i];
+    int64_t nf = 0, oper = 0, iters = 0, nodes = 0;
+    if (local) {
+        F[nf++] = s;
+    } else {
+        for (int64_t u = 0; u < n; u++)
+            if (degree[u] > 0) F[nf++] = u;
+    }
+    int rc = ORA_OK;
+    while (nf > 0) {
+        if (iters >= max_iters) { rc = ORA_EARG; break; }
+        iters++;
+        /* stable bucket of F by colour */
+        memset(coff, 0, sizeof(int64_t) * (ncol + 1));
+        for (int64_t i = 0; i < nf; i++) coff[colour[F[i]] + 1]++;
+        for (int64_t c = 0; c < ncol; c++) coff[c + 1] += coff[c];
+        for (int64_t i = 0; i < nf; i++) B[coff[colour[F[i]]]++] = F[i];
+        for (int64_t c = ncol; c > 0; c--) coff[c] = coff[c - 1];
+        coff[0] = 0;
+        int64_t ng = 0;
+        for (int64_t c = 0; c < ncol; c++) {
+            for (int64_t i = coff[c]; i < coff[c + 1]; i++) {
+                const int64_t u = B[i];
+                res[i] = 0.0;
+                if (local && eps_vec[u] > fabs(rt[u])) continue;
+                if (local) { oper += degree[u]; nodes++; }
+                const double rs = omega * rt[u];
+                res[i] = rs;
+                const double push_res = (1. - alpha) * rs / (double)degree[u];
+                for (int64_t j = indptr[u]; j < indptr[u] + degree[u]; j++) {
+                    const int64_t v = indices[j];
+                    rt[v] += push_res;
+                    if (local && !mark[v]) { mark[v] = 1; G[ng++] = v; }   /* touched */
+                }
+            }
+            for (int64_t i = coff[c]; i < coff[c + 1]; i++) {
+                const int64_t u = B[i];
+                if (res[i] == 0.0) continue;   /* inactive (local) or zero residual */
+                xt[u] += res[i];
+                rt[u] -= res[i];
+                if (local && !mark[u]) { mark[u] = 1; G[ng++] = u; }
+            }
+        }
+        if (local) {
+            nf = 0;
+            for (int64_t i = 0; i < ng; i++) {
+                const int64_t v = G[i];
+                mark[v] = 0;
+                if (degree[v] > 0 && eps_vec[v] <= fabs(rt[v])) F[nf++] = v;
+            }
+        } else {
+            oper += n + m;
+            int64_t act = 0;
+            for (int64_t i = 0; i < n; i++)
+                if (degree[i] > 0 && eps_vec[i] <= fabs(rt[i])) act++;
+            if (act == 0) nf = 0;
+        }
+    }
+    free(eps_vec); free(F); free(G); free(B); free(res); free(coff); free(mark);
+    if (oper_out) *oper_out = oper;
+    if (iters_out) *iters_out = iters;
+    if (nodes_out) *nodes_out = nodes;
+    return rc;
+}

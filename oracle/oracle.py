@@ -19,6 +19,14 @@ reference function:
                                              truncated Taylor series
   ignn_column     InstantGNN.py:12-71        pinned: invariant p + a(I-(1-a)M)^-1 r
   update_graph    InstantGNN.py:137-180      pinned: resulting adjacency sets
+  ppr_sor         ppr_solver.py:331-461      pinned: one global sweep = the dense SOR
+                                             update in natural order, invariants,
+                                             hand-traced path graph
+  colour          (this build's GPU colouring, restated) pinned: proper colouring,
+                                             first-fit minimality, rounds by hand
+  ppr_sor_coloured (this build's GPU SOR, restated)  pinned: one global sweep = the
+                                             dense SOR update in colour order,
+                                             certificate vs. ppr_sor's solution
   ignn_update     InstantGNN.py:298-329      pinned: invariant on the new graph
 The reference itself cannot run here (numba / cupy are not importable:
 an ordinary ImportError, not a denial), so the pins are the mathematics and
@@ -77,6 +85,11 @@ def lib():
         L.ora_ignn_update_column.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, vp, vp, vp, i64, vp]
         L.ora_ppr_batch.argtypes = [i64, vp, vp, vp, vp, i64, dbl, dbl, i32, i32, vp, vp]
         L.ora_max_threads.restype = i32
+        L.ora_ppr_sor.argtypes = [i64, vp, vp, vp, i64, dbl, dbl, dbl, i32, vp, vp, vp, vp, vp]
+        L.ora_colour.argtypes = [i64, vp, vp, vp, vp, vp]
+        L.ora_colour.restype = i64
+        L.ora_ppr_sor_coloured.argtypes = [i64, vp, vp, vp, vp, i64, i64, dbl, dbl, dbl, i32, i64, vp, vp, vp, vp,
+                                           vp]
         L.ora_near_threshold.argtypes = []
         L.ora_near_threshold.restype = i64
         _lib = L
@@ -265,6 +278,52 @@ def ppr_batch(g, seeds, alpha, eps, kind=0, nthreads=0):
     if rc:
         raise RuntimeError(f"oracle error {rc}")
     return opers, times
+
+
+def ppr_sor(g, s, alpha, eps, omega, local=True):
+    """The reference's ppr_sor (local FIFO or natural-order sweeps)."""
+    indptr, indices, degree = _csr(g)
+    x = np.zeros(g.n)
+    r = np.zeros(g.n)
+    oper = np.zeros(1)
+    iters = np.zeros(1, dtype=np.int64)
+    t = np.zeros(1)
+    rc = lib().ora_ppr_sor(g.n, _p(indptr), _p(indices), _p(degree), int(s), float(alpha), float(eps),
+                           float(omega), int(local), _p(x), _p(r), _p(oper), _p(iters), _p(t))
+    if rc == ORA_EASSERT:
+        raise AssertionError("eps * degree[s_node] <= 1.")
+    if rc != ORA_OK:
+        raise RuntimeError(f"oracle error {rc}")
+    return {"x": x, "r": r, "oper": float(oper[0]), "iters": int(iters[0]), "time": float(t[0])}
+
+
+def colour(g):
+    """Jones-Plassmann colouring restated from the GPU variant:
+    (colour[n] int32, number of colours, rounds)."""
+    indptr, indices, degree = _csr(g)
+    col = np.zeros(g.n, dtype=np.int32)
+    rounds = np.zeros(1, dtype=np.int64)
+    k = lib().ora_colour(g.n, _p(indptr), _p(indices), _p(degree), _p(col), _p(rounds))
+    if k < 0:
+        raise MemoryError("oracle colouring")
+    return col, int(k), int(rounds[0])
+
+
+def ppr_sor_coloured(g, s, alpha, eps, omega, col, ncol, local=True, max_iters=100000):
+    """Sequential restatement of the GPU coloured SOR push."""
+    indptr, indices, degree = _csr(g)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    x = np.zeros(g.n)
+    r = np.zeros(g.n)
+    out = np.zeros(3, dtype=np.int64)
+    rc = lib().ora_ppr_sor_coloured(g.n, _p(indptr), _p(indices), _p(degree), _p(col), int(ncol), int(s),
+                                    float(alpha), float(eps), float(omega), int(local), int(max_iters),
+                                    _p(x), _p(r), _p(out[0:]), _p(out[1:]), _p(out[2:]))
+    if rc == ORA_EASSERT:
+        raise AssertionError("eps * degree[s_node] <= 1.")
+    if rc != ORA_OK:
+        raise RuntimeError(f"oracle error {rc}")
+    return {"x": x, "r": r, "oper": int(out[0]), "iters": int(out[1]), "nodes": int(out[2])}
 
 
 def max_threads() -> int:

@@ -18,6 +18,10 @@ EXPECTED = {
                                           "n indptr indices degree alpha eps omega graph_name s_nodes"),
     ("ppr_solver", "solve_a_graph_cheby"): ("ppr_solver.py:665",
                                             "n indptr indices degree alpha eps omega graph_name s_nodes"),
+    ("ppr_solver", "ppr_sor"): ("ppr_solver.py:332",
+                                "n indptr indices degree s_node alpha eps omega local opt_x debug"),
+    ("ppr_solver", "solve_a_graph_omega"): ("ppr_solver.py:465",
+                                            "n indptr indices degree alpha eps omega graph_name s_nodes"),
     # katz_solver.py
     ("katz_solver", "katz_gd"): ("katz_solver.py:168", "n indptr indices degree s_node alpha eps local"),
     ("katz_solver", "katz_cheby"): ("katz_solver.py:341",

@@ -1,0 +1,218 @@
+"""Pins of the SOR oracle functions (CPU only).
+
+* ``ppr_sor`` restates the reference's SOR / Gauss-Southwell push
+  (ppr_solver.py:331-461).  Pinned by a hand trace (FIFO with the iteration
+  flag, the float op count) and, for the global sweeps, by the matrix form of
+  SOR: with Q = I + L + U (L strictly lower in the sweep order) one sweep is
+  x <- (I + w L)^-1 [((1 - w) I - w U) x + w b], iterated with dense numpy
+  until the reference's stopping test holds.
+* ``colour`` / ``ppr_sor_coloured`` restate the GPU's coloured variant.
+  Pinned by properness and first-fit minimality of the colouring, a hand
+  trace, and the same matrix form with L taken in colour order.
+* Both end on the certificate |r_u| < eps*alpha*d_u with r == b - Q x.
+"""
+import numpy as np
+import pytest
+
+from flsg.graph import CSRGraph
+from oracle import oracle as O
+
+OMEGA_OPT = 1.3928644583850192   # demo.py:31-35 at alpha = 0.1
+
+
+def path3():
+    return CSRGraph.from_edges(3, np.array([0, 1]), np.array([1, 2]))
+
+
+def random_graph(n=40, p=0.12, seed=0):
+    rng = np.random.default_rng(seed)
+    iu = np.triu_indices(n, 1)
+    keep = rng.random(iu[0].size) < p
+    u = np.concatenate([iu[0][keep], np.arange(n - 1)])
+    v = np.concatenate([iu[1][keep], np.arange(1, n)])
+    return CSRGraph.from_edges(n, u, v)
+
+
+def rows(g):
+    return np.repeat(np.arange(g.n), np.asarray(g.degree, dtype=np.int64))
+
+
+def sor_matrix_form(g, s, alpha, eps, omega, order_key):
+    """Dense SOR on Q x = alpha e_s in the sweep order given by order_key
+    (row u is updated before row v when order_key[u] < order_key[v]; equal
+    keys are never adjacent).  Returns (x, sweeps)."""
+    Q = O.ppr_operator(g, alpha).toarray()
+    n = g.n
+    b = np.zeros(n)
+    b[s] = alpha
+    key = np.asarray(order_key)
+    before = key[None, :] < key[:, None]          # column v is updated before row u
+    L = np.where(before, Q, 0.0)
+    np.fill_diagonal(L, 0.0)
+    U = Q - np.eye(n) - L
+    M = np.eye(n) + omega * L
+    N = (1.0 - omega) * np.eye(n) - omega * U
+    thr = eps * alpha * np.asarray(g.degree, dtype=np.float64)
+    x = np.zeros(n)
+    for sweeps in range(1, 100000):
+        x = np.linalg.solve(M, N @ x + omega * b)
+        r = b - Q @ x
+        if not np.any(thr <= np.abs(r)):
+            return x, sweeps
+    raise RuntimeError("no convergence")
+
+
+# ------------------------------------------------------------------ reference ppr_sor
+def test_ppr_sor_hand_traced_path():
+    """P3, seed 0, alpha = 1/2, eps = 1/100, omega = 1 (Gauss-Southwell).
+    Thresholds (1/200) d = [.005, .01, .005].  Pops: flag, 0 (r0 = .5 ->
+    r1 = .25), flag, 1 (-> r0 = r2 = .0625), flag, 0 (-> r1 = .03125),
+    2 (-> r1 = .0625), flag, 1 (-> r0 = r2 = .015625), flag, 0 (-> r1 =
+    .0078125 < .01, not queued), 2 (-> r1 = .015625, queued), flag, 1
+    (-> r0 = r2 = .00390625 < .005), flag with the queue empty: stop.
+    oper = 1+2+1+1+2+1+1+2 = 11 (a float), 6 iterations."""
+    o = O.ppr_sor(path3(), 0, 0.5, 0.01, 1.0, local=True)
+    assert o["oper"] == 11.0 and o["iters"] == 6
+    assert np.array_equal(o["x"], [0.578125, 0.328125, 0.078125])
+    assert np.array_equal(o["r"], [0.00390625, 0.0, 0.00390625])
+
+
+def test_ppr_sor_overrelaxed_step_by_hand():
+    """omega = 3/2 on P3 (alpha = 1/2, eps = 1/100): the first push moves
+    res = .75 out of r0 = .5, leaving r0 = -.25, which is still active
+    (|r0| >= .005) and is re-queued after the push (ppr_solver.py:415-419)."""
+    o = O.ppr_sor(path3(), 0, 0.5, 0.01, 1.5, local=True)
+    # first pops by hand: 0 -> r0 = -.25, r1 = .375 (queue 1, 0); both active
+    assert o["oper"] >= 1 + 2 + 1
+    r = O.ppr_residual(path3(), 0, 0.5, o["x"])
+    assert np.abs(r - o["r"]).max() < 1e-15
+    assert np.all(np.abs(r) < 0.005 * np.array([1, 2, 1]))
+
+
+@pytest.mark.parametrize("omega", [1.0, OMEGA_OPT])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_ppr_sor_global_matches_matrix_form(seed, omega):
+    g = random_graph(seed=seed)
+    s = 3
+    o = O.ppr_sor(g, s, 0.1, 1e-4, omega, local=False)
+    x, sweeps = sor_matrix_form(g, s, 0.1, 1e-4, omega, np.arange(g.n))
+    assert o["iters"] == sweeps
+    assert o["oper"] == sweeps * (g.n + int(g.degree.sum()))
+    assert np.abs(o["x"] - x).sum() <= 1e-12 * np.abs(x).sum()
+
+
+def test_jacobi_order_would_fail_the_matrix_form():
+    """The matrix-form pin distinguishes sweep orders: natural order and the
+    reversed order give different iterates on the same graph."""
+    g = random_graph(seed=2)
+    x1, k1 = sor_matrix_form(g, 3, 0.1, 1e-4, OMEGA_OPT, np.arange(g.n))
+    x2, k2 = sor_matrix_form(g, 3, 0.1, 1e-4, OMEGA_OPT, -np.arange(g.n))
+    o = O.ppr_sor(g, 3, 0.1, 1e-4, OMEGA_OPT, local=False)
+    assert np.abs(o["x"] - x1).sum() < 1e-12 < np.abs(o["x"] - x2).sum()
+
+
+@pytest.mark.parametrize("omega", [0.7, 1.0, OMEGA_OPT, 1.8])
+@pytest.mark.parametrize("local", [True, False])
+def test_ppr_sor_certificate_and_identity(local, omega):
+    g = random_graph(n=60, p=0.08, seed=5)
+    s = 7
+    o = O.ppr_sor(g, s, 0.1, 1e-5, omega, local=local)
+    r = O.ppr_residual(g, s, 0.1, o["x"])
+    assert np.abs(r - o["r"]).max() < 1e-15
+    thr = 1e-6 * g.degree
+    assert np.all(np.abs(r) < thr)
+    # x* - x = Q^-1 r exactly
+    xs = O.ppr_dense_solve(g, s, 0.1)
+    Q = O.ppr_operator(g, 0.1).toarray()
+    assert np.abs((xs - o["x"]) - np.linalg.solve(Q, r)).max() < 1e-14
+
+
+def test_ppr_sor_assert():
+    with pytest.raises(AssertionError):
+        O.ppr_sor(path3(), 1, 0.1, 0.6, 1.0, local=True)
+
+
+# ------------------------------------------------------------------ colouring
+def check_colouring(g, col, k):
+    r = rows(g)
+    c = np.asarray(col)
+    assert c.min() >= 0 and c.max() == k - 1
+    loops = r == g.indices
+    assert not np.any((c[r] == c[g.indices]) & ~loops), "adjacent nodes share a colour"
+    # first fit: a node of colour c has neighbours of every colour below c
+    for u in range(g.n):
+        nb = set(c[g.indices[g.indptr[u]:g.indptr[u] + g.degree[u]]].tolist()) - {c[u]}
+        assert set(range(c[u])) <= nb, u
+
+
+def test_colour_path_by_hand():
+    """P3: node 1 (degree 2) has the highest key and takes colour 0 in round
+    1; nodes 0 and 2 take colour 1 in round 2."""
+    col, k, rounds = O.colour(path3())
+    assert list(col) == [1, 0, 1] and k == 2 and rounds == 2
+
+
+def test_colour_clique_and_isolated():
+    """K5 plus two isolated nodes: five colours, one per round (keys are
+    distinct); isolated nodes take colour 0 without a round."""
+    u, v = np.triu_indices(5, 1)
+    g = CSRGraph.from_edges(7, u, v)
+    col, k, rounds = O.colour(g)
+    assert sorted(col[:5]) == [0, 1, 2, 3, 4] and list(col[5:]) == [0, 0]
+    assert k == 5 and rounds == 5
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_colour_proper_first_fit(seed):
+    g = random_graph(n=80, p=0.1, seed=seed)
+    col, k, rounds = O.colour(g)
+    check_colouring(g, col, k)
+    assert rounds >= k
+
+
+def test_colour_powerlaw_small(small_graph):
+    col, k, rounds = O.colour(small_graph)
+    check_colouring(small_graph, col, k)
+
+
+# ------------------------------------------------------------------ coloured SOR
+def test_coloured_sor_hand_traced_path():
+    """P3 with colours [1, 0, 1], omega = 1, same parameters as the FIFO
+    trace: the frontier passes are [0], [1], [0, 2], [1], [0, 2], [1] and the
+    values equal the FIFO run's (every push here sees the same residuals)."""
+    g = path3()
+    col, k, _ = O.colour(g)
+    o = O.ppr_sor_coloured(g, 0, 0.5, 0.01, 1.0, col, k, local=True)
+    assert o["oper"] == 11 and o["iters"] == 6 and o["nodes"] == 8
+    assert np.array_equal(o["x"], [0.578125, 0.328125, 0.078125])
+    assert np.array_equal(o["r"], [0.00390625, 0.0, 0.00390625])
+
+
+@pytest.mark.parametrize("omega", [1.0, OMEGA_OPT])
+@pytest.mark.parametrize("seed", [0, 3])
+def test_coloured_sor_global_matches_matrix_form(seed, omega):
+    g = random_graph(seed=seed)
+    col, k, _ = O.colour(g)
+    o = O.ppr_sor_coloured(g, 3, 0.1, 1e-4, omega, col, k, local=False)
+    x, sweeps = sor_matrix_form(g, 3, 0.1, 1e-4, omega, col)
+    assert o["iters"] == sweeps
+    assert np.abs(o["x"] - x).sum() <= 1e-12 * np.abs(x).sum()
+
+
+@pytest.mark.parametrize("omega", [1.0, OMEGA_OPT, 1.8])
+@pytest.mark.parametrize("local", [True, False])
+def test_coloured_sor_certificate_and_reference_distance(local, omega, small_graph):
+    g = small_graph
+    col, k, _ = O.colour(g)
+    s = 17 if g.degree[17] > 0 else int(np.argmax(g.degree > 0))
+    a = O.ppr_sor(g, s, 0.1, 1e-6, omega, local=local)
+    b = O.ppr_sor_coloured(g, s, 0.1, 1e-6, omega, col, k, local=local)
+    live = g.degree > 0
+    thr = 1e-7 * g.degree
+    for o in (a, b):
+        r = O.ppr_residual(g, s, 0.1, o["x"])
+        assert np.abs(r - o["r"]).max() < 1e-15
+        assert np.all(np.abs(r[live]) < thr[live])
+    # both certified: ||x_a - x_b||_1 <= (||r_a||_1 + ||r_b||_1) / alpha
+    bound = (np.abs(a["r"]).sum() + np.abs(b["r"]).sum()) / 0.1
+    assert np.abs(a["x"] - b["x"]).sum() <= bound
