@@ -1,7 +1,8 @@
-// flsg_sor.hip -- graph-coloured SOR push for PPR: the GPU counterpart of the
-// reference's ppr_sor (ppr_solver.py:331-461), local (frontier) and global
-// (sweeps), any omega in (0, 2) (omega = 1 is the Gauss-Southwell / APPR push
-// of solve_a_graph_all, ppr_solver.py:559-570).
+// flsg_sor.hip -- graph-coloured SOR push: the GPU counterpart of the
+// reference's ppr_sor (ppr_solver.py:331-461) and katz_sor
+// (katz_solver.py:252-337), local (frontier) and global (sweeps), any omega
+// in (0, 2) (omega = 1 is the Gauss-Southwell / APPR push of the
+// solve_a_graph_all drivers, ppr_solver.py:559-570, katz_solver.py:630-640).
 //
 // The reference's SOR is sequential: each push reads the residual the
 // previous pushes left.  Two neighbours pushing at once would each move their
@@ -23,13 +24,15 @@
 //     which has colour c, so r_u of colour c is stable during the launch and
 //     hub pieces recompute the same push value -- and applies the previous
 //     colour's x_u += res, r_u -= res (atomically: the colour-c pushes add to
-//     those residuals in the same launch).  A residual's first change in the
-//     iteration lists its node (a mark dedups); after the last colour the
-//     listed nodes with eps*alpha*d_v <= |r_v| form the next frontier.  That
-//     is the active set after the sweep whatever order the adds of one colour
+//     those residuals in the same launch).  The pushes are non-returning
+//     atomics with kSU edges in flight per thread; nothing waits on them.
+//   * after the last colour a check pass walks the pushed nodes and their
+//     rows again (gathers, no atomics) and lists each one that is active
+//     (eps*alpha*d_v <= |r_v|, a mark dedups) as the next frontier.  That is
+//     the active set after the sweep whatever order the adds of one colour
 //     took (residuals change sign for omega > 1, so a test after every add
-//     would depend on it), and every node whose residual did not change was
-//     inactive already, so an empty frontier certifies |r_v| < eps*alpha*d_v.
+//     would depend on it); only pushed nodes and their neighbours changed,
+//     so an empty frontier certifies |r_v| < eps*alpha*d_v everywhere.
 // One iteration of the local variant is a Gauss-Seidel sweep over the frontier
 // in colour order (the reference's FIFO order differs), the global variant a
 // full multicolour SOR sweep; both end on the reference's certificate.
@@ -44,8 +47,9 @@
 namespace flsg {
 
 constexpr int kSNT = 256;        // threads per block
-constexpr int kSHub = 4096;      // entries with more edges are pushed in pieces
-constexpr int kSPiece = 4096;    // edges per piece
+constexpr int kSHub = 128;       // entries with more edges are pushed in pieces, so one
+                                 // 256-entry chunk holds at most 32K edges
+constexpr int kSPiece = 1024;    // edges per piece (one round of kSU edges per thread)
 constexpr int kColLong = 64;     // colouring: longer rows are scanned by a wavefront
 constexpr int kLdsBins = 1024;   // bucket kernels: colours below this use LDS histograms
 constexpr int kColRoundsPerCheck = 8;
@@ -60,7 +64,7 @@ struct SorCtr {
     unsigned int act;            // global: live nodes at or above the threshold
     unsigned int selfloop;       // a row holds its own node
     unsigned int nlong;          // colouring: rows longer than kColLong
-    unsigned int ntl;            // local: nodes whose residual changed this iteration
+    unsigned int pad;
 };
 
 struct SorWs {
@@ -82,8 +86,7 @@ struct SorWs {
     int32_t *front[2] = {nullptr, nullptr};
     int32_t *bucket = nullptr;       // n entries by colour
     double *res = nullptr;           // n: omega r_u of each entry (0 = not pushed)
-    int32_t *mark = nullptr;         // n: residual changed this iteration (listed in tl)
-    int32_t *tl = nullptr;           // n: those nodes
+    int32_t *mark = nullptr;         // n: listed in the next frontier
     int64_t *pieces = nullptr;       // (entry << 32) | piece
     int64_t pieces_cap = 0;
     // every live node by colour (global sweeps)
@@ -110,7 +113,6 @@ void sor_free(flsg_graph g) {
     (void)hipFree(w->bucket);
     (void)hipFree(w->res);
     (void)hipFree(w->mark);
-    (void)hipFree(w->tl);
     (void)hipFree(w->pieces);
     (void)hipFree(w->ball);
     (void)hipFree(w->pall);
@@ -313,23 +315,21 @@ __device__ __forceinline__ unsigned int n_pieces(int d) {
     return d > kSHub ? (unsigned int)((d + kSPiece - 1) / kSPiece) : 0u;
 }
 
-// counts per colour (entries, pieces); zeroes *zero_a and *zero_b (the next
-// frontier's and the touched list's counters) when given
+// counts per colour (entries, pieces); clears the entries' marks and zeroes
+// *zero_me (the next frontier's counter) when given
 __global__ __launch_bounds__(kSNT) void k_bkt_count(const int32_t *F, const unsigned int *nfp, unsigned int nF_all,
                                                     const int32_t *colour, const int32_t *degree,
-                                                    unsigned int *cnts, int ncol, unsigned int *zero_a,
-                                                    unsigned int *zero_b) {
+                                                    unsigned int *cnts, int ncol, int32_t *mark,
+                                                    unsigned int *zero_me) {
     __shared__ unsigned int h[kLdsBins];
     const unsigned int nF = nfp ? *nfp : nF_all;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (zero_a) *zero_a = 0;
-        if (zero_b) *zero_b = 0;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && zero_me) *zero_me = 0;
     for (int b = threadIdx.x; b < kLdsBins; b += kSNT) h[b] = 0;
     __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * kSNT + threadIdx.x; i < (int64_t)nF; i += (int64_t)gridDim.x * kSNT) {
         int u;
         if (!entry_at(F, nF, degree, i, u)) continue;
+        if (mark) mark[u] = 0;
         const int c = colour[u];
         if (c < kLdsBins) atomicAdd(&h[c], 1u);
         else atomicAdd(&cnts[c], 1u);
@@ -417,52 +417,60 @@ struct SorArgs {
     const int32_t *B;        // entries by colour
     const int64_t *pieces;   // hub pieces by colour
     double *res;             // per entry: omega r_u pushed (0 = not pushed)
-    int32_t *mark;           // local: residual changed this iteration
-    int32_t *tl;             // local: those nodes
-    unsigned int *ntl;       // local: their count
+    int32_t *mark;           // local: listed in the next frontier
+    int32_t *next;           // local: next frontier
+    unsigned int *nnext;     // local: its counter
     SorCtr *ctr;
-    double om_alpha;         // 1 - alpha
+    double coef;             // PPR: 1 - alpha (push = coef*res/d_u); Katz: beta (push = coef*res)
     double omega;
-    double ea;               // eps * alpha
+    double ea;               // threshold per unit degree: PPR eps*alpha, Katz eps
     int local;
+    int katz;
 };
 
-// first change of v's residual in this iteration (local)
-__device__ __forceinline__ bool first_touch(const SorArgs &a, int v) {
-    return a.mark[v] == 0 && atomicExch(&a.mark[v], 1) == 0;
+constexpr int kSU = 4;   // edges in flight per thread
+
+enum { kPush = 0, kCheck = 1 };
+
+__device__ __forceinline__ double sor_push_value(const SorArgs &a, double rs, int d) {
+    return a.katz ? a.coef * rs                  // katz_solver.py:302
+                  : a.coef * rs / (double)d;     // ppr_solver.py:407
 }
 
-// One colour: finalize entries [fbeg, fend) of the previous colour, push the
-// active entries [beg, end) (chunks of kSNT, their edges flattened by an LDS
-// prefix sum) and the hub pieces [pbeg, pend).
-__global__ __launch_bounds__(kSNT) void k_sor_colour(SorArgs a, unsigned int beg, unsigned int end,
-                                                     unsigned int pbeg, unsigned int pend, unsigned int fbeg,
-                                                     unsigned int fend) {
+// v active at the end of the sweep, listed once (mark)
+__device__ __forceinline__ bool active_claim(const SorArgs &a, int v) {
+    const int d = a.degree[v];
+    return d > 0 && a.ea * (double)d <= fabs(a.r[v]) && a.mark[v] == 0 && atomicExch(&a.mark[v], 1) == 0;
+}
+
+// Work over bucketed entries [beg, end) (chunks of kSNT, their edges
+// flattened by an LDS prefix sum) and hub pieces [pbeg, pend):
+//   kPush  (one colour): finalize entries [fbeg, fend) of the previous colour
+//          (x_u += res, r_u -= res), then push the entries still active --
+//          res = omega r_u, its push value atomically added to every neighbour
+//          (non-returning atomics, kSU edges in flight per thread);
+//   kCheck (local, after the last colour, over every entry of the iteration):
+//          each pushed node and each neighbour of one -- every node whose
+//          residual changed -- joins the next frontier if active.
+template <int MODE>
+__global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg, unsigned int end,
+                                                    unsigned int pbeg, unsigned int pend, unsigned int fbeg,
+                                                    unsigned int fend) {
     __shared__ int s_incl[kSNT];
     __shared__ int s_start[kSNT];
     __shared__ double s_val[kSNT];
     __shared__ int s_sh[kSNT / kWave];
     const int tid = threadIdx.x;
-    // (1) the previous colour: x_u += res, r_u -= res
-    for (unsigned int base = fbeg + blockIdx.x * kSNT; base < fend; base += gridDim.x * kSNT) {
-        const unsigned int i = base + tid;
-        bool enq = false;
-        int u = 0;
-        if (i < fend) {
+    if (MODE == kPush) {
+        for (unsigned int i = fbeg + blockIdx.x * kSNT + tid; i < fend; i += gridDim.x * kSNT) {
             const double rs = a.res[i];
             if (rs != 0.0) {
-                u = a.B[i];
-                a.x[u] += rs;
-                (void)atomic_add_dev(&a.r[u], -rs);
-                if (a.local) enq = first_touch(a, u);
+                const int u = a.B[i];
+                a.x[u] += rs;                                // ppr_solver.py:405
+                (void)atomic_add_dev(&a.r[u], -rs);          // ppr_solver.py:406
             }
         }
-        if (a.local) {
-            const long long slot = wave_append(enq, a.ntl);
-            if (slot >= 0) a.tl[slot] = u;
-        }
     }
-    // (2) this colour
     const unsigned int nchunks = (end - beg + kSNT - 1) / kSNT;
     const unsigned int nitems = nchunks + (pend - pbeg);
     unsigned long long oper = 0, nodes = 0;
@@ -473,23 +481,37 @@ __global__ __launch_bounds__(kSNT) void k_sor_colour(SorArgs a, unsigned int beg
         if (chunk) {
             const unsigned int i = beg + item * kSNT + tid;
             int cnt = 0;
+            bool self = false;
+            int u = 0;
             if (i < end) {
-                const int u = a.B[i];
+                u = a.B[i];
                 const int d = a.degree[u];
-                const double ru = a.r[u];
-                const bool act = d > 0 && (a.local ? (a.ea * (double)d <= fabs(ru)) : (ru != 0.0));
-                double rs = 0.0;
+                bool act;
+                if (MODE == kPush) {
+                    const double ru = a.r[u];
+                    act = d > 0 && (a.local ? (a.ea * (double)d <= fabs(ru)) : (ru != 0.0));
+                    double rs = 0.0;
+                    if (act) {
+                        rs = a.omega * ru;                   // ppr_solver.py:404, katz_solver.py:299
+                        val = sor_push_value(a, rs, d);
+                        if (a.local) {
+                            oper += (unsigned long long)d;   // ppr_solver.py:401
+                            nodes++;
+                        }
+                    }
+                    a.res[i] = rs;
+                } else {
+                    act = a.res[i] != 0.0;
+                    self = act && active_claim(a, u);
+                }
                 if (act) {
-                    rs = a.omega * ru;                        // ppr_solver.py:404
-                    val = a.om_alpha * rs / (double)d;        // ppr_solver.py:407
                     st = a.indptr[u];
                     cnt = d > kSHub ? 0 : d;
-                    if (a.local) {
-                        oper += (unsigned long long)d;        // ppr_solver.py:401
-                        nodes++;
-                    }
                 }
-                a.res[i] = rs;
+            }
+            if (MODE == kCheck) {
+                const long long slot = wave_append(self, a.nnext);
+                if (slot >= 0) a.next[slot] = u;
             }
             int tot;
             const int incl = block_incl_scan<kSNT>(cnt, s_sh, &tot);
@@ -504,39 +526,57 @@ __global__ __launch_bounds__(kSNT) void k_sor_colour(SorArgs a, unsigned int beg
             const int k = (int)(pc & 0xffffffffLL);
             const int u = a.B[i];
             const int d = a.degree[u];
-            const double ru = a.r[u];
-            const bool act = d > 0 && (a.local ? (a.ea * (double)d <= fabs(ru)) : (ru != 0.0));
-            if (!act) continue;   // uniform: every thread read the same entry
-            val = a.om_alpha * (a.omega * ru) / (double)d;   // bit-equal to the chunk's value
+            if (MODE == kPush) {
+                const double ru = a.r[u];
+                const bool act = d > 0 && (a.local ? (a.ea * (double)d <= fabs(ru)) : (ru != 0.0));
+                if (!act) continue;   // uniform: every thread read the same entry
+                val = sor_push_value(a, a.omega * ru, d);   // bit-equal to the chunk's value
+            } else if (a.res[i] == 0.0) {
+                continue;
+            }
             st = a.indptr[u];
             e_beg = k * kSPiece;
             e_end = min(d, e_beg + kSPiece);
         }
-        for (int e0 = e_beg; e0 < e_end; e0 += kSNT) {
-            const int e = e0 + tid;
-            bool enq = false;
-            int v = 0;
-            if (e < e_end) {
-                double vv = val;
-                if (chunk) {
-                    const int k = upper_bound_lds(s_incl, 0, kSNT, e);
-                    const int o = e - (k ? s_incl[k - 1] : 0);
-                    v = a.indices[s_start[k] + o];
-                    vv = s_val[k];
-                } else {
-                    v = a.indices[st + e];
+        for (int e0 = e_beg; e0 < e_end; e0 += kSNT * kSU) {
+            int v[kSU];
+            double vv[kSU];
+            bool ok[kSU];
+#pragma unroll
+            for (int q = 0; q < kSU; q++) {
+                const int e = e0 + q * kSNT + tid;
+                ok[q] = e < e_end;
+                v[q] = 0;
+                vv[q] = val;
+                if (ok[q]) {
+                    if (chunk) {
+                        const int k = upper_bound_lds(s_incl, 0, kSNT, e);
+                        const int o = e - (k ? s_incl[k - 1] : 0);
+                        v[q] = a.indices[s_start[k] + o];
+                        vv[q] = s_val[k];
+                    } else {
+                        v[q] = a.indices[st + e];
+                    }
                 }
-                (void)atomic_add_dev(&a.r[v], vv);   // ppr_solver.py:409
-                if (a.local) enq = first_touch(a, v);
             }
-            if (a.local) {
-                const long long slot = wave_append(enq, a.ntl);
-                if (slot >= 0) a.tl[slot] = v;
+            if (MODE == kPush) {
+#pragma unroll
+                for (int q = 0; q < kSU; q++)
+                    if (ok[q]) (void)atomic_add_dev(&a.r[v[q]], vv[q]);   // ppr_solver.py:409
+            } else {
+                bool enq[kSU];
+#pragma unroll
+                for (int q = 0; q < kSU; q++) enq[q] = ok[q] && active_claim(a, v[q]);
+#pragma unroll
+                for (int q = 0; q < kSU; q++) {
+                    const long long slot = wave_append(enq[q], a.nnext);
+                    if (slot >= 0) a.next[slot] = v[q];
+                }
             }
         }
         if (chunk) __syncthreads();   // s_incl / s_start / s_val are rewritten by the next chunk
     }
-    if (a.local) {
+    if (MODE == kPush && a.local) {
         oper = wave_sum(oper);
         nodes = wave_sum(nodes);
         if (__lane_id() == 0 && nodes) {
@@ -546,34 +586,15 @@ __global__ __launch_bounds__(kSNT) void k_sor_colour(SorArgs a, unsigned int beg
     }
 }
 
-__global__ void k_sor_init(double *r, int32_t *F, SorCtr *c, int64_t s, double alpha, int local) {
-    r[s] = alpha;   // ppr_solver.py:339
+__global__ void k_sor_init(double *r, int32_t *F, SorCtr *c, int64_t s, double r0, int local) {
+    r[s] = r0;   // ppr_solver.py:339 (alpha), katz_solver.py:258 (1)
     if (local) {
         F[0] = (int32_t)s;
         c->nf[0] = 1;
     }
 }
 
-// end of a local iteration: the touched nodes still at or above their
-// threshold form the next frontier; their marks are cleared
-__global__ __launch_bounds__(kSNT) void k_sor_frontier(const int32_t *tl, const unsigned int *ntl, int32_t *mark,
-                                                       const double *r, const int32_t *degree, double ea,
-                                                       int32_t *next, unsigned int *nnext) {
-    const unsigned int cnt = *ntl;
-    for (unsigned int base = blockIdx.x * kSNT; base < cnt; base += gridDim.x * kSNT) {
-        const unsigned int i = base + threadIdx.x;
-        bool act = false;
-        int v = 0;
-        if (i < cnt) {
-            v = tl[i];
-            mark[v] = 0;
-            const int d = degree[v];
-            act = d > 0 && ea * (double)d <= fabs(r[v]);
-        }
-        const long long slot = wave_append(act, nnext);
-        if (slot >= 0) next[slot] = v;
-    }
-}
+__global__ void k_sor_add(double *x, int64_t s, double v) { x[s] += v; }
 
 // live nodes at or above the threshold (the global loop's stopping test,
 // ppr_solver.py:449-452; isolated nodes are never counted)
@@ -603,7 +624,6 @@ static int sor_ws(flsg_graph g, SorWs **out) {
         FLSG_HIP(hipMalloc((void **)&w->bucket, sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->res, sizeof(double) * n));
         FLSG_HIP(hipMalloc((void **)&w->mark, sizeof(int32_t) * n));
-        FLSG_HIP(hipMalloc((void **)&w->tl, sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->ball, sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->ctr, sizeof(SorCtr)));
         FLSG_HIP(hipHostMalloc((void **)&w->h_ctr, sizeof(SorCtr), hipHostMallocDefault));
@@ -679,13 +699,13 @@ static int ensure_colouring(flsg_graph g, SorWs *w, hipStream_t st) {
 
 // bucket entries (F with its device count, or every live node) by colour;
 // returns the offsets on the host (entries in [0, ncol], pieces after)
-static int bucket(flsg_graph g, SorWs *w, const int32_t *F, const unsigned int *nfp, unsigned int *zero_a,
-                  unsigned int *zero_b, int32_t *B, int64_t *pieces, hipStream_t st) {
+static int bucket(flsg_graph g, SorWs *w, const int32_t *F, const unsigned int *nfp, int32_t *mark,
+                  unsigned int *zero_me, int32_t *B, int64_t *pieces, hipStream_t st) {
     const int ncol = w->ncol;
     const unsigned int nall = (unsigned int)g->n;
     const int grid = grid_of(g, (g->n + kSNT - 1) / kSNT, 4);
     hipLaunchKernelGGL(k_bkt_count, dim3(grid), dim3(kSNT), 0, st, F, nfp, nall, w->colour, g->degree, w->cnts,
-                       ncol, zero_a, zero_b);
+                       ncol, mark, zero_me);
     hipLaunchKernelGGL(k_bkt_scan, dim3(1), dim3(1024), 0, st, w->cnts, w->offs, w->curs, ncol);
     hipLaunchKernelGGL(k_bkt_scatter, dim3(grid), dim3(kSNT), 0, st, F, nfp, nall, w->colour, g->degree, w->curs,
                        ncol, B, pieces);
@@ -702,7 +722,7 @@ static int ensure_all_lists(flsg_graph g, SorWs *w, hipStream_t st) {
     const int grid = grid_of(g, (g->n + kSNT - 1) / kSNT, 4);
     hipLaunchKernelGGL(k_bkt_count, dim3(grid), dim3(kSNT), 0, st, (const int32_t *)nullptr,
                        (const unsigned int *)nullptr, (unsigned int)g->n, w->colour, g->degree, w->cnts, ncol,
-                       (unsigned int *)nullptr, (unsigned int *)nullptr);
+                       (int32_t *)nullptr, (unsigned int *)nullptr);
     hipLaunchKernelGGL(k_bkt_scan, dim3(1), dim3(1024), 0, st, w->cnts, w->offs, w->curs, ncol);
     FLSG_HIP(hipMemcpyAsync(h.data(), w->offs, sizeof(unsigned int) * h.size(), hipMemcpyDeviceToHost, st));
     FLSG_HIP(hipStreamSynchronize(st));
@@ -748,7 +768,7 @@ static int colour_pass(flsg_graph g, SorWs *w, SorArgs a, const unsigned int *of
         const int64_t items = (int64_t)(end - beg + kSNT - 1) / kSNT + (pend - pbeg);
         const int64_t fin = (int64_t)(fend - fbeg + kSNT - 1) / kSNT;
         const int grid = grid_of(g, std::max(items, fin), 8);
-        hipLaunchKernelGGL(k_sor_colour, dim3(grid), dim3(kSNT), 0, st, a, beg, end, pbeg, pend, fbeg, fend);
+        hipLaunchKernelGGL(k_sor_items<kPush>, dim3(grid), dim3(kSNT), 0, st, a, beg, end, pbeg, pend, fbeg, fend);
         fbeg = beg;
         fend = end;
     }
@@ -779,23 +799,23 @@ static int sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t s
     a.r = d_r;
     a.res = w->res;
     a.ctr = w->ctr;
-    a.om_alpha = 1. - p->alpha;
+    const bool katz = p->kind == FLSG_KATZ;
+    a.coef = katz ? p->alpha : 1. - p->alpha;
     a.omega = omega;
-    a.ea = p->eps * p->alpha;   // ppr_solver.py:340 (eps * alpha) * degree
+    a.ea = katz ? p->eps : p->eps * p->alpha;   // katz_solver.py:259 eps * degree; ppr_solver.py:340
     a.local = p->local ? 1 : 0;
+    a.katz = katz ? 1 : 0;
+    const double r0 = katz ? 1.0 : p->alpha;
     int64_t iters = 0;
     if (p->local) {
         FLSG_HIP(hipMemsetAsync(w->mark, 0, sizeof(int32_t) * n, st));
-        hipLaunchKernelGGL(k_sor_init, dim3(1), dim3(1), 0, st, d_r, w->front[0], w->ctr, s, p->alpha, 1);
+        hipLaunchKernelGGL(k_sor_init, dim3(1), dim3(1), 0, st, d_r, w->front[0], w->ctr, s, r0, 1);
         a.B = w->bucket;
         a.pieces = w->pieces;
         a.mark = w->mark;
-        a.tl = w->tl;
-        a.ntl = &w->ctr->ntl;
-        const int gridt = grid_of(g, (n + kSNT - 1) / kSNT, 4);
         int par = 0;
         for (;;) {
-            rc = bucket(g, w, w->front[par], &w->ctr->nf[par], &w->ctr->nf[par ^ 1], &w->ctr->ntl, w->bucket,
+            rc = bucket(g, w, w->front[par], &w->ctr->nf[par], w->mark, &w->ctr->nf[par ^ 1], w->bucket,
                         w->pieces, st);
             if (rc) return rc;
             if (w->h_ctr->nf[par] == 0) break;
@@ -806,14 +826,19 @@ static int sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t s
             iters++;
             rc = colour_pass(g, w, a, w->h_offs, st);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_sor_frontier, dim3(gridt), dim3(kSNT), 0, st, w->tl, &w->ctr->ntl, w->mark, d_r,
-                               g->degree, a.ea, w->front[par ^ 1], &w->ctr->nf[par ^ 1]);
+            // every entry of the iteration and all its pieces
+            const unsigned int ne = w->h_offs[w->ncol], np = w->h_offs[2 * w->ncol + 1];
+            a.next = w->front[par ^ 1];
+            a.nnext = &w->ctr->nf[par ^ 1];
+            const int gridc = grid_of(g, (int64_t)(ne + kSNT - 1) / kSNT + np, 8);
+            hipLaunchKernelGGL(k_sor_items<kCheck>, dim3(gridc), dim3(kSNT), 0, st, a, 0u, ne, 0u, np, 0u, 0u);
+            FLSG_HIP(hipGetLastError());
             par ^= 1;
         }
         res->oper = (int64_t)w->h_ctr->oper;
         res->nodes = (int64_t)w->h_ctr->nodes;
     } else {
-        hipLaunchKernelGGL(k_sor_init, dim3(1), dim3(1), 0, st, d_r, (int32_t *)nullptr, w->ctr, s, p->alpha, 0);
+        hipLaunchKernelGGL(k_sor_init, dim3(1), dim3(1), 0, st, d_r, (int32_t *)nullptr, w->ctr, s, r0, 0);
         a.B = w->ball;
         a.pieces = w->pall;
         const int grida = grid_of(g, (n + kSNT - 1) / kSNT, 4);
@@ -833,6 +858,7 @@ static int sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t s
         }
         res->oper = iters * (n + g->m_live);   // ppr_solver.py:447-448 (oper += n + m per sweep)
     }
+    if (katz) hipLaunchKernelGGL(k_sor_add, dim3(1), dim3(1), 0, st, d_x, s, -1.0);   // katz_solver.py:332
     res->iters = iters;
     return FLSG_OK;
 }
@@ -866,17 +892,17 @@ extern "C" int flsg_sor_solve(flsg_graph g, const flsg_params *p, double omega, 
     if (rc) return rc;
     if (!p || !d_x || !d_r || !h_res) return fail(FLSG_EARG, "NULL argument");
     if (seed < 0 || seed >= g->n) return fail(FLSG_EARG, "seed out of range");
-    if (p->kind != FLSG_PPR) return fail(FLSG_EARG, "coloured SOR solves PPR");
+    if (p->kind != FLSG_PPR && p->kind != FLSG_KATZ) return fail(FLSG_EARG, "coloured SOR solves PPR or Katz");
     if (p->precision != FLSG_FP64) return fail(FLSG_EARG, "coloured SOR runs the FP64 path");
     if (!(omega > 0.0 && omega < 2.0)) return fail(FLSG_EARG, "omega must lie in (0, 2)");
-    if (!(p->alpha > 0.0 && p->alpha < 1.0)) return fail(FLSG_EARG, "alpha must lie in (0, 1)");
+    if (!(p->alpha > 0.0 && p->alpha < 1.0)) return fail(FLSG_EARG, "alpha (PPR) / beta (Katz) must lie in (0, 1)");
     hipStream_t st = (hipStream_t)stream;
     int32_t deg_s = 0;
     FLSG_HIP(hipMemcpyAsync(&deg_s, g->degree + seed, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     FLSG_HIP(hipStreamSynchronize(st));
     if (deg_s <= 0) return fail(FLSG_EARG, "seed node has degree 0 (the reference divides by zero)");
-    if (!(p->eps * (double)deg_s <= 1.))
-        return fail(FLSG_EASSERT, "assert eps * degree[s_node] <= 1. failed");   // ppr_solver.py:333
+    if (p->kind == FLSG_PPR && !(p->eps * (double)deg_s <= 1.))
+        return fail(FLSG_EASSERT, "assert eps * degree[s_node] <= 1. failed");   // ppr_solver.py:333 (katz_sor has none)
     *h_res = flsg_result{};
     hipEvent_t e0, e1;
     FLSG_HIP(hipEventCreate(&e0));

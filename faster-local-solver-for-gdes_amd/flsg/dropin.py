@@ -73,27 +73,27 @@ def per_seed(n, indptr, indices, degree, s_nodes, params):
     return opers, times
 
 
-def single_sor(n, indptr, indices, degree, s_node, alpha, eps, omega, local) -> tuple:
+def single_sor(n, indptr, indices, degree, s_node, alpha, eps, omega, local, kind=_abi.FLSG_PPR) -> tuple:
     """(xt, rt, oper (float), algo_time) of the coloured SOR push."""
     dg = device_graph(n, indptr, indices, degree)
     import torch
-    if not (eps * float(np.asarray(degree)[int(s_node)]) <= 1.):
+    if kind == _abi.FLSG_PPR and not (eps * float(np.asarray(degree)[int(s_node)]) <= 1.):
         raise AssertionError("eps * degree[s_node] <= 1.")   # ppr_solver.py:333
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    out = engine.sor_solve(dg, float(alpha), float(eps), float(omega), int(s_node), bool(local))
+    out = engine.sor_solve(dg, float(alpha), float(eps), float(omega), int(s_node), bool(local), kind=kind)
     torch.cuda.synchronize()
     algo_time = time.perf_counter() - t0
     return out.x.cpu().numpy(), out.r.cpu().numpy(), float(out.result["oper"]), algo_time
 
 
-def per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, local):
+def per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, local, kind=_abi.FLSG_PPR):
     """Coloured SOR per seed; returns (opers int64[k], algo_times f64[k])."""
     k = len(s_nodes)
     opers = np.zeros(k, dtype=np.int64)
     times = np.zeros(k)
     for i in range(k):
-        _, _, o, times[i] = single_sor(n, indptr, indices, degree, s_nodes[i], alpha, eps, omega, local)
+        _, _, o, times[i] = single_sor(n, indptr, indices, degree, s_nodes[i], alpha, eps, omega, local, kind)
         opers[i] = int(o)
     return opers, times
 

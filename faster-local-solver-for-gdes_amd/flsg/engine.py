@@ -184,15 +184,16 @@ def solve(dg: DeviceGraph, params: _abi.Params, seed: int, trace: bool = False,
 
 
 def sor_solve(dg: DeviceGraph, alpha: float, eps: float, omega: float, seed: int, local: bool = True,
-              max_iters: int = 0) -> SolveOut:
-    """Graph-coloured SOR push for PPR (ppr_sor, ppr_solver.py:331-461) on the
-    whole GPU: FP64, local (frontier) or global (sweeps)."""
+              max_iters: int = 0, kind: int = _abi.FLSG_PPR) -> SolveOut:
+    """Graph-coloured SOR push on the whole GPU: FP64, local (frontier) or
+    global (sweeps); PPR (ppr_sor, ppr_solver.py:331-461) or Katz with
+    alpha = beta (katz_sor, katz_solver.py:252-337)."""
     torch = dg.torch
     n = dg.n
     x = torch.empty(n, dtype=torch.float64, device=dg.device)
     r = torch.empty(n, dtype=torch.float64, device=dg.device)
     res = _abi.Result()
-    p = make_params(_abi.FLSG_PPR, _abi.FLSG_GD, local, _abi.FLSG_FP64, alpha=alpha, eps=eps, max_iters=max_iters)
+    p = make_params(kind, _abi.FLSG_GD, local, _abi.FLSG_FP64, alpha=alpha, eps=eps, max_iters=max_iters)
     seed = int(dg.to_internal([seed])[0])
     _check(dg.lib.flsg_sor_solve(dg.handle, C.byref(p), float(omega), seed, _ptr(x), _ptr(r), C.byref(res),
                                  _stream_ptr(torch)))

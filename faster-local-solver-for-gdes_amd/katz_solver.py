@@ -74,13 +74,17 @@ def solve_a_graph_gpu(n, indptr, indices, degree, alpha, eps, s_nodes):
 
 
 def solve_a_graph_all(n, indptr, indices, degree, alpha, eps, omega, graph_name, s_nodes):
-    """katz_solver.py:605-666; SOR slots are CPU baselines not run here (NaN)."""
-    k = len(s_nodes)
-    nan = np.full(k, np.nan)
+    """katz_solver.py:605-666: (s_nodes, local_sor_*, global_sor_*,
+    local_akatz_* (omega = 1), global_akatz_*, local_gd_*, global_gd_*), every
+    slot on the GPU (SOR / AKatz by the coloured variant)."""
+    K = _abi.FLSG_KATZ
+    lso, lst = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, True, K)
+    gso, gst = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, False, K)
+    lao, lat = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, 1.0, True, K)
+    gao, gat = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, 1.0, False, K)
     lo, lt = _d.per_seed(n, indptr, indices, degree, s_nodes, _params(_abi.FLSG_GD, True, alpha, eps))
     go, gt = _d.per_seed(n, indptr, indices, degree, s_nodes, _params(_abi.FLSG_GD, False, alpha, eps))
-    return (s_nodes, nan, nan.copy(), nan.copy(), nan.copy(), nan.copy(), nan.copy(), nan.copy(), nan.copy(),
-            lo, lt, go, gt)
+    return (s_nodes, lso, lst, gso, gst, lao, lat, gao, gat, lo, lt, go, gt)
 
 
 def solve_a_graph_cheby(n, indptr, indices, degree, alpha, l_max, mu_min, eps, graph_name, s_nodes):
@@ -104,5 +108,8 @@ def eigen_bounds(n, indptr, indices, alpha):
     return 1.0 - alpha * lam_min, 1.0 - alpha * lam_max
 
 
-def katz_sor(*args, **kwargs):
-    _d.sor_not_built("katz_sor")
+def katz_sor(n, indptr, indices, degree, s_node, alpha, eps, omega, local):
+    """SOR / Gauss-Southwell push for Katz (katz_solver.py:252-337),
+    graph-coloured on the GPU.  Returns (xt, rt, oper (float), algo_time),
+    xt without the e_s term as the reference's."""
+    return _d.single_sor(n, indptr, indices, degree, s_node, alpha, eps, omega, local, _abi.FLSG_KATZ)

@@ -243,13 +243,16 @@ int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, int64_t n_ke
 
 /* ---------------------------------------------------------------- coloured SOR */
 
-/* Graph-coloured SOR / Gauss-Southwell push for PPR.  Replaces
+/* Graph-coloured SOR / Gauss-Southwell push.  Replaces
  *   ppr_sor(..., omega, local=True/False)     ppr_solver.py:331-461
- * (omega = 1 is the APPR push of solve_a_graph_all, ppr_solver.py:559-570).
+ *   katz_sor(..., omega, local=True/False)    katz_solver.py:252-337
+ * (omega = 1 is the APPR / AKatz push of the solve_a_graph_all drivers,
+ * ppr_solver.py:559-570, katz_solver.py:630-640).
  * Colours are visited in order and the nodes of one colour, which share no
  * edge, push at once; the local variant sweeps its frontier this way, the
  * global variant every live node.  Both stop on the reference's certificate
- * |r_u| < eps*alpha*d_u for every live u.  p->kind must be FLSG_PPR,
+ * |r_u| < eps*alpha*d_u (Katz: eps*d_u) for every live u.  p->kind is
+ * FLSG_PPR or FLSG_KATZ (p->alpha = beta; x excludes e_s as katz_sor's),
  * p->precision FLSG_FP64, p->local selects the variant; 0 < omega < 2.
  * d_x, d_r: dense FP64 outputs of n entries.  h_res->oper is the reference's
  * operation count (local: sum of degree over pushed nodes; global: n + m per

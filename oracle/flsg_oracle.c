@@ -973,6 +973,91 @@ int ora_ppr_sor(int64_t n, const int64_t *indptr, const int32_t *indices, const 
 }
 
 /* ------------------------------------------------------------------------
+ * Katz SOR / Gauss-Southwell push (katz_solver.py:252-337, katz_sor): the
+ * same control flow as ppr_sor with r_s = 1 (:258), eps_vec = eps*degree
+ * (:259), push value alpha*res (alpha is Katz's beta, :302, :321), full CSR
+ * rows (:303, :322), no assert, and x_s -= 1 at the end (:332).  The 4-hour
+ * wall-clock bail-outs (:288, :330) are not restated; isolated nodes are not
+ * counted as active, as in ora_ppr_sor.
+ * -------------------------------------------------------------------- */
+int ora_katz_sor(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                 int64_t s, double alpha, double eps, double omega, int local, double *xt, double *rt,
+                 double *oper_out, int64_t *iters_out, double *time_out) {
+    if (degree[s] == 0) return ORA_EARG;
+    double t0 = now_s();
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++) m += degree[i];            /* :255 */
+    memset(xt, 0, sizeof(double) * n);
+    memset(rt, 0, sizeof(double) * n);
+    rt[s] = 1.0;                                                 /* :258 */
+    double *eps_vec = (double *)malloc(sizeof(double) * n);
+    if (!eps_vec) return ORA_ENOMEM;
+    for (int64_t i = 0; i < n; i++) eps_vec[i] = eps * (double)degree[i];   /* :259 */
+    double oper = 0.;
+    int64_t iters = 0;
+    if (local) {
+        const int64_t q_max = n + 2;                             /* :264-274 */
+        int64_t *queue = (int64_t *)malloc(sizeof(int64_t) * q_max);
+        uint8_t *q_mark = (uint8_t *)calloc(q_max, 1);
+        if (!queue || !q_mark) { free(queue); free(q_mark); free(eps_vec); return ORA_ENOMEM; }
+        int64_t front = 0, rear = 0;
+        queue[rear] = n; rear = (rear + 1) % q_max; q_mark[n] = 1;
+        queue[rear] = s; rear = (rear + 1) % q_max; q_mark[s] = 1;
+        for (;;) {
+            const int64_t u = queue[front];                                    /* :277-279 */
+            front = (front + 1) % q_max;
+            q_mark[u] = 0;
+            if (u == n) {                                                      /* :281-293 */
+                if (rear - front == 0) break;
+                queue[rear] = n; rear = (rear + 1) % q_max; q_mark[u] = 1;
+                iters++;
+                continue;
+            }
+            if (eps_vec[u] > fabs(rt[u])) continue;                            /* :295-296 */
+            oper += (double)degree[u];                                         /* :298 */
+            const double res = omega * rt[u];                                  /* :299-302 */
+            xt[u] += res;
+            rt[u] -= res;
+            const double push_res = alpha * res;
+            for (int64_t j = indptr[u]; j < indptr[u + 1]; j++) {             /* :303-308 */
+                const int64_t v = indices[j];
+                rt[v] += push_res;
+                if (!q_mark[v] && eps_vec[v] <= fabs(rt[v])) {
+                    queue[rear] = v; rear = (rear + 1) % q_max; q_mark[v] = 1;
+                }
+            }
+            if (!q_mark[u] && eps_vec[u] <= fabs(rt[u])) {                    /* :310-313 */
+                queue[rear] = u; rear = (rear + 1) % q_max; q_mark[u] = 1;
+            }
+        }
+        free(queue); free(q_mark);
+    } else {
+        for (;;) {
+            for (int64_t u = 0; u < n; u++) {                                  /* :317-323 */
+                if (degree[u] == 0) continue;
+                const double res = omega * rt[u];
+                xt[u] += res;
+                rt[u] -= res;
+                const double push_res = alpha * res;
+                for (int64_t j = indptr[u]; j < indptr[u + 1]; j++) rt[indices[j]] += push_res;
+            }
+            oper += (double)(n + m);                                           /* :324 */
+            iters++;
+            int64_t act = 0;                                                   /* :325-326 */
+            for (int64_t i = 0; i < n; i++)
+                if (degree[i] > 0 && eps_vec[i] <= fabs(rt[i])) act++;
+            if (act <= 0) break;
+        }
+    }
+    xt[s] -= 1.;                                                               /* :332 */
+    free(eps_vec);
+    if (oper_out) *oper_out = oper;
+    if (iters_out) *iters_out = iters;
+    if (time_out) *time_out = now_s() - t0;
+    return ORA_OK;
+}
+
+/* ------------------------------------------------------------------------
  * Graph colouring and the coloured SOR push: NOT the reference's algorithm.
  * They restate, one step after another, what libflsg's GPU SOR variant does,
  * so its results can be checked exactly (the colouring) or to rounding (the
@@ -1053,18 +1138,19 @@ int64_t ora_colour(int64_t n, const int64_t *indptr, const int32_t *indices, con
  * and satisfies eps_vec[v] <= |r_v| when the iteration ends -- the active set
  * after the sweep, whatever the order of the adds within a colour.  Nodes of
  * F are processed in F's order within a colour. */
-int ora_ppr_sor_coloured(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
-                         const int32_t *colour, int64_t ncol, int64_t s, double alpha, double eps,
-                         double omega, int local, int64_t max_iters, double *xt, double *rt,
-                         int64_t *oper_out, int64_t *iters_out, int64_t *nodes_out) {
-    if (!(eps * (double)degree[s] <= 1.)) return ORA_EASSERT;
+int ora_sor_coloured(int kind, int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                     const int32_t *colour, int64_t ncol, int64_t s, double alpha, double eps,
+                     double omega, int local, int64_t max_iters, double *xt, double *rt,
+                     int64_t *oper_out, int64_t *iters_out, int64_t *nodes_out) {
+    const int katz = kind == 1;   /* 0: PPR (ppr_sor), 1: Katz (katz_sor, alpha = beta) */
+    if (!katz && !(eps * (double)degree[s] <= 1.)) return ORA_EASSERT;
     if (degree[s] == 0) return ORA_EARG;
     int64_t m = 0;
     for (int64_t i = 0; i < n; i++) m += degree[i];
     memset(xt, 0, sizeof(double) * n);
     memset(rt, 0, sizeof(double) * n);
-    rt[s] = alpha;
-    const double ea = eps * alpha;
+    rt[s] = katz ? 1.0 : alpha;
+    const double ea = katz ? eps : eps * alpha;
     double *eps_vec = (double *)malloc(sizeof(double) * n);
     int64_t *F = (int64_t *)malloc(sizeof(int64_t) * n);
     int64_t *G = (int64_t *)malloc(sizeof(int64_t) * n);
@@ -1104,7 +1190,7 @@ int ora_ppr_sor_coloured(int64_t n, const int64_t *indptr, const int32_t *indice
                 if (local) { oper += degree[u]; nodes++; }
                 const double rs = omega * rt[u];
                 res[i] = rs;
-                const double push_res = (1. - alpha) * rs / (double)degree[u];
+                const double push_res = katz ? alpha * rs : (1. - alpha) * rs / (double)degree[u];
                 for (int64_t j = indptr[u]; j < indptr[u] + degree[u]; j++) {
                     const int64_t v = indices[j];
                     rt[v] += push_res;
@@ -1134,6 +1220,7 @@ int ora_ppr_sor_coloured(int64_t n, const int64_t *indptr, const int32_t *indice
             if (act == 0) nf = 0;
         }
     }
+    if (katz) xt[s] -= 1.;   /* katz_solver.py:332 */
     free(eps_vec); free(F); free(G); free(B); free(res); free(coff); free(mark);
     if (oper_out) *oper_out = oper;
     if (iters_out) *iters_out = iters;

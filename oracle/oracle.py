@@ -22,6 +22,8 @@ reference function:
   ppr_sor         ppr_solver.py:331-461      pinned: one global sweep = the dense SOR
                                              update in natural order, invariants,
                                              hand-traced path graph
+  katz_sor        katz_solver.py:252-337     pinned: the same matrix form with
+                                             Q = I - bA, dense Katz solve
   colour          (this build's GPU colouring, restated) pinned: proper colouring,
                                              first-fit minimality, rounds by hand
   ppr_sor_coloured (this build's GPU SOR, restated)  pinned: one global sweep = the
@@ -88,8 +90,9 @@ def lib():
         L.ora_ppr_sor.argtypes = [i64, vp, vp, vp, i64, dbl, dbl, dbl, i32, vp, vp, vp, vp, vp]
         L.ora_colour.argtypes = [i64, vp, vp, vp, vp, vp]
         L.ora_colour.restype = i64
-        L.ora_ppr_sor_coloured.argtypes = [i64, vp, vp, vp, vp, i64, i64, dbl, dbl, dbl, i32, i64, vp, vp, vp, vp,
-                                           vp]
+        L.ora_katz_sor.argtypes = [i64, vp, vp, vp, i64, dbl, dbl, dbl, i32, vp, vp, vp, vp, vp]
+        L.ora_sor_coloured.argtypes = [i32, i64, vp, vp, vp, vp, i64, i64, dbl, dbl, dbl, i32, i64, vp, vp, vp, vp,
+                                       vp]
         L.ora_near_threshold.argtypes = []
         L.ora_near_threshold.restype = i64
         _lib = L
@@ -280,7 +283,7 @@ def ppr_batch(g, seeds, alpha, eps, kind=0, nthreads=0):
     return opers, times
 
 
-def ppr_sor(g, s, alpha, eps, omega, local=True):
+def ppr_sor(g, s, alpha, eps, omega, local=True, _fn="ora_ppr_sor"):
     """The reference's ppr_sor (local FIFO or natural-order sweeps)."""
     indptr, indices, degree = _csr(g)
     x = np.zeros(g.n)
@@ -288,13 +291,18 @@ def ppr_sor(g, s, alpha, eps, omega, local=True):
     oper = np.zeros(1)
     iters = np.zeros(1, dtype=np.int64)
     t = np.zeros(1)
-    rc = lib().ora_ppr_sor(g.n, _p(indptr), _p(indices), _p(degree), int(s), float(alpha), float(eps),
-                           float(omega), int(local), _p(x), _p(r), _p(oper), _p(iters), _p(t))
+    rc = getattr(lib(), _fn)(g.n, _p(indptr), _p(indices), _p(degree), int(s), float(alpha), float(eps),
+                             float(omega), int(local), _p(x), _p(r), _p(oper), _p(iters), _p(t))
     if rc == ORA_EASSERT:
         raise AssertionError("eps * degree[s_node] <= 1.")
     if rc != ORA_OK:
         raise RuntimeError(f"oracle error {rc}")
     return {"x": x, "r": r, "oper": float(oper[0]), "iters": int(iters[0]), "time": float(t[0])}
+
+
+def katz_sor(g, s, beta, eps, omega, local=True):
+    """The reference's katz_sor (katz_solver.py:252-337)."""
+    return ppr_sor(g, s, beta, eps, omega, local=local, _fn="ora_katz_sor")
 
 
 def colour(g):
@@ -309,21 +317,26 @@ def colour(g):
     return col, int(k), int(rounds[0])
 
 
-def ppr_sor_coloured(g, s, alpha, eps, omega, col, ncol, local=True, max_iters=100000):
-    """Sequential restatement of the GPU coloured SOR push."""
+def ppr_sor_coloured(g, s, alpha, eps, omega, col, ncol, local=True, max_iters=100000, kind=0):
+    """Sequential restatement of the GPU coloured SOR push (kind 0 PPR,
+    1 Katz with alpha = beta)."""
     indptr, indices, degree = _csr(g)
     col = np.ascontiguousarray(col, dtype=np.int32)
     x = np.zeros(g.n)
     r = np.zeros(g.n)
     out = np.zeros(3, dtype=np.int64)
-    rc = lib().ora_ppr_sor_coloured(g.n, _p(indptr), _p(indices), _p(degree), _p(col), int(ncol), int(s),
-                                    float(alpha), float(eps), float(omega), int(local), int(max_iters),
-                                    _p(x), _p(r), _p(out[0:]), _p(out[1:]), _p(out[2:]))
+    rc = lib().ora_sor_coloured(int(kind), g.n, _p(indptr), _p(indices), _p(degree), _p(col), int(ncol), int(s),
+                                float(alpha), float(eps), float(omega), int(local), int(max_iters),
+                                _p(x), _p(r), _p(out[0:]), _p(out[1:]), _p(out[2:]))
     if rc == ORA_EASSERT:
         raise AssertionError("eps * degree[s_node] <= 1.")
     if rc != ORA_OK:
         raise RuntimeError(f"oracle error {rc}")
     return {"x": x, "r": r, "oper": int(out[0]), "iters": int(out[1]), "nodes": int(out[2])}
+
+
+def katz_sor_coloured(g, s, beta, eps, omega, col, ncol, local=True, max_iters=100000):
+    return ppr_sor_coloured(g, s, beta, eps, omega, col, ncol, local=local, max_iters=max_iters, kind=1)
 
 
 def max_threads() -> int:

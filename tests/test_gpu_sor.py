@@ -213,3 +213,73 @@ def test_sor_global_certified_large(large_graph, dev_large):
     out = engine.sor_solve(dev_large, ALPHA, 1e-7, OMEGA_OPT, s, local=False, max_iters=5000)
     _certified(g, s, out.x.cpu().numpy(), out.r.cpu().numpy(), 1e-7, tol=1e-14)
     print(f"global SOR sweeps {out.result['iters']} ms {out.result['ms']:.1f}")
+
+
+# ------------------------------------------------------------------ Katz (katz_sor)
+def _katz_certified(g, s, beta, x, r_ret, eps, tol=1e-15):
+    A = O.adjacency(g)
+    y = x.copy()
+    y[s] += 1.0
+    r = -(y - beta * (A @ y))
+    r[s] += 1.0
+    live = g.degree > 0
+    assert np.abs(r - r_ret).max() < tol
+    worst = float((np.abs(r[live]) / (eps * g.degree[live])).max())
+    assert worst < 1.0, worst
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+@pytest.mark.parametrize("local", [True, False])
+def test_katz_sor_parity_small(small_graph, dev_small, local, omega):
+    """Katz (beta = 1/(d_max+1), eps = 1e-7) against the coloured
+    restatement (same bars as PPR) and the reference-order katz_sor (both
+    certified: ||x - x_ref||_1 <= ||(I - bA)^-1||_1 (||r||_1 + ||r_ref||_1) with
+    ||(I - bA)^-1||_1 <= 1 / (1 - b d_max))."""
+    from flsg import _abi
+    g = small_graph
+    beta = 1.0 / (int(g.degree.max()) + 1)
+    col, k, _ = engine.colouring(dev_small)
+    eps = 1e-7
+    for s in graph.uniform_seeds(g, 4, seed=5):
+        s = int(s)
+        out = engine.sor_solve(dev_small, beta, eps, omega, s, local=local, max_iters=5000, kind=_abi.FLSG_KATZ)
+        x = out.x.cpu().numpy()
+        oc = O.katz_sor_coloured(g, s, beta, eps, omega, col, k, local=local)
+        res = out.result
+        assert res["iters"] == oc["iters"] and res["oper"] == oc["oper"], (s, res, oc["iters"], oc["oper"])
+        rel = np.abs(x - oc["x"]).sum() / np.abs(oc["x"]).sum()
+        assert rel <= REL_L1, (s, rel)
+        r = out.r.cpu().numpy()
+        _katz_certified(g, s, beta, x, r, eps)
+        ref = O.katz_sor(g, s, beta, eps, omega, local=local)
+        bound = (np.abs(r).sum() + np.abs(ref["r"]).sum()) / (1.0 - beta * g.degree.max())
+        assert np.abs(x - ref["x"]).sum() <= bound
+
+
+def test_katz_sor_dropin(small_graph):
+    import katz_solver
+    g = small_graph
+    beta = 1.0 / (int(g.degree.max()) + 1)
+    s = int(graph.uniform_seeds(g, 1, seed=6)[0])
+    x, r, oper, t = katz_solver.katz_sor(g.n, g.indptr, g.indices, g.degree, s, beta, 1e-7, 1.2, True)
+    assert isinstance(oper, float) and oper > 0
+    _katz_certified(g, s, beta, x, r, 1e-7)
+    out = katz_solver.solve_a_graph_all(g.n, g.indptr, g.indices, g.degree, beta, 1e-7, 1.2, "synthetic",
+                                        np.array([s], dtype=np.int32))
+    assert len(out) == 13 and out[1][0] == int(oper) and min(o[0] for o in out[1::2]) > 0
+
+
+def test_katz_sor_local_parity_large(large_graph, dev_large):
+    from flsg import _abi
+    g = large_graph
+    beta = 1.0 / (int(g.degree.max()) + 1)
+    col, k, _ = engine.colouring(dev_large)
+    s = int(graph.uniform_seeds(g, 1, seed=13)[0])
+    out = engine.sor_solve(dev_large, beta, 1e-7, 1.0, s, local=True, max_iters=5000, kind=_abi.FLSG_KATZ)
+    x = out.x.cpu().numpy()
+    oc = O.katz_sor_coloured(g, s, beta, 1e-7, 1.0, col, k, local=True)
+    rel = np.abs(x - oc["x"]).sum() / np.abs(oc["x"]).sum()
+    print(f"katz seed {s}: iters {out.result['iters']}/{oc['iters']} oper {out.result['oper']}/{oc['oper']} "
+          f"rel L1 {rel:.2e} ms {out.result['ms']:.2f}")
+    assert out.result["oper"] == oc["oper"] and rel <= REL_L1
+    _katz_certified(g, s, beta, x, out.r.cpu().numpy(), 1e-7, tol=1e-14)

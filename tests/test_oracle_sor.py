@@ -216,3 +216,92 @@ def test_coloured_sor_certificate_and_reference_distance(local, omega, small_gra
     # both certified: ||x_a - x_b||_1 <= (||r_a||_1 + ||r_b||_1) / alpha
     bound = (np.abs(a["r"]).sum() + np.abs(b["r"]).sum()) / 0.1
     assert np.abs(a["x"] - b["x"]).sum() <= bound
+
+
+# ------------------------------------------------------------------ Katz SOR
+def katz_matrix_form(g, s, beta, eps, omega, order_key):
+    """Dense SOR on (I - beta A) y = e_s in the given sweep order, stopped by
+    the reference's test (eps*d_v <= |r_v| for no live v).  Returns (y - e_s,
+    sweeps): katz_sor's x convention."""
+    A = O.adjacency(g).toarray()
+    n = g.n
+    Q = np.eye(n) - beta * A
+    b = np.zeros(n)
+    b[s] = 1.0
+    key = np.asarray(order_key)
+    L = np.where(key[None, :] < key[:, None], Q, 0.0)
+    np.fill_diagonal(L, 0.0)
+    U = Q - np.eye(n) - L
+    M = np.eye(n) + omega * L
+    N = (1.0 - omega) * np.eye(n) - omega * U
+    thr = eps * np.asarray(g.degree, dtype=np.float64)
+    y = np.zeros(n)
+    for sweeps in range(1, 100000):
+        y = np.linalg.solve(M, N @ y + omega * b)
+        if not np.any(thr <= np.abs(b - Q @ y)):
+            y[s] -= 1.0
+            return y, sweeps
+    raise RuntimeError("no convergence")
+
+
+def test_katz_sor_hand_traced_path():
+    """P3, seed 0, beta = 1/4, eps = 1/16, omega = 1; thresholds eps*d =
+    [1/16, 1/8, 1/16].  Pops: flag, 0 (x0 = 1, r1 = 1/4), flag, 1 (x1 = 1/4,
+    r0 = r2 = 1/16, both queued on equality), flag, 0 (r1 = 1/64), 2
+    (r1 = 1/32), flag with the queue empty.  oper = 1+2+1+1 = 5, 3
+    iterations, x_s -= 1 at the end."""
+    o = O.katz_sor(path3(), 0, 0.25, 1.0 / 16, 1.0, local=True)
+    assert o["oper"] == 5.0 and o["iters"] == 3
+    assert np.array_equal(o["x"], [0.0625, 0.25, 0.0625])
+    assert np.array_equal(o["r"], [0.0, 0.03125, 0.0])
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_katz_sor_global_matches_matrix_form(seed, omega):
+    g = random_graph(seed=seed)
+    beta = 1.0 / (g.degree.max() + 1)
+    o = O.katz_sor(g, 3, beta, 1e-6, omega, local=False)
+    x, sweeps = katz_matrix_form(g, 3, beta, 1e-6, omega, np.arange(g.n))
+    assert o["iters"] == sweeps
+    assert o["oper"] == sweeps * (g.n + int(g.degree.sum()))
+    assert np.abs(o["x"] - x).sum() <= 1e-12 * np.abs(x).sum()
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+@pytest.mark.parametrize("local", [True, False])
+def test_katz_sor_certificate(local, omega):
+    g = random_graph(n=60, p=0.08, seed=5)
+    beta = 1.0 / (g.degree.max() + 1)
+    s = 7
+    o = O.katz_sor(g, s, beta, 1e-7, omega, local=local)
+    A = O.adjacency(g)
+    y = o["x"].copy()
+    y[s] += 1.0
+    r = -(y - beta * (A @ y))
+    r[s] += 1.0
+    assert np.abs(r - o["r"]).max() < 1e-15
+    assert np.all(np.abs(r) < 1e-7 * g.degree)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_coloured_katz_global_matches_matrix_form(omega):
+    g = random_graph(seed=4)
+    beta = 1.0 / (g.degree.max() + 1)
+    col, k, _ = O.colour(g)
+    o = O.katz_sor_coloured(g, 3, beta, 1e-6, omega, col, k, local=False)
+    x, sweeps = katz_matrix_form(g, 3, beta, 1e-6, omega, col)
+    assert o["iters"] == sweeps
+    assert np.abs(o["x"] - x).sum() <= 1e-12 * np.abs(x).sum()
+
+
+def test_coloured_katz_hand_traced_path():
+    """Colours [1, 0, 1]: iteration 1 pushes 0 (r1 = 1/4), iteration 2 pushes
+    1 (r0 = r2 = 1/16 = their thresholds), iteration 3 pushes 0 and 2 (r1 =
+    1/32 < 1/8): the same pushes as the FIFO run."""
+    g = path3()
+    col, k, _ = O.colour(g)
+    o = O.katz_sor_coloured(g, 0, 0.25, 1.0 / 16, 1.0, col, k, local=True)
+    assert o["oper"] == 5 and o["iters"] == 3
+    assert np.array_equal(o["x"], [0.0625, 0.25, 0.0625])
+    assert np.array_equal(o["r"], [0.0, 0.03125, 0.0])
