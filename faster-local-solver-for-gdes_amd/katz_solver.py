@@ -5,7 +5,8 @@ Same signatures and returns; solves run in libflsg.so on the GPU.
 ``katz_gd`` returns x - e_s (katz_solver.py:243), ``katz_cheby`` returns x
 with the e_s component (katz_solver.py:431), as the reference does.  The
 reference's 4-hour wall-clock bail-out is replaced by an iteration cap.
-SOR variants remain CPU reference baselines (NotImplementedError).
+``katz_sor`` runs the graph-coloured SOR push (colour order, not the
+reference's FIFO order; same certificate |r_u| < eps*d_u).
 """
 from __future__ import annotations
 
@@ -15,7 +16,8 @@ from flsg import _abi, engine
 from flsg import dropin as _d
 
 __all__ = ["katz_gd", "katz_cheby", "katz_gd_local_gpu", "katz_gd_standard_gpu", "solve_a_graph_gpu",
-           "solve_a_graph_all", "solve_a_graph_cheby", "katz_sor", "katz_gd_batch", "eigen_bounds"]
+           "solve_a_graph_all", "solve_a_graph_cheby", "solve_a_graph_omega", "katz_sor", "katz_gd_batch",
+           "eigen_bounds"]
 
 
 def _params(method, local, alpha, eps, l_max=0.0, mu_min=0.0, precision=_abi.FLSG_FP64):
@@ -106,6 +108,20 @@ def eigen_bounds(n, indptr, indices, alpha):
     lam_max = float(sla.eigsh(A, 1, which="LA")[0][0])
     lam_min = float(sla.eigsh(A, 1, which="SA")[0][0])
     return 1.0 - alpha * lam_min, 1.0 - alpha * lam_max
+
+
+def solve_a_graph_omega(n, indptr, indices, degree, alpha, eps, omega, graph_name, s_nodes):
+    """katz_solver.py:561-603: the 13-tuple with only the local SOR and local
+    AKatz (omega = 1) slots filled; the others stay 0 as the reference
+    leaves them."""
+    K = _abi.FLSG_KATZ
+    k = len(s_nodes)
+    z_o = np.zeros(k, dtype=np.int64)
+    z_t = np.zeros(k)
+    lso, lst = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, True, K)
+    lao, lat = _d.per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, 1.0, True, K)
+    return (s_nodes, lso, lst, z_o, z_t.copy(), lao, lat, z_o.copy(), z_t.copy(), z_o.copy(), z_t.copy(),
+            z_o.copy(), z_t.copy())
 
 
 def katz_sor(n, indptr, indices, degree, s_node, alpha, eps, omega, local):

@@ -27,6 +27,8 @@ EXPECTED = {
     ("katz_solver", "katz_sor"): ("katz_solver.py:252", "n indptr indices degree s_node alpha eps omega local"),
     ("katz_solver", "solve_a_graph_all"): ("katz_solver.py:606",
                                            "n indptr indices degree alpha eps omega graph_name s_nodes"),
+    ("katz_solver", "solve_a_graph_omega"): ("katz_solver.py:562",
+                                             "n indptr indices degree alpha eps omega graph_name s_nodes"),
     ("katz_solver", "katz_cheby"): ("katz_solver.py:341",
                                     "n indptr indices degree s_node alpha l_max mu_min eps local"),
     ("katz_solver", "katz_gd_local_gpu"): ("katz_solver.py:81", "n indptr indices degree s_node alpha eps"),

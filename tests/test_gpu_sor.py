@@ -267,6 +267,9 @@ def test_katz_sor_dropin(small_graph):
     out = katz_solver.solve_a_graph_all(g.n, g.indptr, g.indices, g.degree, beta, 1e-7, 1.2, "synthetic",
                                         np.array([s], dtype=np.int32))
     assert len(out) == 13 and out[1][0] == int(oper) and min(o[0] for o in out[1::2]) > 0
+    om = katz_solver.solve_a_graph_omega(g.n, g.indptr, g.indices, g.degree, beta, 1e-7, 1.2, "synthetic",
+                                         np.array([s], dtype=np.int32))
+    assert len(om) == 13 and om[1][0] == int(oper) and om[5][0] > 0 and om[3][0] == 0
 
 
 def test_katz_sor_local_parity_large(large_graph, dev_large):
