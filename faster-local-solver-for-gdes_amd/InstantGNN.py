@@ -11,7 +11,10 @@ the two residuals allow (both satisfy the same invariant and certificate).
 ``runtimes`` are the device time split evenly over the columns.
 ``instantGNNPush`` / ``instantGNNUpdateBatch`` (one column, the reference's
 queue arguments) run the same device propagation in FP64.
-SOR pushes and edge deletion stay CPU reference baselines (NotImplementedError).
+``instantGNNSorPush`` / ``instantGNNSorParallel`` and the 'sor' update
+algorithms run a graph-coloured SOR push per column in FP64 (colour order, not
+the reference's FIFO order; same invariant and certificate).  Edge deletion
+is not built (NotImplementedError).
 """
 from __future__ import annotations
 
@@ -53,8 +56,6 @@ def instantGNNUpdateBatchParallel(n, indptr, indices, degree, alpha, eps, omega,
     """InstantGNN.py:332-359 with algo 'fwd...insert': inserts (us, vs) into
     the slack CSR (indices / degree updated in place), corrects P and R and
     pushes.  Returns per-column runtimes."""
-    if "sor" in algo:
-        _d.sor_not_built("instantGNNUpdateBatchParallel(algo=%r)" % algo)
     if "insert" not in algo:
         raise NotImplementedError("edge deletion is not built on the GPU (algo must contain 'insert')")
     F = R.shape[1]
@@ -63,7 +64,7 @@ def instantGNNUpdateBatchParallel(n, indptr, indices, degree, alpha, eps, omega,
         prop = DynamicPropagation(_slack(indptr, indices, degree), X=X, alpha=float(alpha), eps=float(eps),
                                   beta=float(beta), precision=_abi.FLSG_FP32, R0=R, P0=P)
         _STATE[id(R)] = prop
-    st = prop.insert(us, vs)
+    st = prop.insert(us, vs, sor_omega=float(omega) if "sor" in algo else None)
     P[:] = prop.P.cpu().numpy()
     R[:] = prop.R.cpu().numpy()
     ind, deg = prop.graph_arrays()
@@ -172,8 +173,6 @@ def instantGNNUpdateBatch(n, indptr, indices, degree, old_degree, p, r, x, alpha
     import time
     import ctypes as C
     from flsg.engine import _check, _ptr, _stream_ptr
-    if "sor" in algo:
-        _d.sor_not_built("instantGNNUpdateBatch(algo=%r)" % algo)
     if "insert" not in algo and any(len(v) for v in dict(delete_neighbor).values()):
         raise NotImplementedError("edge deletion is not built on the GPU")
     t0 = time.perf_counter()
@@ -190,7 +189,10 @@ def instantGNNUpdateBatch(n, indptr, indices, degree, old_degree, p, r, x, alpha
                                             _ptr(prop.R), _ptr(d_keys), int(keys.size), _ptr(d_old),
                                             _stream_ptr(torch)))
         nodes = np.union1d(keys, _queue_nodes(queue, front, rear, n)).astype(np.int32)
-        _push_from(prop, nodes)
+        if "sor" in algo:
+            _sor_from(prop, nodes, omega)
+        else:
+            _push_from(prop, nodes)
         p[:] = prop.P.cpu().numpy()[:, 0].astype(p.dtype)
         r[:] = prop.R.cpu().numpy()[:, 0].astype(r.dtype)
     finally:
@@ -199,12 +201,63 @@ def instantGNNUpdateBatch(n, indptr, indices, degree, old_degree, p, r, x, alpha
     return time.perf_counter() - t0
 
 
-def instantGNNSorPush(*args, **kwargs):
-    _d.sor_not_built("instantGNNSorPush")
+def _sor_from(prop, nodes, omega):
+    from flsg import engine
+    p = prop.P[:, 0].contiguous()
+    r = prop.R[:, 0].contiguous()
+    st = engine.ignn_sor_columns(prop.dg, prop.params.alpha, prop.params.eps, prop.params.beta, float(omega), p, r,
+                                 nodes=nodes)
+    prop.P[:, 0] = p
+    prop.R[:, 0] = r
+    return st
 
 
-def instantGNNSorParallel(*args, **kwargs):
-    _d.sor_not_built("instantGNNSorParallel")
+def instantGNNSorPush(n, indptr, indices, degree, alpha, eps_vec, omega, p, r, queue, front, rear, q_mark, beta):
+    """InstantGNN.py:74-105 for one feature column: SOR push from the nodes in
+    queue positions front+1 .. rear (circular, length n = node count + 2 as
+    the reference's drivers pass) until every |r_u| < eps_vec_u; p and r are
+    updated in place and q_mark cleared.  Graph-coloured on the device (the
+    nodes of one colour push at once, colours in order), so p matches the
+    reference's through the propagation invariant and the certificate rather
+    than bitwise.  Returns the runtime in seconds."""
+    import time
+    t0 = time.perf_counter()
+    eps = _eps_from_vec(degree, eps_vec, beta)
+    nodes = _queue_nodes(queue, front, rear, n)
+    prop = _column_prop(indptr, indices, degree, r, p, r, alpha, eps, beta)
+    try:
+        if nodes.size:
+            _sor_from(prop, nodes, omega)
+        p[:] = prop.P.cpu().numpy()[:, 0].astype(p.dtype)
+        r[:] = prop.R.cpu().numpy()[:, 0].astype(r.dtype)
+    finally:
+        prop.close()
+    q_mark[:] = False
+    return time.perf_counter() - t0
+
+
+def instantGNNSorParallel(n, indptr, indices, degree, alpha, eps, omega, P, R, beta, batchsize=80):
+    """InstantGNN.py:108-134: per column, p starts at zero and r at R's
+    column, the queue holds every node with |r| >= eps d^(1-beta); P and R are
+    overwritten in place.  Returns per-column device times (seconds)."""
+    from flsg import engine
+    F = R.shape[1]
+    times = np.zeros(F)
+    prop = DynamicPropagation(_slack(indptr, indices, degree), X=R, alpha=float(alpha), eps=float(eps),
+                              beta=float(beta), precision=_abi.FLSG_FP64, R0=R, P0=np.zeros_like(R))
+    try:
+        for c in range(F):
+            p = prop.P[:, c].contiguous()
+            r = prop.R[:, c].contiguous()
+            st = engine.ignn_sor_columns(prop.dg, float(alpha), float(eps), float(beta), float(omega), p, r)
+            prop.P[:, c] = p
+            prop.R[:, c] = r
+            times[c] = st["ms"] * 1e-3
+        P[:] = prop.P.cpu().numpy()
+        R[:] = prop.R.cpu().numpy()
+    finally:
+        prop.close()
+    return times
 
 
 def instantGNNUpdateEdgeParallel(*args, **kwargs):

@@ -2,7 +2,8 @@
 // reference's ppr_sor (ppr_solver.py:331-461) and katz_sor
 // (katz_solver.py:252-337), local (frontier) and global (sweeps), any omega
 // in (0, 2) (omega = 1 is the Gauss-Southwell / APPR push of the
-// solve_a_graph_all drivers, ppr_solver.py:559-570, katz_solver.py:630-640).
+// solve_a_graph_all drivers, ppr_solver.py:559-570, katz_solver.py:630-640),
+// and of instantGNNSorPush (InstantGNN.py:74-105) for one feature column.
 //
 // The reference's SOR is sequential: each push reads the residual the
 // previous pushes left.  Two neighbours pushing at once would each move their
@@ -71,6 +72,7 @@ struct SorWs {
     int32_t *colour = nullptr;   // n
     int32_t *sel = nullptr;      // n: selected in the current colouring round
     int32_t *longs = nullptr;    // n: live rows longer than kColLong
+    int32_t *loops = nullptr;    // n: self-loops in each row
     int64_t nlong = 0;
     int ncol = 0;
     int64_t rounds = 0;
@@ -104,6 +106,7 @@ void sor_free(flsg_graph g) {
     (void)hipFree(w->colour);
     (void)hipFree(w->sel);
     (void)hipFree(w->longs);
+    (void)hipFree(w->loops);
     (void)hipFree(w->cnts);
     (void)hipFree(w->offs);
     (void)hipFree(w->curs);
@@ -145,10 +148,10 @@ __device__ __forceinline__ unsigned long long colour_key(int d, int u) {
 }
 
 // colour 0 for isolated nodes, -1 (uncoloured) for the rest; counts live
-// nodes and lists the long rows; flags self-loops
+// nodes and lists the long rows; counts each row's self-loops
 __global__ __launch_bounds__(kSNT) void k_col_init(const int32_t *indptr, const int32_t *indices,
                                                    const int32_t *degree, int64_t n, int32_t *colour,
-                                                   int32_t *longs, SorCtr *c) {
+                                                   int32_t *longs, int32_t *loops, SorCtr *c) {
     unsigned int live = 0;
     bool loop = false;
     const int64_t stride = (int64_t)gridDim.x * kSNT;
@@ -161,8 +164,11 @@ __global__ __launch_bounds__(kSNT) void k_col_init(const int32_t *indptr, const 
             live += d != 0;
             is_long = d > kColLong;
             const int st = indptr[u];
+            int nl = 0;
 #pragma unroll 8
-            for (int j = 0; j < d; j++) loop |= indices[st + j] == (int)u;
+            for (int j = 0; j < d; j++) nl += indices[st + j] == (int)u;
+            loops[u] = nl;
+            loop |= nl > 0;
         }
         const long long slot = wave_append(is_long, &c->nlong);
         if (slot >= 0) longs[slot] = (int32_t)u;
@@ -421,26 +427,50 @@ struct SorArgs {
     int32_t *next;           // local: next frontier
     unsigned int *nnext;     // local: its counter
     SorCtr *ctr;
-    double coef;             // PPR: 1 - alpha (push = coef*res/d_u); Katz: beta (push = coef*res)
+    const double *thr;       // InstantGNN: eps_vec (eps d^(1-beta)); else null (threshold ea * d)
+    const double *dp1;       // InstantGNN: d^(1-beta)
+    const double *dpb;       // InstantGNN: d^beta
+    const int32_t *loops;    // self-loops per row
+    double coef;             // PPR / InstantGNN: 1 - alpha (push = coef*res/d_u, /d_u^(1-beta)); Katz: beta
+    double xcoef;            // x_u += xcoef * res: InstantGNN alpha, else 1
     double omega;
     double ea;               // threshold per unit degree: PPR eps*alpha, Katz eps
     int local;
     int katz;
+    int ignn;
 };
 
 constexpr int kSU = 4;   // edges in flight per thread
 
 enum { kPush = 0, kCheck = 1 };
 
-__device__ __forceinline__ double sor_push_value(const SorArgs &a, double rs, int d) {
+// push value of u (InstantGNN: before the 1/d_v^beta of each edge)
+__device__ __forceinline__ double sor_push_value(const SorArgs &a, double rs, int u, int d) {
+    if (a.ignn) return a.coef * rs / a.dp1[u];  // InstantGNN.py:90
     return a.katz ? a.coef * rs                  // katz_solver.py:302
                   : a.coef * rs / (double)d;     // ppr_solver.py:407
+}
+
+// u is pushed (local: at or above its threshold; global: nonzero residual).
+// InstantGNN: |r_u| >= eps_vec[u] (InstantGNN.py:82) and r_u != 0, so an
+// isolated node (threshold 0) is taken once; other kinds need a live row.
+__device__ __forceinline__ bool sor_active(const SorArgs &a, int u, int d, double ru) {
+    if (a.ignn) return a.thr[u] <= fabs(ru) && ru != 0.0;
+    return d > 0 && (a.local ? (a.ea * (double)d <= fabs(ru)) : (ru != 0.0));
+}
+
+// InstantGNN: an isolated node or one whose only edge is a self-loop takes
+// p_u = r_u, r_u = 0 instead of pushing (InstantGNN.py:84-87)
+__device__ __forceinline__ bool sor_absorbs(const SorArgs &a, int u, int d) {
+    return a.ignn && (d == 0 || (d == 1 && a.loops[u] == 1));
 }
 
 // v active at the end of the sweep, listed once (mark)
 __device__ __forceinline__ bool active_claim(const SorArgs &a, int v) {
     const int d = a.degree[v];
-    return d > 0 && a.ea * (double)d <= fabs(a.r[v]) && a.mark[v] == 0 && atomicExch(&a.mark[v], 1) == 0;
+    const double rv = a.r[v];
+    const bool act = a.ignn ? (a.thr[v] <= fabs(rv) && rv != 0.0) : (d > 0 && a.ea * (double)d <= fabs(rv));
+    return act && a.mark[v] == 0 && atomicExch(&a.mark[v], 1) == 0;
 }
 
 // Work over bucketed entries [beg, end) (chunks of kSNT, their edges
@@ -458,6 +488,7 @@ __global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg,
                                                     unsigned int fend) {
     __shared__ int s_incl[kSNT];
     __shared__ int s_start[kSNT];
+    __shared__ int s_node[kSNT];
     __shared__ double s_val[kSNT];
     __shared__ int s_sh[kSNT / kWave];
     const int tid = threadIdx.x;
@@ -466,8 +497,20 @@ __global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg,
             const double rs = a.res[i];
             if (rs != 0.0) {
                 const int u = a.B[i];
-                a.x[u] += rs;                                // ppr_solver.py:405
-                (void)atomic_add_dev(&a.r[u], -rs);          // ppr_solver.py:406
+                if (a.ignn) {
+                    const int d = a.degree[u];
+                    if (sor_absorbs(a, u, d)) {              // InstantGNN.py:85-86: nothing else writes r_u
+                        a.x[u] = rs;
+                        a.r[u] = 0.0;
+                    } else {                                 // InstantGNN.py:89, 91 and the self-loops of :92-93
+                        a.x[u] += a.xcoef * rs;
+                        const double push_u = sor_push_value(a, rs, u, d);
+                        (void)atomic_add_dev(&a.r[u], (double)a.loops[u] * (push_u / a.dpb[u]) - rs);
+                    }
+                } else {
+                    a.x[u] += rs;                            // ppr_solver.py:405
+                    (void)atomic_add_dev(&a.r[u], -rs);      // ppr_solver.py:406
+                }
             }
         }
     }
@@ -475,7 +518,7 @@ __global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg,
     const unsigned int nitems = nchunks + (pend - pbeg);
     unsigned long long oper = 0, nodes = 0;
     for (unsigned int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        int st = 0, e_beg = 0, e_end = 0;
+        int st = 0, e_beg = 0, e_end = 0, pu = 0;
         double val = 0.0;
         const bool chunk = item < nchunks;
         if (chunk) {
@@ -489,11 +532,14 @@ __global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg,
                 bool act;
                 if (MODE == kPush) {
                     const double ru = a.r[u];
-                    act = d > 0 && (a.local ? (a.ea * (double)d <= fabs(ru)) : (ru != 0.0));
+                    act = sor_active(a, u, d, ru);
                     double rs = 0.0;
-                    if (act) {
+                    if (act && sor_absorbs(a, u, d)) {
+                        rs = ru;                             // applied by the finalize; no edges
+                        act = false;
+                    } else if (act) {
                         rs = a.omega * ru;                   // ppr_solver.py:404, katz_solver.py:299
-                        val = sor_push_value(a, rs, d);
+                        val = sor_push_value(a, rs, u, d);
                         if (a.local) {
                             oper += (unsigned long long)d;   // ppr_solver.py:401
                             nodes++;
@@ -517,6 +563,7 @@ __global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg,
             const int incl = block_incl_scan<kSNT>(cnt, s_sh, &tot);
             s_incl[tid] = incl;
             s_start[tid] = st;
+            s_node[tid] = u;
             s_val[tid] = val;
             __syncthreads();
             e_end = tot;
@@ -524,17 +571,16 @@ __global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg,
             const int64_t pc = a.pieces[pbeg + (item - nchunks)];
             const unsigned int i = (unsigned int)(pc >> 32);
             const int k = (int)(pc & 0xffffffffLL);
-            const int u = a.B[i];
-            const int d = a.degree[u];
+            pu = a.B[i];
+            const int d = a.degree[pu];
             if (MODE == kPush) {
-                const double ru = a.r[u];
-                const bool act = d > 0 && (a.local ? (a.ea * (double)d <= fabs(ru)) : (ru != 0.0));
-                if (!act) continue;   // uniform: every thread read the same entry
-                val = sor_push_value(a, a.omega * ru, d);   // bit-equal to the chunk's value
+                const double ru = a.r[pu];
+                if (!sor_active(a, pu, d, ru)) continue;   // uniform: every thread read the same entry
+                val = sor_push_value(a, a.omega * ru, pu, d);   // bit-equal to the chunk's value
             } else if (a.res[i] == 0.0) {
                 continue;
             }
-            st = a.indptr[u];
+            st = a.indptr[pu];
             e_beg = k * kSPiece;
             e_end = min(d, e_beg + kSPiece);
         }
@@ -549,13 +595,21 @@ __global__ __launch_bounds__(kSNT) void k_sor_items(SorArgs a, unsigned int beg,
                 v[q] = 0;
                 vv[q] = val;
                 if (ok[q]) {
+                    int src = pu;
                     if (chunk) {
                         const int k = upper_bound_lds(s_incl, 0, kSNT, e);
                         const int o = e - (k ? s_incl[k - 1] : 0);
                         v[q] = a.indices[s_start[k] + o];
                         vv[q] = s_val[k];
+                        src = s_node[k];
                     } else {
                         v[q] = a.indices[st + e];
+                    }
+                    if (a.ignn) {
+                        // self-loops go through the finalize (this colour's
+                        // residuals are read while it pushes); InstantGNN.py:93
+                        ok[q] = v[q] != src;
+                        if (MODE == kPush && ok[q]) vv[q] = vv[q] / a.dpb[v[q]];
                     }
                 }
             }
@@ -596,6 +650,25 @@ __global__ void k_sor_init(double *r, int32_t *F, SorCtr *c, int64_t s, double r
 
 __global__ void k_sor_add(double *x, int64_t s, double v) { x[s] += v; }
 
+// starting frontier of an InstantGNN column: the listed nodes (distinct), or
+// every node with eps_vec <= |r| and r != 0 (InstantGNN.py:118-123)
+__global__ __launch_bounds__(kSNT) void k_sor_seed(const int32_t *nodes, int64_t n_nodes, int64_t n,
+                                                   const double *thr, const double *r, int32_t *F,
+                                                   unsigned int *nf) {
+    const int64_t cnt = nodes ? n_nodes : n;
+    for (int64_t base = (int64_t)blockIdx.x * kSNT; base < cnt; base += (int64_t)gridDim.x * kSNT) {
+        const int64_t i = base + threadIdx.x;
+        bool act = false;
+        int v = 0;
+        if (i < cnt) {
+            v = nodes ? nodes[i] : (int)i;
+            act = nodes ? true : (thr[v] <= fabs(r[v]) && r[v] != 0.0);
+        }
+        const long long slot = wave_append(act, nf);
+        if (slot >= 0) F[slot] = v;
+    }
+}
+
 // live nodes at or above the threshold (the global loop's stopping test,
 // ppr_solver.py:449-452; isolated nodes are never counted)
 __global__ __launch_bounds__(kSNT) void k_sor_active(const double *r, const int32_t *degree, int64_t n, double ea,
@@ -619,6 +692,7 @@ static int sor_ws(flsg_graph g, SorWs **out) {
         FLSG_HIP(hipMalloc((void **)&w->colour, sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->sel, sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->longs, sizeof(int32_t) * n));
+        FLSG_HIP(hipMalloc((void **)&w->loops, sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->front[0], sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->front[1], sizeof(int32_t) * n));
         FLSG_HIP(hipMalloc((void **)&w->bucket, sizeof(int32_t) * n));
@@ -651,7 +725,7 @@ static int ensure_colouring(flsg_graph g, SorWs *w, hipStream_t st) {
     const int gridn = grid_of(g, (n + kSNT - 1) / kSNT, 8);
     FLSG_HIP(hipMemsetAsync(w->ctr, 0, sizeof(SorCtr), st));
     hipLaunchKernelGGL(k_col_init, dim3(gridn), dim3(kSNT), 0, st, g->indptr, g->indices, g->degree, n,
-                       w->colour, w->longs, w->ctr);
+                       w->colour, w->longs, w->loops, w->ctr);
     int rc = sor_sync_ctr(w, st);
     if (rc) return rc;
     w->selfloop = w->h_ctr->selfloop != 0;
@@ -776,6 +850,39 @@ static int colour_pass(flsg_graph g, SorWs *w, SorArgs a, const unsigned int *of
     return FLSG_OK;
 }
 
+// local iterations from the frontier in front[0] (count ctr->nf[0]) until
+// it is empty
+static int run_local(flsg_graph g, SorWs *w, SorArgs a, int64_t max_iters, hipStream_t st, int64_t *iters_out) {
+    a.B = w->bucket;
+    a.pieces = w->pieces;
+    a.mark = w->mark;
+    int64_t iters = 0;
+    int par = 0;
+    for (;;) {
+        int rc = bucket(g, w, w->front[par], &w->ctr->nf[par], w->mark, &w->ctr->nf[par ^ 1], w->bucket,
+                        w->pieces, st);
+        if (rc) return rc;
+        if (w->h_ctr->nf[par] == 0) break;
+        if (iters >= max_iters) {
+            *iters_out = iters;
+            return fail(FLSG_ENOCONV, "iteration cap reached");
+        }
+        iters++;
+        rc = colour_pass(g, w, a, w->h_offs, st);
+        if (rc) return rc;
+        // every entry of the iteration and all its pieces
+        const unsigned int ne = w->h_offs[w->ncol], np = w->h_offs[2 * w->ncol + 1];
+        a.next = w->front[par ^ 1];
+        a.nnext = &w->ctr->nf[par ^ 1];
+        const int gridc = grid_of(g, (int64_t)(ne + kSNT - 1) / kSNT + np, 8);
+        hipLaunchKernelGGL(k_sor_items<kCheck>, dim3(gridc), dim3(kSNT), 0, st, a, 0u, ne, 0u, np, 0u, 0u);
+        FLSG_HIP(hipGetLastError());
+        par ^= 1;
+    }
+    *iters_out = iters;
+    return FLSG_OK;
+}
+
 static int sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t s, double *d_x, double *d_r,
                      flsg_result *res, hipStream_t st) {
     SorWs *w = nullptr;
@@ -800,6 +907,8 @@ static int sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t s
     a.res = w->res;
     a.ctr = w->ctr;
     const bool katz = p->kind == FLSG_KATZ;
+    a.loops = w->loops;
+    a.xcoef = 1.0;
     a.coef = katz ? p->alpha : 1. - p->alpha;
     a.omega = omega;
     a.ea = katz ? p->eps : p->eps * p->alpha;   // katz_solver.py:259 eps * degree; ppr_solver.py:340
@@ -810,30 +919,10 @@ static int sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t s
     if (p->local) {
         FLSG_HIP(hipMemsetAsync(w->mark, 0, sizeof(int32_t) * n, st));
         hipLaunchKernelGGL(k_sor_init, dim3(1), dim3(1), 0, st, d_r, w->front[0], w->ctr, s, r0, 1);
-        a.B = w->bucket;
-        a.pieces = w->pieces;
-        a.mark = w->mark;
-        int par = 0;
-        for (;;) {
-            rc = bucket(g, w, w->front[par], &w->ctr->nf[par], w->mark, &w->ctr->nf[par ^ 1], w->bucket,
-                        w->pieces, st);
-            if (rc) return rc;
-            if (w->h_ctr->nf[par] == 0) break;
-            if (iters >= max_iters) {
-                res->iters = iters;
-                return fail(FLSG_ENOCONV, "iteration cap reached");
-            }
-            iters++;
-            rc = colour_pass(g, w, a, w->h_offs, st);
-            if (rc) return rc;
-            // every entry of the iteration and all its pieces
-            const unsigned int ne = w->h_offs[w->ncol], np = w->h_offs[2 * w->ncol + 1];
-            a.next = w->front[par ^ 1];
-            a.nnext = &w->ctr->nf[par ^ 1];
-            const int gridc = grid_of(g, (int64_t)(ne + kSNT - 1) / kSNT + np, 8);
-            hipLaunchKernelGGL(k_sor_items<kCheck>, dim3(gridc), dim3(kSNT), 0, st, a, 0u, ne, 0u, np, 0u, 0u);
-            FLSG_HIP(hipGetLastError());
-            par ^= 1;
+        rc = run_local(g, w, a, max_iters, st, &iters);
+        if (rc) {
+            res->iters = iters;
+            return rc;
         }
         res->oper = (int64_t)w->h_ctr->oper;
         res->nodes = (int64_t)w->h_ctr->nodes;
@@ -920,6 +1009,68 @@ extern "C" int flsg_sor_solve(flsg_graph g, const flsg_params *p, double omega, 
     }
     hipError_t es = hipEventRecord(e1, st);
     if (es == hipSuccess) es = hipEventSynchronize(e1);   // also reports a fault of the solve's kernels
+    float ms = 0.f;
+    if (es == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == FLSG_OK && es != hipSuccess) rc = fail(FLSG_EHIP, std::string("solve: ") + hipGetErrorString(es));
+    h_res->ms = ms;
+    h_res->status = rc;
+    return rc;
+}
+
+extern "C" int flsg_sor_ignn_column(flsg_graph g, double alpha, double omega, const double *d_eps_vec,
+                                    const double *d_dpow1, const double *d_dpowb, double *d_p, double *d_r,
+                                    const int32_t *d_nodes, int64_t n_nodes, int64_t max_iters,
+                                    flsg_result *h_res, void *stream) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (!d_eps_vec || !d_dpow1 || !d_dpowb || !d_p || !d_r || !h_res) return fail(FLSG_EARG, "NULL argument");
+    if (n_nodes < 0 || n_nodes > g->n || (n_nodes > 0 && !d_nodes)) return fail(FLSG_EARG, "bad node list");
+    if (!(omega > 0.0 && omega < 2.0)) return fail(FLSG_EARG, "omega must lie in (0, 2)");
+    if (!(alpha > 0.0 && alpha < 1.0)) return fail(FLSG_EARG, "alpha must lie in (0, 1)");
+    hipStream_t st = (hipStream_t)stream;
+    *h_res = flsg_result{};
+    SorWs *w = nullptr;
+    rc = sor_ws(g, &w);
+    if (rc == FLSG_OK) rc = ensure_colouring(g, w, st);
+    if (rc == FLSG_OK) rc = ensure_all_lists(g, w, st);   // sizes the piece buffers
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    FLSG_HIP(hipEventCreate(&e0));
+    FLSG_HIP(hipEventCreate(&e1));
+    FLSG_HIP(hipEventRecord(e0, st));
+    const int64_t n = g->n;
+    FLSG_HIP(hipMemsetAsync(w->ctr, 0, sizeof(SorCtr), st));
+    FLSG_HIP(hipMemsetAsync(w->mark, 0, sizeof(int32_t) * n, st));
+    const int64_t cnt = d_nodes ? n_nodes : n;
+    if (cnt > 0)
+        hipLaunchKernelGGL(k_sor_seed, dim3(grid_of(g, (cnt + kSNT - 1) / kSNT, 4)), dim3(kSNT), 0, st,
+                           d_nodes, n_nodes, n, d_eps_vec, (const double *)d_r, w->front[0], &w->ctr->nf[0]);
+    SorArgs a{};
+    a.indptr = g->indptr;
+    a.indices = g->indices;
+    a.degree = g->degree;
+    a.x = d_p;
+    a.r = d_r;
+    a.res = w->res;
+    a.ctr = w->ctr;
+    a.thr = d_eps_vec;
+    a.dp1 = d_dpow1;
+    a.dpb = d_dpowb;
+    a.loops = w->loops;
+    a.coef = 1. - alpha;
+    a.xcoef = alpha;   // InstantGNN.py:89 p_u += alpha * res_u
+    a.omega = omega;
+    a.local = 1;
+    a.ignn = 1;
+    int64_t iters = 0;
+    rc = run_local(g, w, a, max_iters > 0 ? max_iters : 100000, st, &iters);
+    h_res->iters = iters;
+    h_res->oper = (int64_t)w->h_ctr->oper;
+    h_res->nodes = (int64_t)w->h_ctr->nodes;
+    hipError_t es = hipEventRecord(e1, st);
+    if (es == hipSuccess) es = hipEventSynchronize(e1);
     float ms = 0.f;
     if (es == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);

@@ -81,6 +81,9 @@ SIGNATURES = {
     "flsg_boundary_bound": (C.c_double, [C.c_float, C.c_int32, C.c_int64]),
     "flsg_sor_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_double, C.c_int64, C.c_void_p, C.c_void_p,
                                  C.POINTER(Result), C.c_void_p]),
+    "flsg_sor_ignn_column": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                                       C.POINTER(Result), C.c_void_p]),
     "flsg_graph_colouring": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                        C.c_void_p]),
     "flsg_graph_insert_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,

@@ -203,6 +203,46 @@ def sor_solve(dg: DeviceGraph, alpha: float, eps: float, omega: float, seed: int
     return SolveOut(x=x, r=r, result=res.as_dict(), trace=None, extra={})
 
 
+def ignn_sor_columns(dg: DeviceGraph, alpha: float, eps: float, beta: float, omega: float, P, R, nodes=None,
+                     max_iters: int = 0) -> dict:
+    """Graph-coloured SOR push (instantGNNSorPush, InstantGNN.py:74-105) of
+    every column of the FP64 n x C device tensors P, R (updated in place),
+    one column at a time.  nodes: optional starting frontier (numpy, distinct
+    ids) shared by every column; by default each column starts from its
+    nodes above threshold.  Returns the summed device ms, iterations and
+    pushed degrees."""
+    torch = dg.torch
+    deg = dg.degree.to(torch.float64)
+    # eps_vec, d^(1-beta), d^beta as the reference builds them (np.power on
+    # the host, InstantGNN.py:118)
+    d_np = deg.cpu().numpy()
+    ev = torch.from_numpy(np.power(d_np, 1 - beta) * eps).to(dg.device)
+    dp1 = torch.from_numpy(np.power(d_np, 1 - beta)).to(dg.device)
+    dpb = torch.from_numpy(np.power(d_np, beta)).to(dg.device)
+    d_nodes = None
+    nn = 0
+    if nodes is not None:
+        nodes = np.unique(np.asarray(nodes, dtype=np.int32))
+        d_nodes = torch.from_numpy(nodes).to(dg.device)
+        nn = int(nodes.size)
+    out = {"ms": 0.0, "iters": 0, "oper": 0}
+    cols = 1 if P.dim() == 1 else P.shape[1]
+    for c in range(cols):
+        p = P if P.dim() == 1 else P[:, c].contiguous()
+        r = R if R.dim() == 1 else R[:, c].contiguous()
+        res = _abi.Result()
+        _check(dg.lib.flsg_sor_ignn_column(dg.handle, float(alpha), float(omega), _ptr(ev), _ptr(dp1), _ptr(dpb),
+                                           _ptr(p), _ptr(r), _ptr(d_nodes) if d_nodes is not None else None, nn,
+                                           int(max_iters), C.byref(res), _stream_ptr(torch)))
+        if P.dim() != 1:
+            P[:, c] = p
+            R[:, c] = r
+        out["ms"] += res.ms
+        out["iters"] += res.iters
+        out["oper"] += res.oper
+    return out
+
+
 def colouring(dg: DeviceGraph):
     """The SOR colouring of the device graph: (colour[n] numpy int32 in the
     caller's node ids, number of colours, rounds)."""

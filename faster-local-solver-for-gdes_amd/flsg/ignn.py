@@ -76,9 +76,11 @@ class DynamicPropagation:
         """instantGNNParallel: push from every node above threshold."""
         return self._propagate()
 
-    def insert(self, us, vs) -> PropagationStats:
+    def insert(self, us, vs, sor_omega: float | None = None) -> PropagationStats:
         """One snapshot: in-place edge insertion, invariant correction of the
-        endpoints (instantGNNUpdateBatch), then the push from them."""
+        endpoints (instantGNNUpdateBatch), then the push from them -- the
+        coloured SOR push per column (FP64) when sor_omega is given (the
+        reference's 'sor' algorithms, InstantGNN.py:320-322)."""
         torch = self.torch
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
@@ -98,7 +100,15 @@ class DynamicPropagation:
         _check(lib.flsg_ignn_update(self.dg.handle, C.byref(self.params), _ptr(self.X), _ptr(self.P),
                                     _ptr(self.R), _ptr(self.keys), nk.value, _ptr(self.old_degree),
                                     _stream_ptr(torch)))
-        st = self._propagate(self.keys, nk.value)
+        if sor_omega is None:
+            st = self._propagate(self.keys, nk.value)
+        else:
+            from .engine import ignn_sor_columns
+            keys = self.keys[:nk.value].cpu().numpy()
+            so = ignn_sor_columns(self.dg, self.params.alpha, self.params.eps, self.params.beta, sor_omega,
+                                  self.P, self.R, nodes=keys)
+            st = PropagationStats(ms=so["ms"], rounds=so["iters"], rounds_fp64=so["iters"], oper=so["oper"],
+                                  oper_cert=0, cert=0.0)
         e1.record()
         torch.cuda.synchronize()
         st.ms = e0.elapsed_time(e1)

@@ -262,6 +262,23 @@ int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, int64_t n_ke
 int flsg_sor_solve(flsg_graph g, const flsg_params *p, double omega, int64_t seed, double *d_x, double *d_r,
                    flsg_result *h_res, void *stream);
 
+/* Graph-coloured SOR push of one InstantGNN feature column on a (slack) CSR
+ * that may hold self-loops.  Replaces instantGNNSorPush (InstantGNN.py:74-105;
+ * instantGNNSorParallel, :108-134, calls it per column).  d_p, d_r: FP64 n,
+ * updated in place.  d_eps_vec = eps * d^(1-beta), d_dpow1 = d^(1-beta),
+ * d_dpowb = d^beta (device, n doubles each; the caller computes them as the
+ * reference does, np.power).  The frontier starts from d_nodes (n_nodes
+ * distinct ids) or, when d_nodes is NULL, from every node with
+ * eps_vec <= |r| and r != 0.  A pushed node u moves res = omega r_u:
+ * p_u += alpha res, and (1-alpha) res / d_u^(1-beta) / d_v^beta to each live
+ * neighbour v (its self-loops included); an isolated node or one whose only
+ * edge is a self-loop takes p_u = r_u, r_u = 0.  Ends with |r_u| < eps_vec[u]
+ * (r_u = 0 where eps_vec is 0).  max_iters <= 0: 1e5. */
+int flsg_sor_ignn_column(flsg_graph g, double alpha, double omega, const double *d_eps_vec,
+                         const double *d_dpow1, const double *d_dpowb, double *d_p, double *d_r,
+                         const int32_t *d_nodes, int64_t n_nodes, int64_t max_iters, flsg_result *h_res,
+                         void *stream);
+
 /* The colouring the SOR solve uses (Jones-Plassmann rounds, priority
  * (degree, hash), first-fit colours): d_colour_out (device, n int32, may be
  * NULL), *h_num_colours, *h_rounds.  No counterpart in the reference. */

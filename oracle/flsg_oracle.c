@@ -1227,3 +1227,143 @@ int ora_sor_coloured(int kind, int64_t n, const int64_t *indptr, const int32_t *
     if (nodes_out) *nodes_out = nodes;
     return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * InstantGNN SOR push for one feature column (InstantGNN.py:74-105,
+ * instantGNNSorPush).  Ring queue of length qn (the drivers pass n + 2),
+ * entries front+1 .. rear; a popped u is skipped while |r_u| < eps_vec[u]
+ * (:82-83); an isolated node, or one whose only edge is a self-loop, takes
+ * p_u = r_u, r_u = 0 (:84-87); otherwise res = omega r_u, p_u += alpha res,
+ * push = (1-alpha) res / d_u^(1-beta), r_u = (1-omega) r_u, and every live
+ * neighbour (self-loops included) gets push / d_v^beta with the enqueue test
+ * (:88-97); u is re-queued while active (:98-101).  pow() is the host libm's,
+ * as numba's ** on floats.  Returns the sum of degrees pushed in *oper_out.
+ * -------------------------------------------------------------------- */
+int ora_ignn_sor_push(int64_t qn, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                      double alpha, const double *eps_vec, double omega, double *p, double *r,
+                      int64_t *queue, int64_t front, int64_t rear, uint8_t *q_mark, double beta,
+                      int64_t *oper_out) {
+    int64_t oper = 0;
+    while (rear != front) {
+        front = (front + 1) % qn;                                               /* :79-81 */
+        const int64_t u = queue[front];
+        q_mark[u] = 0;
+        if (fabs(r[u]) < eps_vec[u]) continue;                                  /* :82-83 */
+        if ((degree[u] == 1 && indices[indptr[u]] == u) || degree[u] == 0) {   /* :84-87 */
+            p[u] = r[u];
+            r[u] = 0;
+            continue;
+        }
+        oper += degree[u];
+        const double res_u = omega * r[u];                                      /* :88-91 */
+        p[u] += alpha * res_u;
+        const double push_u = (1. - alpha) * res_u / pow((double)degree[u], 1 - beta);
+        r[u] = (1 - omega) * r[u];
+        for (int64_t j = indptr[u]; j < indptr[u] + degree[u]; j++) {          /* :92-97 */
+            const int64_t v = indices[j];
+            r[v] += push_u / pow((double)degree[v], beta);
+            if (!q_mark[v] && fabs(r[v]) >= eps_vec[v]) {
+                rear = (rear + 1) % qn;
+                queue[rear] = v;
+                q_mark[v] = 1;
+            }
+        }
+        if (!q_mark[u] && fabs(r[u]) >= eps_vec[u]) {                         /* :98-101 */
+            rear = (rear + 1) % qn;
+            queue[rear] = u;
+            q_mark[u] = 1;
+        }
+    }
+    if (oper_out) *oper_out = oper;
+    return ORA_OK;
+}
+
+/* Coloured InstantGNN SOR for one column: the GPU variant, restated.  The
+ * frontier starts as F0 (distinct nodes) and each iteration visits colours in
+ * order: a listed node u of colour c with eps_vec[u] <= |r_u| and r_u != 0
+ * either absorbs (isolated or self-loop only: p_u = r_u, r_u = 0) or takes
+ * res = omega r_u and sends (1-alpha) res / dp1[u] / dpb[v] to each live
+ * neighbour v != u; after the colour each pushed u applies p_u += alpha res
+ * and r_u += loops[u] * ((1-alpha) res / dp1[u] / dpb[u]) - res (its
+ * self-loops).  The next frontier is every pushed node or neighbour of one
+ * with eps_vec <= |r| and r != 0 at the end of the iteration.  dp1 / dpb are
+ * d^(1-beta) / d^beta, loops[u] the self-loops in u's row. */
+int ora_ignn_sor_coloured(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                          const int32_t *colour, int64_t ncol, double alpha, double omega,
+                          const double *eps_vec, const double *dp1, const double *dpb, const int32_t *loops,
+                          const int32_t *F0, int64_t nF0, double *p, double *r, int64_t max_iters,
+                          int64_t *oper_out, int64_t *iters_out) {
+    int64_t *F = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+    int64_t *G = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+    int64_t *B = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+    double *res = (double *)malloc(sizeof(double) * (n + 1));
+    int64_t *coff = (int64_t *)calloc(ncol + 1, sizeof(int64_t));
+    uint8_t *mark = (uint8_t *)calloc(n, 1);
+    uint8_t *seen = (uint8_t *)calloc(n, 1);
+    if (!F || !G || !B || !res || !coff || !mark || !seen) {
+        free(F); free(G); free(B); free(res); free(coff); free(mark); free(seen);
+        return ORA_ENOMEM;
+    }
+    int64_t nf = 0, oper = 0, iters = 0;
+    for (int64_t i = 0; i < nF0; i++) F[nf++] = F0[i];
+    int rc = ORA_OK;
+    while (nf > 0) {
+        if (iters >= max_iters) { rc = ORA_EARG; break; }
+        iters++;
+        for (int64_t i = 0; i < nf; i++) mark[F[i]] = 0;
+        memset(coff, 0, sizeof(int64_t) * (ncol + 1));
+        for (int64_t i = 0; i < nf; i++) coff[colour[F[i]] + 1]++;
+        for (int64_t c = 0; c < ncol; c++) coff[c + 1] += coff[c];
+        for (int64_t i = 0; i < nf; i++) B[coff[colour[F[i]]]++] = F[i];
+        for (int64_t c = ncol; c > 0; c--) coff[c] = coff[c - 1];
+        coff[0] = 0;
+        int64_t ng = 0;   /* pushed nodes and their neighbours, listed once */
+        for (int64_t c = 0; c < ncol; c++) {
+            for (int64_t i = coff[c]; i < coff[c + 1]; i++) {
+                const int64_t u = B[i];
+                res[i] = 0.0;
+                if (!(eps_vec[u] <= fabs(r[u]) && r[u] != 0.0)) continue;
+                const int d = degree[u];
+                if (d == 0 || (d == 1 && loops[u] == 1)) {
+                    res[i] = r[u];
+                    continue;
+                }
+                oper += d;
+                const double rs = omega * r[u];
+                res[i] = rs;
+                const double push_u = (1. - alpha) * rs / dp1[u];
+                for (int64_t j = indptr[u]; j < indptr[u] + d; j++) {
+                    const int64_t v = indices[j];
+                    if (v == u) continue;
+                    r[v] += push_u / dpb[v];
+                    if (!seen[v]) { seen[v] = 1; G[ng++] = v; }
+                }
+            }
+            for (int64_t i = coff[c]; i < coff[c + 1]; i++) {
+                const int64_t u = B[i];
+                const double rs = res[i];
+                if (rs == 0.0) continue;
+                const int d = degree[u];
+                if (d == 0 || (d == 1 && loops[u] == 1)) {
+                    p[u] = rs;
+                    r[u] = 0.0;
+                } else {
+                    p[u] += alpha * rs;
+                    const double push_u = (1. - alpha) * rs / dp1[u];
+                    r[u] += (double)loops[u] * (push_u / dpb[u]) - rs;
+                }
+                if (!seen[u]) { seen[u] = 1; G[ng++] = u; }
+            }
+        }
+        nf = 0;
+        for (int64_t i = 0; i < ng; i++) {
+            const int64_t v = G[i];
+            seen[v] = 0;
+            if (eps_vec[v] <= fabs(r[v]) && r[v] != 0.0 && !mark[v]) { mark[v] = 1; F[nf++] = v; }
+        }
+    }
+    free(F); free(G); free(B); free(res); free(coff); free(mark); free(seen);
+    if (oper_out) *oper_out = oper;
+    if (iters_out) *iters_out = iters;
+    return rc;
+}

@@ -24,6 +24,10 @@ reference function:
                                              hand-traced path graph
   katz_sor        katz_solver.py:252-337     pinned: the same matrix form with
                                              Q = I - bA, dense Katz solve
+  ignn_sor_column InstantGNN.py:74-105       pinned: propagation invariant,
+                                             certificate, hand trace
+  ignn_sor_coloured (this build's GPU variant, restated)  pinned: the same
+                                             invariant and certificate, hand trace
   colour          (this build's GPU colouring, restated) pinned: proper colouring,
                                              first-fit minimality, rounds by hand
   ppr_sor_coloured (this build's GPU SOR, restated)  pinned: one global sweep = the
@@ -91,6 +95,9 @@ def lib():
         L.ora_colour.argtypes = [i64, vp, vp, vp, vp, vp]
         L.ora_colour.restype = i64
         L.ora_katz_sor.argtypes = [i64, vp, vp, vp, i64, dbl, dbl, dbl, i32, vp, vp, vp, vp, vp]
+        L.ora_ignn_sor_push.argtypes = [i64, vp, vp, vp, dbl, vp, dbl, vp, vp, vp, i64, i64, vp, dbl, vp]
+        L.ora_ignn_sor_coloured.argtypes = [i64, vp, vp, vp, vp, i64, dbl, dbl, vp, vp, vp, vp, vp, i64, vp, vp,
+                                            i64, vp, vp]
         L.ora_sor_coloured.argtypes = [i32, i64, vp, vp, vp, vp, i64, i64, dbl, dbl, dbl, i32, i64, vp, vp, vp, vp,
                                        vp]
         L.ora_near_threshold.argtypes = []
@@ -337,6 +344,70 @@ def ppr_sor_coloured(g, s, alpha, eps, omega, col, ncol, local=True, max_iters=1
 
 def katz_sor_coloured(g, s, beta, eps, omega, col, ncol, local=True, max_iters=100000):
     return ppr_sor_coloured(g, s, beta, eps, omega, col, ncol, local=local, max_iters=max_iters, kind=1)
+
+
+def _slack_arrays(indptr, indices, degree):
+    return (np.ascontiguousarray(indptr, dtype=np.int64), np.ascontiguousarray(indices, dtype=np.int32),
+            np.ascontiguousarray(degree, dtype=np.int32))
+
+
+def ignn_eps_vec(degree, eps, beta):
+    """eps_vec = np.power(degree, 1 - beta) * eps (InstantGNN.py:57, 118)."""
+    return np.power(np.asarray(degree, dtype=np.float64), 1 - beta) * eps
+
+
+def ignn_sor_column(indptr, indices, degree, alpha, eps, omega, beta, p, r, nodes=None):
+    """instantGNNSorPush on one column (p, r FP64, updated in place), the
+    queue filled as instantGNNSorParallel does (every node with |r| >=
+    eps_vec, in node order, InstantGNN.py:118-123) or from ``nodes``.
+    Returns the sum of degrees pushed."""
+    ip, ix, dg = _slack_arrays(indptr, indices, degree)
+    n = dg.size
+    ev = ignn_eps_vec(dg, eps, beta)
+    qn = n + 2
+    queue = np.zeros(qn, dtype=np.int64)
+    q_mark = np.zeros(qn, dtype=np.uint8)
+    rear = 0
+    cand = np.arange(n) if nodes is None else np.asarray(nodes, dtype=np.int64)
+    for i in cand:
+        if np.abs(r[i]) >= ev[i] and not q_mark[i]:
+            rear = (rear + 1) % qn
+            queue[rear] = i
+            q_mark[i] = 1
+    oper = np.zeros(1, dtype=np.int64)
+    rc = lib().ora_ignn_sor_push(qn, _p(ip), _p(ix), _p(dg), float(alpha), _p(ev), float(omega), _p(p), _p(r),
+                                 _p(queue), 0, int(rear), _p(q_mark), float(beta), _p(oper))
+    if rc != ORA_OK:
+        raise RuntimeError(f"oracle error {rc}")
+    return int(oper[0])
+
+
+def self_loops(indptr, indices, degree):
+    ip, ix, dg = _slack_arrays(indptr, indices, degree)
+    rows = np.repeat(np.arange(dg.size), dg)
+    cols = np.concatenate([ix[ip[u]:ip[u] + dg[u]] for u in range(dg.size)]) if dg.size else np.zeros(0, int)
+    return np.bincount(rows[rows == cols], minlength=dg.size).astype(np.int32)
+
+
+def ignn_sor_coloured(indptr, indices, degree, col, ncol, alpha, eps, omega, beta, p, r, nodes,
+                      max_iters=100000):
+    """The GPU's coloured InstantGNN SOR on one column, restated (p, r
+    updated in place); nodes = the starting frontier (distinct)."""
+    ip, ix, dg = _slack_arrays(indptr, indices, degree)
+    d = dg.astype(np.float64)
+    ev = ignn_eps_vec(dg, eps, beta)
+    dp1 = np.power(d, 1 - beta)
+    dpb = np.power(d, beta)
+    loops = self_loops(ip, ix, dg)
+    F0 = np.ascontiguousarray(nodes, dtype=np.int32)
+    out = np.zeros(2, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    rc = lib().ora_ignn_sor_coloured(dg.size, _p(ip), _p(ix), _p(dg), _p(col), int(ncol), float(alpha),
+                                     float(omega), _p(ev), _p(dp1), _p(dpb), _p(loops), _p(F0), F0.size, _p(p),
+                                     _p(r), int(max_iters), _p(out[0:]), _p(out[1:]))
+    if rc != ORA_OK:
+        raise RuntimeError(f"oracle error {rc}")
+    return {"oper": int(out[0]), "iters": int(out[1])}
 
 
 def max_threads() -> int:

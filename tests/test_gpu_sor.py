@@ -286,3 +286,116 @@ def test_katz_sor_local_parity_large(large_graph, dev_large):
           f"rel L1 {rel:.2e} ms {out.result['ms']:.2f}")
     assert out.result["oper"] == oc["oper"] and rel <= REL_L1
     _katz_certified(g, s, beta, x, out.r.cpu().numpy(), 1e-7, tol=1e-14)
+
+
+# ------------------------------------------------------------------ InstantGNN (instantGNNSorPush)
+def _ignn_slack(n=400, p=0.02, seed=31, loops=(0, 5, 17)):
+    """Slack CSR (2 spare slots per row) of a random connected graph with
+    self-loops, one self-loop-only node and one isolated node."""
+    rng = np.random.default_rng(seed)
+    iu = np.triu_indices(n, 1)
+    keep = rng.random(iu[0].size) < p
+    g = CSRGraph.from_edges(n, np.concatenate([iu[0][keep], np.arange(n - 1)]),
+                            np.concatenate([iu[1][keep], np.arange(1, n)]))
+    rows = [list(g.indices[g.indptr[u]:g.indptr[u + 1]]) for u in range(n)]
+    for u in loops:
+        rows[u] = sorted(rows[u] + [u])
+    rows.append([n])
+    rows.append([])
+    N = len(rows)
+    cap = max(len(r) for r in rows) + 2
+    indptr = np.arange(0, cap * (N + 1), cap, dtype=np.int64)
+    indices = np.full(cap * N, N, dtype=np.int32)
+    degree = np.zeros(N, dtype=np.int32)
+    for u, rw in enumerate(rows):
+        indices[cap * u:cap * u + len(rw)] = rw
+        degree[u] = len(rw)
+    return indptr, indices, degree
+
+
+def _ignn_check(indptr, indices, degree, X, P, R, alpha, eps, beta):
+    from flsg.ignn import ignn_residual
+    loops = O.self_loops(indptr, indices, degree)
+    live = ~((degree == 0) | ((degree == 1) & (loops == 1)))
+    Rr = ignn_residual(indptr, indices, degree, X, P, alpha, beta)
+    # invariant R == X - P/alpha + ((1-alpha)/alpha) M P on pushing nodes
+    assert np.abs((Rr - R)[live]).max() < 1e-12
+    ev = O.ignn_eps_vec(degree, eps, beta)
+    assert np.all((np.abs(R) < ev[:, None]) | ((ev[:, None] == 0) & (R == 0)))
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_ignn_sor_column_parity(omega):
+    """One column against the coloured restatement (same colouring): equal
+    iteration and operation counts, p and r within 1e-12 relative L1; the
+    invariant and certificate on the host."""
+    import torch
+    ip, ix, dg = _ignn_slack()
+    N = dg.size
+    alpha, eps, beta = 0.1, 1e-7, 0.5
+    X = np.random.default_rng(7).standard_normal(N)
+    sg = graph.SlackGraph(n=N, indptr=ip, indices=ix, degree=dg, us=np.zeros((0, 0), np.int32),
+                          vs=np.zeros((0, 0), np.int32))
+    dev = engine.DeviceGraph(sg)
+    col, k, _ = engine.colouring(dev)
+    p = torch.zeros(N, dtype=torch.float64, device=dev.device)
+    r = torch.from_numpy(X.copy()).to(dev.device)
+    st = engine.ignn_sor_columns(dev, alpha, eps, beta, omega, p, r, max_iters=5000)
+    po, ro = np.zeros(N), X.copy()
+    ev = O.ignn_eps_vec(dg, eps, beta)
+    nodes = np.flatnonzero((np.abs(X) >= ev) & (X != 0))
+    oc = O.ignn_sor_coloured(ip, ix, dg, col, k, alpha, eps, omega, beta, po, ro, nodes)
+    assert st["iters"] == oc["iters"] and st["oper"] == oc["oper"], (st, oc)
+    pg, rg = p.cpu().numpy(), r.cpu().numpy()
+    assert np.abs(pg - po).sum() <= 1e-12 * np.abs(po).sum()
+    assert np.abs(rg - ro).sum() <= 1e-12 * max(np.abs(ro).sum(), 1e-300) + 1e-18
+    _ignn_check(ip, ix, dg, X[:, None], pg[:, None], rg[:, None], alpha, eps, beta)
+    dev.close()
+
+
+def test_ignn_sor_parallel_dropin():
+    """instantGNNSorParallel on 3 columns: P, R overwritten in place, the
+    invariant and certificate hold, and P is within the certificates of the
+    reference-order SOR (||p - p_ref||_1 <= ||K||_1 (||r||_1 + ||r_ref||_1))."""
+    import InstantGNN
+    ip, ix, dg = _ignn_slack(n=120, p=0.05, seed=32)
+    N = dg.size
+    alpha, eps, beta, omega = 0.1, 1e-6, 0.5, 1.2
+    X = np.random.default_rng(8).standard_normal((N, 3))
+    P = np.full((N, 3), 7.0)   # the reference starts every column from p = 0
+    R = X.copy()
+    times = InstantGNN.instantGNNSorParallel(N, ip, ix.copy(), dg.copy(), alpha, eps, omega, P, R, beta)
+    assert times.shape == (3,) and np.all(times >= 0)
+    _ignn_check(ip, ix, dg, X, P, R, alpha, eps, beta)
+    M = np.zeros((N, N))
+    for u in range(N):
+        for v in ix[ip[u]:ip[u] + dg[u]]:
+            M[v, u] += 1.0 / (dg[v] ** beta * dg[u] ** (1 - beta))
+    K = alpha * np.linalg.inv(np.eye(N) - (1 - alpha) * M)
+    kn = np.abs(K).sum(axis=0).max()
+    live = ~((dg == 0) | ((dg == 1) & (O.self_loops(ip, ix, dg) == 1)))
+    for c in range(3):
+        pr, rr = np.zeros(N), X[:, c].copy()
+        O.ignn_sor_column(ip, ix, dg, alpha, eps, omega, beta, pr, rr)
+        bound = kn * (np.abs(R[:, c]).sum() + np.abs(rr).sum())
+        assert np.abs((P[:, c] - pr)[live]).sum() <= bound + 1e-15
+
+
+def test_ignn_sor_update_insert():
+    """instantGNNUpdateBatchParallel with a 'sor' algorithm: after inserting
+    edges the invariant and certificate hold on the new graph."""
+    import InstantGNN
+    ip, ix, dg = _ignn_slack(n=150, p=0.03, seed=33)
+    N = dg.size
+    alpha, eps, beta = 0.1, 1e-6, 0.5
+    X = np.random.default_rng(9).standard_normal((N, 2))
+    P, R = np.zeros((N, 2)), X.copy()
+    ix, dg = ix.copy(), dg.copy()
+    InstantGNN.instantGNNParallel(N, ip, ix, dg, alpha, eps, P, R, beta)
+    rng = np.random.default_rng(10)
+    us = rng.integers(0, N - 2, size=10).astype(np.int32)
+    vs = rng.integers(0, N - 2, size=10).astype(np.int32)
+    keep = us != vs
+    InstantGNN.instantGNNUpdateBatchParallel(N, ip, ix, dg, alpha, eps, 1.2, P, R, X, us[keep], vs[keep], beta,
+                                             "sorbatchinsert")
+    _ignn_check(ip, ix, dg, X, P, R, alpha, eps, beta)

@@ -305,3 +305,88 @@ def test_coloured_katz_hand_traced_path():
     assert o["oper"] == 5 and o["iters"] == 3
     assert np.array_equal(o["x"], [0.0625, 0.25, 0.0625])
     assert np.array_equal(o["r"], [0.0, 0.03125, 0.0])
+
+
+# ------------------------------------------------------------------ InstantGNN SOR (instantGNNSorPush)
+def ignn_graph(n=30, p=0.15, seed=21, loops=(0, 3), isolated=True):
+    """Slack CSR (capacity 2 spare slots per row) of a random graph with
+    self-loops at ``loops``, a self-loop-only node and an isolated node."""
+    g = random_graph(n=n, p=p, seed=seed)
+    rows = [list(g.indices[g.indptr[u]:g.indptr[u + 1]]) for u in range(g.n)]
+    for u in loops:
+        rows[u] = sorted(rows[u] + [u])
+    rows.append([g.n])          # node n: only a self-loop
+    if isolated:
+        rows.append([])         # node n+1: isolated
+    N = len(rows)
+    cap = max(len(r) for r in rows) + 2
+    indptr = np.arange(0, cap * (N + 1), cap, dtype=np.int64)
+    indices = np.full(cap * N, N, dtype=np.int32)
+    degree = np.zeros(N, dtype=np.int32)
+    for u, rw in enumerate(rows):
+        indices[cap * u:cap * u + len(rw)] = rw
+        degree[u] = len(rw)
+    return indptr, indices, degree
+
+
+def ignn_operator(indptr, indices, degree, beta):
+    n = degree.size
+    M = np.zeros((n, n))
+    for u in range(n):
+        for v in indices[indptr[u]:indptr[u] + degree[u]]:
+            M[v, u] += 1.0 / (degree[v] ** beta * degree[u] ** (1 - beta))
+    return M
+
+
+def check_ignn_column(indptr, indices, degree, X, p, r, alpha, eps, beta):
+    n = degree.size
+    live = ~((degree == 0) | ((degree == 1) & (O.self_loops(indptr, indices, degree) == 1)))
+    M = ignn_operator(indptr, indices, degree, beta)
+    K = alpha * np.linalg.inv(np.eye(n) - (1 - alpha) * M)
+    # invariant p + K r = K X on the pushed part; absorbing nodes take p = X
+    assert np.abs((p + K @ r - K @ X)[live]).max() < 1e-13
+    assert np.array_equal(p[~live], X[~live]) and np.all(r[~live] == 0)
+    ev = O.ignn_eps_vec(degree, eps, beta)
+    assert np.all((np.abs(r) < ev) | ((ev == 0) & (r == 0)))   # isolated nodes: threshold 0, r = 0
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_ignn_sor_column_invariant(omega):
+    ip, ix, dg = ignn_graph()
+    alpha, beta, eps = 0.1, 0.5, 1e-6
+    X = np.random.default_rng(3).standard_normal(dg.size)
+    p, r = np.zeros(dg.size), X.copy()
+    O.ignn_sor_column(ip, ix, dg, alpha, eps, omega, beta, p, r)
+    check_ignn_column(ip, ix, dg, X, p, r, alpha, eps, beta)
+
+
+def test_ignn_sor_hand_traced():
+    """Node 0 with a self-loop and an edge to node 1 (degrees 2, 1), alpha =
+    1/2, beta = 0 (eps_vec = eps d = [1/4, 1/8]), eps = 1/8, omega = 1,
+    r = [1, 0].  Pop 0: p0 = 1/2, push (1/2)(1)/2 = 1/4 to 0 itself (r0 =
+    1/4, queued on equality) and to 1 (queued).  Pop 0: p0 = 5/8, push 1/16
+    -> r0 = 1/16 (not queued), r1 = 5/16.  Pop 1: p1 = 5/32, push 5/32 to 0
+    -> r0 = 7/32 < 1/4.  Queue empty."""
+    ip = np.array([0, 2, 3], dtype=np.int64)
+    ix = np.array([0, 1, 0], dtype=np.int32)
+    dg = np.array([2, 1], dtype=np.int32)
+    p, r = np.zeros(2), np.array([1.0, 0.0])
+    oper = O.ignn_sor_column(ip, ix, dg, 0.5, 0.125, 1.0, 0.0, p, r)
+    assert np.array_equal(p, [0.625, 0.15625]) and np.array_equal(r, [0.21875, 0.0]) and oper == 5
+    check_ignn_column(ip, ix, dg, np.array([1.0, 0.0]), p, r, 0.5, 0.125, 0.0)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_ignn_sor_coloured_invariant(omega):
+    from flsg.graph import CSRGraph
+    ip, ix, dg = ignn_graph(seed=22)
+    alpha, beta, eps = 0.1, 0.5, 1e-6
+    X = np.random.default_rng(4).standard_normal(dg.size)
+    live_rows = CSRGraph(n=dg.size, indptr=ip, indices=ix, degree=dg)
+    col, k, _ = O.colour(live_rows)
+    p, r = np.zeros(dg.size), X.copy()
+    ev = O.ignn_eps_vec(dg, eps, beta)
+    nodes = np.flatnonzero((np.abs(r) >= ev) & (r != 0))
+    o = O.ignn_sor_coloured(ip, ix, dg, col, k, alpha, eps, omega, beta, p, r, nodes)
+    assert o["iters"] > 0
+    check_ignn_column(ip, ix, dg, X, p, r, alpha, eps, beta)
