@@ -152,7 +152,8 @@ def synthetic_workload(snapshots: int = 16, cols: int = 128):
 
 
 def snapshot_bench(snapshots: int = 16, cols: int = 128, alpha: float = 0.1, eps: float = 1e-7, beta: float = 0.5,
-                   plain_cols: int = 8, log=print, dataset: str | None = None) -> list:
+                   plain_cols: int = 8, log=print, dataset: str | None = None, sor_cols: int = 0,
+                   batched: bool = True) -> list:
     """The dynamic feature-propagation workload: the 169,343-node citation
     graph with 950,000 edges held out and re-inserted in `snapshots` batches,
     X = 128 row-normalised N(0,1) FP32 feature columns.  After the initial
@@ -161,6 +162,9 @@ def snapshot_bench(snapshots: int = 16, cols: int = 128, alpha: float = 0.1, eps
     columns at once in FP64, and -- the per-column form of the reference,
     InstantGNN.py:332-359 -- one FP64 propagation per column for `plain_cols`
     columns (its per-snapshot time scaled to all columns is reported as such).
+    With `sor_cols` > 0 the coloured SOR push (instantGNNSorParallel and the
+    'sorbatchinsert' updates, omega from the experiment's alpha round-trip)
+    runs on that many columns, scaled to all columns the same way.
     With `dataset`, the graph, insert batches and features come from a file
     in the reference's experiment layout (graph.save_exp_npz).
     Returns one dict per (path, snapshot)."""
@@ -180,12 +184,26 @@ def snapshot_bench(snapshots: int = 16, cols: int = 128, alpha: float = 0.1, eps
         out.append(d)
         log(d)
 
-    for prec, name in ((_abi.FLSG_FP32, "batched-fp32-cert"), (_abi.FLSG_FP64, "batched-fp64")):
+    for prec, name in ((_abi.FLSG_FP32, "batched-fp32-cert"), (_abi.FLSG_FP64, "batched-fp64")) if batched else ():
         prop = DynamicPropagation(sg.copy(), X, alpha, eps, beta, precision=prec)
         try:
             rec(name, -1, prop.initial())
             for k in range(snapshots):
                 rec(name, k, prop.insert(sg.us[k], sg.vs[k]))
+        finally:
+            prop.close()
+    if sor_cols > 0:
+        from .engine import ignn_sor_columns
+        omega = _graph.alpha_roundtrip(alpha)[1]
+        prop = DynamicPropagation(sg.copy(), X[:, :sor_cols], alpha, eps, beta, precision=_abi.FLSG_FP64)
+        try:
+            so = ignn_sor_columns(prop.dg, alpha, eps, beta, omega, prop.P, prop.R)
+            st = PropagationStats(ms=so["ms"], rounds=so["iters"], rounds_fp64=so["iters"], oper=so["oper"],
+                                  oper_cert=0, cert=0.0)
+            rec(f"coloured-sor-fp64 omega {omega:.4f} (x{cols}/{sor_cols})", -1, st, scale=cols / sor_cols)
+            for k in range(snapshots):
+                rec(f"coloured-sor-fp64 omega {omega:.4f} (x{cols}/{sor_cols})", k,
+                    prop.insert(sg.us[k], sg.vs[k], sor_omega=omega), scale=cols / sor_cols)
         finally:
             prop.close()
     if plain_cols > 0:
@@ -211,6 +229,8 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="dynamic feature-propagation snapshot benchmark (one GPU)")
     ap.add_argument("--snapshots", type=int, default=16)
     ap.add_argument("--plain-cols", type=int, default=8)
+    ap.add_argument("--sor-cols", type=int, default=0, help="columns of the coloured SOR path (0 = off)")
+    ap.add_argument("--skip-batched", action="store_true", help="only the per-column paths")
     ap.add_argument("--out", default=None)
     ap.add_argument("--dataset", default=None, help="workload file in the reference's experiment layout")
     ap.add_argument("--write-dataset", default=None, help="write the synthetic workload to this file and exit")
@@ -229,7 +249,8 @@ if __name__ == "__main__":
             fh.write(line + "\n")
             fh.flush()
 
-    snapshot_bench(snapshots=args.snapshots, plain_cols=args.plain_cols, log=_log, dataset=args.dataset)
+    snapshot_bench(snapshots=args.snapshots, plain_cols=args.plain_cols, log=_log, dataset=args.dataset,
+                   sor_cols=args.sor_cols, batched=not args.skip_batched)
     if fh:
         fh.close()
     sys.exit(0)
