@@ -1596,8 +1596,16 @@ extern "C" int flsg_batch_set_degree_bins(flsg_graph g, int32_t split_edges, int
 extern "C" int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes) {
     int rc = check_graph(g);
     if (rc) return rc;
-    if (bytes < -1 || bytes > 120 * 1024 || (bytes > 0 && bytes % 8 != 0))
-        return fail(FLSG_EARG, "hub cache bytes: -1, 0 or a multiple of 8 up to 122880");
+    // the largest hub cache: the device's LDS per workgroup (160 KiB on
+    // gfx950) less the static LDS of the 1024-thread kernel
+    int lds_max = 0;
+    FLSG_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device));
+    hipFuncAttributes fa;
+    FLSG_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_batch<1024>)));
+    const int64_t cap = ((int64_t)lds_max - (int64_t)fa.sharedSizeBytes) / 8 * 8;
+    if (bytes < -1 || bytes > cap || (bytes > 0 && bytes % 8 != 0))
+        return fail(FLSG_EARG, "hub cache bytes: -1, 0 or a multiple of 8 up to " + std::to_string(cap) +
+                                   " (LDS per workgroup less the kernel's static LDS)");
     g->batch_hub_bytes = bytes;
     return FLSG_OK;
 }

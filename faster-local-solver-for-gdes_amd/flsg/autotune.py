@@ -15,7 +15,7 @@ import time
 import numpy as np
 
 from . import _abi
-from .engine import DeviceGraph, _check, batch_solve
+from .engine import DeviceGraph, FlsgError, _check, batch_solve
 
 
 def _apply(dg: DeviceGraph, cfg: dict) -> None:
@@ -41,7 +41,7 @@ def tune_batched(dg: DeviceGraph, params: _abi.Params, seeds, rounds: int = 2, l
         shapes.append((512, 2 * cus))
     space = [
         ("shape", shapes),
-        ("hub_bytes", [-1, 0, 64 * 1024, 122880]),
+        ("hub_bytes", [-1, 0, 64 * 1024, 122880, 136 * 1024, 143 * 1024]),
         ("split_edges", [256, 512, 1024, 2048]),
         ("piece_edges", [0, 2048, 4096, 8192]),
     ]
@@ -50,7 +50,11 @@ def tune_batched(dg: DeviceGraph, params: _abi.Params, seeds, rounds: int = 2, l
     seeds = np.asarray(seeds)
 
     def measure(c):
-        _apply(dg, c)
+        try:
+            _apply(dg, c)
+        except FlsgError:   # a setting this device cannot take (e.g. a hub cache beyond its LDS)
+            _apply(dg, cfg)
+            return 0.0
         best = 0.0
         for _ in range(rounds):
             bo = batch_solve(dg, params, seeds)

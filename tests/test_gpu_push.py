@@ -559,3 +559,20 @@ def test_autotune_small(small_graph):
         _check_batch(g, seeds, bo, 1e-6, n_check=8)
     finally:
         dg.close()
+
+
+@pytest.mark.parametrize("hub_bytes", [136 * 1024])
+def test_batch_hub_cache_beyond_120k_large(large_graph, dev_large, hub_bytes):
+    """A hub cache past 120 KiB (up to the 160 KiB of LDS per workgroup less
+    the kernel's static LDS) keeps every query certified; the whole-LDS
+    maximum is accepted and one byte more is refused."""
+    g = large_graph
+    eps = 1e-7
+    seeds = graph.mixed_seeds(g, 64, seed=21)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
+    try:
+        bo = engine.batch_solve(dev_large, p, seeds, hub_bytes=hub_bytes)
+        _check_batch(g, seeds, bo, eps, n_check=6)
+        assert dev_large.lib.flsg_batch_set_hub_cache(dev_large.handle, 160 * 1024) != 0
+    finally:
+        assert dev_large.lib.flsg_batch_set_hub_cache(dev_large.handle, -1) == 0
