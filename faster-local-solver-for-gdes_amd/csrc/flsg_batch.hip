@@ -50,6 +50,7 @@ constexpr int kMaxHkTerms = 96;      // heat-kernel Taylor order cap of a batche
 constexpr unsigned kInQ = 1u;        // flags: in the correction queue
 constexpr unsigned kSeen = 2u;       // flags: listed in the support
 constexpr unsigned kTouched = 4u;    // flags: in the query's touched list (reset at the end)
+constexpr unsigned kBound = 8u;      // flags: support record certified by sup_bound, not recomputed
 constexpr int kSink = 64;            // per-slot sink records (d = 0) for the idle lanes of an edge batch
 
 // Boundary certificate of the FP32 push.  A node the push reached but never
@@ -86,6 +87,44 @@ __host__ __device__ __forceinline__ double lazy_bound(float rl, int d, int64_t i
     return lazy_g(d, it) * fabs((double)rl) + ((double)d * (double)it + 1.0) * 0x1p-125;
 }
 
+// Error bound of a pushed node v certified without the FP64 pull.  The true
+// residual of the returned x32 is r_v = b_v - x32_v + S_v with
+// S_v = sum over neighbours u of c_u x32_u (c_u = (1-alpha)/d_u, Katz beta),
+// and the FP32 push tracked r32_v; with u = 2^-24, K pushes at most per node
+// and every FP32 add within 2u of its exact result:
+//   r_v - r32_v = -db_v - H_v - dx_v - E_v + sum_u c_u dx_u, where
+//   db_v  = fl32(b_v) - b_v                       |db| <= 2u b
+//   dx_v  = x32_v - (sum of the r moved out of v)   |dx_v| <= 2uK x32_v (1 + 4uK)
+//   E_v   = received push values - sum_u c_u (moved out of u)
+//   H_v   = rounding of every add into r32_v    |H_v| <= 2u d K max(x32_v(1+4uK), r32_v) + d K 2^-126
+// and |E_v| + sum_u c_u |dx_u| <= (2K + 1.01) u (1 + 4uK) S_v, while
+// S_v = r_v - b_v + x32_v <= |r32_v| + B + x32_v -- the residual identity
+// itself bounds the neighbour sum, so no gather is needed.  Solving for B:
+//   B (1 - g) <= 2u b + 2uK x (1+4uK) + g (|r32| + x) + 2u d K max(x(1+4uK), |r32|) + d K 2^-126,
+//   g = (2K + 1.01) u (1 + 4uK),
+// returned with a 1.01 margin.  It is small next to eps*alpha*d_v unless
+// x32_v is large (about x32 > 2e-3 at eps 1e-7 and K = 45): the seed and
+// its near neighbourhood stay in the FP64 pull.
+__host__ __device__ __forceinline__ double sup_bound(float x32, float rl, int d, int64_t K, double b) {
+    const double u = 0x1p-24;
+    const double x = fabs((double)x32), r = fabs((double)rl), k = (double)K, dd = (double)d;
+    const double c = 1.0 + 4.0 * u * k;
+    const double g = (2.0 * k + 1.01) * u * c;
+    const double num = 2.0 * u * b + 2.0 * u * k * x * c + g * (r + x) + 2.0 * u * dd * k * fmax(x * c, r) +
+                       dd * k * 0x1p-126;
+    return 1.01 * num / (1.0 - g);
+}
+
+// bound of a record with flags fl and FP32 residual field rl: a kBound
+// record keeps its support bound, rounded up, as -rl (its pending FP32
+// residual moved into r64 when it was certified); others the boundary bound
+__host__ __device__ __forceinline__ double rec_bound(uint32_t fl, float rl, int d, int64_t it) {
+    return (fl & kBound) ? fabs((double)rl) : lazy_bound(rl, d, it);
+}
+
+struct NodeRec;
+__device__ __attribute__((noinline)) bool certify_support(NodeRec *r, int64_t iters, double b, double thr_scale);
+
 struct __attribute__((aligned(32))) NodeRec {
     union {
         struct {
@@ -103,6 +142,28 @@ struct __attribute__((aligned(32))) NodeRec {
     double x64;         // FP64 estimate; FP64 heat kernel: odd-level residual
 };
 static_assert(sizeof(NodeRec) == 32, "node record must be 32 bytes");
+
+// Certifies a pushed node by sup_bound when |r32| + bound < (1 - 2^-8) thr:
+// r64 takes the pending FP32 residual (exactly), the FP32 field the bound as
+// -fl32_up(bound), x64 the FP32 estimate, flags kSeen | kBound.  Returns
+// whether it did (else the node joins the FP64 pull).  Out of line: the
+// bound's FP64 arithmetic would otherwise raise the persistent kernel's
+// register pressure around its hot loops.
+__device__ __attribute__((noinline)) bool certify_support(NodeRec *r, int64_t iters, double b, double thr_scale) {
+    float4 lo;
+    __builtin_memcpy(&lo, r, 16);
+    const int d = __float_as_int(lo.z);
+    const double bnd = sup_bound(lo.y, lo.x, d, iters, b);
+    if (!(fabs((double)lo.x) + bnd < kNearFrac * thr_scale * (double)d)) return false;
+    float bf = (float)bnd;
+    if ((double)bf < bnd) bf = __uint_as_float(__float_as_uint(bf) + 1u);   // next float up (bf > 0)
+    char *pr = reinterpret_cast<char *>(r);
+    const double2 hv = make_double2((double)lo.x, (double)lo.y);
+    __builtin_memcpy(pr + 16, &hv, 16);
+    const float4 lv = make_float4(-bf, lo.y, lo.z, __uint_as_float(kSeen | kBound));
+    __builtin_memcpy(pr, &lv, 16);
+    return true;
+}
 
 // A frontier entry as its node phase leaves it for the edge phase: row start,
 // edge count (0 = nothing to push) and push value -- one 16-byte load.
@@ -752,7 +813,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __shared__ WaveLds s_w[NT / kWave];
     __shared__ unsigned long long s_red[NT / kWave];
     __shared__ double s_redd[NT / kWave];
-    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl;
+    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl, s_ex;
     __shared__ unsigned long long s_base, s_vol;
     // hub cache: the residuals of nodes [0, H) live here for the whole query
     // (ids are degree-ranked on relabelled graphs, so these take the largest
@@ -1061,7 +1122,31 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 }
             }
             __syncthreads();
-            const unsigned int NLIST = S32 + s_ncnt;
+            // support rows the local bound certifies (sup_bound) keep their FP32
+            // residual: x64 = x32, flag kBound; the others are compacted to the
+            // front of sup[] / sst[] for the pull (positions never pass the
+            // entries still to be read: a chunk is read, then written)
+            if (tid == 0) s_ex = 0;
+            __syncthreads();
+            for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
+                const unsigned int i = i0 + tid;
+                int v = -1, st = 0;
+                bool exact = false;
+                if (i < S32) {
+                    v = sup[i];
+                    st = sst[i];
+                    exact = !certify_support(rec + v, iters, v == s ? P.r0 : 0.0, P.thr_scale);
+                }
+                __syncthreads();
+                const long long k = wave_append(exact, &s_ex);
+                if (k >= 0) {
+                    sup[k] = v;
+                    sst[k] = st;
+                }
+                __syncthreads();
+            }
+            const unsigned int E32 = s_ex;
+            const unsigned int NLIST = E32 + s_ncnt;
             // a listed record, finished: r64 = b - x + (pulled pushes) in FP64,
             // x64 = x32, FP32 residual field -0.0 (exact record), support flag
             // for the pushed nodes, queued for the correction push if |r64| >= thr
@@ -1088,11 +1173,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             pull_rows(
                 NLIST,
                 [&](unsigned int i, int &v, int &st) {
-                    if (i < S32) {
+                    if (i < E32) {
                         v = sup[i];
                         st = sst[i];
                     } else {
-                        v = qn[i - S32];
+                        v = qn[i - E32];
                         st = indptr[v];
                     }
                 },
@@ -1152,7 +1237,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     e.st = 0;
                     e.d = 0;
                     e.v = 0.0;
-                    if (fabs(tmp) + lazy_bound(rl, d, iters) >= P.thr_scale * (double)d) {
+                    if (fabs(tmp) + rec_bound(fl, rl, d, iters) >= P.thr_scale * (double)d) {
                         ru.r64 = 0.0;
                         if (rl > 0.0f) ru.r = -rl;
                         ru.x64 = ru.x64 + tmp;
@@ -1187,7 +1272,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                         // records the main push never reached still hold r == +0.0
                         if (old == 0.0 && pristine32(rl) && (fl_or(&rv.fl, kTouched) & kTouched) == 0) res = 2;
                         const double eff = nw + (rl > 0.0f ? (double)rl : 0.0);
-                        if (fabs(eff) + lazy_bound(rl, dv, iters) >= P.thr_scale * (double)dv &&
+                        const double bv = rec_bound(rv.fl, rl, dv, iters);
+                        if (fabs(eff) + bv >= P.thr_scale * (double)dv &&
                             (fl_or(&rv.fl, kInQ) & kInQ) == 0)
                             res |= 1;
                         return res;
@@ -1261,7 +1347,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 // (pending, rl > 0) and carries the error bound of that sum
                 const float rl = P.fp64 ? 0.0f : lo[j].x;
                 const double res = hi[j].x + (rl > 0.0f ? (double)rl : 0.0);
-                const double bnd = lazy_bound(rl, d, iters);
+                const double bnd = P.fp64 ? 0.0 : rec_bound(__float_as_uint(lo[j].w), rl, d, iters);
                 if (res != 0.0 || bnd != 0.0) cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
                 is_sup[j] = P.fp64 ? hi[j].y != 0.0 : (__float_as_uint(lo[j].w) & kSeen) != 0u;
                 if (is_sup[j] && a.out_map) id[j] = a.out_map[vv[j]];
@@ -1572,6 +1658,10 @@ using namespace flsg;
 extern "C" int64_t flsg_batch_slot_bytes(int64_t n) { return slot_bytes(n); }
 
 extern "C" double flsg_boundary_bound(float r32, int32_t d, int64_t iters) { return lazy_bound(r32, d, iters); }
+
+extern "C" double flsg_support_bound(float x32, float r32, int32_t d, int64_t iters, double b) {
+    return sup_bound(x32, r32, d, iters, b);
+}
 
 extern "C" int flsg_batch_set_slots(flsg_graph g, int32_t slots) {
     int rc = check_graph(g);
