@@ -231,6 +231,13 @@ int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queri
  * reference (the certificate's error bound, exposed for tests). */
 double flsg_boundary_bound(float r32, int32_t d, int64_t iters);
 
+/* The error bound a pushed node of the batched FP32 path is certified with
+ * when it is not recomputed: |r_v - r32_v| <= flsg_support_bound(x32_v,
+ * r32_v, d_v, iters, b_v), r_v the exact residual of the returned x32 and b_v
+ * the right-hand side at v (alpha or 1 at the seed, else 0).  Host-callable,
+ * for tests.  No counterpart in the reference. */
+double flsg_support_bound(float x32, float r32, int32_t d, int64_t iters, double b);
+
 /* Number of slot-arena records not back in the between-queries state (every
  * field zero except the degree copy) -- 0 after any completed launch.
  * Synchronises the stream.  No counterpart in the reference (a check of the

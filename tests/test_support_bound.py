@@ -1,0 +1,119 @@
+"""The error bound that certifies a pushed node of the batched FP32 path
+without an FP64 recompute (flsg_support_bound, the function the kernel
+evaluates).
+
+The kernel's FP32 push is restated on the 10,000-node power-law graph:
+Jacobi iterations; a frontier node u moves t = r32_u into x32_u
+(x32_u = fl32(x32_u + t)), sets r32_u = 0 and sends the push value
+fl32(fl64(fl64(1 - alpha) * t) / d_u) to each neighbour (r32_v =
+fl32(r32_v + value)); the next frontier is every node with r32 at or above
+eps*alpha*d.  Every FP32 addition rounds toward +infinity -- the bound
+allows 2u per add for atomics of any rounding mode, and rounding one way
+makes the errors add up instead of cancelling.  The exact residual of the
+returned x32, alpha e_s - x32 + (1 - alpha) A D^-1 x32, is taken in FP64 (its
+own rounding, ~1e-17 of x, is far below the bound) and every pushed node's
+|r - r32| must stay below flsg_support_bound(x32, r32, d, iterations, b).
+Measured: the largest error reaches 2-6% of the bound (eps 1e-5 and 1e-6,
+25-63 iterations), so the bound is neither violated nor loose by orders of
+magnitude; the boundary bound, which has no x term, does not cover these
+nodes (shown below).
+"""
+import numpy as np
+import pytest
+
+from flsg import _build
+from oracle import oracle as O
+
+ALPHA = 0.1
+OM = 1.0 - ALPHA
+
+
+def sup_bound(x32, r32, d, iters, b):
+    return _build.load_lib().flsg_support_bound(float(x32), float(r32), int(d), int(iters), float(b))
+
+
+def round_up32(s: float) -> np.float32:
+    f = np.float32(s)
+    if float(f) < s:
+        f = np.nextafter(f, np.float32(np.inf))
+    return f
+
+
+def fp32_push(g, s, eps, max_iters=100000):
+    """(x32, r32, iterations, pushed mask) of the FP32 push, additions rounded up."""
+    n = g.n
+    deg = np.asarray(g.degree, dtype=np.int64)
+    ip = np.asarray(g.indptr, dtype=np.int64)
+    ix = np.asarray(g.indices, dtype=np.int64)
+    thr = eps * ALPHA * deg.astype(np.float64)
+    x32 = np.zeros(n, dtype=np.float32)
+    r32 = np.zeros(n, dtype=np.float32)
+    r32[s] = np.float32(ALPHA)
+    pushed = np.zeros(n, dtype=bool)
+    front = [s]
+    iters = 0
+    while front and iters < max_iters:
+        iters += 1
+        vals = []
+        for u in front:
+            t = r32[u]
+            x32[u] = round_up32(float(x32[u]) + float(t))
+            r32[u] = np.float32(0.0)
+            vals.append(np.float32((OM * float(t)) / float(deg[u])))
+            pushed[u] = True
+        touched = set()
+        for u, val in zip(front, vals):
+            for v in ix[ip[u]:ip[u + 1]]:
+                r32[v] = round_up32(float(r32[v]) + float(val))
+                touched.add(int(v))
+        front = [v for v in sorted(touched) if thr[v] <= float(r32[v])]
+    return x32, r32, iters, pushed
+
+
+@pytest.fixture(scope="module")
+def runs(small_graph):
+    from flsg import graph
+    g = small_graph
+    out = []
+    seeds = [int(v) for v in graph.uniform_seeds(g, 3, seed=5)]
+    for s, eps in [(seeds[0], 1e-5), (seeds[1], 1e-5), (seeds[2], 1e-6)]:
+        x32, r32, iters, pushed = fp32_push(g, s, eps)
+        r = O.ppr_residual(g, s, ALPHA, x32.astype(np.float64))
+        out.append((s, x32, r32, iters, pushed, r))
+    return out
+
+
+def test_support_bound_covers_rounded_up_push(small_graph, runs):
+    g = small_graph
+    worst = 0.0
+    for s, x32, r32, iters, pushed, r in runs:
+        for v in np.flatnonzero(pushed):
+            b = ALPHA if v == s else 0.0
+            bnd = sup_bound(x32[v], r32[v], g.degree[v], iters, b)
+            err = abs(r[v] - float(r32[v]))
+            assert err <= bnd, (s, v, err, bnd)
+            worst = max(worst, err / bnd)
+    assert worst > 1e-2, worst   # within two orders of magnitude of the errors it covers
+
+
+def test_boundary_bound_alone_would_not_cover_pushed_nodes(small_graph, runs):
+    """The x-dependent terms are needed: the boundary bound (r32 only) is
+    exceeded at some pushed node."""
+    g = small_graph
+    lib = _build.load_lib()
+    exceeded = 0
+    for s, x32, r32, iters, pushed, r in runs:
+        for v in np.flatnonzero(pushed):
+            err = abs(r[v] - float(r32[v]))
+            if err > lib.flsg_boundary_bound(float(r32[v]), int(g.degree[v]), int(iters)):
+                exceeded += 1
+    assert exceeded > 0
+
+
+def test_support_bound_monotone_and_finite():
+    base = sup_bound(1e-3, 1e-9, 10, 40, 0.0)
+    assert np.isfinite(base) and base > 0
+    assert sup_bound(2e-3, 1e-9, 10, 40, 0.0) > base
+    assert sup_bound(1e-3, 1e-9, 20, 40, 0.0) > base
+    assert sup_bound(1e-3, 1e-9, 10, 80, 0.0) > base
+    assert sup_bound(1e-3, 1e-9, 10, 40, 0.1) > base
