@@ -79,6 +79,11 @@ __host__ __device__ __forceinline__ double lazy_g(int d, int64_t it) {
     return 1.03 * (2.0 * (double)d * (double)it + (double)it + 4.0) * 0x1p-24;
 }
 
+// lazy_bound holds while (2 d iters + iters + 4) u <= 1e-2 (the 1.03 margin)
+__host__ __device__ __forceinline__ bool lazy_valid(int d, int64_t it) {
+    return (2.0 * (double)d * (double)it + (double)it + 4.0) * 0x1p-24 <= 1e-2;
+}
+
 // error bound of a record whose FP32 residual field holds rl: > 0 a boundary
 // residual still pending, < 0 one the correction push has moved into x (its
 // error stays), +-0 an exact (FP64) record
@@ -1109,7 +1114,19 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     int v = -1;
                     if (i < lim) {
                         v = src[i];
-                        if (!pristine32(rec[v].x)) v = -1;   // pushed: already in the support list
+                        float4 lo;
+                        __builtin_memcpy(&lo, rec + v, 16);
+                        if (!pristine32(lo.y)) {
+                            v = -1;   // pushed: already in the support list
+                        } else if (lazy) {
+                            // a candidate listed for its degree (> kLazyDeg) whose
+                            // final FP32 residual the bound still certifies keeps it
+                            const int d = __float_as_int(lo.z);
+                            if (d > kLazyDeg && lazy_valid(d, iters) &&
+                                fabs((double)lo.x) + lazy_bound(lo.x, d, iters) <
+                                    kNearFrac * P.thr_scale * (double)d)
+                                v = -1;
+                        }
                     }
                     const long long k = wave_append(v >= 0, &s_ncnt);
                     if (k >= 0) qn[k] = v;
