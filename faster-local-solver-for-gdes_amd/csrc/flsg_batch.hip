@@ -59,29 +59,41 @@ constexpr int kSink = 64;            // per-slot sink records (d = 0) for the id
 // each neighbour pushes at most once per iteration).  Against the FP64 residual
 // of the returned x -- the pushed nodes' x32 -- r32 is off by at most
 //     g(d, iters) * r32 + (d * iters + 1) * 2^-125,
-//     g(d, iters) = 1.03 * (2 d iters + iters + 4) * 2^-24,
-// because (u = 2^-24):
+// with u = 2^-24, u' = u + 2^-51, gm(k) = 2ku / (1 - 2ku) and
+//     g = (gm(iters) + u' + gm(m) + u' gm(m)) / (1 - u' - gm(m) - u' gm(m)),
+// because, writing a_j = c_u rho for the exact push terms (rho the FP32
+// residuals the neighbours moved) and A = sum a_j:
 //   - every FP32 add (memory-side or LDS atomic) lands within 2u of its exact
-//     result, so the sum of m >= 0 terms is within 2mu / (1 - 2mu) of theirs;
-//   - every push value fl32(push(r)) is within u (+ 2^-52) of push(r);
-//   - every x32_u is an FP32 sum of at most iters residuals r >= 0, within
-//     iters * u / (1 - iters * u) of their sum;
-//   - a flushed denormal result loses at most 2^-126 per add;
-// and (2 d iters + iters + 4) u <= 1e-2 keeps the product of these factors
-// below the 1.03 margin.  Such a node keeps its FP32 residual (plus the bound)
-// when that certifies it; the rest are recomputed in FP64 like the support.
+//     result, so a sum of k >= 0 terms carries each term within gm(k);
+//   - every push value fl32(fl64(fl64(1 - alpha) rho) / d) is within u' of a_j;
+//   - so r32 = sum a_j (1 + eta_j), |eta_j| <= u' + gm(m) + u' gm(m), and the
+//     true residual sum_u c_u x32_u = sum a_j (1 + theta_j), |theta_j| <=
+//     gm(iters) (each x32_u an FP32 sum of at most iters terms), whence
+//     |r - r32| <= A (gm(iters) + max|eta|) and A <= r32 / (1 - max|eta|);
+//   - a flushed denormal result loses at most 2^-126 per add.
+// It holds while 2 m u < 1; lazy_valid asks 2 m u <= 1/4.  Such a node keeps
+// its FP32 residual (plus the bound) when that certifies it; the rest are
+// recomputed in FP64 like the support.
 constexpr int kLazyDeg = 64;                 // boundary nodes of higher degree are recomputed
 constexpr double kNearFrac = 0.99609375;     // 1 - 2^-8: boundary residuals above this fraction of
                                              // the threshold are recomputed
 constexpr double kLazyGmax = 0.0039;         // g(kLazyDeg, iters) limit: kNearFrac * (1 + g) < 1 - 1e-6
 
-__host__ __device__ __forceinline__ double lazy_g(int d, int64_t it) {
-    return 1.03 * (2.0 * (double)d * (double)it + (double)it + 4.0) * 0x1p-24;
+__host__ __device__ __forceinline__ double lazy_gm(double k) {
+    const double t = 2.0 * k * 0x1p-24;
+    return t / (1.0 - t);
 }
 
-// lazy_bound holds while (2 d iters + iters + 4) u <= 1e-2 (the 1.03 margin)
+__host__ __device__ __forceinline__ double lazy_g(int d, int64_t it) {
+    const double up = 0x1p-24 + 0x1p-51;
+    const double gm = lazy_gm((double)d * (double)it);
+    const double eta = up + gm + up * gm;
+    return (lazy_gm((double)it) + eta) / (1.0 - eta);
+}
+
+// where lazy_g is used: 2 m u <= 1/4 (m = d iters)
 __host__ __device__ __forceinline__ bool lazy_valid(int d, int64_t it) {
-    return (2.0 * (double)d * (double)it + (double)it + 4.0) * 0x1p-24 <= 1e-2;
+    return 2.0 * (double)d * (double)it * 0x1p-24 <= 0.25;
 }
 
 // error bound of a record whose FP32 residual field holds rl: > 0 a boundary
@@ -89,6 +101,7 @@ __host__ __device__ __forceinline__ bool lazy_valid(int d, int64_t it) {
 // error stays), +-0 an exact (FP64) record
 __host__ __device__ __forceinline__ double lazy_bound(float rl, int d, int64_t it) {
     if (rl == 0.0f) return 0.0;
+    if (!lazy_valid(d, it)) return __builtin_inf();   // never certifies
     return lazy_g(d, it) * fabs((double)rl) + ((double)d * (double)it + 1.0) * 0x1p-125;
 }
 
