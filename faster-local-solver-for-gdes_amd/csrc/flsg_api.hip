@@ -166,3 +166,19 @@ int flsg_graph_destroy(flsg_graph g) {
 }
 
 }  // extern "C"
+
+extern "C" int flsg_stream_poll(void *stream, int32_t *h_done) {
+    if (!h_done) return fail(FLSG_EARG, "h_done is NULL");
+    const hipError_t e = hipStreamQuery((hipStream_t)stream);
+    if (e == hipSuccess) {
+        *h_done = 1;
+        return FLSG_OK;
+    }
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();   // not-ready is a state, not an error to keep
+        *h_done = 0;
+        return FLSG_OK;
+    }
+    *h_done = 0;
+    return fail(FLSG_EHIP, std::string("stream poll: ") + hipGetErrorString(e));
+}

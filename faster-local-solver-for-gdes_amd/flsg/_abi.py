@@ -76,6 +76,7 @@ SIGNATURES = {
     "flsg_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
                                     C.c_void_p, C.c_void_p]),
+    "flsg_stream_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "flsg_batch_refresh_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "flsg_batch_arena_dirty": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "flsg_boundary_bound": (C.c_double, [C.c_float, C.c_int32, C.c_int64]),

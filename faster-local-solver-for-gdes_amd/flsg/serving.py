@@ -16,7 +16,8 @@ graph to the device on every call (ppr_solver.py:101-107).  Here:
 * edge-insert snapshots are applied between launches: the loop stops admitting,
   lets in-flight launches finish, inserts on the device and refreshes the slots'
   degree copies;
-* latency = completion (the host observes the lane's completion event) minus
+* latency = completion (the host observes, by flsg_stream_poll, that the
+  lane's stream has drained) minus
   arrival; p50 / p99 and queries/s are reported for the FP32-batched path and
   for the plain FP64 single-source path serving the same arrival stream.
 """
@@ -211,6 +212,13 @@ class Server:
         lane.busy = True
         lane.reqs = reqs
 
+    def _poll(self, lane: _Lane) -> bool:
+        """Whether the lane's launch and result copies have completed
+        (flsg_stream_poll on the lane's stream; never blocks)."""
+        done = C.c_int32(0)
+        _check(self.lib.flsg_stream_poll(C.c_void_p(lane.stream.cuda_stream), C.byref(done)))
+        return bool(done.value)
+
     def _complete(self, lane: _Lane, now: float, collect: bool, requeue: deque) -> int:
         res = (_abi.Result * len(lane.reqs)).from_buffer_copy(
             lane.h_res[:len(lane.reqs) * C.sizeof(_abi.Result)].numpy().tobytes())
@@ -279,7 +287,7 @@ class Server:
                 (pend_hk if has_hk_lane and r.kind == _abi.FLSG_HK else pend_short).append(r)
                 nxt += 1
             for lane in self.lanes:
-                if lane.busy and lane.event.query():
+                if lane.busy and self._poll(lane):
                     done += self._complete(lane, now, collect, requeue)
             while requeue:
                 r = requeue.pop()

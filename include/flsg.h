@@ -170,8 +170,9 @@ int flsg_batch_set_block(flsg_graph g, int32_t block);
  * (FP32 push) or [0, bytes/8) (FP64 push and certificate sums) for a whole
  * query: on degree-relabelled graphs these are the hubs, whose pushes then
  * use LDS atomics.  -1 = default for the workgroup size (120 KiB at 1024,
- * 48 KiB at 512, 24 KiB at 256), 0 = off, else a multiple of 8 up to
- * 120 KiB.  No counterpart in the reference (tuning knob). */
+ * 48 KiB at 512, 24 KiB at 256), 0 = off, else a multiple of 8 up to the
+ * device's LDS per workgroup less the kernel's static LDS (147,144 bytes on
+ * gfx950).  No counterpart in the reference (tuning knob). */
 int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes);
 
 /* Degree bins of the batched edge phases: a list entry with more than
@@ -215,19 +216,27 @@ typedef struct {
  * n_queries records) receives the results; nothing is synchronised.  Launches
  * whose slot ranges are disjoint may run concurrently on different streams
  * (the serving loop's continuous batching).  The graph must not be modified
- * while a launch runs. */
+ * while a launch runs.  This is the enqueue half of a serving session;
+ * flsg_stream_poll is the other. */
 int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queries, int32_t slot_base,
                       int32_t n_slots, int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off, int64_t out_cap,
                       uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters, const flsg_hk_config *hk,
                       void *stream);
+
+/* Poll half of a serving session: *h_done = 1 when every operation queued
+ * on `stream` so far (a flsg_batch_launch and the caller's copies of its
+ * results) has completed, 0 while any is still running; never blocks.  A
+ * fault of that work is returned as FLSG_EHIP.  No counterpart in the
+ * reference (it has no serving loop). */
+int flsg_stream_poll(void *stream, int32_t *h_done);
 
 /* Rounding-error bound the batched FP32 certificate applies to a boundary
  * node (reached by the push, never pushed) of degree d whose FP32 residual is
  * r32 after `iters` push iterations: |r_FP64 - r32| <= bound, r_FP64 being the
  * FP64 residual of the returned x (0 for r32 == 0; |r32| is used, the sign
  * marks a residual the correction push has moved into x).  Valid while
- * (2 d iters + iters + 4) * 2^-24 <= 1e-2; the kernels recompute every other
- * node in FP64.  Host function, no GPU needed.  No counterpart in the
+ * 2 d iters 2^-24 <= 1/4 (+inf beyond); the kernels recompute every node the
+ * bound does not certify in FP64.  Host function, no GPU needed.  No counterpart in the
  * reference (the certificate's error bound, exposed for tests). */
 double flsg_boundary_bound(float r32, int32_t d, int64_t iters);
 
