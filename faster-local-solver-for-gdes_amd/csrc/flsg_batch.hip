@@ -284,6 +284,18 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
 }
 
 template <int NT>
+__device__ __forceinline__ double block_sum_f64(double v, double *sh) {
+    v = wave_sum(v);
+    const int wid = threadIdx.x / kWave;
+    if (__lane_id() == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < NT / kWave; w++) t += sh[w];
+    __syncthreads();
+    return t;
+}
+
+template <int NT>
 __device__ __forceinline__ double block_max(double v, double *sh) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -1335,7 +1347,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         // skipped) instead of gathering the touched list's records one by one.
         if (tid == 0) s_ncnt = 0;
         __syncthreads();
-        double cmax = 0.0;
+        double cmax = 0.0, rl1 = 0.0;
         const bool stream = (int64_t)T * 4 > a.n;
         const unsigned int lim = stream ? (unsigned int)a.n : T;
         for (unsigned int i0 = 0; i0 < lim; i0 += NT * kOutUnroll) {
@@ -1378,7 +1390,10 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 const float rl = P.fp64 ? 0.0f : lo[j].x;
                 const double res = hi[j].x + (rl > 0.0f ? (double)rl : 0.0);
                 const double bnd = P.fp64 ? 0.0 : rec_bound(__float_as_uint(lo[j].w), rl, d, iters);
-                if (res != 0.0 || bnd != 0.0) cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
+                if (res != 0.0 || bnd != 0.0) {
+                    cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
+                    rl1 += fabs(res) + bnd;
+                }
                 is_sup[j] = P.fp64 ? hi[j].y != 0.0 : (__float_as_uint(lo[j].w) & kSeen) != 0u;
                 if (is_sup[j] && a.out_map) id[j] = a.out_map[vv[j]];
                 char *pr = reinterpret_cast<char *>(rec + vv[j]);
@@ -1399,6 +1414,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             }
         }
         cmax = block_max<NT>(cmax, s_redd);
+        rl1 = block_sum_f64<NT>(rl1, s_redd);
         const unsigned long long cops = block_sum_u64<NT>(cert_ops, s_red);
         if (tid == 0) {
             const uint64_t t_end = now_ticks();
@@ -1410,6 +1426,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             rr.supp = (int64_t)min(s_ncnt, S);   // entries written (== S: every support record is listed)
             rr.nodes = (int64_t)fnodes;
             rr.cert = cmax;
+            rr.r_l1 = rl1;
             rr.ms = (double)(t_end - t_start) * 1e-5;       // 100 MHz constant clock
             rr.ms_push = (double)(t_push - t_start) * 1e-5;
             rr.ms_recompute = (double)(t_rec - t_start) * 1e-5;

@@ -28,7 +28,7 @@ class Result(C.Structure):
         ("oper", C.c_int64), ("iters", C.c_int64), ("oper_cert", C.c_int64), ("n_corr", C.c_int64),
         ("supp", C.c_int64), ("nodes", C.c_int64), ("cert", C.c_double), ("ms", C.c_double),
         ("ms_push", C.c_double), ("status", C.c_int32), ("touched", C.c_int32),
-        ("ms_recompute", C.c_double), ("ms_correct", C.c_double),
+        ("ms_recompute", C.c_double), ("ms_correct", C.c_double), ("r_l1", C.c_double),
     ]
 
     def as_dict(self) -> dict:

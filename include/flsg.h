@@ -88,6 +88,10 @@ typedef struct {
                             residual recompute                                */
     double ms_correct;   /* batched FP32: time from the start to the end of the FP64
                             correction push (then: output and record reset)   */
+    double r_l1;         /* batched: sum over the touched records of |r_v| plus
+                            its FP32 error bound -- the L1 norm of the certified
+                            residual, so ||x - x*||_1 <= r_l1 / alpha for PPR
+                            (Katz: r_l1 / (1 - beta d_max))                  */
 } flsg_result;
 
 /* Optional per-iteration frontier trace (device buffers): iteration t

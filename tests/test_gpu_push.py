@@ -576,3 +576,22 @@ def test_batch_hub_cache_beyond_120k_large(large_graph, dev_large, hub_bytes):
         assert dev_large.lib.flsg_batch_set_hub_cache(dev_large.handle, 160 * 1024) != 0
     finally:
         assert dev_large.lib.flsg_batch_set_hub_cache(dev_large.handle, -1) == 0
+
+
+def test_batch_residual_l1_bounds_the_error(small_graph, dev_small):
+    """Each query's r_l1 (device sum of |r_v| plus the FP32 error bounds)
+    covers the host FP64 residual's L1 norm, and r_l1 / alpha covers the L1
+    distance to the FP64 global solve (residual 1e-15)."""
+    g = small_graph
+    eps = 1e-6
+    seeds = graph.mixed_seeds(g, 32, seed=13)
+    p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
+    bo = engine.batch_solve(dev_small, p, seeds)
+    for q in range(6):
+        s = int(seeds[q])
+        x = bo.dense(q, g.n)
+        rl1 = float(bo.results[q]["r_l1"])
+        rr = O.ppr_residual(g, s, ALPHA, x)
+        assert np.abs(rr).sum() <= rl1 * (1 + 1e-9) + 1e-18, (q, np.abs(rr).sum(), rl1)
+        xs = O.ppr_global_solve(g, s, ALPHA)
+        assert np.abs(x - xs).sum() <= rl1 / ALPHA * (1 + 1e-9) + 1e-14
