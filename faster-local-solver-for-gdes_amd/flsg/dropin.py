@@ -100,9 +100,10 @@ def per_seed_sor(n, indptr, indices, degree, s_nodes, alpha, eps, omega, local, 
 
 def sor_not_built(name: str):
     raise NotImplementedError(
-        f"{name}: SOR / Gauss-Southwell stay CPU reference baselines (they are "
-        "inherently sequential; a graph-coloured GPU variant would have to meet the same residual "
-        "certificate before replacing them).")
+        f"{name}: not built on the GPU.  With omega != 1 the reference re-queues a node within a "
+        "level only while it is absent from the next level's queue (hk_solver.py:262), so how often "
+        "it is pushed depends on the FIFO order; the level push is built for omega = 1, and the "
+        "graph-coloured SOR covers ppr_sor, katz_sor and instantGNNSorPush.")
 
 
 P = _abi
