@@ -1165,28 +1165,23 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             }
             __syncthreads();
             // support rows the local bound certifies (sup_bound) keep their FP32
-            // residual: x64 = x32, flag kBound; the others are compacted to the
-            // front of sup[] / sst[] for the pull (positions never pass the
-            // entries still to be read: a chunk is read, then written)
+            // residual: x64 = x32, flag kBound; the others are listed in nl[]
+            // (the boundary candidates are consumed above) for the pull.  No
+            // block barrier in the loop: every wave runs its rows on its own.
             if (tid == 0) s_ex = 0;
             __syncthreads();
             for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
                 const unsigned int i = i0 + tid;
-                int v = -1, st = 0;
+                int v = -1;
                 bool exact = false;
                 if (i < S32) {
                     v = sup[i];
-                    st = sst[i];
                     exact = !certify_support(rec + v, iters, v == s ? P.r0 : 0.0, P.thr_scale);
                 }
-                __syncthreads();
                 const long long k = wave_append(exact, &s_ex);
-                if (k >= 0) {
-                    sup[k] = v;
-                    sst[k] = st;
-                }
-                __syncthreads();
+                if (k >= 0) nl[k] = v;
             }
+            __syncthreads();
             const unsigned int E32 = s_ex;
             const unsigned int NLIST = E32 + s_ncnt;
             // a listed record, finished: r64 = b - x + (pulled pushes) in FP64,
@@ -1215,13 +1210,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             pull_rows(
                 NLIST,
                 [&](unsigned int i, int &v, int &st) {
-                    if (i < E32) {
-                        v = sup[i];
-                        st = sst[i];
-                    } else {
-                        v = qn[i - E32];
-                        st = indptr[v];
-                    }
+                    v = i < E32 ? nl[i] : qn[i - E32];
+                    st = indptr[v];
                 },
                 rec, colx, xc, xn, hw, pw, ent, pcs, &s_npc, &s_nlong, &s_work, W, push_of, finish, cert_ops);
             __syncthreads();
