@@ -399,3 +399,23 @@ def test_ignn_sor_update_insert():
     InstantGNN.instantGNNUpdateBatchParallel(N, ip, ix, dg, alpha, eps, 1.2, P, R, X, us[keep], vs[keep], beta,
                                              "sorbatchinsert")
     _ignn_check(ip, ix, dg, X, P, R, alpha, eps, beta)
+
+
+def test_sor_iteration_cap_then_recovers(small_graph, dev_small):
+    """A solve stopped by its iteration cap raises, and the next solve on the
+    same resident graph (its workspace reset at the start of every solve)
+    equals a solve that ran before the failure: iteration and operation
+    counts equal, estimate within 1e-12 relative L1."""
+    g = small_graph
+    s = int(graph.uniform_seeds(g, 1, seed=11)[0])
+    for local in (True, False):
+        before = engine.sor_solve(dev_small, ALPHA, 1e-6, OMEGA_OPT, s, local=local, max_iters=5000)
+        assert before.result["iters"] > 2
+        with pytest.raises(FlsgError):
+            engine.sor_solve(dev_small, ALPHA, 1e-6, OMEGA_OPT, s, local=local, max_iters=2)
+        after = engine.sor_solve(dev_small, ALPHA, 1e-6, OMEGA_OPT, s, local=local, max_iters=5000)
+        assert after.result["iters"] == before.result["iters"]
+        assert after.result["oper"] == before.result["oper"]
+        xb, xa = before.x.cpu().numpy(), after.x.cpu().numpy()
+        assert np.abs(xa - xb).sum() / np.abs(xb).sum() <= REL_L1
+        _certified(g, s, xa, after.r.cpu().numpy(), 1e-6)
