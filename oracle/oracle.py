@@ -9,14 +9,18 @@ reference function:
 
   ppr_gd          ppr_solver.py:208-328      pinned: dense solve identity,
                                              hand-traced tiny graphs, invariants
-  ppr_cheby       ppr_solver.py:582-661      pinned: dense solve, coefficient
-                                             closed form, hand-traced init
-  katz_gd         katz_solver.py:167-248     pinned: dense solve (I - bA)^-1
-  katz_cheby      katz_solver.py:340-431     pinned: dense solve (converged)
+  ppr_cheby       ppr_solver.py:582-661      pinned: exact-rational local hand
+                                             trace (queues, x, r), global sweeps =
+                                             Chebyshev semi-iterates (closed form)
+  katz_gd         katz_solver.py:167-248     pinned: dense Jacobi frontier replay
+                                             (eps*d thresholds), dense solve
+  katz_cheby      katz_solver.py:340-431     pinned: as ppr_cheby, Katz interval
   estimate_n      hk_solver.py:28-48         pinned: Taylor-tail bound property
   hk_taylor       hk_solver.py:51-80         pinned: scipy expm
   hk_push         hk_solver.py:206-295       pinned: level identity vs. the
-                                             truncated Taylor series
+                                             truncated Taylor series, dense level
+                                             replay with closed-form psi, hk-relax
+                                             bound vs expm
   ignn_column     InstantGNN.py:12-71        pinned: invariant p + a(I-(1-a)M)^-1 r
   update_graph    InstantGNN.py:137-180      pinned: resulting adjacency sets
   ppr_sor         ppr_solver.py:331-461      pinned: one global sweep = the dense SOR
@@ -36,7 +40,9 @@ reference function:
   ignn_update     InstantGNN.py:298-329      pinned: invariant on the new graph
 The reference itself cannot run here (numba / cupy are not importable:
 an ordinary ImportError, not a denial), so the pins are the mathematics and
-hand-worked cases in tests/test_oracle.py.
+hand-worked cases in tests/test_oracle.py and tests/test_oracle_pins.py;
+oracle/mutate.py shows that each of a list of plausible restatement slips
+fails at least one of them.
 
 numpy helpers (``ppr_dense_solve``, ``ppr_residual`` ...) give the FP64 global
 solves and residual recomputations the certificates are checked against.
@@ -50,7 +56,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libflsg_oracle.so")
+# FLSG_ORACLE_LIB names another build of the same source (oracle/mutate.py
+# loads deliberately altered copies through it to show which test fails)
+LIB_PATH = os.environ.get("FLSG_ORACLE_LIB") or os.path.join(HERE, "libflsg_oracle.so")
 
 ORA_OK, ORA_EASSERT, ORA_ENOMEM, ORA_EARG, ORA_ETRACE = 0, 1, 2, 3, 4
 
