@@ -18,6 +18,8 @@ is not built (NotImplementedError).
 """
 from __future__ import annotations
 
+from collections import OrderedDict
+
 import numpy as np
 
 from flsg import _abi
@@ -28,7 +30,28 @@ from flsg.ignn import DynamicPropagation
 __all__ = ["instantGNNParallel", "instantGNNUpdateBatchParallel", "updetaGraph", "instantGNNPush",
            "instantGNNSorPush", "instantGNNSorParallel", "instantGNNUpdateBatch", "instantGNNUpdateEdgeParallel"]
 
-_STATE: dict = {}
+# device state of the last propagations, for the updates that follow them:
+# id(R) -> (DynamicPropagation, key): the key holds alpha, eps, beta, the
+# features X the state was built for and the content fingerprint of P, R and
+# the graph as the call returned them.  An update reuses an entry only when
+# its arguments match that key exactly; otherwise it starts a new
+# propagation state from the arrays it is given.
+_STATE: "OrderedDict[int, tuple]" = OrderedDict()
+_STATE_MAX = 2
+
+
+def _state_key(alpha, eps, beta, fx, P, R, indptr, indices, degree):
+    return (float(alpha), float(eps), float(beta), fx) + _d.fingerprint(P, R, indptr, indices, degree)
+
+
+def _remember(R, prop, fp):
+    old = _STATE.pop(id(R), None)
+    if old is not None and old[0] is not prop:
+        old[0].close()
+    _STATE[id(R)] = (prop, fp)
+    while len(_STATE) > _STATE_MAX:
+        _, (p_old, _) = _STATE.popitem(last=False)
+        p_old.close()
 
 
 def _slack(indptr, indices, degree):
@@ -42,12 +65,13 @@ def instantGNNParallel(n, indptr, indices, degree, alpha, eps, P, R, beta, batch
     """InstantGNN.py:46-71.  P (zeros) and R (features) are n x F; both are
     updated in place.  Returns per-column runtimes (seconds)."""
     F = R.shape[1]
+    fx = _d.fingerprint(R)   # the features the state is built for
     prop = DynamicPropagation(_slack(indptr, indices, degree), X=R, alpha=float(alpha), eps=float(eps),
                               beta=float(beta), precision=_abi.FLSG_FP32, R0=R, P0=P)
     st = prop.initial()
     P[:] = prop.P.cpu().numpy()
     R[:] = prop.R.cpu().numpy()
-    _STATE[id(R)] = prop
+    _remember(R, prop, _state_key(alpha, eps, beta, fx, P, R, indptr, indices, degree))
     return np.full(F, st.ms * 1e-3 / F)
 
 
@@ -59,46 +83,61 @@ def instantGNNUpdateBatchParallel(n, indptr, indices, degree, alpha, eps, omega,
     if "insert" not in algo:
         raise NotImplementedError("edge deletion is not built on the GPU (algo must contain 'insert')")
     F = R.shape[1]
-    prop = _STATE.get(id(R))
-    if prop is None or prop.n != n or prop.cols != F:
+    fx = _d.fingerprint(X)
+    hit = _STATE.get(id(R))
+    prop = None
+    if (hit is not None and hit[0].n == n and hit[0].cols == F
+            and hit[1] == _state_key(alpha, eps, beta, fx, P, R, indptr, indices, degree)):
+        prop = hit[0]
+    if prop is None:
         prop = DynamicPropagation(_slack(indptr, indices, degree), X=X, alpha=float(alpha), eps=float(eps),
                                   beta=float(beta), precision=_abi.FLSG_FP32, R0=R, P0=P)
-        _STATE[id(R)] = prop
     st = prop.insert(us, vs, sor_omega=float(omega) if "sor" in algo else None)
     P[:] = prop.P.cpu().numpy()
     R[:] = prop.R.cpu().numpy()
     ind, deg = prop.graph_arrays()
     indices[:] = ind.astype(indices.dtype)
     degree[:] = deg.astype(degree.dtype)
+    _remember(R, prop, _state_key(alpha, eps, beta, fx, P, R, indptr, indices, degree))
     return np.full(F, st.ms * 1e-3 / F)
 
 
 def updetaGraph(indptr, indices, degree, us, vs, onlyinsert=True):
-    """InstantGNN.py:137-180 (insert only): mutates indices/degree in place and
-    returns (insert_neighbor, delete_neighbor) dicts {u: [new neighbours]}."""
+    """InstantGNN.py:137-180 (insert only): mutates indices/degree in place
+    with the reference's byte layout (each row's new neighbours appended in
+    batch order) and returns (insert_neighbor, delete_neighbor) dicts
+    {u: [new neighbours]}, keys in first-insertion order.  A self-loop the
+    reference would insert raises FlsgError (FLSG_EARG; the reference would
+    leave an unwritten entry in the live row) and nothing is inserted;
+    endpoints outside [0, n) raise ValueError."""
     if not onlyinsert:
         raise NotImplementedError("edge deletion is not built on the GPU")
     from flsg.engine import DeviceGraph, _check, _ptr, _stream_ptr
+    from flsg.graph import edge_ids
     import ctypes as C
     g = _slack(indptr, indices, degree)
+    hu, hv = edge_ids(us, vs, g.n)
     dg = DeviceGraph(g)
     torch = dg.torch
-    d_us = torch.from_numpy(np.ascontiguousarray(us, dtype=np.int32)).to(dg.device)
-    d_vs = torch.from_numpy(np.ascontiguousarray(vs, dtype=np.int32)).to(dg.device)
+    d_us = torch.from_numpy(hu).to(dg.device)
+    d_vs = torch.from_numpy(hv).to(dg.device)
     keys = torch.empty(max(2 * d_us.numel(), 1), dtype=torch.int32, device=dg.device)
     old = torch.empty(g.n, dtype=torch.int32, device=dg.device)
     nk, ni = C.c_int64(0), C.c_int64(0)
-    _check(dg.lib.flsg_graph_insert_edges(dg.handle, _ptr(d_us), _ptr(d_vs), int(d_us.numel()), _ptr(keys),
-                                          C.byref(nk), _ptr(old), C.byref(ni), _stream_ptr(torch)))
-    ind = dg.indices.cpu().numpy()
-    deg = dg.degree.cpu().numpy()
-    od = old.cpu().numpy()
-    ins = {}
-    for u in sorted(keys[:nk.value].cpu().numpy().tolist()):
-        ins[u] = ind[g.indptr[u] + od[u]:g.indptr[u] + deg[u]].tolist()
+    try:
+        _check(dg.lib.flsg_graph_insert_edges(dg.handle, _ptr(d_us), _ptr(d_vs), int(d_us.numel()), _ptr(keys),
+                                              C.byref(nk), _ptr(old), C.byref(ni), _stream_ptr(torch)))
+        ind = dg.indices.cpu().numpy()
+        deg = dg.degree.cpu().numpy()
+        od = old.cpu().numpy()
+        # keys in first-insertion order; each row's appended tail is its list
+        ins = {}
+        for u in keys[:nk.value].cpu().numpy().tolist():
+            ins[u] = ind[g.indptr[u] + od[u]:g.indptr[u] + deg[u]].tolist()
+    finally:
+        dg.close()
     indices[:] = ind.astype(indices.dtype)
     degree[:] = deg.astype(degree.dtype)
-    dg.close()
     return ins, {}
 
 

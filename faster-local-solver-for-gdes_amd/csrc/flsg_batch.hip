@@ -1762,15 +1762,21 @@ extern "C" int flsg_batch_reserve(flsg_graph g, int32_t slots, void *stream) {
     return colx_ensure(g, (hipStream_t)stream);
 }
 
+namespace flsg {
+int batch_degrees_changed(flsg_graph g, const int32_t *d_rows, int64_t n_rows, hipStream_t st) {
+    BatchArena *A = g->arena;
+    if (!A || n_rows <= 0) return FLSG_OK;
+    hipLaunchKernelGGL(k_slot_degrees, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree,
+                       g->n, A->slots, d_rows, n_rows);
+    FLSG_HIP(hipGetLastError());
+    return FLSG_OK;
+}
+}  // namespace flsg
+
 extern "C" int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, int64_t n_keys, void *stream) {
     int rc = check_graph(g);
     if (rc) return rc;
-    BatchArena *A = g->arena;
-    if (!A || n_keys <= 0) return FLSG_OK;
-    hipLaunchKernelGGL(k_slot_degrees, dim3(g->num_cus * 8), dim3(256), 0, (hipStream_t)stream, A->rec,
-                       (const int32_t *)g->degree, g->n, A->slots, d_keys, n_keys);
-    FLSG_HIP(hipGetLastError());
-    return FLSG_OK;
+    return batch_degrees_changed(g, d_keys, n_keys, (hipStream_t)stream);
 }
 
 extern "C" int flsg_batch_arena_dirty(flsg_graph g, int64_t *h_count, void *stream) {

@@ -119,6 +119,8 @@ int64_t host_estimate_n(double t, double eps);
 int check_graph(flsg_graph g);
 void batch_free(flsg_graph g);
 int colx_refresh(flsg_graph g, hipStream_t st);
+// after an edge insertion: the batched slots' copies of the listed rows' degrees
+int batch_degrees_changed(flsg_graph g, const int32_t *d_rows, int64_t n_rows, hipStream_t st);
 void cheb_free(flsg_graph g);
 void ignn_free(flsg_graph g);
 void dyn_free(flsg_graph g);

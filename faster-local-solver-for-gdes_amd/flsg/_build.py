@@ -14,6 +14,7 @@ import ctypes
 import glob
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(PKG_DIR, "csrc")
@@ -54,9 +55,24 @@ def build_graphgen() -> str:
 
 
 def build_lib() -> str:
+    """One object per source, compiled in parallel, then linked (each
+    translation unit's device code is self-contained: no relocatable device
+    code)."""
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    objdir = os.path.join(CSRC, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def obj(src):
+        o = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+        _run([HIPCC, *cflags, f"-I{INCLUDE}", "-c", "-o", o, src])
+        return o
+
+    workers = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(obj, srcs))
     tmp = LIB_PATH + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", "-o", tmp, *srcs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -208,6 +208,19 @@ def citation_dynamic_graph(n: int = 169_343, m_directed: int = 1_166_243, delnum
                       name=f"citation-n{n}")
 
 
+def edge_ids(us, vs, n: int) -> tuple[np.ndarray, np.ndarray]:
+    """Edge endpoints as int32 arrays, checked to lie in [0, n) before the
+    cast (an id of 2^31 or more would otherwise wrap to another node)."""
+    a = np.asarray(us).reshape(-1)
+    b = np.asarray(vs).reshape(-1)
+    if a.shape != b.shape:
+        raise ValueError("us and vs must have the same length")
+    for x in (a, b):
+        if x.size and (x.dtype.kind not in "iu" or int(x.min()) < 0 or int(x.max()) >= n):
+            raise ValueError(f"edge endpoints must be integers in [0, {n})")
+    return np.ascontiguousarray(a, dtype=np.int32), np.ascontiguousarray(b, dtype=np.int32)
+
+
 def demo_seeds(n: int, test_num: int = 50, seed: int = 17) -> np.ndarray:
     """``demo.py:19,45-46``: ``np.random.seed(17); np.unique(randint(n, 50))``."""
     rs = np.random.RandomState(seed)

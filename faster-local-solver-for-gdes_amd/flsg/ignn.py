@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _abi
 from .engine import DeviceGraph, _check, _ptr, _stream_ptr
-from .graph import SlackGraph
+from .graph import SlackGraph, edge_ids
 
 
 class IgnnParams(C.Structure):
@@ -85,8 +85,9 @@ class DynamicPropagation:
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        d_us = torch.from_numpy(np.ascontiguousarray(us, dtype=np.int32)).to(self.dg.device)
-        d_vs = torch.from_numpy(np.ascontiguousarray(vs, dtype=np.int32)).to(self.dg.device)
+        hu, hv = edge_ids(us, vs, self.n)
+        d_us = torch.from_numpy(hu).to(self.dg.device)
+        d_vs = torch.from_numpy(hv).to(self.dg.device)
         k = int(d_us.numel())
         if self.keys.numel() < 2 * k:
             self.keys = torch.empty(2 * k, dtype=torch.int32, device=self.dg.device)

@@ -309,13 +309,21 @@ int flsg_graph_colouring(flsg_graph g, int32_t *d_colour_out, int32_t *h_num_col
 
 /* In-place insertion of k undirected edges (d_us[i], d_vs[i]) into a slack-CSR
  * handle.  Replaces updetaGraph(onlyinsert=True) (InstantGNN.py:137-180,
- * ppr_solver.py:723-766): an edge already present (or repeated in the batch)
- * is skipped.  If any row's slack would overflow nothing is written and
- * FLSG_ECAP is returned.  Outputs (any may be NULL): d_keys_out (device, up to
- * 2k entries) the endpoints that gained edges, *n_keys their number,
- * d_old_degree (device, n) the degrees before the batch, *n_inserted the
- * number of edges added.  The new neighbours of u are
- * indices[indptr[u]+old_degree[u] : indptr[u]+degree[u]]. */
+ * ppr_solver.py:723-766) with the same bytes: pairs are taken in batch order,
+ * a pair whose v is already in u's live row (present before the batch, or
+ * inserted by an earlier pair of it, in either orientation) is skipped, and
+ * each row's new neighbours are appended in batch order.  Nothing is written
+ * and an error is returned if an endpoint is outside [0, n) (FLSG_EARG), if a
+ * self-loop would be inserted (FLSG_EARG: the reference raises the degree by
+ * two but writes one slot, leaving an unwritten entry in the live row; a
+ * self-loop already present is skipped), or if a row's slack would overflow
+ * (FLSG_ECAP).  Outputs (any may be NULL): d_keys_out (device, up to 2k
+ * entries) the endpoints that gained edges in the reference's insert_neighbor
+ * key order (first insertion), *n_keys their number, d_old_degree (device, n)
+ * the degrees before the batch, *n_inserted the number of edges added.  The
+ * new neighbours of u -- its insert_neighbor list -- are
+ * indices[indptr[u]+old_degree[u] : indptr[u]+degree[u]].  The slot arena's
+ * degree copies of those rows are refreshed in the same call. */
 int flsg_graph_insert_edges(flsg_graph g, const int32_t *d_us, const int32_t *d_vs, int64_t k, int32_t *d_keys_out,
                             int64_t *n_keys, int32_t *d_old_degree, int64_t *n_inserted, void *stream);
 

@@ -146,3 +146,41 @@ def test_instantgnn_column_dropins():
     InstantGNN.instantGNNUpdateBatch(n, ip, ind, deg, old, p, r, x, ALPHA, queue, 0, 0, q_mark, eps_vec, ins, dele,
                                      algo="fwd_insert", omega=1, beta=beta)
     _certified(ip, ind, deg, x, p, r, eps, beta)
+
+
+def test_dropin_sees_arrays_changed_in_place(small_graph):
+    """The resident-graph cache is keyed by content: after the caller rewires
+    edges in the same arrays (a degree-preserving double edge swap, rows kept
+    sorted), ppr_gd / katz_gd solve the new graph."""
+    g = small_graph
+    ip, ind, deg = g.indptr.copy(), g.indices.copy(), g.degree.copy()
+    s = int(graph.uniform_seeds(g, 1, seed=41)[0])
+    xt0, _, oper0, _ = ppr_solver.ppr_gd(g.n, ip, ind, deg, s, ALPHA, 1e-6, True, None)
+    nbrs = lambda u: set(ind[ip[u]:ip[u + 1]].tolist())
+    # a swap among the seed's neighbourhood, so the PPR vector changes
+    a = s
+    b = int(ind[ip[a]])
+    rng = np.random.default_rng(0)
+    for c in rng.permutation(g.n):
+        c = int(c)
+        if c in (a, b) or deg[c] == 0:
+            continue
+        d = int(ind[ip[c]])
+        if d in (a, b) or d in nbrs(a) or b in nbrs(c):
+            continue
+        break
+    for u, old, new in ((a, b, d), (b, a, c), (c, d, b), (d, c, a)):
+        row = ind[ip[u]:ip[u + 1]]
+        row[row == old] = new
+        row.sort()
+    g2 = graph.CSRGraph(n=g.n, indptr=ip, indices=ind, degree=deg)
+    xt1, _, oper1, _ = ppr_solver.ppr_gd(g.n, ip, ind, deg, s, ALPHA, 1e-6, True, None)
+    o = O.ppr_gd(g2, s, ALPHA, 1e-6, local=True)
+    assert oper1 == o["oper"] and _rel_l1(xt1, o["x"]) <= 1e-12
+    assert _rel_l1(xt1, xt0) > 1e-6
+    # (beta, eps) with no enqueue test within an ulp of its threshold, so the
+    # operation counts must agree exactly
+    xk, _, ok, _ = katz_solver.katz_gd(g.n, ip, ind, deg, s, 0.0037, 1.3e-8, True)
+    ko = O.katz_gd(g2, s, 0.0037, 1.3e-8, local=True)
+    assert ko["near_thr"] == 0
+    assert ok == ko["oper"] and _rel_l1(xk, ko["x"]) <= 1e-12

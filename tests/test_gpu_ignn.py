@@ -1,8 +1,9 @@
 """GPU InstantGNN propagation and in-place edge insertion.
 
 Bars:
-  * graph mutation (integer work): after a snapshot the device rows hold
-    exactly the oracle's rows as sets (row order is the device's);
+  * graph mutation (integer work): after a snapshot indices and degree equal
+    the oracle's updetaGraph byte for byte, and the changed rows come back in
+    the reference's insert_neighbor key order;
   * propagation: on return every entry satisfies |R| < eps d^(1-beta) in FP64
     AND R equals the host FP64 recompute X - P/alpha + ((1-alpha)/alpha) M P
     (relative 1e-9 of max|X|), which is the invariant the reference's push
@@ -76,25 +77,61 @@ def test_ignn_initial_small_dense(precision):
         assert diff < 1e-5   # p_o is float32 in the reference's initial pass
 
 
-def test_insert_edges_matches_oracle_rows():
+def _mixed_batch(sg, rng):
+    """A snapshot's pairs plus repeats in both orientations and pairs already
+    in the graph, shuffled."""
+    us, vs = sg.us[0], sg.vs[0]
+    rep = rng.choice(us.size, 60, replace=False)
+    rows = np.repeat(np.arange(sg.n), sg.degree)
+    live = np.concatenate([sg.indices[sg.indptr[u]:sg.indptr[u] + sg.degree[u]] for u in range(sg.n)])
+    ex = rng.choice(rows.size, 40, replace=False)
+    bu = np.concatenate([us, us[rep[:30]], vs[rep[30:]], rows[ex]]).astype(np.int32)
+    bv = np.concatenate([vs, vs[rep[:30]], us[rep[30:]], live[ex]]).astype(np.int32)
+    perm = rng.permutation(bu.size)
+    return bu[perm], bv[perm]
+
+
+def test_insert_edges_byte_exact():
+    """indices and degree equal the oracle's updetaGraph byte for byte (each
+    row's new neighbours appended in batch order, repeats and present pairs
+    skipped), and the endpoints come back in insert_neighbor key order."""
     sg = graph.citation_dynamic_graph(n=2000, m_directed=14000, delnum=3000, snapshots=4, seed=7)
     X = _features(sg.n, 4)
     prop = DynamicPropagation(sg.copy(), X, ALPHA, 1e-6, BETA, precision=_abi.FLSG_FP64)
     prop.initial()
     ind_o = sg.indices.copy()
     deg_o = sg.degree.copy()
-    us = np.concatenate([sg.us[0], sg.us[0][:50]])          # repeats are skipped
-    vs = np.concatenate([sg.vs[0], sg.vs[0][:50]])
+    us, vs = _mixed_batch(sg, np.random.default_rng(3))
     keys, off, nbr = O.update_graph(sg.indptr, ind_o, deg_o, us, vs)
     st = prop.insert(us, vs)
     ind, deg = prop.graph_arrays()
     assert np.array_equal(deg, deg_o)
+    assert np.array_equal(ind, ind_o)
     assert st.inserted == sg.us.shape[1]
-    assert sorted(keys.tolist()) == sorted(np.flatnonzero(deg != sg.degree).tolist())
-    for u in range(sg.n):
-        a = np.sort(ind[sg.indptr[u]:sg.indptr[u] + deg[u]])
-        b = np.sort(ind_o[sg.indptr[u]:sg.indptr[u] + deg_o[u]])
-        assert np.array_equal(a, b), u
+    assert np.array_equal(prop.keys[:keys.size].cpu().numpy(), keys)
+
+
+def test_updetagraph_dropin_returns_reference_dict():
+    import InstantGNN
+    from flsg.engine import FlsgError
+    sg = graph.citation_dynamic_graph(n=1500, m_directed=9000, delnum=2000, snapshots=4, seed=8)
+    ind_o, deg_o = sg.indices.copy(), sg.degree.copy()
+    ind, deg = sg.indices.copy(), sg.degree.copy()
+    us, vs = _mixed_batch(sg, np.random.default_rng(4))
+    keys, off, nbr = O.update_graph(sg.indptr, ind_o, deg_o, us, vs)
+    ins, dele = InstantGNN.updetaGraph(sg.indptr, ind, deg, us, vs)
+    assert dele == {}
+    assert list(ins.keys()) == keys.tolist()
+    assert all(ins[int(k)] == nbr[off[i]:off[i + 1]].tolist() for i, k in enumerate(keys))
+    assert np.array_equal(ind, ind_o) and np.array_equal(deg, deg_o)
+    # a self-loop the reference would insert fails and changes nothing
+    free = int(np.flatnonzero(deg < np.diff(sg.indptr))[0])
+    before = (ind.copy(), deg.copy())
+    with pytest.raises(FlsgError):
+        InstantGNN.updetaGraph(sg.indptr, ind, deg, np.array([0, free]), np.array([1, free]))
+    assert np.array_equal(ind, before[0]) and np.array_equal(deg, before[1])
+    with pytest.raises(ValueError):
+        InstantGNN.updetaGraph(sg.indptr, ind, deg, np.array([0]), np.array([sg.n]))
 
 
 @pytest.mark.parametrize("precision", [_abi.FLSG_FP64, _abi.FLSG_FP32])
