@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--no-relabel", action="store_true", help="keep the generator's node ids on the device")
     ap.add_argument("--hub-bytes", type=int, default=None,
                     help="LDS hub cache bytes per workgroup (-1 default, 0 off)")
+    ap.add_argument("--bin-min", type=int, default=None,
+                    help="binned push for iterations of at least this many edges (0 = never; default: the library's)")
+    ap.add_argument("--dense-min", type=int, default=None, help="binned push: pairs below which a bin takes atomics")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (workgroup size / slots / hub cache / degree bins)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -135,12 +138,17 @@ def main():
     # generator's ids); the plain FP64 path is timed on the same resident graph
     dg = engine.DeviceGraph(g, relabel=not args.no_relabel)
     seeds = graph.mixed_seeds(g, args.nq, seed=0)
+    if args.bin_min is not None or args.dense_min is not None:
+        from flsg.engine import _check
+        _check(dg.lib.flsg_batch_set_binning(dg.handle, 16384 if args.bin_min is None else args.bin_min,
+                                             256 if args.dense_min is None else args.dense_min))
     p32 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=args.alpha, eps=args.eps)
     p64 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=args.alpha, eps=args.eps)
 
     # ------------------------------------------------------------ autotune (untimed)
     tuned = None
-    if not args.no_autotune and not (args.slots or args.block or args.hub_bytes is not None):
+    if not args.no_autotune and not (args.slots or args.block or args.hub_bytes is not None
+                                     or args.bin_min is not None or args.dense_min is not None):
         from flsg import autotune
         t_at = time.perf_counter()
         tn = autotune.tune_batched(dg, p32, seeds[:min(args.nq, 512)], log=log)
@@ -169,7 +177,10 @@ def main():
     for k, ms in enumerate(kern):
         log(f"step {k}: kernel {ms:.1f} ms")
     log(f"last step, per query: push {res['ms_push'].mean():.2f} ms, recompute {res['ms_recompute'].mean():.2f}, "
-        f"correct {res['ms_correct'].mean():.2f}, total {res['ms'].mean():.2f} ms (times from the query's start)")
+        f"correct {res['ms_correct'].mean():.2f}, total {res['ms'].mean():.2f} ms (times from the query's start); "
+        f"push phases: node {res['ms_node'].mean():.2f}, pairs {res['ms_pairs'].mean():.2f}, "
+        f"bins {res['ms_bins'].mean():.2f}, direct edges {res['ms_direct'].mean():.2f} ms, "
+        f"{res['bin_iters'].mean():.1f} of {res['iters'].mean():.1f} iterations binned")
     certified = bool(np.all((res["status"] == 0) & (res["cert"] < 1.0)))
     ms_step = wall * 1e3 / args.steps
     qps = args.nq * args.steps / wall

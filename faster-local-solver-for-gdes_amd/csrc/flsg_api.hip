@@ -81,7 +81,7 @@ using namespace flsg;
 
 extern "C" {
 
-int flsg_version(void) { return 1; }
+int flsg_version(void) { return 2; }
 
 const char *flsg_last_error(void) { return g_last_error.c_str(); }
 

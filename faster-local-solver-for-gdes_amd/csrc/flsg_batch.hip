@@ -3,35 +3,41 @@
 // heat-kernel level push, mixed freely within a launch.
 //
 // Layout.  Workgroup b of a launch owns slot (slot_base + b) for the whole
-// launch.  A slot holds, per node v, one 32-byte record
-//     { float r; float x; int32 d; uint32 flags; double r64; double x64; }
-// so the residual atomic of a push, the degree the node phase needs and the
-// FP64 fields of that node share one line, 64 sink records (degree 0) for
-// idle lanes, the frontier double buffer, the support and touched lists, the
-// hub-piece list and the 16-byte frontier entries.  Records are all-zero
-// (except d) between queries: every query lists each record it touches once
+// launch.  A slot holds, per node (and for 64 sink entries after the n nodes,
+// the targets of idle lanes):
+//     rx32  {float r, x}    FP32 push state: residual and estimate, 8 bytes
+//     rx64  {double r, x}   FP64 push state (FP64 push, FP32 certificate)
+//     fl    uint32          certificate flags
+//     hk2   double          heat-kernel level residuals (FP32: two levels as floats)
+// so the FP32 push moves 8-byte node records -- half of what the FP64 push
+// moves -- with a node's residual and estimate on one line, and the
+// certificate's FP64 fields stay out of its lines; degrees and row starts
+// come from the graph's own arrays (shared by every slot).
+// Every field is zero between queries: a query lists each entry it touches
 // and resets exactly those.  Launches with disjoint slot ranges may run
 // concurrently on different streams.
 //
-// The residuals of the first ids (the hubs, on a degree-ranked graph) live in
-// an LDS hub cache for the whole query, so their pushes are LDS atomics: every
-// global atomic on this part is executed on the memory side, whatever its
-// locality.
-//
-// Per PPR / Katz query (workgroup-local, barriers are __syncthreads only):
-//   1. Local push (gradient descent = Jacobi push, ppr_solver.py:246-288 /
-//      katz_solver.py:183-224): node phase (r_u -> x_u, push value into the
-//      frontier entry), then an edge phase that streams the column entries
-//      {v, d_v}; v joins the next frontier iff the atomic that moved r_v from
-//      below to above thr_v = eps*alpha*d_v (FP64 compare) was this one
-//      (pushes are >= 0, so exactly one atomic sees it).
-//   2. FP32 precision only -- FP64 certificate: r64 = b - Q x recomputed from
-//      the FP32 x (support sweep, which also queues the crossings outside the
-//      support; then the support itself), FP64 correction push (mark-based
-//      queue: the recomputed residual has both signs) until none is left -- on
-//      return |r_v| < thr_v for every v, in FP64, on every query.
-//   3. One pass over the touched list: certificate value, output (support ids
-//      + FP64 values), record reset.
+// Push of one PPR / Katz iteration (gradient descent = Jacobi push,
+// ppr_solver.py:246-288 / katz_solver.py:183-224): the node phase moves r_u to
+// x_u and leaves each frontier entry's push value; then the edges, in one of
+// two ways chosen per iteration:
+//   * direct: each edge is one atomic add into r_v -- an LDS atomic for the
+//     hub ids [0, H) the LDS hub cache holds (degree-ranked ids: the hubs),
+//     a memory-side atomic for the rest (every global atomic on this part is
+//     executed on the memory side, at ~20 G/s whatever its locality);
+//   * binned (iterations of at least bin_min edges): the edges are written as
+//     (v, value) pairs into one bin per destination range of R = 2^s ids (a
+//     count pass, a scan, a write pass), and each bin is then summed into an
+//     LDS slice of R residuals (LDS atomics) and added to r in one coalesced
+//     sweep over the touched part of the range -- sparse bins (fewer than
+//     dense_min pairs) take memory-side atomics instead.  An iteration that
+//     touches ~10% of the graph costs ~30 streamed bytes per edge instead of
+//     a memory-side atomic per edge.
+// v joins the next frontier iff the add that moved r_v from below thr_v to at
+// least thr_v = eps*alpha*d_v (FP64 compare) was this one: pushes are >= 0,
+// so exactly one add sees the crossing (a bin's summed add counts as one).
+// The FP32 path then certifies its result in FP64 (below), and one pass over
+// the touched entries writes the certificate value, the output and the reset.
 // Queries are claimed through a per-launch counter, so uneven queries balance
 // across workgroups.
 #include <math.h>
@@ -42,6 +48,8 @@
 #include <utility>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "flsg_internal.h"
 
 namespace flsg {
@@ -50,8 +58,10 @@ constexpr int kMaxHkTerms = 96;      // heat-kernel Taylor order cap of a batche
 constexpr unsigned kInQ = 1u;        // flags: in the correction queue
 constexpr unsigned kSeen = 2u;       // flags: listed in the support
 constexpr unsigned kTouched = 4u;    // flags: in the query's touched list (reset at the end)
-constexpr unsigned kBound = 8u;      // flags: support record certified by sup_bound, not recomputed
-constexpr int kSink = 64;            // per-slot sink records (d = 0) for the idle lanes of an edge batch
+constexpr unsigned kBound = 8u;      // flags: support entry certified by sup_bound, not recomputed
+constexpr int kSink = 64;            // per-slot sink entries (degree 0) for the idle lanes of an edge batch
+constexpr int kBinShiftMin = 11;     // binned push: at least 2^11 ids per destination range
+constexpr int kMaxBins = 512;        // binned push: at most this many destination ranges (static LDS)
 
 // Boundary certificate of the FP32 push.  A node the push reached but never
 // pushed (a boundary node) holds r32, the FP32 sum of the m <= d * iters push
@@ -63,8 +73,9 @@ constexpr int kSink = 64;            // per-slot sink records (d = 0) for the id
 //     g = (gm(iters) + u' + gm(m) + u' gm(m)) / (1 - u' - gm(m) - u' gm(m)),
 // because, writing a_j = c_u rho for the exact push terms (rho the FP32
 // residuals the neighbours moved) and A = sum a_j:
-//   - every FP32 add (memory-side or LDS atomic) lands within 2u of its exact
-//     result, so a sum of k >= 0 terms carries each term within gm(k);
+//   - every FP32 add (memory-side or LDS atomic, or a bin's summed add) lands
+//     within 2u of its exact result, and r32 is a summation tree of its m
+//     terms whatever order the adds took, so each term is carried within gm(m);
 //   - every push value fl32(fl64(fl64(1 - alpha) rho) / d) is within u' of a_j;
 //   - so r32 = sum a_j (1 + eta_j), |eta_j| <= u' + gm(m) + u' gm(m), and the
 //     true residual sum_u c_u x32_u = sum a_j (1 + theta_j), |theta_j| <=
@@ -96,9 +107,9 @@ __host__ __device__ __forceinline__ bool lazy_valid(int d, int64_t it) {
     return 2.0 * (double)d * (double)it * 0x1p-24 <= 0.25;
 }
 
-// error bound of a record whose FP32 residual field holds rl: > 0 a boundary
+// error bound of an entry whose FP32 residual field holds rl: > 0 a boundary
 // residual still pending, < 0 one the correction push has moved into x (its
-// error stays), +-0 an exact (FP64) record
+// error stays), +-0 an exact (FP64) entry
 __host__ __device__ __forceinline__ double lazy_bound(float rl, int d, int64_t it) {
     if (rl == 0.0f) return 0.0;
     if (!lazy_valid(d, it)) return __builtin_inf();   // never certifies
@@ -115,9 +126,11 @@ __host__ __device__ __forceinline__ double lazy_bound(float rl, int d, int64_t i
 //   dx_v  = x32_v - (sum of the r moved out of v)   |dx_v| <= 2uK x32_v (1 + 4uK)
 //   E_v   = received push values - sum_u c_u (moved out of u)
 //   H_v   = rounding of every add into r32_v    |H_v| <= 2u d K max(x32_v(1+4uK), r32_v) + d K 2^-126
-// and |E_v| + sum_u c_u |dx_u| <= (2K + 1.01) u (1 + 4uK) S_v, while
-// S_v = r_v - b_v + x32_v <= |r32_v| + B + x32_v -- the residual identity
-// itself bounds the neighbour sum, so no gather is needed.  Solving for B:
+// (at most d K adds: one per received push value, a bin's summed add
+// included), and |E_v| + sum_u c_u |dx_u| <= (2K + 1.01) u (1 + 4uK) S_v,
+// while S_v = r_v - b_v + x32_v <= |r32_v| + B + x32_v -- the residual
+// identity itself bounds the neighbour sum, so no gather is needed.  Solving
+// for B:
 //   B (1 - g) <= 2u b + 2uK x (1+4uK) + g (|r32| + x) + 2u d K max(x(1+4uK), |r32|) + d K 2^-126,
 //   g = (2K + 1.01) u (1 + 4uK),
 // returned with a 1.01 margin.  It is small next to eps*alpha*d_v unless
@@ -133,33 +146,29 @@ __host__ __device__ __forceinline__ double sup_bound(float x32, float rl, int d,
     return 1.01 * num / (1.0 - g);
 }
 
-// bound of a record with flags fl and FP32 residual field rl: a kBound
-// record keeps its support bound, rounded up, as -rl (its pending FP32
-// residual moved into r64 when it was certified); others the boundary bound
+// bound of an entry with flags fl and FP32 residual field rl: a kBound entry
+// keeps its support bound, rounded up, as -rl (its pending FP32 residual moved
+// into r64 when it was certified); others the boundary bound
 __host__ __device__ __forceinline__ double rec_bound(uint32_t fl, float rl, int d, int64_t it) {
     return (fl & kBound) ? fabs((double)rl) : lazy_bound(rl, d, it);
 }
 
-struct NodeRec;
-__device__ __attribute__((noinline)) bool certify_support(NodeRec *r, int64_t iters, double b, double thr_scale);
-
-struct __attribute__((aligned(32))) NodeRec {
-    union {
-        struct {
-            float r;    // residual (FP32 push); FP32 heat kernel: even-level residual
-            float x;    // estimate (FP32 push and FP32 heat kernel)
-        };
-        double hx;      // FP64 heat kernel: estimate
-    };
-    int32_t d;          // degree (copy of the graph's, same line as r)
-    uint32_t fl;        // certificate flags
-    union {
-        double r64;     // FP64 residual; FP64 heat kernel: even-level residual
-        float rh[2];    // FP32 heat kernel: rh[0] = odd-level residual
-    };
-    double x64;         // FP64 estimate; FP64 heat kernel: odd-level residual
+// One slot's arrays (see the file header).
+struct SlotView {
+    float2 *rx32;
+    uint32_t *fl;
+    double2 *rx64;
+    double *hk2;
 };
-static_assert(sizeof(NodeRec) == 32, "node record must be 32 bytes");
+
+// an array of T with a stride: the residual / estimate halves of a record
+// array, or a plain array
+template <typename T>
+struct Strided {
+    T *p;
+    int s;
+    __device__ __forceinline__ T &operator[](int64_t i) const { return p[i * s]; }
+};
 
 // Certifies a pushed node by sup_bound when |r32| + bound < (1 - 2^-8) thr:
 // r64 takes the pending FP32 residual (exactly), the FP32 field the bound as
@@ -167,19 +176,16 @@ static_assert(sizeof(NodeRec) == 32, "node record must be 32 bytes");
 // whether it did (else the node joins the FP64 pull).  Out of line: the
 // bound's FP64 arithmetic would otherwise raise the persistent kernel's
 // register pressure around its hot loops.
-__device__ __attribute__((noinline)) bool certify_support(NodeRec *r, int64_t iters, double b, double thr_scale) {
-    float4 lo;
-    __builtin_memcpy(&lo, r, 16);
-    const int d = __float_as_int(lo.z);
-    const double bnd = sup_bound(lo.y, lo.x, d, iters, b);
-    if (!(fabs((double)lo.x) + bnd < kNearFrac * thr_scale * (double)d)) return false;
+__device__ __attribute__((noinline)) bool certify_support(SlotView S, int v, int d, int64_t iters, double b,
+                                                          double thr_scale) {
+    const float2 rx = S.rx32[v];
+    const double bnd = sup_bound(rx.y, rx.x, d, iters, b);
+    if (!(fabs((double)rx.x) + bnd < kNearFrac * thr_scale * (double)d)) return false;
     float bf = (float)bnd;
     if ((double)bf < bnd) bf = __uint_as_float(__float_as_uint(bf) + 1u);   // next float up (bf > 0)
-    char *pr = reinterpret_cast<char *>(r);
-    const double2 hv = make_double2((double)lo.x, (double)lo.y);
-    __builtin_memcpy(pr + 16, &hv, 16);
-    const float4 lv = make_float4(-bf, lo.y, lo.z, __uint_as_float(kSeen | kBound));
-    __builtin_memcpy(pr, &lv, 16);
+    S.rx64[v] = make_double2((double)rx.x, (double)rx.y);
+    S.rx32[v] = make_float2(-bf, rx.y);
+    S.fl[v] = kSeen | kBound;
     return true;
 }
 
@@ -194,7 +200,11 @@ struct __attribute__((aligned(16))) Ent {
 struct BatchArena {
     int64_t n = 0;
     int32_t slots = 0;
-    NodeRec *rec = nullptr;      // slots * (n + kSink): per slot n records, then kSink sink records
+    int64_t rn = 0;              // per-node array stride: n + kSink
+    float2 *rx32 = nullptr;      // slots * rn
+    uint32_t *fl = nullptr;      // slots * rn
+    double2 *rx64 = nullptr;     // slots * rn
+    double *hk2 = nullptr;       // slots * rn
     int32_t *q0 = nullptr;       // slots * n
     int32_t *q1 = nullptr;
     Ent *ent = nullptr;          // slots * n: frontier entries (heat kernel: two touched lists)
@@ -202,8 +212,11 @@ struct BatchArena {
     int32_t *sst = nullptr;      // slots * n: row start of sup[k] (main push)
     uint32_t *pcs = nullptr;     // slots * pcs_stride: hub pieces of the current list
     int64_t pcs_stride = 0;
-    int32_t *tl = nullptr;       // slots * n: records touched by the current query
+    int32_t *tl = nullptr;       // slots * n: entries touched by the current query
     int32_t *nl = nullptr;       // slots * n: FP32 push: boundary candidates for an exact recompute
+    int32_t *pv = nullptr;       // slots * pcap: binned push, pair destinations
+    double *pval = nullptr;      // slots * pcap: pair values (FP32 pairs use the first pcap floats)
+    int64_t pcap = 0;            // pairs per slot (0: no binned push)
     uint64_t *ctl = nullptr;     // launch control for the synchronous entry point
     flsg_result *d_res = nullptr;
     int64_t d_res_cap = 0;
@@ -216,12 +229,17 @@ struct BatchArena {
 struct BatchArgs {
     const int32_t *indptr;
     const int32_t *indices;
+    const int32_t *degree;
     const int2 *colx;            // {v, degree[v]} per column entry
     int64_t n;
     const flsg_query *queries;   // per-query (seed, kind, alpha, eps)
     int64_t nq;
     const int32_t *order;        // k-th claim runs query order[k] (NULL = k)
-    NodeRec *rec;                // slot stride n + kSink
+    float2 *rx32;                // slot arrays (see BatchArena)
+    uint32_t *fl;
+    double2 *rx64;
+    double *hk2;
+    int64_t rn;
     int32_t *q0;
     int32_t *q1;
     Ent *ent;
@@ -231,10 +249,16 @@ struct BatchArgs {
     int64_t pcs_stride;
     int32_t *tl;
     int32_t *nl;
+    int32_t *pv;
+    double *pval;
+    int64_t pcap;
+    int64_t bin_min;             // binned push for iterations of at least this many edges
+    int32_t dense_min;           // bins with fewer pairs use memory-side atomics
     int32_t piece_w;             // edges per hub piece
     int32_t hub_w;               // entries with more edges go through pieces
-    int32_t hub_bytes;           // dynamic LDS of the hub cache (0 = none)
+    int32_t hub_bytes;           // dynamic LDS: hub cache / bin slice (0 = none)
     int32_t slot_base;
+    int32_t undirected;          // 1 if every edge (u, v) has its (v, u): FP32 certificate precondition
     uint64_t *ctl;               // [0] next query (low 32 bits used), [1] output entries used
     const int32_t *out_map;      // internal id -> caller id (NULL = identity)
     int32_t *out_ids;
@@ -316,10 +340,15 @@ __device__ __forceinline__ uint32_t fl_or(uint32_t *p, uint32_t v) {
 __device__ __forceinline__ uint32_t fl_and(uint32_t *p, uint32_t v) {
     return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-
+// a value other waves of the workgroup wrote before the last barrier (a
+// vector load: the scalar cache is not kept coherent with vector stores)
+__device__ __forceinline__ int32_t load_wg(const int32_t *p) {
+    return __hip_atomic_load(const_cast<int32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 constexpr int kUnroll = 4;           // edges in flight per lane
-constexpr int kOutUnroll = 8;        // records in flight per lane in the output pass
+constexpr int kOutUnroll = 8;        // entries in flight per lane in the output pass
+constexpr int kBinUnroll = 4;        // pairs / destinations in flight per lane in the binned push
 
 struct WaveLds {
     int incl[kWave];
@@ -333,6 +362,22 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// appends of one touched destination: bit 0 -> next[] (counter *s_next), bit 1
+// -> tl[] (*s_tl), bit 2 -> nl[] (*s_nl, entries past nl_cap dropped but
+// counted); every lane of the wave calls it
+__device__ __forceinline__ void append_bits(int bits, int v, int32_t *next, unsigned int *s_next, int32_t *tl,
+                                            unsigned int *s_tl, int32_t *nl, unsigned int *s_nl,
+                                            unsigned int nl_cap) {
+    const long long slot = wave_append((bits & 1) != 0, s_next);
+    if (slot >= 0) next[slot] = v;
+    const long long ts = wave_append((bits & 2) != 0, s_tl);
+    if (ts >= 0) tl[ts] = v;
+    if (s_nl) {
+        const long long ns = wave_append((bits & 4) != 0, s_nl);
+        if (ns >= 0 && ns < (long long)nl_cap) nl[ns] = v;
+    }
+}
+
 // Edges of the entries [0, cnt) of a list, wavefront by wavefront with no
 // block barrier: a wave claims either a group of 64 consecutive entries (its
 // edges are flattened by a wave prefix sum; entries with more than hub_w edges
@@ -343,16 +388,13 @@ __device__ __forceinline__ void wave_sync() {
 //   load(i, st, d, v): row start, edge count (0 = nothing) and value of entry i
 //   pre(vtx, v, S &, stage): issue one edge's LDS (stage 0) or memory-side
 //                           (stage 1) operations, results into S
-//   post(vtx, dv, v, S, act) -> int: bit 0 appends vtx to next[] (counter *s_next),
-//                           bit 1 appends it to tl[] (counter *s_tl), bit 2 to
-//                           nl[] (counter *s_nl, entries past nl_cap dropped but
-//                           counted); the result is masked by act, side effects
-//                           must test it
+//   post(vtx, dv, v, S, act) -> int: append bits (append_bits); the result is
+//                           masked by act, side effects must test it
 // A lane issues pre() for all its kUnroll edges before post() consumes any
 // result, so kUnroll atomics per lane are in flight at once.  Both run
 // unpredicated (a predicated atomic whose result is used makes the compiler
 // wait inside the predicated block, one round trip per edge): an idle lane
-// aims at its own sink record (id sink + lane, degree 0) with value -0.0,
+// aims at its own sink entry (id sink + lane, degree 0) with value -0.0,
 // which leaves any value unchanged under IEEE addition.
 template <typename S, typename Load, typename Pre, typename Post>
 __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w, int hub_w,
@@ -432,16 +474,77 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 7 : 0);
 #pragma unroll
+            for (int u = 0; u < kUnroll; u++) append_bits(cr[u], vtx[u], next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+        }
+        if (group) wave_sync();
+    }
+}
+
+// The same walk over the edges of a list's entries for the binned push's count
+// and write passes: f(col, value, act) for every edge slot (act false for the
+// idle ones), reading the 4-byte column indices.
+template <typename Load, typename F>
+__device__ __forceinline__ void wave_cols(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w,
+                                          int hub_w, unsigned int *s_work, WaveLds &W, const int32_t *cols, Load load,
+                                          F f) {
+    const int lane = __lane_id();
+    const unsigned int ngroups = (cnt + kWave - 1) / kWave;
+    const unsigned int nitems = ngroups + npc;
+    for (;;) {
+        unsigned int item = 0;
+        if (lane == 0) item = atomicAdd(s_work, 1u);
+        item = __shfl(item, 0);
+        if (item >= nitems) break;
+        int st = 0, d = 0, beg = 0, end = 0;
+        double v = 0.0;
+        const bool group = item < ngroups;
+        if (group) {
+            const unsigned int i = item * kWave + lane;
+            if (i < cnt) {
+                load(i, st, d, v);
+                if (d > hub_w) d = 0;
+            }
+            int incl = d;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            W.incl[lane] = incl;
+            W.start[lane] = st;
+            W.val[lane] = v;
+            wave_sync();
+            end = __shfl(incl, kWave - 1);
+        } else {
+            const uint32_t pc = pcs[item - ngroups];
+            load(pc >> 8, st, d, v);
+            beg = (int)(pc & 255u) * piece_w;
+            end = min(d, beg + piece_w);
+        }
+        for (int e0 = beg; e0 < end; e0 += kWave * kUnroll) {
+            int c[kUnroll];
+            double vv[kUnroll];
+            bool act[kUnroll];
+#pragma unroll
             for (int u = 0; u < kUnroll; u++) {
-                const long long slot = wave_append((cr[u] & 1) != 0, s_next);
-                if (slot >= 0) next[slot] = vtx[u];
-                const long long ts = wave_append((cr[u] & 2) != 0, s_tl);
-                if (ts >= 0) tl[ts] = vtx[u];
-                if (s_nl) {
-                    const long long ns = wave_append((cr[u] & 4) != 0, s_nl);
-                    if (ns >= 0 && ns < (long long)nl_cap) nl[ns] = vtx[u];
+                const int e = e0 + u * kWave + lane;
+                act[u] = e < end;
+                c[u] = 0;
+                vv[u] = 0.0;
+                if (act[u]) {
+                    if (group) {
+                        const int k = upper_bound_lds(W.incl, 0, kWave, e);
+                        const int o = e - (k ? W.incl[k - 1] : 0);
+                        c[u] = cols[W.start[k] + o];
+                        vv[u] = W.val[k];
+                    } else {
+                        c[u] = cols[st + e];
+                        vv[u] = v;
+                    }
                 }
             }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) f(c[u], vv[u], act[u]);
         }
         if (group) wave_sync();
     }
@@ -457,18 +560,21 @@ __device__ __forceinline__ void add_pieces(unsigned int i, int d, int pw, int hw
     }
 }
 
+__device__ __forceinline__ bool pristine(float v) { return __float_as_uint(v) == 0u; }
+__device__ __forceinline__ bool pristine(double v) { return __double_as_longlong(v) == 0ll; }
+
 // FP64 pull of the certificate: for the listed rows v, sum over the row's
 // column entries u of push(x32_u) -- a gather per column entry where a
 // scatter would take a memory-side atomic.  Rows of at most hub_w edges go 64
 // at a time per wavefront: the row's lane gets node and row start from
-// row_of(i, v, st) and loads the record, the
-// edges are flattened over the wave, their terms summed in LDS, and the row's
-// lane then finishes the record (finish(ok, v, record lo half, sum), called by
-// every lane).  Longer rows are listed in ent[j] = {start, degree, v} for
-// j < *s_nlong, with their pieces in pcs[] for pull_pieces().  x32 of the
-// nodes below xn comes from the LDS copy xc[].  ops counts the rows' edges.
+// row_of(i, v, st), the edges are flattened over the wave, their terms summed
+// in LDS, and the row's lane then finishes the entry (finish(ok, v, x32_v,
+// d_v, sum), called by every lane).  Longer rows are listed in ent[j] =
+// {start, degree, v} for j < *s_nlong, with their pieces in pcs[] for
+// pull_pieces().  x32 of the nodes below xn comes from the LDS copy xc[].
+// ops counts the rows' edges.
 template <typename Row, typename PushOf, typename Finish>
-__device__ __forceinline__ void pull_rows(unsigned int cnt, Row row_of, NodeRec *rec,
+__device__ __forceinline__ void pull_rows(unsigned int cnt, Row row_of, const SlotView &S, const int32_t *degree,
                                           const int2 *colx, const float *xc, int xn, int hub_w, int piece_w,
                                           Ent *ent, uint32_t *pcs, unsigned int *s_npc, unsigned int *s_nlong,
                                           unsigned int *s_work, WaveLds &W, PushOf push_of, Finish finish,
@@ -482,11 +588,11 @@ __device__ __forceinline__ void pull_rows(unsigned int cnt, Row row_of, NodeRec 
         if (item >= ngroups) break;
         const unsigned int i = item * kWave + lane;
         int v = -1, st = 0, d = 0;
-        float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
+        float xv0 = 0.0f;
         if (i < cnt) {
             row_of(i, v, st);
-            __builtin_memcpy(&lo, rec + v, 16);
-            d = __float_as_int(lo.z);
+            d = degree[v];
+            xv0 = S.rx32[v].y;
         }
         ops += (unsigned long long)d;
         const bool lng = d > hub_w;
@@ -527,13 +633,13 @@ __device__ __forceinline__ void pull_rows(unsigned int cnt, Row row_of, NodeRec 
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; u++)
-                xv[u] = k[u] < 0 ? 0.0f : c[u].x < xn ? xc[c[u].x] : rec[c[u].x].x;
+                xv[u] = k[u] < 0 ? 0.0f : c[u].x < xn ? xc[c[u].x] : S.rx32[c[u].x].y;
 #pragma unroll
             for (int u = 0; u < kUnroll; u++)
                 if (xv[u] != 0.0f) atomicAdd(&W.val[k[u]], push_of(xv[u], c[u].y));
         }
         wave_sync();
-        finish(v >= 0 && !lng, v, lo, W.val[lane]);
+        finish(v >= 0 && !lng, v, xv0, d, W.val[lane]);
         wave_sync();
     }
 }
@@ -542,7 +648,7 @@ __device__ __forceinline__ void pull_rows(unsigned int cnt, Row row_of, NodeRec 
 // atomic per piece into the row's r64 (zero before).
 template <typename PushOf>
 __device__ __forceinline__ void pull_pieces(unsigned int npc, const uint32_t *pcs, const Ent *ent, int piece_w,
-                                            NodeRec *rec, const int2 *colx, const float *xc, int xn,
+                                            const SlotView &S, const int2 *colx, const float *xc, int xn,
                                             unsigned int *s_work, PushOf push_of) {
     const int lane = __lane_id();
     for (;;) {
@@ -565,77 +671,68 @@ __device__ __forceinline__ void pull_pieces(unsigned int npc, const uint32_t *pc
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; u++)
-                xv[u] = c[u].x < 0 ? 0.0f : c[u].x < xn ? xc[c[u].x] : rec[c[u].x].x;
+                xv[u] = c[u].x < 0 ? 0.0f : c[u].x < xn ? xc[c[u].x] : S.rx32[c[u].x].y;
 #pragma unroll
             for (int u = 0; u < kUnroll; u++)
                 if (xv[u] != 0.0f) acc += push_of(xv[u], c[u].y);
         }
         acc = wave_sum(acc);
-        if (lane == 0 && acc != 0.0) atomic_add_wg(&rec[(int)e.v].r64, acc);
+        if (lane == 0 && acc != 0.0) atomic_add_wg(&S.rx64[(int)e.v].x, acc);
     }
 }
 
-// Touched-record bookkeeping of the main push: a record untouched by the
+// Touched-entry bookkeeping of the main push: an entry untouched by the
 // current query has r == +0.0 (bit pattern 0); the node phase leaves -0.0 in
 // the residual it moves to x and pushes are >= 0, so r never returns to +0.0
-// once it has left it: "old == +0.0 and new != 0" in the edge atomic's return
-// marks exactly the first touch, with no flag atomic.  The correction push (residuals of both signs) falls back to the
-// flag for records the main push never reached.
+// once it has left it: "old == +0.0 and new != 0" in an add's result marks
+// exactly the first touch, with no flag atomic.  The correction push
+// (residuals of both signs) falls back to the flag for entries the main push
+// never reached.
 // an edge atomic's result (from the hub cache or from memory)
-struct Res32 {
-    float v;
-    bool lds;
-};
-struct Res64 {
-    double v;
-    bool lds;
-};
-
-__device__ __forceinline__ bool pristine32(float old) { return __float_as_uint(old) == 0u; }
-__device__ __forceinline__ bool pristine64(double old) { return __double_as_longlong(old) == 0ll; }
-
-
-// Heat-kernel record fields by precision: level residuals (parity 0 / 1)
-// and the estimate.
-template <bool F64>
-struct HkF;
-template <>
-struct HkF<false> {
-    using T = float;
-    static __device__ __forceinline__ float *lvl(NodeRec &r, bool odd) { return odd ? &r.rh[0] : &r.r; }
-    static __device__ __forceinline__ float *xp(NodeRec &r) { return &r.x; }
-};
-template <>
-struct HkF<true> {
-    using T = double;
-    static __device__ __forceinline__ double *lvl(NodeRec &r, bool odd) { return odd ? &r.x64 : &r.r64; }
-    static __device__ __forceinline__ double *xp(NodeRec &r) { return &r.hx; }
-};
-
-__device__ __forceinline__ bool pristine(float v) { return __float_as_uint(v) == 0u; }
-__device__ __forceinline__ bool pristine(double v) { return __double_as_longlong(v) == 0ll; }
-
 template <typename T>
 struct ResT {
     T v;
     bool lds;
 };
 
+// Heat-kernel fields by precision: level residuals (parity 0 / 1) and the
+// estimate.  FP32: both levels in hk2 (as floats), the estimate in rx32.x;
+// FP64: the even level in rx64.r, the odd one in hk2, the estimate in rx64.x.
+template <bool F64>
+struct HkF;
+template <>
+struct HkF<false> {
+    using T = float;
+    static __device__ __forceinline__ Strided<float> lvl(const SlotView &S, int64_t rn, bool odd) {
+        return Strided<float>{reinterpret_cast<float *>(S.hk2) + (odd ? rn : 0), 1};
+    }
+    static __device__ __forceinline__ Strided<float> est(const SlotView &S) { return Strided<float>{&S.rx32[0].y, 2}; }
+};
+template <>
+struct HkF<true> {
+    using T = double;
+    static __device__ __forceinline__ Strided<double> lvl(const SlotView &S, int64_t rn, bool odd) {
+        return odd ? Strided<double>{S.hk2, 1} : Strided<double>{&S.rx64[0].x, 2};
+    }
+    static __device__ __forceinline__ Strided<double> est(const SlotView &S) { return Strided<double>{&S.rx64[0].y, 2}; }
+};
+
 // Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), with
 // FP32 or FP64 level residuals (the query's precision) and FP64 threshold
-// tests.  Level j's residual lives in the record (HkF::lvl, parity j & 1) --
-// for the hub cache's nodes in one of two LDS level arrays.  Each level keeps
+// tests.  Level j's residual lives in the level array of parity j & 1 -- for
+// the hub cache's nodes in one of two LDS level arrays.  Each level keeps
 // the list of nodes it touched; the node phase walks that list, pushes every
 // node at or above the level threshold (level residuals only grow, so this is
 // the set the reference's crossing queue holds, hk_solver.py:258) and clears
 // every entry (the reference zeroes the whole level, hk_solver.py:282-285),
 // so no separate clearing pass is needed.
 template <int NT, bool F64>
-__device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, NodeRec *rec, int32_t *qa,
-                         int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs, WaveLds &W, double *hub_raw,
-                         unsigned int *s_cnt, unsigned int *s_ncnt, unsigned int *s_sup, unsigned int *s_npc,
-                         unsigned int *s_work, unsigned int *s_tn, unsigned long long *s_vol,
-                         unsigned long long *s_base, uint64_t t_start, int sink) {
+__device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, const SlotView &S,
+                                         int32_t *qa, int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs, WaveLds &W,
+                                         double *hub_raw, unsigned int *s_cnt, unsigned int *s_ncnt,
+                                         unsigned int *s_sup, unsigned int *s_npc, unsigned int *s_work,
+                                         unsigned int *s_tn, unsigned long long *s_vol, unsigned long long *s_base,
+                                         uint64_t t_start, int sink) {
     using F = HkF<F64>;
     using T = typename F::T;
     const int tid = threadIdx.x;
@@ -644,6 +741,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
     const int pw = a.piece_w;
     const int hw = a.hub_w;
     const int32_t *indptr = a.indptr;
+    const Strided<T> est = F::est(S);
     T *hub = reinterpret_cast<T *>(hub_raw);
     // hub cache: two level arrays of hub_n entries, then kWave sink slots
     const int cap = a.hub_bytes / (int)sizeof(T);
@@ -653,7 +751,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
     __syncthreads();
     if (tid == 0) {
         if (s < hub_n) hub[s] = (T)1;   // hk_solver.py:223
-        else *F::lvl(rec[s], false) = (T)1;
+        else F::lvl(S, a.rn, false)[s] = (T)1;
         qa[0] = s;
         *s_cnt = 1;
         *s_sup = 0;
@@ -678,6 +776,8 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
         const double coef = a.hk_tau / ((double)j + 1.);
         const double psic = a.hk_psi[j];
         T *hc = hub + (odd ? hub_n : 0);
+        const Strided<T> lv = F::lvl(S, a.rn, odd);
+        const Strided<T> lnx = F::lvl(S, a.rn, !odd);
         const int nb = odd ? 0 : hub_n;   // LDS base of level j + 1
         __syncthreads();
         if (tid == 0) {
@@ -703,20 +803,9 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
                 ss[k] = 0;
                 dd[k] = 0;
                 if (uu[k] >= 0) {
-                    NodeRec &ru = rec[uu[k]];
-                    float4 lo;
-                    __builtin_memcpy(&lo, &ru, 16);
-                    dd[k] = __float_as_int(lo.z);
-                    if constexpr (F64) {
-                        double2 hi;
-                        __builtin_memcpy(&hi, reinterpret_cast<const char *>(&ru) + 16, 16);
-                        __builtin_memcpy(&xo[k], &lo, 8);
-                        ro[k] = odd ? hi.y : hi.x;
-                    } else {
-                        xo[k] = lo.y;
-                        ro[k] = odd ? ru.rh[0] : lo.x;
-                    }
-                    if (uu[k] < hub_n) ro[k] = hc[uu[k]];
+                    dd[k] = a.degree[uu[k]];
+                    xo[k] = est[uu[k]];
+                    ro[k] = uu[k] < hub_n ? hc[uu[k]] : lv[uu[k]];
                     ss[k] = indptr[uu[k]];
                 }
             }
@@ -728,13 +817,13 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
                 const int d = dd[k];
                 const T tmp = ro[k];
                 if (u < hub_n) hc[u] = (T)0;            // hk_solver.py:282-285
-                else *F::lvl(rec[u], odd) = (T)0;
+                else lv[u] = (T)0;
                 Ent e;
                 e.st = 0;
                 e.d = 0;
                 e.v = 0.0;
                 if (!(fabs((double)tmp) < ((a.hk_ete * (double)d) / (double)N) / psic)) {   // hk_solver.py:244
-                    *F::xp(rec[u]) = xo[k] + tmp;
+                    est[u] = xo[k] + tmp;
                     if (pristine(xo[k])) sup[atomicAdd(s_sup, 1u)] = u;
                     e.st = ss[k];
                     e.d = d;
@@ -759,7 +848,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
                         r.lds = li >= 0;
                         if (li >= 0) r.v = atomicAdd(&hub[li], (T)val);
                     } else if (li < 0) {
-                        r.v = atomic_add_wg(F::lvl(rec[vtx], !odd), (T)val);
+                        r.v = atomic_add_wg(&lnx[vtx], (T)val);
                     }
                 },
                 [&](int, int, double val, const ResT<T> &r, bool) -> int {
@@ -771,7 +860,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
             wave_edges<T>(
                 cnt, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
                 [&](int vtx, double val, T &old, int stage) {
-                    if (stage == 1) old = atomic_add_wg(F::xp(rec[vtx]), (T)val);
+                    if (stage == 1) old = atomic_add_wg(&est[vtx], (T)val);
                 },
                 [&](int, int, double val, T old, bool) -> int {
                     return (pristine(old) && old + (T)val != (T)0) ? 2 : 0;
@@ -785,26 +874,26 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
         ln = t;
         __syncthreads();
     }
-    const unsigned int S = *s_sup;
-    if (tid == 0) *s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
+    const unsigned int Sn = *s_sup;
+    if (tid == 0) *s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)Sn);
     __syncthreads();
     const unsigned long long obase = *s_base;
-    const bool fits = obase + S <= (unsigned long long)a.out_cap;
+    const bool fits = obase + Sn <= (unsigned long long)a.out_cap;
     const double et = exp(a.hk_tau);
-    for (unsigned int i0 = tid; i0 < S; i0 += NT * kUnroll) {
+    for (unsigned int i0 = tid; i0 < Sn; i0 += NT * kUnroll) {
         int uu[kUnroll], id[kUnroll];
         T xv[kUnroll];
 #pragma unroll
         for (int k = 0; k < kUnroll; k++) {
             const unsigned int i = i0 + k * NT;
-            uu[k] = i < S ? sup[i] : -1;
+            uu[k] = i < Sn ? sup[i] : -1;
         }
 #pragma unroll
         for (int k = 0; k < kUnroll; k++) {
             id[k] = 0;
             xv[k] = (T)0;
             if (uu[k] >= 0) {
-                xv[k] = *F::xp(rec[uu[k]]);
+                xv[k] = est[uu[k]];
                 id[k] = a.out_map ? a.out_map[uu[k]] : uu[k];
             }
         }
@@ -816,7 +905,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
                 a.out_ids[obase + i] = id[k];
                 a.out_vals[obase + i] = (double)xv[k] / et;   // hk_solver.py:291
             }
-            *F::xp(rec[uu[k]]) = (T)0;
+            est[uu[k]] = (T)0;
         }
     }
     __syncthreads();
@@ -825,7 +914,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
         flsg_result rr{};
         rr.oper = (int64_t)*s_vol;
         rr.iters = levels;
-        rr.supp = (int64_t)S;
+        rr.supp = (int64_t)Sn;
         rr.nodes = (int64_t)*s_tn;
         rr.ms = (double)(t_end - t_start) * 1e-5;
         rr.ms_push = rr.ms;
@@ -838,34 +927,495 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
     __syncthreads();
 }
 
+// Binned edge phase of one push iteration (see the file header), with the
+// next iteration's node phase folded into it for the nodes it sends across
+// their threshold.
+//   write pass: a push into a hub id (v < hub_n) is an LDS atomic on the hub
+//     cache, whose result decides the crossing at once (-> next[], the nodes
+//     whose node phase the next iteration runs); any other push becomes a
+//     (v, value) pair in the region of its destination range k (ids
+//     [k 2^shift, (k+1) 2^shift)): pv/pval[indptr[k 2^shift] ...].  An
+//     iteration pushes each column entry at most once (the frontier is a set)
+//     and, on an undirected graph, the entries pointing into a range number
+//     its row volume, so every range has room with no counting pass.  The
+//     ranges' write cursors (then their ends) are bins[] (static LDS).
+//   then the hub cache goes back to the record array (the dynamic LDS becomes
+//     the slice of one range: 2^shift residual sums, zero between ranges);
+//   per range: the pairs are summed into the slice (LDS atomics), and then
+//     drained by the pairs themselves: the first pair of each destination to
+//     swap its slice entry with 0 gets the summed add and applies it to the
+//     record in one read and one write.  A destination it sends across its
+//     threshold has all of this iteration's pushes in (they all come through
+//     its range), so its node phase runs right there: x += r, r -> -0, and its
+//     frontier entry {row start, degree, push value} goes to ent[] of the next
+//     iteration (counter *s_nf; its hub pieces to pcs[], counter *s_npc; its
+//     volume to *s_vol and *s_fvol).  Ranges with fewer than dense_min pairs
+//     take one memory-side atomic per pair instead (crossings -> next[]).
+// r and x are the residual and estimate halves of the slot's {r, x} records;
+// post(old, new, d_v) -> append bits of one destination's add;
+// push(r, d) -> the push value of a node moving r with degree d.
+template <int NT, typename T, typename Load, typename Post, typename Push>
+__device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int pw, int hw,
+                                             unsigned int *s_work, WaveLds &W, const int2 *colx,
+                                             const int32_t *indptr, const int32_t *degree, Load load, int shift,
+                                             int K, int64_t n, T *hub, int hub_n, void *lds, unsigned int *bins,
+                                             int32_t *pv, T *pval, Strided<T> r, Strided<T> x, int dense_min,
+                                             Post post, Push push,
+                                             Ent *ent, uint32_t *pcs_next, unsigned int *s_npc, unsigned int *s_nf,
+                                             unsigned long long *s_vol, unsigned long long *s_fvol, int32_t *sup,
+                                             int32_t *sst, unsigned int *s_sup, int32_t *next, unsigned int *s_next,
+                                             int32_t *tl, unsigned int *s_tl, int32_t *nl, unsigned int *s_nl,
+                                             unsigned int nl_cap, uint64_t &t_pairs) {
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    T *slice = reinterpret_cast<T *>(lds);
+    // 1. write pass
+    for (int k = tid; k < K; k += NT) bins[k] = (unsigned int)indptr[min((int64_t)k << shift, n)];
+    if (tid == 0) *s_work = 0;
+    __syncthreads();
+    {
+        const int lane0 = lane;
+        wave_edges<ResT<T>>(
+            cnt, pcs, npc, pw, hw, s_work, W, colx, load,
+            [&](int vtx, double val, ResT<T> &res, int stage) {
+                if (stage != 0) return;
+                res.lds = vtx < hub_n;
+                res.v = (T)0;
+                if (vtx < hub_n) {
+                    res.v = atomicAdd(&hub[vtx], (T)val);
+                } else if (vtx < n) {
+                    const unsigned int p = atomicAdd(&bins[vtx >> shift], 1u);
+                    pv[p] = vtx;
+                    pval[p] = (T)val;
+                }
+            },
+            [&](int vtx, int dv, double val, const ResT<T> &res, bool) -> int {
+                return res.lds ? post(res.v, res.v + (T)val, dv) : 0;
+            },
+            next, s_next, tl, s_tl, (int)n, nl, s_nl, nl_cap);
+        (void)lane0;
+    }
+    __syncthreads();
+    // 2. the hub cache back to the records; the dynamic LDS becomes the slice
+    for (int v = tid; v < hub_n; v += NT) {
+        const T h = hub[v];
+        if (!pristine(h)) r[v] = h;
+    }
+    if (tid == 0) {
+        *s_nf = 0;
+        *s_npc = 0;
+        *s_fvol = 0;
+    }
+    __syncthreads();
+    {
+        uint4 *z = reinterpret_cast<uint4 *>(lds);
+        const int nz = (int)((((int64_t)1 << shift) * (int64_t)sizeof(T)) / 16);
+        for (int i = tid; i < nz; i += NT) z[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    t_pairs = now_ticks();
+    unsigned long long fvol = 0;
+    // 3. range by range; every loop issues the loads of kBinUnroll items per
+    // lane before it uses any (one memory round trip per batch, not per item)
+    for (int k = 0; k < K; k++) {
+        const int64_t lo = (int64_t)k << shift;
+        const int c0 = indptr[lo];
+        const int c1 = (int)bins[k];
+        if (c1 == c0) continue;
+        if (c1 - c0 < dense_min) {
+            // few pairs: one memory-side atomic each (idle lanes add -0 to a sink)
+            for (int p0 = c0; p0 < c1; p0 += NT * kBinUnroll) {
+                int v[kBinUnroll], dv[kBinUnroll];
+                T val[kBinUnroll], old[kBinUnroll];
+#pragma unroll
+                for (int u = 0; u < kBinUnroll; u++) {
+                    const int p = p0 + u * NT + tid;
+                    v[u] = (int)n + lane;
+                    val[u] = -(T)0;
+                    if (p < c1) {
+                        v[u] = pv[p];
+                        val[u] = pval[p];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kBinUnroll; u++) old[u] = atomic_add_wg(&r[v[u]], val[u]);
+#pragma unroll
+                for (int u = 0; u < kBinUnroll; u++) dv[u] = v[u] < n ? degree[v[u]] : 0;
+#pragma unroll
+                for (int u = 0; u < kBinUnroll; u++) {
+                    const int bits = v[u] < n ? post(old[u], old[u] + val[u], dv[u]) : 0;
+                    append_bits(bits, v[u], next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+                }
+            }
+            continue;
+        }
+        // a range's pairs stay in registers from the sum to the drain when
+        // they fit one batch of the workgroup
+        const bool one = c1 - c0 <= NT * kBinUnroll;
+        int v[kBinUnroll];
+        for (int p0 = c0; p0 < c1; p0 += NT * kBinUnroll) {
+            T val[kBinUnroll];
+#pragma unroll
+            for (int u = 0; u < kBinUnroll; u++) {
+                const int p = p0 + u * NT + tid;
+                v[u] = -1;
+                val[u] = (T)0;
+                if (p < c1) {
+                    v[u] = pv[p];
+                    val[u] = pval[p];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kBinUnroll; u++)
+                if (v[u] >= 0) atomicAdd(&slice[v[u] - (int)lo], val[u]);
+        }
+        __syncthreads();
+        for (int p0 = c0; p0 < c1; p0 += NT * kBinUnroll) {
+            if (!one) {
+#pragma unroll
+                for (int u = 0; u < kBinUnroll; u++) {
+                    const int p = p0 + u * NT + tid;
+                    v[u] = p < c1 ? pv[p] : -1;
+                }
+            }
+            T dl[kBinUnroll];
+#pragma unroll
+            for (int u = 0; u < kBinUnroll; u++) {
+                dl[u] = (T)0;
+                if (v[u] >= 0)
+                    dl[u] = __hip_atomic_exchange(&slice[v[u] - (int)lo], (T)0, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            T ro[kBinUnroll], xo[kBinUnroll];
+            int dv[kBinUnroll], st[kBinUnroll];
+#pragma unroll
+            for (int u = 0; u < kBinUnroll; u++) {
+                ro[u] = (T)0;
+                xo[u] = (T)0;
+                dv[u] = 0;
+                st[u] = 0;
+                if (!pristine(dl[u])) {
+                    ro[u] = r[v[u]];   // r and x share the record: one load
+                    xo[u] = x[v[u]];
+                    dv[u] = degree[v[u]];
+                    st[u] = indptr[v[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kBinUnroll; u++) {
+                int bits = 0;
+                bool fuse = false;
+                double pval_u = 0.0;
+                if (!pristine(dl[u])) {
+                    const T nw = ro[u] + dl[u];
+                    bits = post(ro[u], nw, dv[u]);
+                    fuse = (bits & 1) != 0;
+                    if (fuse) {
+                        // the next iteration's node phase of v (ppr_solver.py:273-282)
+                        r[v[u]] = -(T)0;
+                        x[v[u]] = xo[u] + nw;
+                        pval_u = push(nw, dv[u]);
+                        fvol += (unsigned long long)dv[u];
+                    } else {
+                        r[v[u]] = nw;
+                    }
+                }
+                append_bits(bits & 6, v[u], next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+                const long long f = wave_append(fuse, s_nf);
+                if (f >= 0) {
+                    Ent e;
+                    e.st = st[u];
+                    e.d = dv[u];
+                    e.v = pval_u;
+                    ent[f] = e;
+                    add_pieces((unsigned int)f, dv[u], pw, hw, pcs_next, s_npc);
+                }
+                // first push of a node: list it in the support
+                const long long sk = wave_append(fuse && xo[u] == (T)0, s_sup);
+                if (sk >= 0) {
+                    sup[sk] = v[u];
+                    sst[sk] = st[u];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (fvol) {
+        atomicAdd(s_vol, fvol);
+        atomicAdd(s_fvol, fvol);
+    }
+}
+
+// Local push of one PPR / Katz query in residual type T (float: the FP32
+// push, double: the FP64 one), from the seed's residual to an empty frontier;
+// rb is the slot's {r, x} record array seen as T (r_v = rb[2v], x_v = rb[2v+1]).  An iteration's frontier is ent[0, F) -- entries a binned
+// iteration already prepared (its drain ran their node phase) -- followed by
+// the nodes listed in qc[0, cu), whose node phase runs at the iteration's
+// start.  Returns the status (FLSG_ENOCONV past the iteration cap); *iters,
+// *fnodes and s_vol (operation count) are filled.
+template <int NT, typename T>
+__device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, int s, T *rb, int32_t *qa,
+                                          int32_t *qb, Ent *ent, int32_t *sup, int32_t *sst, uint32_t *pcs,
+                                          int32_t *tl, int32_t *nl, int32_t *pv, T *pval, WaveLds &W,
+                                          double *s_hub, unsigned int *bins, unsigned int *s_cnt,
+                                          unsigned int *s_ncnt, unsigned int *s_sup, unsigned int *s_npc,
+                                          unsigned int *s_work, unsigned int *s_tl, unsigned int *s_nl,
+                                          unsigned int *s_nf, unsigned long long *s_vol, unsigned long long *s_ivol,
+                                          unsigned long long *s_fvol, int sink, int64_t *iters_out,
+                                          unsigned long long *fnodes_out, int32_t **final_q, uint64_t *tick) {
+    const int tid = threadIdx.x;
+    const int pw = a.piece_w;
+    const int hw = a.hub_w;
+    const bool f32 = sizeof(T) == 4;
+    const Strided<T> r{rb, 2};        // residual and estimate halves of the records
+    const Strided<T> x{rb + 1, 2};
+    T *hub = reinterpret_cast<T *>(s_hub);
+    // cache capacity in elements; its last kWave entries take the identity
+    // adds of idle lanes (so they cost no memory-side atomic)
+    const int hub_cap = a.hub_bytes / (int)sizeof(T);
+    const int hub_n = hub_cap > kWave ? (int)min((int64_t)(hub_cap - kWave), a.n) : 0;
+    const int hub_sink = hub_cap > kWave ? hub_cap - kWave - sink : INT32_MIN;   // + vtx of an idle lane
+    // binned push: destination ranges of 2^shift ids in the dynamic LDS, at
+    // most kMaxBins of them (the pair regions are sized by the ranges' row
+    // volumes: undirected graphs only)
+    int shift = 0;
+    while (((int64_t)sizeof(T) << (shift + 1)) <= (int64_t)a.hub_bytes) shift++;
+    const int K = (int)((a.n + ((int64_t)1 << shift) - 1) >> shift);
+    const bool can_bin = a.undirected && a.pcap >= (int64_t)a.indptr[a.n] && shift >= kBinShiftMin &&
+                         K <= kMaxBins;
+    for (int i = tid; i < a.hub_bytes / 8; i += NT) s_hub[i] = 0.0;
+    __syncthreads();
+    if (tid == 0) {
+        if (s < hub_n) hub[s] = (T)P.r0;
+        else r[s] = (T)P.r0;
+        tl[0] = s;
+        qa[0] = s;
+        *s_cnt = 1;
+        *s_nf = 0;
+        *s_npc = 0;
+        *s_fvol = 0;
+        *s_sup = 0;
+        *s_tl = 1;
+        *s_nl = 0;
+        *s_vol = 0;
+    }
+    __syncthreads();
+    bool hub_on = true;   // the hub cache holds the residuals of [0, hub_n)
+    int32_t *qc = qa, *qn = qb;
+    unsigned long long fnodes = 0;
+    int64_t iters = 0;
+    int status = FLSG_OK;
+    auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
+        const Ent e = ent[i];
+        st = e.st;
+        d = e.d;
+        v = e.v;
+    };
+    // one destination's add: crossing (-> next frontier), first touch (->
+    // touched list) and, FP32, boundary candidates for an exact recompute:
+    // first touch of a high-degree node, or a residual that entered
+    // [kNearFrac * thr, thr) with this add
+    auto post = [&](T old, T nw, int dv) -> int {
+        const double thr = P.thr_scale * (double)dv;
+        const bool cross = (double)old < thr && thr <= (double)nw;
+        const bool first = pristine(old) && nw != (T)0;
+        int bits = (cross ? 1 : 0) | (first ? 2 : 0);
+        if (f32) {
+            const double cn = kNearFrac * thr;
+            const bool near = dv > kLazyDeg ? first : (!cross && (double)old < cn && cn <= (double)nw);
+            bits |= near ? 4 : 0;
+        }
+        return bits;
+    };
+    // push value of a node moving r (ppr_solver.py:282, katz_solver.py:212);
+    // the FP32 push rounds it to FP32
+    auto push = [&](T r, int d) -> double {
+        const double v = bpush_value(P, (double)r, d);
+        return f32 ? (double)(float)v : v;
+    };
+    int32_t *nlp = f32 ? nl : nullptr;
+    unsigned int *s_nlp = f32 ? s_nl : nullptr;
+    for (;;) {
+        const unsigned int cu = *s_cnt;   // nodes still to run their node phase
+        const unsigned int F = *s_nf;     // entries already prepared
+        const unsigned int cnt = F + cu;
+        if (cnt == 0) break;
+        if (iters >= a.max_iters) {
+            status = FLSG_ENOCONV;
+            break;
+        }
+        iters++;
+        fnodes += cnt;
+        const uint64_t t_it = now_ticks();
+        __syncthreads();
+        if (tid == 0) {
+            *s_ncnt = 0;
+            *s_work = 0;
+            *s_ivol = *s_fvol;
+        }
+        __syncthreads();
+        // node phase: r_u -> 0, x_u += r_u, push value (ppr_solver.py:273-282);
+        // every entry of the unrolled frontier slots is read before any is written
+        unsigned long long vol = 0;
+        for (unsigned int i00 = 0; i00 < cu; i00 += NT * kUnroll) {
+            const unsigned int i0 = i00 + tid;
+            int uu[kUnroll];
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                const unsigned int i = i0 + j * NT;
+                uu[j] = i < cu ? qc[i] : -1;
+            }
+            T rr[kUnroll], xx[kUnroll];
+            int dd[kUnroll], ss[kUnroll];
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                rr[j] = (T)0;
+                xx[j] = (T)0;
+                dd[j] = 0;
+                ss[j] = 0;
+                if (uu[j] >= 0) {
+                    const int u = uu[j];
+                    xx[j] = x[u];
+                    rr[j] = (hub_on && u < hub_n) ? hub[u] : r[u];
+                    dd[j] = a.degree[u];
+                    ss[j] = a.indptr[u];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                if (uu[j] < 0) continue;
+                const int u = uu[j];
+                const unsigned int i = F + i0 + j * NT;
+                Ent e;
+                e.st = ss[j];
+                e.d = dd[j];
+                if (hub_on && u < hub_n) hub[u] = -(T)0;
+                else r[u] = -(T)0;
+                x[u] = xx[j] + rr[j];
+                e.v = push(rr[j], dd[j]);
+                ent[i] = e;
+                vol += (unsigned long long)dd[j];
+                add_pieces(i, dd[j], pw, hw, pcs, s_npc);
+            }
+            // first push of a node: list it in the support (one LDS atomic per wave)
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                const long long k = wave_append(uu[j] >= 0 && xx[j] == (T)0, s_sup);
+                if (k >= 0) {
+                    sup[k] = uu[j];
+                    sst[k] = ss[j];
+                }
+            }
+        }
+        if (vol) {
+            atomicAdd(s_vol, vol);
+            atomicAdd(s_ivol, vol);
+        }
+        __syncthreads();
+        const uint64_t t0 = now_ticks();
+        tick[0] += t0 - t_it;
+        const bool binned = can_bin && *s_ivol >= (unsigned long long)a.bin_min;
+        const unsigned int npc = *s_npc;
+        if (!hub_on) {
+            for (int v = tid; v < hub_n; v += NT) hub[v] = r[v];
+            hub_on = true;
+            __syncthreads();
+        }
+        if (binned) {
+            uint64_t tp = t0;
+            binned_edges<NT, T>(cnt, pcs, npc, pw, hw, s_work, W, a.colx, a.indptr, a.degree, load_ent, shift, K,
+                                a.n, hub, hub_n, s_hub, bins, pv, pval, r, x, a.dense_min, post, push, ent, pcs,
+                                s_npc, s_nf, s_vol, s_fvol, sup, sst, s_sup, qn, s_ncnt, tl, s_tl, nlp, s_nlp,
+                                (unsigned int)a.n, tp);
+            hub_on = false;
+            const uint64_t t1 = now_ticks();
+            tick[1] += tp - t0;
+            tick[2] += t1 - tp;
+            tick[4] += 1;
+        } else {
+            // edge phase: v joins the next frontier iff this atomic moved r_v across thr_v
+            wave_edges<ResT<T>>(
+                cnt, pcs, npc, pw, hw, s_work, W, a.colx, load_ent,
+                [&](int vtx, double val, ResT<T> &res, int stage) {
+                    const int li = vtx < hub_n ? vtx : vtx >= sink ? vtx + hub_sink : -1;
+                    if (stage == 0) {
+                        res.lds = li >= 0;
+                        if (li >= 0) res.v = atomicAdd(&hub[li], (T)val);
+                    } else if (li < 0) {
+                        res.v = atomic_add_wg(&r[vtx], (T)val);
+                    }
+                },
+                [&](int, int dv, double val, const ResT<T> &res, bool) -> int {
+                    return post(res.v, res.v + (T)val, dv);
+                },
+                qn, s_ncnt, tl, s_tl, sink, nlp, s_nlp, (unsigned int)a.n);
+            __syncthreads();
+            if (tid == 0) {
+                *s_nf = 0;
+                *s_npc = 0;
+                *s_fvol = 0;
+            }
+            tick[3] += now_ticks() - t0;
+        }
+        __syncthreads();
+        if (tid == 0) *s_cnt = *s_ncnt;
+        int32_t *t = qc;
+        qc = qn;
+        qn = t;
+        __syncthreads();
+    }
+    __syncthreads();
+    if (hub_on) {
+        // hub residuals back into the records (touched hubs have nonzero bits)
+        for (int v = tid; v < hub_n; v += NT) {
+            const T h = hub[v];
+            if (!pristine(h)) r[v] = h;
+        }
+    }
+    __syncthreads();
+    *iters_out = iters;
+    *fnodes_out = fnodes;
+    *final_q = qc;
+    return status;
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __shared__ WaveLds s_w[NT / kWave];
     __shared__ unsigned long long s_red[NT / kWave];
     __shared__ double s_redd[NT / kWave];
-    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl, s_ex;
-    __shared__ unsigned long long s_base, s_vol;
-    // hub cache: the residuals of nodes [0, H) live here for the whole query
-    // (ids are degree-ranked on relabelled graphs, so these take the largest
-    // share of the pushes): LDS atomics instead of memory-side ones
+    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl, s_ex, s_nf;
+    __shared__ unsigned long long s_base, s_vol, s_ivol, s_fvol;
+    __shared__ unsigned int s_bins[kMaxBins];   // binned push: write cursors, then ends of the pair regions
+    // dynamic LDS: the hub cache (residuals of nodes [0, H) for the whole
+    // query -- ids are degree-ranked on relabelled graphs, so these take the
+    // largest share of the pushes: LDS atomics instead of memory-side ones),
+    // or the binned push's counters and destination slice, or the
+    // certificate's copy of the hubs' x32
     extern __shared__ double s_hub[];
-    float *hub32 = reinterpret_cast<float *>(s_hub);
-    double *hub64 = s_hub;
 
     const int tid = threadIdx.x;
     WaveLds &W = s_w[tid / kWave];
-    const int64_t off = (int64_t)(a.slot_base + (int)blockIdx.x) * a.n;
-    NodeRec *rec = a.rec + (int64_t)(a.slot_base + (int)blockIdx.x) * (a.n + kSink);
+    const int64_t slot = (int64_t)(a.slot_base + (int)blockIdx.x);
+    const int64_t off = slot * a.n;
+    SlotView S;
+    S.rx32 = a.rx32 + slot * a.rn;
+    S.fl = a.fl + slot * a.rn;
+    S.rx64 = a.rx64 + slot * a.rn;
+    S.hk2 = a.hk2 + slot * a.rn;
     const int sink = (int)a.n;
     int32_t *qa = a.q0 + off;
     int32_t *qb = a.q1 + off;
     Ent *ent = a.ent + off;
     int32_t *sup = a.sup + off;
     int32_t *sst = a.sst + off;
-    uint32_t *pcs = a.pcs + (int64_t)(a.slot_base + (int)blockIdx.x) * a.pcs_stride;
+    uint32_t *pcs = a.pcs + slot * a.pcs_stride;
     int32_t *tl = a.tl + off;
     int32_t *nl = a.nl + off;
+    int32_t *pv = a.pv ? a.pv + slot * a.pcap : nullptr;
+    double *pval = a.pval ? a.pval + slot * a.pcap : nullptr;
     const int32_t *indptr = a.indptr;
+    const int32_t *degree = a.degree;
     const int2 *colx = a.colx;
     const int pw = a.piece_w;
     const int hw = a.hub_w;
@@ -881,13 +1431,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const flsg_query qq = a.queries[q];
         // a query the launch cannot run fails on its own record, the other
         // queries of the launch go on: seed outside the graph, unknown kind,
-        // a heat-kernel query in a launch without heat-kernel settings, or PPR /
+        // a heat-kernel query in a launch without heat-kernel settings, PPR /
         // Katz parameters outside the range where every push is >= 0
-        // (0 < alpha < 1 for PPR, beta > 0 for Katz, eps > 0)
+        // (0 < alpha < 1 for PPR, beta > 0 for Katz, eps > 0), or an FP32
+        // push on a graph that is not undirected (its certificate's error
+        // bounds count a node's in-edges by its degree)
         const bool bad_kind = qq.kind != FLSG_PPR && qq.kind != FLSG_KATZ && qq.kind != FLSG_HK;
         const bool bad_par = qq.kind == FLSG_HK ? a.hk_n <= 0
                                                 : !(qq.eps > 0.0) || !(qq.alpha > 0.0) ||
-                                                      (qq.kind == FLSG_PPR && !(qq.alpha < 1.0));
+                                                      (qq.kind == FLSG_PPR && !(qq.alpha < 1.0)) ||
+                                                      (qq.precision != FLSG_FP64 && !a.undirected);
         if (qq.seed < 0 || qq.seed >= a.n || bad_kind || bad_par) {
             if (tid == 0) {
                 flsg_result rr{};
@@ -900,211 +1453,43 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         }
         if (qq.kind == FLSG_HK) {
             if (qq.precision == FLSG_FP64)
-                hk_query<NT, true>(a, qq, q, rec, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
+                hk_query<NT, true>(a, qq, q, S, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
                                    &s_work, &s_tn, &s_vol, &s_base, t_start, sink);
             else
-                hk_query<NT, false>(a, qq, q, rec, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
+                hk_query<NT, false>(a, qq, q, S, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
                                     &s_work, &s_tn, &s_vol, &s_base, t_start, sink);
             continue;
         }
         const QParams P = qparams(qq);
         const int s = (int)qq.seed;
-        // cache capacity in elements; its last kWave entries take the identity
-        // adds of idle lanes (so they cost no memory-side atomic)
-        const int hub_cap = P.fp64 ? a.hub_bytes / 8 : a.hub_bytes / 4;
-        const int hub_n = hub_cap > kWave ? (int)min((int64_t)(hub_cap - kWave), a.n) : 0;
-        const int hub_sink = hub_cap > kWave ? hub_cap - kWave - sink : INT32_MIN;   // + vtx of an idle lane
-        for (int i = tid; i < a.hub_bytes / 8; i += NT) s_hub[i] = 0.0;
-        __syncthreads();
-        if (tid == 0) {
-            if (s < hub_n) {
-                if (P.fp64) hub64[s] = P.r0;
-                else hub32[s] = (float)P.r0;
-            } else if (P.fp64) {
-                rec[s].r64 = P.r0;
-            } else {
-                rec[s].r = (float)P.r0;
-            }
-            tl[0] = s;
-            qa[0] = s;
-            s_cnt = 1;
-            s_sup = 0;
-            s_tl = 1;
-            s_nl = 0;
-            s_vol = 0;
-        }
-        __syncthreads();
-        int32_t *qc = qa, *qn = qb;
-        unsigned long long fnodes = 0;
         int64_t iters = 0;
-        int status = FLSG_OK;
+        unsigned long long fnodes = 0;
+        int32_t *qc = qa;
+        int status;
+        uint64_t tick[5] = {0, 0, 0, 0, 0};   // node, pairs, bins, direct (ticks), binned iterations
+        // ---------------- 1. local push (ppr_solver.py:246-288, katz_solver.py:183-224)
+        if (P.fp64)
+            status = local_push<NT, double>(a, P, s, reinterpret_cast<double *>(S.rx64), qa, qb, ent, sup, sst, pcs,
+                                            tl, nl, pv, pval, W, s_hub, s_bins, &s_cnt, &s_ncnt, &s_sup, &s_npc,
+                                            &s_work, &s_tl, &s_nl, &s_nf, &s_vol, &s_ivol, &s_fvol, sink, &iters,
+                                            &fnodes, &qc, tick);
+        else
+            status = local_push<NT, float>(a, P, s, reinterpret_cast<float *>(S.rx32), qa, qb, ent, sup, sst, pcs,
+                                           tl, nl, pv, reinterpret_cast<float *>(pval), W, s_hub, s_bins, &s_cnt,
+                                           &s_ncnt, &s_sup, &s_npc, &s_work, &s_tl, &s_nl, &s_nf, &s_vol, &s_ivol,
+                                           &s_fvol, sink, &iters, &fnodes, &qc, tick);
+        int32_t *qn = qc == qa ? qb : qa;
+        const unsigned long long oper = s_vol;
+        const uint64_t t_push = now_ticks();
+        unsigned long long cert_ops = 0;
+        unsigned long long n_corr = 0;
+        uint64_t t_rec = t_push, t_corr = t_push;
         auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
             const Ent e = ent[i];
             st = e.st;
             d = e.d;
             v = e.v;
         };
-
-        // ---------------- 1. local push (ppr_solver.py:246-288, katz_solver.py:183-224)
-        for (;;) {
-            const unsigned int cnt = s_cnt;
-            if (cnt == 0) break;
-            if (iters >= a.max_iters) {
-                status = FLSG_ENOCONV;
-                break;
-            }
-            iters++;
-            fnodes += cnt;
-            __syncthreads();
-            if (tid == 0) {
-                s_ncnt = 0;
-                s_npc = 0;
-                s_work = 0;
-            }
-            __syncthreads();
-            // node phase: r_u -> 0, x_u += r_u, push value (ppr_solver.py:273-282);
-            // every record of the unrolled entries is read before any is written
-            unsigned long long vol = 0;
-            for (unsigned int i00 = 0; i00 < cnt; i00 += NT * kUnroll) {
-                const unsigned int i0 = i00 + tid;
-                int uu[kUnroll];
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    const unsigned int i = i0 + j * NT;
-                    uu[j] = i < cnt ? qc[i] : -1;
-                }
-                double rr[kUnroll], xx[kUnroll];
-                int dd[kUnroll], ss[kUnroll];
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    rr[j] = 0.0;
-                    xx[j] = 0.0;
-                    dd[j] = 0;
-                    ss[j] = 0;
-                    if (uu[j] >= 0) {
-                        // 16-byte record halves: one request each
-                        const char *pr = reinterpret_cast<const char *>(rec + uu[j]);
-                        float4 lo;
-                        __builtin_memcpy(&lo, pr, 16);
-                        dd[j] = __float_as_int(lo.z);
-                        if (P.fp64) {
-                            double2 hi;
-                            __builtin_memcpy(&hi, pr + 16, 16);
-                            rr[j] = uu[j] < hub_n ? hub64[uu[j]] : hi.x;
-                            xx[j] = hi.y;
-                        } else {
-                            rr[j] = (double)(uu[j] < hub_n ? hub32[uu[j]] : lo.x);
-                            xx[j] = (double)lo.y;
-                        }
-                        ss[j] = indptr[uu[j]];
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    if (uu[j] < 0) continue;
-                    const unsigned int i = i0 + j * NT;
-                    char *pr = reinterpret_cast<char *>(rec + uu[j]);
-                    Ent e;
-                    e.st = ss[j];
-                    e.d = dd[j];
-                    if (P.fp64) {
-                        const double2 hv = make_double2(-0.0, xx[j] + rr[j]);
-                        __builtin_memcpy(pr + 16, &hv, 16);
-                        if (uu[j] < hub_n) hub64[uu[j]] = -0.0;
-                        e.v = bpush_value(P, rr[j], dd[j]);
-                    } else {
-                        const float2 fv = make_float2(-0.0f, (float)xx[j] + (float)rr[j]);
-                        __builtin_memcpy(pr, &fv, 8);
-                        if (uu[j] < hub_n) hub32[uu[j]] = -0.0f;
-                        e.v = (double)(float)bpush_value(P, rr[j], dd[j]);
-                    }
-                    ent[i] = e;
-                    vol += (unsigned long long)dd[j];
-                    add_pieces(i, dd[j], pw, hw, pcs, &s_npc);
-                }
-                // first push of a node: list it in the support (one LDS atomic per wave)
-#pragma unroll
-                for (int j = 0; j < kUnroll; j++) {
-                    const long long k = wave_append(uu[j] >= 0 && xx[j] == 0.0, &s_sup);
-                    if (k >= 0) {
-                        sup[k] = uu[j];
-                        sst[k] = ss[j];
-                    }
-                }
-            }
-            if (vol) atomicAdd(&s_vol, vol);
-            __syncthreads();
-            // edge phase: v joins the next frontier iff this atomic moved r_v across thr_v
-            if (P.fp64) {
-                wave_edges<Res64>(
-                    cnt, pcs, s_npc, pw, hw, &s_work, W, colx, load_ent,
-                    [&](int vtx, double val, Res64 &r, int stage) {
-                        const int li = vtx < hub_n ? vtx : vtx >= sink ? vtx + hub_sink : -1;
-                        if (stage == 0) {
-                            r.lds = li >= 0;
-                            if (li >= 0) r.v = atomicAdd(&hub64[li], val);
-                        } else if (li < 0) {
-                            r.v = atomic_add_wg(&rec[vtx].r64, val);
-                        }
-                    },
-                    [&](int, int dv, double val, const Res64 &r, bool) -> int {
-                        const double old = r.v;
-                        const double nw = old + val;
-                        const double thr = P.thr_scale * (double)dv;
-                        return ((old < thr && thr <= nw) ? 1 : 0) | ((pristine64(old) && nw != 0.0) ? 2 : 0);
-                    },
-                    qn, &s_ncnt, tl, &s_tl, sink);
-            } else {
-                wave_edges<Res32>(
-                    cnt, pcs, s_npc, pw, hw, &s_work, W, colx, load_ent,
-                    [&](int vtx, double val, Res32 &r, int stage) {
-                        const int li = vtx < hub_n ? vtx : vtx >= sink ? vtx + hub_sink : -1;
-                        if (stage == 0) {
-                            r.lds = li >= 0;
-                            if (li >= 0) r.v = atomicAdd(&hub32[li], (float)val);
-                        } else if (li < 0) {
-                            r.v = atomic_add_wg(&rec[vtx].r, (float)val);
-                        }
-                    },
-                    [&](int, int dv, double val, const Res32 &r, bool) -> int {
-                        const float old = r.v;
-                        const float nw = old + (float)val;
-                        const double thr = P.thr_scale * (double)dv;
-                        const bool cross = (double)old < thr && thr <= (double)nw;
-                        const bool first = pristine32(old) && nw != 0.0f;
-                        // boundary candidates for an exact recompute: first touch
-                        // of a high-degree node, or a residual that entered
-                        // [kNearFrac * thr, thr) with this add
-                        const double cn = kNearFrac * thr;
-                        const bool near = dv > kLazyDeg ? first : (!cross && (double)old < cn && cn <= (double)nw);
-                        return (cross ? 1 : 0) | (first ? 2 : 0) | (near ? 4 : 0);
-                    },
-                    qn, &s_ncnt, tl, &s_tl, sink, nl, &s_nl, (unsigned int)a.n);
-            }
-            __syncthreads();
-            if (tid == 0) s_cnt = s_ncnt;
-            int32_t *t = qc;
-            qc = qn;
-            qn = t;
-            __syncthreads();
-        }
-        __syncthreads();
-        // hub residuals back into their records (touched hubs have nonzero bits)
-        for (int v = tid; v < hub_n; v += NT) {
-            if (P.fp64) {
-                const double h = hub64[v];
-                if (!pristine64(h)) rec[v].r64 = h;
-            } else {
-                const float h = hub32[v];
-                if (!pristine32(h)) rec[v].r = h;
-            }
-        }
-        __syncthreads();
-        const unsigned long long oper = s_vol;
-        const uint64_t t_push = now_ticks();
-        unsigned long long cert_ops = 0;
-        unsigned long long n_corr = 0;
-        uint64_t t_rec = t_push, t_corr = t_push;
         if (!P.fp64) {
             // ---------------- 2. FP64 certificate of the FP32 result
             // Exact FP64 residual r64 = b - x + push(x) of the support (from the
@@ -1119,10 +1504,10 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             const unsigned int NL = s_nl;
             const unsigned int T32 = s_tl;
             const bool lazy = (int64_t)NL <= a.n && lazy_g(kLazyDeg, iters) <= kLazyGmax;
-            // the hub cache's LDS now holds x32 of the same nodes for the pull
-            const int xn = hub_n;
-            float *xc = hub32;
-            for (int v = tid; v < xn; v += NT) xc[v] = rec[v].x;
+            // the dynamic LDS now holds x32 of the first ids for the pull
+            const int xn = (int)min((int64_t)(a.hub_bytes / 4), a.n);
+            float *xc = reinterpret_cast<float *>(s_hub);
+            for (int v = tid; v < xn; v += NT) xc[v] = S.rx32[v].y;
             if (tid == 0) {
                 s_cnt = 0;
                 s_ncnt = 0;
@@ -1139,17 +1524,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     int v = -1;
                     if (i < lim) {
                         v = src[i];
-                        float4 lo;
-                        __builtin_memcpy(&lo, rec + v, 16);
-                        if (!pristine32(lo.y)) {
+                        const float xv = S.rx32[v].y;
+                        if (!pristine(xv)) {
                             v = -1;   // pushed: already in the support list
                         } else if (lazy) {
                             // a candidate listed for its degree (> kLazyDeg) whose
                             // final FP32 residual the bound still certifies keeps it
-                            const int d = __float_as_int(lo.z);
+                            const int d = degree[v];
+                            const float rv = S.rx32[v].x;
                             if (d > kLazyDeg && lazy_valid(d, iters) &&
-                                fabs((double)lo.x) + lazy_bound(lo.x, d, iters) <
-                                    kNearFrac * P.thr_scale * (double)d)
+                                fabs((double)rv) + lazy_bound(rv, d, iters) < kNearFrac * P.thr_scale * (double)d)
                                 v = -1;
                         }
                     }
@@ -1158,8 +1542,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 }
                 // the seed's residual was set, not added: never listed by the push
                 if (lazy && tid == 0) {
-                    const NodeRec &rs = rec[s];
-                    if (pristine32(rs.x) && (rs.d > kLazyDeg || P.r0 >= kNearFrac * P.thr_scale * (double)rs.d))
+                    const int ds = degree[s];
+                    if (pristine(S.rx32[s].y) && (ds > kLazyDeg || P.r0 >= kNearFrac * P.thr_scale * (double)ds))
                         qn[atomicAdd(&s_ncnt, 1u)] = s;
                 }
             }
@@ -1176,7 +1560,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 bool exact = false;
                 if (i < S32) {
                     v = sup[i];
-                    exact = !certify_support(rec + v, iters, v == s ? P.r0 : 0.0, P.thr_scale);
+                    exact = !certify_support(S, v, degree[v], iters, v == s ? P.r0 : 0.0, P.thr_scale);
                 }
                 const long long k = wave_append(exact, &s_ex);
                 if (k >= 0) nl[k] = v;
@@ -1184,26 +1568,23 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             __syncthreads();
             const unsigned int E32 = s_ex;
             const unsigned int NLIST = E32 + s_ncnt;
-            // a listed record, finished: r64 = b - x + (pulled pushes) in FP64,
-            // x64 = x32, FP32 residual field -0.0 (exact record), support flag
+            // a listed entry, finished: r64 = b - x + (pulled pushes) in FP64,
+            // x64 = x32, FP32 residual field -0.0 (exact entry), support flag
             // for the pushed nodes, queued for the correction push if |r64| >= thr
-            auto finish = [&](bool ok, int v, const float4 &lo, double acc) {
+            auto finish = [&](bool ok, int v, float x32v, int d, double acc) {
                 bool enq = false;
                 if (ok) {
-                    const double xd = (double)lo.y;
-                    const int d = __float_as_int(lo.z);
+                    const double xd = (double)x32v;
                     double rt = acc - xd;
                     if (v == s) rt = rt + P.r0;
                     enq = fabs(rt) >= P.thr_scale * (double)d;
-                    char *pr = reinterpret_cast<char *>(rec + v);
-                    const double2 hv = make_double2(rt, xd);
-                    __builtin_memcpy(pr + 16, &hv, 16);
-                    const uint32_t f = (pristine32(lo.y) ? 0u : kSeen) | (enq ? kInQ : 0u);
-                    const float4 lv = make_float4(-0.0f, lo.y, lo.z, __uint_as_float(f));
-                    __builtin_memcpy(pr, &lv, 16);
+                    S.rx64[v].x = rt;
+                    S.rx64[v].y = xd;
+                    S.rx32[v].x = -0.0f;
+                    S.fl[v] = (pristine(x32v) ? 0u : kSeen) | (enq ? kInQ : 0u);
                 }
-                const long long slot = wave_append(enq, &s_cnt);
-                if (slot >= 0) qc[slot] = v;
+                const long long sl = wave_append(enq, &s_cnt);
+                if (sl >= 0) qc[sl] = v;
             };
             auto push_of = [&](float xu, int du) { return bpush_value(P, (double)xu, du); };
             // 2a. rows of the support and of the listed boundary nodes
@@ -1213,26 +1594,27 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     v = i < E32 ? nl[i] : qn[i - E32];
                     st = indptr[v];
                 },
-                rec, colx, xc, xn, hw, pw, ent, pcs, &s_npc, &s_nlong, &s_work, W, push_of, finish, cert_ops);
+                S, degree, colx, xc, xn, hw, pw, ent, pcs, &s_npc, &s_nlong, &s_work, W, push_of, finish, cert_ops);
             __syncthreads();
             if (tid == 0) s_work = 0;
             __syncthreads();
-            // 2b. pieces of the long rows, then their records
-            pull_pieces(s_npc, pcs, ent, pw, rec, colx, xc, xn, &s_work, push_of);
+            // 2b. pieces of the long rows, then their entries
+            pull_pieces(s_npc, pcs, ent, pw, S, colx, xc, xn, &s_work, push_of);
             __syncthreads();
             {
                 const unsigned int nlong = s_nlong;
                 for (unsigned int j0 = 0; j0 < nlong; j0 += NT) {
                     const unsigned int j = j0 + tid;
-                    int v = -1;
-                    float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
+                    int v = -1, d = 0;
+                    float xv = 0.0f;
                     double acc = 0.0;
                     if (j < nlong) {
                         v = (int)ent[j].v;
-                        __builtin_memcpy(&lo, rec + v, 16);
-                        acc = rec[v].r64;
+                        xv = S.rx32[v].y;
+                        d = degree[v];
+                        acc = S.rx64[v].x;
                     }
-                    finish(j < nlong, v, lo, acc);
+                    finish(j < nlong, v, xv, d, acc);
                 }
             }
             __syncthreads();
@@ -1258,23 +1640,22 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 unsigned long long pushed = 0, cvol = 0;
                 for (unsigned int i = tid; i < cnt; i += NT) {
                     const int u = qc[i];
-                    NodeRec &ru = rec[u];
-                    const int d = ru.d;
-                    const uint32_t fl = fl_and(&ru.fl, ~kInQ);
+                    const int d = degree[u];
+                    const uint32_t fl = fl_and(&S.fl[u], ~kInQ);
                     // a pending boundary residual (rl > 0) joins the FP64 one;
                     // once pushed its error bound stays (rl < 0)
-                    const float rl = ru.r;
-                    const double tmp = ru.r64 + (rl > 0.0f ? (double)rl : 0.0);
+                    const float rl = S.rx32[u].x;
+                    const double tmp = S.rx64[u].x + (rl > 0.0f ? (double)rl : 0.0);
                     Ent e;
                     e.st = 0;
                     e.d = 0;
                     e.v = 0.0;
                     if (fabs(tmp) + rec_bound(fl, rl, d, iters) >= P.thr_scale * (double)d) {
-                        ru.r64 = 0.0;
-                        if (rl > 0.0f) ru.r = -rl;
-                        ru.x64 = ru.x64 + tmp;
+                        S.rx64[u].x = 0.0;
+                        if (rl > 0.0f) S.rx32[u].x = -rl;
+                        S.rx64[u].y = S.rx64[u].y + tmp;
                         if ((fl & kSeen) == 0) {
-                            fl_or(&ru.fl, kSeen);
+                            fl_or(&S.fl[u], kSeen);
                             sup[atomicAdd(&s_sup, 1u)] = u;
                         }
                         e.st = indptr[u];
@@ -1293,20 +1674,18 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 wave_edges<double>(
                     cnt, pcs, s_npc, pw, hw, &s_work, W, colx, load_ent,
                     [&](int vtx, double val, double &old, int stage) {
-                        if (stage == 1) old = atomic_add_wg(&rec[vtx].r64, val);
+                        if (stage == 1) old = atomic_add_wg(&S.rx64[vtx].x, val);
                     },
                     [&](int vtx, int dv, double val, double old, bool act) -> int {
                         if (!act) return 0;
-                        NodeRec &rv = rec[vtx];
                         const double nw = old + val;
-                        const float rl = rv.r;
+                        const float rl = S.rx32[vtx].x;
                         int res = 0;
-                        // records the main push never reached still hold r == +0.0
-                        if (old == 0.0 && pristine32(rl) && (fl_or(&rv.fl, kTouched) & kTouched) == 0) res = 2;
+                        // entries the main push never reached still hold r == +0.0
+                        if (old == 0.0 && pristine(rl) && (fl_or(&S.fl[vtx], kTouched) & kTouched) == 0) res = 2;
                         const double eff = nw + (rl > 0.0f ? (double)rl : 0.0);
-                        const double bv = rec_bound(rv.fl, rl, dv, iters);
-                        if (fabs(eff) + bv >= P.thr_scale * (double)dv &&
-                            (fl_or(&rv.fl, kInQ) & kInQ) == 0)
+                        const double bv = rec_bound(S.fl[vtx], rl, dv, iters);
+                        if (fabs(eff) + bv >= P.thr_scale * (double)dv && (fl_or(&S.fl[vtx], kInQ) & kInQ) == 0)
                             res |= 1;
                         return res;
                     },
@@ -1321,20 +1700,20 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             __syncthreads();
             t_corr = now_ticks();
         }
-        const unsigned int S = s_sup;
+        const unsigned int Sn = s_sup;
         const unsigned int T = s_tl;
-        // ---------------- 3. output, certificate value, reset of every touched record
-        if (tid == 0) s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)S);
+        // ---------------- 3. output, certificate value, reset of every touched entry
+        if (tid == 0) s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)Sn);
         __syncthreads();
         const unsigned long long obase = s_base;
-        const bool fits = obase + S <= (unsigned long long)a.out_cap;
-        // one pass over the touched records: certificate value, output of the
+        const bool fits = obase + Sn <= (unsigned long long)a.out_cap;
+        // one pass over the touched entries: certificate value, output of the
         // support (listed by flag in the FP32 path, by x != 0 in the FP64 path,
-        // whose x only grows; in any order), record reset; the loads of kUnroll
+        // whose x only grows; in any order), reset; the loads of kOutUnroll
         // entries are issued before any is used.  When the query touched more
-        // than a quarter of the graph, the pass streams over all n records of
-        // the slot (coalesced, untouched ones -- all bits zero but d -- are
-        // skipped) instead of gathering the touched list's records one by one.
+        // than a quarter of the graph, the pass streams over all n entries of
+        // the slot (coalesced, untouched ones -- all bits zero -- are skipped)
+        // instead of gathering the touched list's entries one by one.
         if (tid == 0) s_ncnt = 0;
         __syncthreads();
         double cmax = 0.0, rl1 = 0.0;
@@ -1342,64 +1721,75 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const unsigned int lim = stream ? (unsigned int)a.n : T;
         for (unsigned int i0 = 0; i0 < lim; i0 += NT * kOutUnroll) {
             int vv[kOutUnroll];
-            float4 lo[kOutUnroll];
-            double2 hi[kOutUnroll];
+            float r32v[kOutUnroll], x32v[kOutUnroll];
+            uint32_t flv[kOutUnroll];
+            double r64v[kOutUnroll], x64v[kOutUnroll];
 #pragma unroll
             for (int j = 0; j < kOutUnroll; j++) {
                 const unsigned int i = i0 + j * NT + tid;
                 vv[j] = i < lim ? (stream ? (int)i : tl[i]) : -1;
             }
-            // (the FP64 push never writes the FP32 half of a record, so there only
-            // the FP64 half is cleared; the degree is read from the record's own
-            // line, which a load of the shared degree array would not hit)
+            // (the FP64 push writes only r64 and x64, so there only those are read
+            // and cleared)
 #pragma unroll
             for (int j = 0; j < kOutUnroll; j++) {
-                lo[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                hi[j] = make_double2(0.0, 0.0);
+                r32v[j] = 0.0f;
+                x32v[j] = 0.0f;
+                flv[j] = 0u;
+                r64v[j] = 0.0;
+                x64v[j] = 0.0;
                 if (vv[j] >= 0) {
-                    const char *pr = reinterpret_cast<const char *>(rec + vv[j]);
-                    __builtin_memcpy(&lo[j], pr, 16);
-                    __builtin_memcpy(&hi[j], pr + 16, 16);
+                    const int v = vv[j];
+                    if (!P.fp64) {
+                        r32v[j] = S.rx32[v].x;
+                        x32v[j] = S.rx32[v].y;
+                        flv[j] = S.fl[v];
+                    }
+                    r64v[j] = S.rx64[v].x;
+                    x64v[j] = S.rx64[v].y;
                 }
             }
             bool is_sup[kOutUnroll];
             int id[kOutUnroll];
+            double outv[kOutUnroll];
 #pragma unroll
             for (int j = 0; j < kOutUnroll; j++) {
                 is_sup[j] = false;
                 id[j] = vv[j];
+                outv[j] = x64v[j];
                 if (vv[j] < 0) continue;
-                if (stream && (__float_as_uint(lo[j].x) | __float_as_uint(lo[j].y) | __float_as_uint(lo[j].w)) == 0u &&
-                    __double_as_longlong(hi[j].x) == 0ll && __double_as_longlong(hi[j].y) == 0ll) {
-                    id[j] = -1;   // untouched record: nothing to certify, output or reset
+                const int v = vv[j];
+                if (stream && (__float_as_uint(r32v[j]) | __float_as_uint(x32v[j]) | flv[j]) == 0u &&
+                    pristine(r64v[j]) && pristine(x64v[j])) {
+                    id[j] = -1;   // untouched entry: nothing to certify, output or reset
                     continue;
                 }
-                const int d = __float_as_int(lo[j].z);
-                // FP32 push: a boundary record may hold its residual in FP32
+                const int d = degree[v];
+                // FP32 push: a boundary entry may hold its residual in FP32
                 // (pending, rl > 0) and carries the error bound of that sum
-                const float rl = P.fp64 ? 0.0f : lo[j].x;
-                const double res = hi[j].x + (rl > 0.0f ? (double)rl : 0.0);
-                const double bnd = P.fp64 ? 0.0 : rec_bound(__float_as_uint(lo[j].w), rl, d, iters);
+                const float rl = r32v[j];
+                const double res = r64v[j] + (rl > 0.0f ? (double)rl : 0.0);
+                const double bnd = P.fp64 ? 0.0 : rec_bound(flv[j], rl, d, iters);
                 if (res != 0.0 || bnd != 0.0) {
                     cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
                     rl1 += fabs(res) + bnd;
                 }
-                is_sup[j] = P.fp64 ? hi[j].y != 0.0 : (__float_as_uint(lo[j].w) & kSeen) != 0u;
-                if (is_sup[j] && a.out_map) id[j] = a.out_map[vv[j]];
-                char *pr = reinterpret_cast<char *>(rec + vv[j]);
-                const double2 zhi = make_double2(0.0, 0.0);
+                is_sup[j] = P.fp64 ? x64v[j] != 0.0 : (flv[j] & kSeen) != 0u;
+                if (is_sup[j] && a.out_map) id[j] = a.out_map[v];
                 if (!P.fp64) {
-                    const float4 zlo = make_float4(0.f, 0.f, lo[j].z, 0.f);   // keep d
-                    __builtin_memcpy(pr, &zlo, 16);
+                    S.rx32[v].x = 0.0f;
+                    S.rx32[v].y = 0.0f;
+                    S.fl[v] = 0u;
                 }
-                __builtin_memcpy(pr + 16, &zhi, 16);
+                S.rx64[v].x = 0.0;
+                S.rx64[v].y = 0.0;
             }
 #pragma unroll
             for (int j = 0; j < kOutUnroll; j++) {
                 const long long k = wave_append(is_sup[j], &s_ncnt);
-                if (k >= 0 && fits && (unsigned long long)k < S) {
+                if (k >= 0 && fits && (unsigned long long)k < Sn) {
                     a.out_ids[obase + k] = id[j];
-                    a.out_vals[obase + k] = hi[j].y;
+                    a.out_vals[obase + k] = outv[j];
                 }
             }
         }
@@ -1413,7 +1803,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             rr.iters = iters;
             rr.oper_cert = (int64_t)cops;
             rr.n_corr = (int64_t)n_corr;
-            rr.supp = (int64_t)min(s_ncnt, S);   // entries written (== S: every support record is listed)
+            rr.supp = (int64_t)min(s_ncnt, Sn);   // entries written (== Sn: every support entry is listed)
             rr.nodes = (int64_t)fnodes;
             rr.cert = cmax;
             rr.r_l1 = rl1;
@@ -1423,6 +1813,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
             rr.ms_correct = (double)(t_corr - t_start) * 1e-5;
             rr.status = fits ? status : FLSG_ECAP;
             rr.touched = (int32_t)T;
+            rr.ms_node = (double)tick[0] * 1e-5;
+            rr.ms_pairs = (double)tick[1] * 1e-5;
+            rr.ms_bins = (double)tick[2] * 1e-5;
+            rr.ms_direct = (double)tick[3] * 1e-5;
+            rr.bin_iters = (int64_t)tick[4];
             a.res[q] = rr;
             a.out_off[q] = fits ? (int64_t)obase : -1;
         }
@@ -1430,47 +1825,18 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     }
 }
 
-// records of every slot: zero, d = degree (sink records: d = 0)
-__global__ void k_slot_init(NodeRec *rec, const int32_t *degree, int64_t n, int64_t total) {
-    const int64_t rn = n + kSink;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t u = i % rn;
-        NodeRec z{};
-        z.d = u < n ? degree[u] : 0;
-        rec[i] = z;
-    }
-}
-
-// after an edge insertion: refresh d of the listed nodes in every slot
-__global__ void k_slot_degrees(NodeRec *rec, const int32_t *degree, int64_t n, int32_t slots, const int32_t *keys,
-                               int64_t nk) {
-    const int64_t total = nk * slots;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t k = i / slots;
-        const int64_t sl = i - k * slots;
-        const int u = keys[k];
-        rec[sl * (n + kSink) + u].d = degree[u];
-    }
-}
-
-// records of every slot not in the between-queries state (all fields zero but
-// d; d = degree, 0 for sink records), counted into *cnt
-__global__ void k_arena_dirty(const NodeRec *rec, const int32_t *degree, int64_t n, int64_t total,
-                              unsigned long long *cnt) {
-    const int64_t rn = n + kSink;
+// every slot array not in the between-queries state (all zero), counted into *cnt
+template <typename T>
+__global__ void k_nonzero(const T *p, int64_t total, unsigned long long *cnt) {
     unsigned long long c = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t u = i % rn;
-        float4 lo;
-        double2 hi;
-        __builtin_memcpy(&lo, rec + i, 16);
-        __builtin_memcpy(&hi, reinterpret_cast<const char *>(rec + i) + 16, 16);
-        const int dd = u < n ? degree[u] : 0;
-        const bool bad = (__float_as_uint(lo.x) | __float_as_uint(lo.y) | __float_as_uint(lo.w)) != 0u ||
-                         __float_as_int(lo.z) != dd || __double_as_longlong(hi.x) != 0ll ||
-                         __double_as_longlong(hi.y) != 0ll;
-        c += bad ? 1ull : 0ull;
-    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        c += pristine(p[i]) ? 0ull : 1ull;
+    if (c) atomicAdd(cnt, c);
+}
+__global__ void k_nonzero_u32(const uint32_t *p, int64_t total, unsigned long long *cnt) {
+    unsigned long long c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        c += p[i] ? 1ull : 0ull;
     if (c) atomicAdd(cnt, c);
 }
 
@@ -1517,6 +1883,31 @@ __global__ void k_colx(const int32_t *indices, const int32_t *degree, int64_t n,
     }
 }
 
+// live entries (u, v) as sort keys u:v and v:u (dead slack slots: ~0 in both)
+__global__ void k_edge_keys(const int32_t *indptr, const int32_t *indices, const int32_t *degree, int64_t n,
+                            int64_t nnz, uint64_t *fwd, uint64_t *rev) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x) {
+        const int st = indptr[u], en = indptr[u + 1], d = degree[u];
+        for (int e = st; e < en; e++) {
+            uint64_t a = ~0ull, b = ~0ull;
+            if (e - st < d) {
+                const uint32_t v = (uint32_t)indices[e];
+                a = ((uint64_t)u << 32) | v;
+                b = ((uint64_t)v << 32) | (uint64_t)u;
+            }
+            fwd[e] = a;
+            rev[e] = b;
+        }
+    }
+}
+
+__global__ void k_count_diff(const uint64_t *a, const uint64_t *b, int64_t m, unsigned long long *cnt) {
+    unsigned long long c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        c += a[i] != b[i] ? 1ull : 0ull;
+    if (c) atomicAdd(cnt, c);
+}
+
 int colx_refresh(flsg_graph g, hipStream_t st) {
     if (!g->colx || g->nnz_cap == 0) return FLSG_OK;
     hipLaunchKernelGGL(k_colx, dim3(g->num_cus * 8), dim3(256), 0, st, (const int32_t *)g->indices,
@@ -1536,12 +1927,62 @@ static int colx_ensure(flsg_graph g, hipStream_t st) {
     return FLSG_OK;
 }
 
+// Whether every live entry (u, v) has its (v, u) -- the FP32 certificate's
+// error bounds count a node's incoming pushes by its degree.  Decided once per
+// handle by sorting the entries both ways (edge insertions add both
+// directions, so it stays valid).
+static int undirected_ensure(flsg_graph g, hipStream_t st) {
+    if (g->undirected >= 0) return FLSG_OK;
+    const int64_t m = g->nnz_cap;
+    if (m == 0) {
+        g->undirected = 1;
+        return FLSG_OK;
+    }
+    if (m >= INT32_MAX) return fail(FLSG_EARG, "graph too large for the symmetry check");
+    uint64_t *k[4] = {nullptr, nullptr, nullptr, nullptr};
+    unsigned long long *d_cnt = nullptr;
+    void *tmp = nullptr;
+    size_t tb = 0;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipMalloc((void **)&k[i], sizeof(uint64_t) * (size_t)m);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_cnt, sizeof(unsigned long long));
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortKeys(nullptr, tb, (const uint64_t *)nullptr, (uint64_t *)nullptr, (int)m, 0,
+                                              64, st);
+    if (e == hipSuccess) e = hipMalloc(&tmp, tb);
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_edge_keys, dim3(g->num_cus * 4), dim3(256), 0, st, g->indptr,
+                           (const int32_t *)g->indices, (const int32_t *)g->degree, g->n, m, k[0], k[1]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortKeys(tmp, tb, k[0], k[2], (int)m, 0, 64, st);
+    if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortKeys(tmp, tb, k[1], k[3], (int)m, 0, 64, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_count_diff, dim3(g->num_cus * 4), dim3(256), 0, st, (const uint64_t *)k[2],
+                           (const uint64_t *)k[3], m, d_cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    for (int i = 0; i < 4; i++) (void)hipFree(k[i]);
+    (void)hipFree(d_cnt);
+    (void)hipFree(tmp);
+    FLSG_HIP(e);
+    g->undirected = h == 0 ? 1 : 0;
+    return FLSG_OK;
+}
+
 void batch_free(flsg_graph g) {
     (void)hipFree(g->colx);
     g->colx = nullptr;
     BatchArena *A = g->arena;
     if (!A) return;
-    (void)hipFree(A->rec);
+    (void)hipFree(A->rx32);
+    (void)hipFree(A->fl);
+    (void)hipFree(A->rx64);
+    (void)hipFree(A->hk2);
     (void)hipFree(A->q0);
     (void)hipFree(A->q1);
     (void)hipFree(A->ent);
@@ -1550,6 +1991,8 @@ void batch_free(flsg_graph g) {
     (void)hipFree(A->pcs);
     (void)hipFree(A->tl);
     (void)hipFree(A->nl);
+    (void)hipFree(A->pv);
+    (void)hipFree(A->pval);
     (void)hipFree(A->ctl);
     (void)hipFree(A->d_res);
     (void)hipFree(A->d_q);
@@ -1559,11 +2002,15 @@ void batch_free(flsg_graph g) {
     g->arena = nullptr;
 }
 
+// bytes per slot for n nodes (per-node arrays and lists), without the binned
+// push's pair buffer (12 bytes per column entry of the graph)
 int64_t slot_bytes(int64_t n) {
-    // 32 B node record + q0 q1 sup sst pcs tl nl (4 B) + frontier entry (16 B), kSink sink records
-    // (the hub-piece list adds 4 B per 64 column slots of the graph)
-    return n * (32 + 7 * 4 + 16) + (int64_t)kSink * 32;
+    // rx32 {r, x} and fl (8 + 4 B), rx64 {r, x} and hk2 (16 + 8 B) over n + kSink entries; q0 q1 sup sst tl nl (4 B) +
+    // frontier entry (16 B)
+    return (n + kSink) * (3 * 4 + 3 * 8) + n * (6 * 4 + 16);
 }
+
+static int64_t pair_bytes(flsg_graph g) { return g->nnz_cap * 12; }
 
 static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     BatchArena *A = g->arena;
@@ -1573,9 +2020,13 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     g->arena = A;
     A->n = g->n;
     A->slots = slots;
+    A->rn = g->n + kSink;
     const size_t ns = (size_t)g->n * (size_t)slots;
-    const size_t nrec = ((size_t)g->n + kSink) * (size_t)slots;
-    FLSG_HIP(hipMalloc((void **)&A->rec, sizeof(NodeRec) * nrec));
+    const size_t nr = (size_t)A->rn * (size_t)slots;
+    FLSG_HIP(hipMalloc((void **)&A->rx32, sizeof(float2) * nr));
+    FLSG_HIP(hipMalloc((void **)&A->fl, sizeof(uint32_t) * nr));
+    FLSG_HIP(hipMalloc((void **)&A->rx64, sizeof(double2) * nr));
+    FLSG_HIP(hipMalloc((void **)&A->hk2, sizeof(double) * nr));
     FLSG_HIP(hipMalloc((void **)&A->q0, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->q1, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ent, sizeof(Ent) * ns));
@@ -1587,17 +2038,22 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     FLSG_HIP(hipMalloc((void **)&A->pcs, sizeof(uint32_t) * (size_t)A->pcs_stride * (size_t)slots));
     FLSG_HIP(hipMalloc((void **)&A->tl, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->nl, sizeof(int32_t) * ns));
+    // binned push: one iteration's pairs (at most one per column entry)
+    A->pcap = g->nnz_cap > 0 ? g->nnz_cap : 1;
+    FLSG_HIP(hipMalloc((void **)&A->pv, sizeof(int32_t) * (size_t)A->pcap * (size_t)slots));
+    FLSG_HIP(hipMalloc((void **)&A->pval, sizeof(double) * (size_t)A->pcap * (size_t)slots));
     FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
-    hipLaunchKernelGGL(k_slot_init, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree, g->n,
-                       (int64_t)nrec);
-    FLSG_HIP(hipGetLastError());
+    FLSG_HIP(hipMemsetAsync(A->rx32, 0, sizeof(float2) * nr, st));
+    FLSG_HIP(hipMemsetAsync(A->fl, 0, sizeof(uint32_t) * nr, st));
+    FLSG_HIP(hipMemsetAsync(A->rx64, 0, sizeof(double2) * nr, st));
+    FLSG_HIP(hipMemsetAsync(A->hk2, 0, sizeof(double) * nr, st));
     return FLSG_OK;
 }
 
 static int32_t default_slots(flsg_graph g) {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return g->num_cus;
-    const int64_t per = slot_bytes(g->n);
+    const int64_t per = slot_bytes(g->n) + pair_bytes(g);
     int64_t fit = (int64_t)(fr / 2) / (per > 0 ? per : 1);
     int64_t want = (int64_t)g->num_cus;
     if (fit < want) want = fit;
@@ -1615,16 +2071,23 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
         return fail(FLSG_EARG, "slot range outside the reserved arena");
     int rc = colx_ensure(g, st);
     if (rc) return rc;
+    rc = undirected_ensure(g, st);
+    if (rc) return rc;
     FLSG_HIP(hipMemsetAsync(d_ctl, 0, sizeof(uint64_t) * 2, st));
     BatchArgs a{};
     a.indptr = g->indptr;
     a.indices = g->indices;
+    a.degree = g->degree;
     a.colx = g->colx;
     a.n = g->n;
     a.queries = d_q;
     a.nq = nq;
     a.order = d_order;
-    a.rec = A->rec;
+    a.rx32 = A->rx32;
+    a.fl = A->fl;
+    a.rx64 = A->rx64;
+    a.hk2 = A->hk2;
+    a.rn = A->rn;
     a.q0 = A->q0;
     a.q1 = A->q1;
     a.ent = A->ent;
@@ -1634,6 +2097,12 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.pcs_stride = A->pcs_stride;
     a.tl = A->tl;
     a.nl = A->nl;
+    a.pv = A->pv;
+    a.pval = A->pval;
+    a.pcap = A->pcap;
+    a.bin_min = g->batch_bin_min;
+    a.dense_min = g->batch_dense_min;
+    a.undirected = g->undirected;
     // hub pieces are encoded (entry << 8) | piece: entries < 2^24, < 256 pieces per node
     if (g->n >= (1 << 24)) return fail(FLSG_EARG, "batched solves need n < 2^24");
     // degree bins: entries with more than hub_w edges are split into pieces of
@@ -1663,7 +2132,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     }
     const int grid = n_slots;
     int hub = g->batch_hub_bytes;
-    if (hub < 0) hub = block == 1024 ? 120 * 1024 : block == 512 ? 48 * 1024 : 24 * 1024;
+    if (hub < 0) hub = block == 1024 ? 128 * 1024 : block == 512 ? 48 * 1024 : 24 * 1024;
     a.hub_bytes = hub;
     const int bi = block == 1024 ? 0 : block == 256 ? 1 : 2;
     const void *fn = bi == 0   ? reinterpret_cast<const void *>(&k_batch<1024>)
@@ -1685,6 +2154,16 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     else if (block == 256) hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), hub, st, a);
     else hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), hub, st, a);
     FLSG_HIP(hipGetLastError());
+    return FLSG_OK;
+}
+
+// after an edge insertion: nothing per slot follows the degrees any more
+// (they are read from the graph), so only the handle-level state is touched
+int batch_degrees_changed(flsg_graph g, const int32_t *d_rows, int64_t n_rows, hipStream_t st) {
+    (void)g;
+    (void)d_rows;
+    (void)n_rows;
+    (void)st;
     return FLSG_OK;
 }
 
@@ -1720,11 +2199,20 @@ extern "C" int flsg_batch_set_degree_bins(flsg_graph g, int32_t split_edges, int
     return FLSG_OK;
 }
 
+extern "C" int flsg_batch_set_binning(flsg_graph g, int64_t min_edges, int32_t dense_pairs) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (min_edges < 0 || dense_pairs < 0) return fail(FLSG_EARG, "binning: min_edges >= 0 and dense_pairs >= 0");
+    g->batch_bin_min = min_edges == 0 ? INT64_MAX : min_edges;
+    g->batch_dense_min = dense_pairs;
+    return FLSG_OK;
+}
+
 extern "C" int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes) {
     int rc = check_graph(g);
     if (rc) return rc;
-    // the largest hub cache: the device's LDS per workgroup (160 KiB on
-    // gfx950) less the static LDS of the 1024-thread kernel
+    // the largest: the device's LDS per workgroup (160 KiB on gfx950) less
+    // the static LDS of the 1024-thread kernel
     int lds_max = 0;
     FLSG_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device));
     hipFuncAttributes fa;
@@ -1759,19 +2247,10 @@ extern "C" int flsg_batch_reserve(flsg_graph g, int32_t slots, void *stream) {
     g->batch_slots = slots;
     rc = ensure_arena(g, slots, (hipStream_t)stream);
     if (rc) return rc;
-    return colx_ensure(g, (hipStream_t)stream);
+    rc = colx_ensure(g, (hipStream_t)stream);
+    if (rc) return rc;
+    return undirected_ensure(g, (hipStream_t)stream);
 }
-
-namespace flsg {
-int batch_degrees_changed(flsg_graph g, const int32_t *d_rows, int64_t n_rows, hipStream_t st) {
-    BatchArena *A = g->arena;
-    if (!A || n_rows <= 0) return FLSG_OK;
-    hipLaunchKernelGGL(k_slot_degrees, dim3(g->num_cus * 8), dim3(256), 0, st, A->rec, (const int32_t *)g->degree,
-                       g->n, A->slots, d_rows, n_rows);
-    FLSG_HIP(hipGetLastError());
-    return FLSG_OK;
-}
-}  // namespace flsg
 
 extern "C" int flsg_batch_refresh_degrees(flsg_graph g, const int32_t *d_keys, int64_t n_keys, void *stream) {
     int rc = check_graph(g);
@@ -1790,10 +2269,14 @@ extern "C" int flsg_batch_arena_dirty(flsg_graph g, int64_t *h_count, void *stre
     unsigned long long *d_cnt = nullptr;
     FLSG_HIP(hipMalloc((void **)&d_cnt, sizeof(unsigned long long)));
     unsigned long long h = 0;
+    const int64_t tot = A->rn * A->slots;
+    const dim3 gr(g->num_cus * 8), bl(256);
     hipError_t e = hipMemcpyAsync(d_cnt, &h, sizeof(h), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_arena_dirty, dim3(g->num_cus * 8), dim3(256), 0, st, (const NodeRec *)A->rec,
-                           (const int32_t *)g->degree, g->n, (int64_t)(A->n + kSink) * A->slots, d_cnt);
+        hipLaunchKernelGGL(k_nonzero<float>, gr, bl, 0, st, (const float *)A->rx32, 2 * tot, d_cnt);
+        hipLaunchKernelGGL(k_nonzero_u32, gr, bl, 0, st, (const uint32_t *)A->fl, tot, d_cnt);
+        hipLaunchKernelGGL(k_nonzero<double>, gr, bl, 0, st, (const double *)A->rx64, 2 * tot, d_cnt);
+        hipLaunchKernelGGL(k_nonzero<double>, gr, bl, 0, st, (const double *)A->hk2, tot, d_cnt);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st);
@@ -1831,6 +2314,11 @@ extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_
     if (p->method != FLSG_GD || !p->local) return fail(FLSG_EARG, "batched solves: local gradient descent");
     if (p->precision != FLSG_FP32 && p->precision != FLSG_FP64) return fail(FLSG_EARG, "unknown precision");
     hipStream_t st = (hipStream_t)stream;
+    rc = undirected_ensure(g, st);
+    if (rc) return rc;
+    if (p->precision == FLSG_FP32 && !g->undirected)
+        return fail(FLSG_EARG, "batched FP32 push: the graph is not undirected (its certificate bounds count a "
+                               "node's incoming pushes by its degree); use FLSG_FP64");
     const int32_t slots = g->batch_slots > 0 ? g->batch_slots : default_slots(g);
     rc = ensure_arena(g, slots, st);
     if (rc) return rc;

@@ -108,6 +108,9 @@ struct flsg_graph_s {
     int32_t batch_hub_bytes = -1;           // LDS hub cache per workgroup (-1 = default for the block)
     int32_t batch_hub_w = 0;                // degree bin: entries with more edges are split (0 = 512)
     int32_t batch_piece_w = 0;              // edges per piece (0 = 1024, doubled to <= 255 pieces per entry)
+    int64_t batch_bin_min = 16384;          // binned push for iterations of at least this many edges
+    int32_t batch_dense_min = 256;          // binned push: bins with fewer pairs use memory-side atomics
+    int32_t undirected = -1;                // every edge (u, v) has its (v, u): -1 unknown, 0 no, 1 yes
     const int32_t *out_map = nullptr;       // internal id -> caller id for batched outputs
 };
 
@@ -119,7 +122,7 @@ int64_t host_estimate_n(double t, double eps);
 int check_graph(flsg_graph g);
 void batch_free(flsg_graph g);
 int colx_refresh(flsg_graph g, hipStream_t st);
-// after an edge insertion: the batched slots' copies of the listed rows' degrees
+// after an edge insertion (the batched path keeps no per-slot degree copies)
 int batch_degrees_changed(flsg_graph g, const int32_t *d_rows, int64_t n_rows, hipStream_t st);
 void cheb_free(flsg_graph g);
 void ignn_free(flsg_graph g);

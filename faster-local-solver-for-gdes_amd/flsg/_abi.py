@@ -29,6 +29,8 @@ class Result(C.Structure):
         ("supp", C.c_int64), ("nodes", C.c_int64), ("cert", C.c_double), ("ms", C.c_double),
         ("ms_push", C.c_double), ("status", C.c_int32), ("touched", C.c_int32),
         ("ms_recompute", C.c_double), ("ms_correct", C.c_double), ("r_l1", C.c_double),
+        ("ms_node", C.c_double), ("ms_pairs", C.c_double), ("ms_bins", C.c_double), ("ms_direct", C.c_double),
+        ("bin_iters", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -71,6 +73,7 @@ SIGNATURES = {
     "flsg_batch_set_block": (C.c_int, [C.c_void_p, C.c_int32]),
     "flsg_batch_set_hub_cache": (C.c_int, [C.c_void_p, C.c_int32]),
     "flsg_batch_set_degree_bins": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "flsg_batch_set_binning": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
     "flsg_batch_reserve": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "flsg_graph_set_out_map": (C.c_int, [C.c_void_p, C.c_void_p]),
     "flsg_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,

@@ -1,5 +1,6 @@
-"""Autotuner of the batched kernel: workgroup size and slot count, LDS hub
-cache size and the degree bins of the edge phases, picked by measuring the
+"""Autotuner of the batched kernel: workgroup size and slot count, the
+dynamic LDS size (hub cache / binned push slice), the binned push's
+thresholds and the degree bins of the edge phases, picked by measuring the
 throughput of a sample of the caller's own queries on the resident graph.
 
 Coordinate descent: starting from the defaults, each knob in turn is swept
@@ -24,6 +25,7 @@ def _apply(dg: DeviceGraph, cfg: dict) -> None:
     _check(lib.flsg_batch_set_slots(dg.handle, int(cfg["slots"])))
     _check(lib.flsg_batch_set_hub_cache(dg.handle, int(cfg["hub_bytes"])))
     _check(lib.flsg_batch_set_degree_bins(dg.handle, int(cfg["split_edges"]), int(cfg["piece_edges"])))
+    _check(lib.flsg_batch_set_binning(dg.handle, int(cfg["bin_min"]), int(cfg["dense_min"])))
 
 
 def tune_batched(dg: DeviceGraph, params: _abi.Params, seeds, rounds: int = 2, log=None) -> dict:
@@ -35,17 +37,20 @@ def tune_batched(dg: DeviceGraph, params: _abi.Params, seeds, rounds: int = 2, l
     tot = C.c_int64(0)
     _check(dg.lib.flsg_device_info(C.byref(nc), C.byref(free), C.byref(tot)))
     cus = max(1, int(nc.value))
-    slot_b = int(dg.lib.flsg_batch_slot_bytes(dg.n))
+    slot_b = int(dg.lib.flsg_batch_slot_bytes(dg.n)) + 12 * int(dg.nnz_cap)   # + the binned push's pairs
     shapes = [(1024, cus)]
     if 2 * cus * slot_b < free.value // 2:
         shapes.append((512, 2 * cus))
     space = [
         ("shape", shapes),
-        ("hub_bytes", [-1, 0, 64 * 1024, 122880, 136 * 1024, 143 * 1024]),
-        ("split_edges", [256, 512, 1024, 2048]),
-        ("piece_edges", [0, 2048, 4096, 8192]),
+        ("hub_bytes", [-1, 64 * 1024, 136 * 1024]),
+        ("bin_min", [0, 4096, 65536]),
+        ("dense_min", [64, 1024]),
+        ("split_edges", [256, 1024]),
+        ("piece_edges", [0, 4096]),
     ]
-    cfg = {"block": 1024, "slots": cus, "hub_bytes": -1, "split_edges": 512, "piece_edges": 0}
+    cfg = {"block": 1024, "slots": cus, "hub_bytes": -1, "bin_min": 16384, "dense_min": 256, "split_edges": 512,
+           "piece_edges": 0}
     trials = []
     seeds = np.asarray(seeds)
 

@@ -92,6 +92,11 @@ typedef struct {
                             its FP32 error bound -- the L1 norm of the certified
                             residual, so ||x - x*||_1 <= r_l1 / alpha for PPR
                             (Katz: r_l1 / (1 - beta d_max))                  */
+    double ms_node;      /* batched push, summed over its iterations: node phases  */
+    double ms_pairs;     /* binned iterations: count + write passes of the pairs   */
+    double ms_bins;      /* binned iterations: the ranges' sums and adds            */
+    double ms_direct;    /* directly pushed iterations: the edge phases            */
+    int64_t bin_iters;   /* iterations pushed binned                               */
 } flsg_result;
 
 /* Optional per-iteration frontier trace (device buffers): iteration t
@@ -161,7 +166,9 @@ int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_t *d_seeds,
                      flsg_result *h_res, void *stream);
 
 /* Size of the per-query device state a batched launch keeps resident, in
- * bytes per workgroup slot (the unit of memory-footprint admission). */
+ * bytes per workgroup slot (the unit of memory-footprint admission): the
+ * per-node arrays and lists of an n-node graph; the binned push adds 12
+ * bytes per column entry of the graph to every slot. */
 int64_t flsg_batch_slot_bytes(int64_t n);
 
 /* Number of resident workgroup slots the handle uses for batched solves
@@ -170,14 +177,25 @@ int64_t flsg_batch_slot_bytes(int64_t n);
 int flsg_batch_set_slots(flsg_graph g, int32_t slots);
 int flsg_batch_set_block(flsg_graph g, int32_t block);
 
-/* LDS bytes per workgroup that hold the residuals of nodes [0, bytes/4)
- * (FP32 push) or [0, bytes/8) (FP64 push and certificate sums) for a whole
- * query: on degree-relabelled graphs these are the hubs, whose pushes then
- * use LDS atomics.  -1 = default for the workgroup size (120 KiB at 1024,
+/* Dynamic LDS bytes per workgroup of the batched kernel.  In iterations
+ * pushed directly they hold the residuals of nodes [0, bytes/4) (FP32 push)
+ * or [0, bytes/8) (FP64 push): on degree-relabelled graphs these are the
+ * hubs, whose pushes then use LDS atomics; in binned iterations they hold
+ * the slice of 2^s residuals of one destination range (2^s the largest power
+ * of two that fits; binning needs 2^s >= 2048); in the certificate, x32 of
+ * the first ids.  -1 = default for the workgroup size (128 KiB at 1024,
  * 48 KiB at 512, 24 KiB at 256), 0 = off, else a multiple of 8 up to the
- * device's LDS per workgroup less the kernel's static LDS (147,144 bytes on
+ * device's LDS per workgroup less the kernel's static LDS (~147,000 bytes on
  * gfx950).  No counterpart in the reference (tuning knob). */
 int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes);
+
+/* Binned push of the batched kernel: an iteration of at least min_edges
+ * edges (0 = never) writes its pushes as (destination, value) pairs into one
+ * bin per destination range and sums each bin in LDS before one coalesced
+ * add into the residuals; a bin with fewer than dense_pairs pairs is applied
+ * with memory-side atomics instead.  Defaults 16384 and 256.  No counterpart
+ * in the reference (tuning knob, set by the autotuner). */
+int flsg_batch_set_binning(flsg_graph g, int64_t min_edges, int32_t dense_pairs);
 
 /* Degree bins of the batched edge phases: a list entry with more than
  * split_edges edges (0 = 512) is processed as pieces of piece_edges edges

@@ -28,7 +28,7 @@ def test_library_exports_every_symbol():
 
 def test_struct_layouts():
     assert ctypes.sizeof(_abi.Params) == 4 * 4 + 5 * 8 + 2 * 8
-    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 3 * 8 + 2 * 4 + 3 * 8
+    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 3 * 8 + 2 * 4 + 3 * 8 + 5 * 8
     assert ctypes.sizeof(_abi.Query) == 8 + 2 * 4 + 2 * 8
     assert ctypes.sizeof(_abi.Trace) == 5 * 8 + 2 * 4
 
@@ -64,4 +64,6 @@ def test_struct_offsets_match_the_header(tmp_path):
 def test_version_without_gpu():
     lib = _build.load_lib()
     assert lib.flsg_version() >= 1
-    assert lib.flsg_batch_slot_bytes(1000) == 1000 * 76 + 64 * 32
+    # per node and sink entry: {r, x} FP32 (8 B), flags (4 B), {r, x} FP64 (16 B),
+    # heat-kernel levels (8 B); per node: six 4-byte lists and the 16-byte frontier entry
+    assert lib.flsg_batch_slot_bytes(1000) == (1000 + 64) * 36 + 1000 * 40

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 if not gpu_available():
     pytest.skip("no GPU", allow_module_level=True)
 
-from flsg import _abi, graph, serving  # noqa: E402
+from flsg import _abi, engine, graph, serving  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 
@@ -48,6 +48,10 @@ def test_serving_mix_certified_and_same_alone(small_graph, srv_small):
     st = srv_small.run(reqs, collect=True)
     assert st["ok"] == len(reqs) and st["certified"]
     assert st["launches"] >= 2
+    # every launch (PPR, Katz and heat-kernel queries mixed in the same slots)
+    # left every slot array in its between-queries state
+    srv_small.torch.cuda.synchronize()
+    assert engine.arena_dirty(srv_small.dg) == 0
     for r in reqs:
         x = _dense(r, g.n)
         if r.kind == _abi.FLSG_PPR:
