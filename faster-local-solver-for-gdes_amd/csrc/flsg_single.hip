@@ -31,6 +31,8 @@
 
 #include <string.h>
 
+#include <chrono>
+#include <mutex>
 #include <vector>
 
 #include "flsg_internal.h"
@@ -1094,3 +1096,157 @@ extern "C" int flsg_hk_solve(flsg_graph g, const flsg_params *p, int64_t seed, d
     h_res->status = rc;
     return rc;
 }
+
+// ------------------------------------------------------------------ allocation-free local push
+// The local GD / Katz GD push of flsg_solve for callers that cannot allocate
+// device memory: the same kernels, run directly on the caller's x / r (no
+// captured graph, no handle workspace), with the frontier lists, push values,
+// hub pieces, next-frontier sub-lists and counters in `scratch`, a file-scope
+// device array of a fixed size (graphs of up to kNoAllocMaxN nodes and
+// kNoAllocMaxNnz column entries).  Each GPU holds its own copy until the
+// process ends; its address is looked up once per device.  Every call on a
+// GPU shares it, so calls that overlap on one GPU race.
+namespace flsg {
+
+constexpr int64_t kNoAllocMaxN = (int64_t)1 << 20;
+constexpr int64_t kNoAllocMaxNnz = (int64_t)1 << 24;
+constexpr int64_t kNoAllocPieces = kNoAllocMaxNnz / 1024 + kNoAllocMaxN + 16;
+constexpr int64_t kNoAllocFsubCap = (kNoAllocMaxN + kFrontSub - 1) / kFrontSub;
+
+__host__ __device__ constexpr int64_t na_align(int64_t b) { return (b + 255) / 256 * 256; }
+constexpr int64_t kNaFront0 = 0;
+constexpr int64_t kNaFront1 = kNaFront0 + na_align(4 * (kNoAllocMaxN + 1));
+constexpr int64_t kNaVal = kNaFront1 + na_align(4 * (kNoAllocMaxN + 1));
+constexpr int64_t kNaPieces = kNaVal + na_align(8 * (kNoAllocMaxN + 1));
+constexpr int64_t kNaFsub = kNaPieces + na_align(8 * kNoAllocPieces);
+constexpr int64_t kNaFsubCnt = kNaFsub + na_align(4 * kFrontSub * kNoAllocFsubCap);
+constexpr int64_t kNaCtr = kNaFsubCnt + na_align(4 * kFrontCntWords);
+constexpr int64_t kNoAllocBytes = kNaCtr + na_align((int64_t)sizeof(SolveCounters));
+
+}  // namespace flsg
+
+static __device__ __align__(256) unsigned char scratch[flsg::kNoAllocBytes];
+
+namespace flsg {
+
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[64] = {};   // the array's device address, per device
+
+static int scratch_ptr(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= 64) return fail(FLSG_EARG, "device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        FLSG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(FLSG_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FLSG_OK;
+}
+
+__global__ void k_fill_zero2(double *a, double *b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        a[i] = 0.0;
+        b[i] = 0.0;
+    }
+}
+
+template <int KIND>
+static int noalloc_push(flsg_graph g, PushArgs a, int64_t max_iters, hipStream_t st, SolveCounters *h,
+                        int64_t *iters_out) {
+    const int grid = grid_for(g, 8);
+    for (;;) {
+        for (int b = 0; b < kIterBlock; b++) {
+            const int p = b & 1;
+            hipLaunchKernelGGL(k_node<KIND>, dim3(grid), dim3(kNT), 0, st, a, p);
+            hipLaunchKernelGGL(k_push<KIND>, dim3(grid), dim3(kNT), 0, st, a, p);
+            hipLaunchKernelGGL(k_front_compact, dim3(kFrontSub), dim3(kFrontSub), 0, st, a, p);
+        }
+        FLSG_HIP(hipGetLastError());
+        FLSG_HIP(hipMemcpyAsync(h, a.ctr, sizeof(SolveCounters), hipMemcpyDeviceToHost, st));
+        FLSG_HIP(hipStreamSynchronize(st));
+        if (h->cnt[0] == 0) break;
+        if ((int64_t)h->iters >= max_iters) {
+            *iters_out = (int64_t)h->iters;
+            return fail(FLSG_ENOCONV, "iteration cap reached");
+        }
+    }
+    if (h->overflow & 2u) return fail(FLSG_ECAP, "hub piece list overflow");
+    if (h->overflow & 4u) return fail(FLSG_ECAP, "next-frontier sub-list overflow");
+    *iters_out = (int64_t)h->iters;
+    return FLSG_OK;
+}
+
+}  // namespace flsg
+
+extern "C" int flsg_solve_local_noalloc(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, double *d_r,
+                                        flsg_result *h_res, void *stream) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (!p || !d_x || !d_r || !h_res) return fail(FLSG_EARG, "NULL argument");
+    if ((p->kind != FLSG_PPR && p->kind != FLSG_KATZ) || p->method != FLSG_GD || !p->local ||
+        p->precision != FLSG_FP64)
+        return fail(FLSG_EARG, "allocation-free solve: local gradient descent, PPR or Katz, FP64");
+    if (g->n > kNoAllocMaxN || g->nnz_cap > kNoAllocMaxNnz)
+        return fail(FLSG_EARG, "allocation-free solve: the graph exceeds its fixed scratch (at most " +
+                                   std::to_string(kNoAllocMaxN) + " nodes and " + std::to_string(kNoAllocMaxNnz) +
+                                   " column entries)");
+    if (seed < 0 || seed >= g->n) return fail(FLSG_EARG, "seed out of range");
+    int dev = -1;
+    FLSG_HIP(hipGetDevice(&dev));
+    if (dev != g->device) return fail(FLSG_EARG, "allocation-free solve: make the graph's device current first");
+    unsigned char *base = nullptr;
+    rc = scratch_ptr(g->device, &base);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    int32_t deg_s = 0;
+    FLSG_HIP(hipMemcpyAsync(&deg_s, g->degree + seed, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FLSG_HIP(hipStreamSynchronize(st));
+    if (deg_s <= 0) return fail(FLSG_EARG, "seed node has degree 0 (the reference divides by zero)");
+    if (p->kind == FLSG_PPR && !(p->eps * (double)deg_s <= 1.))
+        return fail(FLSG_EASSERT, "assert eps * degree[s_node] <= 1. failed");   // ppr_solver.py:213
+    const int64_t n = g->n;
+    PushArgs a{};
+    a.indptr = g->indptr;
+    a.indices = g->indices;
+    a.degree = g->degree;
+    a.front0 = reinterpret_cast<int32_t *>(base + kNaFront0);
+    a.front1 = reinterpret_cast<int32_t *>(base + kNaFront1);
+    a.val = reinterpret_cast<double *>(base + kNaVal);
+    a.pieces = reinterpret_cast<int64_t *>(base + kNaPieces);
+    a.pieces_cap = g->nnz_cap / 1024 + n + 16;
+    a.ctr = reinterpret_cast<SolveCounters *>(base + kNaCtr);
+    a.fsub = reinterpret_cast<int32_t *>(base + kNaFsub);
+    a.fsub_cap = (n + kFrontSub - 1) / kFrontSub;
+    a.fsubcnt = reinterpret_cast<unsigned int *>(base + kNaFsubCnt);
+    a.x = d_x;
+    a.r = d_r;
+    a.r_push = d_r;
+    a.alpha = p->alpha;
+    a.om_alpha = 1.0 - p->alpha;
+    const bool ppr = p->kind == FLSG_PPR;
+    a.thr_scale = ppr ? p->eps * p->alpha : p->eps;
+    // x and r start at zero; the counters and the seed's frontier entry are set
+    // by k_init_local, and each iteration clears the sub-list counters it fills
+    hipLaunchKernelGGL(k_fill_zero2, dim3(g->num_cus * 4), dim3(256), 0, st, d_x, d_r, n);
+    hipLaunchKernelGGL(k_init_local, dim3(1), dim3(1), 0, st, a.ctr, a.front0, d_r, seed, ppr ? p->alpha : 1.0);
+    FLSG_HIP(hipGetLastError());
+    const int64_t max_iters = p->max_iters > 0 ? p->max_iters : 100000;
+    SolveCounters h{};
+    int64_t iters = 0;
+    *h_res = flsg_result{};
+    rc = ppr ? noalloc_push<K_PPR>(g, a, max_iters, st, &h, &iters) : noalloc_push<K_KATZ>(g, a, max_iters, st, &h, &iters);
+    if (rc == FLSG_OK && !ppr) hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(1), 0, st, d_x, seed, -1.0);   // katz_solver.py:243
+    const hipError_t es = hipStreamSynchronize(st);
+    if (rc == FLSG_OK && es != hipSuccess) rc = fail(FLSG_EHIP, std::string("solve: ") + hipGetErrorString(es));
+    h_res->oper = (int64_t)h.oper;
+    h_res->iters = iters;
+    h_res->nodes = (int64_t)h.trace_off;
+    h_res->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    h_res->status = rc;
+    return rc;
+}
+
+extern "C" int64_t flsg_solve_local_noalloc_scratch_bytes(void) { return flsg::kNoAllocBytes; }

@@ -66,6 +66,9 @@ SIGNATURES = {
     "flsg_hk_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.c_void_p, C.c_void_p,
                                 C.POINTER(Result), C.c_void_p, C.POINTER(C.c_int64), C.POINTER(Trace),
                                 C.c_void_p]),
+    "flsg_solve_local_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.c_void_p, C.c_void_p,
+                                           C.POINTER(Result), C.c_void_p]),
+    "flsg_solve_local_noalloc_scratch_bytes": (C.c_int64, []),
     "flsg_batch_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_void_p, C.c_int64, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Result), C.c_void_p]),
     "flsg_batch_slot_bytes": (C.c_int64, [C.c_int64]),

@@ -149,6 +149,21 @@ int flsg_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, do
 int flsg_hk_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, double *d_r,
                   flsg_result *h_res, double *h_lvl_sums, int64_t *est_n, flsg_trace *trace, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The local GD push of flsg_solve (PPR or Katz, FP64,
+ * params local = 1, method = FLSG_GD), with the same results, run directly on
+ * the caller's d_x / d_r (n doubles each, zeroed here by a fill kernel) and
+ * with every other buffer in a fixed device array the library holds per GPU
+ * until the process ends (flsg_solve_local_noalloc_scratch_bytes).  Calls no
+ * hipMalloc / hipFree.  Graphs beyond 2^20 nodes or 2^24 column entries are
+ * rejected with FLSG_EARG; the graph's device must be current.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU race.  h_res gets
+ * oper, iters, nodes and ms (host wall time). */
+int flsg_solve_local_noalloc(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, double *d_r,
+                             flsg_result *h_res, void *stream);
+int64_t flsg_solve_local_noalloc_scratch_bytes(void);
+
 /* Batched solve: many seeds in one persistent launch (one workgroup owns a
  * query at a time, queries are claimed dynamically).  Replaces the
  * reference's per-seed fan-out solve_a_graph_all / solve_a_graph_gpu
