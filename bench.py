@@ -83,6 +83,15 @@ def l1_against(a, b, n, torch):
             "max_rel_l1": float((d1h / np.maximum(nbh, 1e-300)).max())}
 
 
+def log_phases(what, res):
+    """Per-query phase times of a batched launch (means over its queries)."""
+    log(f"{what}, per query: push {res['ms_push'].mean():.2f} ms, recompute {res['ms_recompute'].mean():.2f}, "
+        f"correct {res['ms_correct'].mean():.2f}, total {res['ms'].mean():.2f} ms (times from the query's start); "
+        f"push phases: node {res['ms_node'].mean():.2f}, pairs {res['ms_pairs'].mean():.2f}, "
+        f"bins {res['ms_bins'].mean():.2f}, direct edges {res['ms_direct'].mean():.2f} ms, "
+        f"{res['bin_iters'].mean():.1f} of {res['iters'].mean():.1f} iterations binned")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,11 +185,7 @@ def main():
     res = last.results
     for k, ms in enumerate(kern):
         log(f"step {k}: kernel {ms:.1f} ms")
-    log(f"last step, per query: push {res['ms_push'].mean():.2f} ms, recompute {res['ms_recompute'].mean():.2f}, "
-        f"correct {res['ms_correct'].mean():.2f}, total {res['ms'].mean():.2f} ms (times from the query's start); "
-        f"push phases: node {res['ms_node'].mean():.2f}, pairs {res['ms_pairs'].mean():.2f}, "
-        f"bins {res['ms_bins'].mean():.2f}, direct edges {res['ms_direct'].mean():.2f} ms, "
-        f"{res['bin_iters'].mean():.1f} of {res['iters'].mean():.1f} iterations binned")
+    log_phases("last step", res)
     certified = bool(np.all((res["status"] == 0) & (res["cert"] < 1.0)))
     ms_step = wall * 1e3 / args.steps
     qps = args.nq * args.steps / wall
@@ -226,6 +231,7 @@ def main():
         b64_qps = args.nq / (time.perf_counter() - tb)
         ok64 = bool(np.all((bo64.results["status"] == 0) & (bo64.results["cert"] < 1.0)))
         log(f"batched FP64: {b64_qps:.1f} q/s (certified {ok64}, kernel {bo64.ms:.1f} ms)")
+        log_phases("batched FP64", bo64.results)
         l1_64 = l1_against(bo, bo64, g.n, torch)
         log(f"batched FP32 vs batched FP64, every query: {l1_64}")
 

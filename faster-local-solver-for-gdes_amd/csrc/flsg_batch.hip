@@ -362,19 +362,35 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// appends of one touched destination: bit 0 -> next[] (counter *s_next), bit 1
-// -> tl[] (*s_tl), bit 2 -> nl[] (*s_nl, entries past nl_cap dropped but
-// counted); every lane of the wave calls it
-__device__ __forceinline__ void append_bits(int bits, int v, int32_t *next, unsigned int *s_next, int32_t *tl,
-                                            unsigned int *s_tl, int32_t *nl, unsigned int *s_nl,
+// appends of U touched destinations per lane: bit 0 of bits[u] -> next[]
+// (counter *s_next), bit 1 -> tl[] (*s_tl), bit 2 -> nl[] (*s_nl, entries
+// past nl_cap dropped but counted); one LDS atomic per list and wave; every
+// lane of the wave calls it
+template <int U>
+__device__ __forceinline__ void append_bits(const int *bits, const int *v, int32_t *next, unsigned int *s_next,
+                                            int32_t *tl, unsigned int *s_tl, int32_t *nl, unsigned int *s_nl,
                                             unsigned int nl_cap) {
-    const long long slot = wave_append((bits & 1) != 0, s_next);
-    if (slot >= 0) next[slot] = v;
-    const long long ts = wave_append((bits & 2) != 0, s_tl);
-    if (ts >= 0) tl[ts] = v;
+    bool pr[U];
+    long long sl[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) pr[u] = (bits[u] & 1) != 0;
+    wave_append_n<U>(pr, s_next, sl);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (sl[u] >= 0) next[sl[u]] = v[u];
+#pragma unroll
+    for (int u = 0; u < U; u++) pr[u] = (bits[u] & 2) != 0;
+    wave_append_n<U>(pr, s_tl, sl);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (sl[u] >= 0) tl[sl[u]] = v[u];
     if (s_nl) {
-        const long long ns = wave_append((bits & 4) != 0, s_nl);
-        if (ns >= 0 && ns < (long long)nl_cap) nl[ns] = v;
+#pragma unroll
+        for (int u = 0; u < U; u++) pr[u] = (bits[u] & 4) != 0;
+        wave_append_n<U>(pr, s_nl, sl);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (sl[u] >= 0 && sl[u] < (long long)nl_cap) nl[sl[u]] = v[u];
     }
 }
 
@@ -473,8 +489,7 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
             int cr[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 7 : 0);
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) append_bits(cr[u], vtx[u], next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+            append_bits<kUnroll>(cr, vtx, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
         }
         if (group) wave_sync();
     }
@@ -694,6 +709,23 @@ struct ResT {
     T v;
     bool lds;
 };
+
+// write pass of the binned push: a hub's add (LDS cache) or a pair's position
+template <typename T>
+struct BinRes {
+    T v;
+    unsigned int p;
+    bool lds;
+};
+
+// A pair's key: the destination's offset in its range (low `shift` bits) and
+// its degree above them, saturated at deg_sat(shift) (a saturated degree is
+// read from the graph in the drain), so the drain needs no degree load.  The
+// saturation value keeps every key below 0xffffffff, the no-pair mark.
+__device__ __forceinline__ unsigned int deg_sat(int shift) { return (1u << (32 - shift)) - 2u; }
+__device__ __forceinline__ unsigned int pair_key(int v, int dv, int shift) {
+    return ((unsigned int)v & ((1u << shift) - 1u)) | (min((unsigned int)dv, deg_sat(shift)) << shift);
+}
 
 // Heat-kernel fields by precision: level residuals (parity 0 / 1) and the
 // estimate.  FP32: both levels in hk2 (as floats), the estimate in rx32.x;
@@ -975,22 +1007,23 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
     __syncthreads();
     {
         const int lane0 = lane;
-        wave_edges<ResT<T>>(
+        wave_edges<BinRes<T>>(
             cnt, pcs, npc, pw, hw, s_work, W, colx, load,
-            [&](int vtx, double val, ResT<T> &res, int stage) {
+            [&](int vtx, double val, BinRes<T> &res, int stage) {
                 if (stage != 0) return;
                 res.lds = vtx < hub_n;
                 res.v = (T)0;
-                if (vtx < hub_n) {
-                    res.v = atomicAdd(&hub[vtx], (T)val);
-                } else if (vtx < n) {
-                    const unsigned int p = atomicAdd(&bins[vtx >> shift], 1u);
-                    pv[p] = vtx;
-                    pval[p] = (T)val;
-                }
+                res.p = 0;
+                if (vtx < hub_n) res.v = atomicAdd(&hub[vtx], (T)val);
+                else if (vtx < n) res.p = atomicAdd(&bins[vtx >> shift], 1u);
             },
-            [&](int vtx, int dv, double val, const ResT<T> &res, bool) -> int {
-                return res.lds ? post(res.v, res.v + (T)val, dv) : 0;
+            [&](int vtx, int dv, double val, const BinRes<T> &res, bool) -> int {
+                if (res.lds) return post(res.v, res.v + (T)val, dv);
+                if (vtx < n) {   // idle lanes aim past n
+                    pv[res.p] = (int32_t)pair_key(vtx, dv, shift);
+                    pval[res.p] = (T)val;
+                }
+                return 0;
             },
             next, s_next, tl, s_tl, (int)n, nl, s_nl, nl_cap);
         (void)lane0;
@@ -1015,8 +1048,61 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
     __syncthreads();
     t_pairs = now_ticks();
     unsigned long long fvol = 0;
-    // 3. range by range; every loop issues the loads of kBinUnroll items per
-    // lane before it uses any (one memory round trip per batch, not per item)
+    // 3. range by range.  Every loop issues the loads of kBinUnroll items per
+    // lane before it uses any (one memory round trip per batch, not per item).
+    const unsigned int rmask = (1u << shift) - 1u;
+    const unsigned int dsat = deg_sat(shift);
+    // one destination's summed add dl (non-zero) into the record of v, with
+    // the next iteration's node phase when it crosses; called by every lane
+    auto apply = [&](const T *dl, const int *v, const int *dv, const T *ro, const T *xo) {
+        int st[kBinUnroll], bits[kBinUnroll];
+        bool fuse[kBinUnroll], first[kBinUnroll];
+        double pv_u[kBinUnroll];
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; u++) {
+            bits[u] = 0;
+            fuse[u] = false;
+            pv_u[u] = 0.0;
+            st[u] = 0;
+            if (!pristine(dl[u])) {
+                const T nw = ro[u] + dl[u];
+                bits[u] = post(ro[u], nw, dv[u]);
+                fuse[u] = (bits[u] & 1) != 0;
+                if (fuse[u]) {
+                    // the next iteration's node phase of v (ppr_solver.py:273-282);
+                    // only a crossing node needs its row start
+                    st[u] = indptr[v[u]];
+                    r[v[u]] = -(T)0;
+                    x[v[u]] = xo[u] + nw;
+                    pv_u[u] = push(nw, dv[u]);
+                    fvol += (unsigned long long)dv[u];
+                } else {
+                    r[v[u]] = nw;
+                }
+            }
+            bits[u] &= 6;
+            first[u] = fuse[u] && xo[u] == (T)0;   // first push of a node: list it in the support
+        }
+        append_bits<kBinUnroll>(bits, v, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+        long long f[kBinUnroll], sk[kBinUnroll];
+        wave_append_n<kBinUnroll>(fuse, s_nf, f);
+        wave_append_n<kBinUnroll>(first, s_sup, sk);
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; u++) {
+            if (f[u] >= 0) {
+                Ent e;
+                e.st = st[u];
+                e.d = dv[u];
+                e.v = pv_u[u];
+                ent[f[u]] = e;
+                add_pieces((unsigned int)f[u], dv[u], pw, hw, pcs_next, s_npc);
+            }
+            if (sk[u] >= 0) {
+                sup[sk[u]] = v[u];
+                sst[sk[u]] = st[u];
+            }
+        }
+    };
     for (int k = 0; k < K; k++) {
         const int64_t lo = (int64_t)k << shift;
         const int c0 = indptr[lo];
@@ -1031,112 +1117,82 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                 for (int u = 0; u < kBinUnroll; u++) {
                     const int p = p0 + u * NT + tid;
                     v[u] = (int)n + lane;
+                    dv[u] = 0;
                     val[u] = -(T)0;
                     if (p < c1) {
-                        v[u] = pv[p];
+                        const unsigned int key = (unsigned int)pv[p];
+                        v[u] = (int)lo + (int)(key & rmask);
+                        dv[u] = (int)(key >> shift);
                         val[u] = pval[p];
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < kBinUnroll; u++) old[u] = atomic_add_wg(&r[v[u]], val[u]);
 #pragma unroll
-                for (int u = 0; u < kBinUnroll; u++) dv[u] = v[u] < n ? degree[v[u]] : 0;
+                for (int u = 0; u < kBinUnroll; u++)
+                    if ((unsigned int)dv[u] == dsat) dv[u] = degree[v[u]];
+                int bits[kBinUnroll];
 #pragma unroll
-                for (int u = 0; u < kBinUnroll; u++) {
-                    const int bits = v[u] < n ? post(old[u], old[u] + val[u], dv[u]) : 0;
-                    append_bits(bits, v[u], next, s_next, tl, s_tl, nl, s_nl, nl_cap);
-                }
+                for (int u = 0; u < kBinUnroll; u++) bits[u] = v[u] < n ? post(old[u], old[u] + val[u], dv[u]) : 0;
+                append_bits<kBinUnroll>(bits, v, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
             }
             continue;
         }
-        // a range's pairs stay in registers from the sum to the drain when
-        // they fit one batch of the workgroup
+        // the range's pairs summed into the slice
         const bool one = c1 - c0 <= NT * kBinUnroll;
-        int v[kBinUnroll];
+        unsigned int key[kBinUnroll];   // pair keys; 0xffffffff: no pair
         for (int p0 = c0; p0 < c1; p0 += NT * kBinUnroll) {
             T val[kBinUnroll];
 #pragma unroll
             for (int u = 0; u < kBinUnroll; u++) {
                 const int p = p0 + u * NT + tid;
-                v[u] = -1;
+                key[u] = ~0u;
                 val[u] = (T)0;
                 if (p < c1) {
-                    v[u] = pv[p];
+                    key[u] = (unsigned int)pv[p];
                     val[u] = pval[p];
                 }
             }
 #pragma unroll
             for (int u = 0; u < kBinUnroll; u++)
-                if (v[u] >= 0) atomicAdd(&slice[v[u] - (int)lo], val[u]);
+                if (key[u] != ~0u) atomicAdd(&slice[key[u] & rmask], val[u]);
         }
         __syncthreads();
+        // drained by the pairs themselves: the first pair of a destination
+        // to swap its slice entry with 0 applies the summed add
         for (int p0 = c0; p0 < c1; p0 += NT * kBinUnroll) {
             if (!one) {
 #pragma unroll
                 for (int u = 0; u < kBinUnroll; u++) {
                     const int p = p0 + u * NT + tid;
-                    v[u] = p < c1 ? pv[p] : -1;
+                    key[u] = p < c1 ? (unsigned int)pv[p] : ~0u;
                 }
             }
-            T dl[kBinUnroll];
+            T dl[kBinUnroll], ro[kBinUnroll], xo[kBinUnroll];
+            int v[kBinUnroll], dv[kBinUnroll];
 #pragma unroll
             for (int u = 0; u < kBinUnroll; u++) {
                 dl[u] = (T)0;
-                if (v[u] >= 0)
-                    dl[u] = __hip_atomic_exchange(&slice[v[u] - (int)lo], (T)0, __ATOMIC_RELAXED,
+                v[u] = 0;
+                dv[u] = 0;
+                if (key[u] != ~0u) {
+                    v[u] = (int)lo + (int)(key[u] & rmask);
+                    dv[u] = (int)(key[u] >> shift);
+                    dl[u] = __hip_atomic_exchange(&slice[key[u] & rmask], (T)0, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             }
-            T ro[kBinUnroll], xo[kBinUnroll];
-            int dv[kBinUnroll], st[kBinUnroll];
 #pragma unroll
             for (int u = 0; u < kBinUnroll; u++) {
                 ro[u] = (T)0;
                 xo[u] = (T)0;
-                dv[u] = 0;
-                st[u] = 0;
                 if (!pristine(dl[u])) {
-                    ro[u] = r[v[u]];   // r and x share the record: one load
+                    ro[u] = r[v[u]];
                     xo[u] = x[v[u]];
-                    dv[u] = degree[v[u]];
-                    st[u] = indptr[v[u]];
+                    if ((unsigned int)dv[u] == dsat) dv[u] = degree[v[u]];
                 }
             }
-#pragma unroll
-            for (int u = 0; u < kBinUnroll; u++) {
-                int bits = 0;
-                bool fuse = false;
-                double pval_u = 0.0;
-                if (!pristine(dl[u])) {
-                    const T nw = ro[u] + dl[u];
-                    bits = post(ro[u], nw, dv[u]);
-                    fuse = (bits & 1) != 0;
-                    if (fuse) {
-                        // the next iteration's node phase of v (ppr_solver.py:273-282)
-                        r[v[u]] = -(T)0;
-                        x[v[u]] = xo[u] + nw;
-                        pval_u = push(nw, dv[u]);
-                        fvol += (unsigned long long)dv[u];
-                    } else {
-                        r[v[u]] = nw;
-                    }
-                }
-                append_bits(bits & 6, v[u], next, s_next, tl, s_tl, nl, s_nl, nl_cap);
-                const long long f = wave_append(fuse, s_nf);
-                if (f >= 0) {
-                    Ent e;
-                    e.st = st[u];
-                    e.d = dv[u];
-                    e.v = pval_u;
-                    ent[f] = e;
-                    add_pieces((unsigned int)f, dv[u], pw, hw, pcs_next, s_npc);
-                }
-                // first push of a node: list it in the support
-                const long long sk = wave_append(fuse && xo[u] == (T)0, s_sup);
-                if (sk >= 0) {
-                    sup[sk] = v[u];
-                    sst[sk] = st[u];
-                }
-            }
+            apply(dl, v, dv, ro, xo);
         }
         __syncthreads();
     }
@@ -1298,12 +1354,16 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
                 add_pieces(i, dd[j], pw, hw, pcs, s_npc);
             }
             // first push of a node: list it in the support (one LDS atomic per wave)
+            bool fp[kUnroll];
+            long long k[kUnroll];
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) fp[j] = uu[j] >= 0 && xx[j] == (T)0;
+            wave_append_n<kUnroll>(fp, s_sup, k);
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) {
-                const long long k = wave_append(uu[j] >= 0 && xx[j] == (T)0, s_sup);
-                if (k >= 0) {
-                    sup[k] = uu[j];
-                    sst[k] = ss[j];
+                if (k[j] >= 0) {
+                    sup[k[j]] = uu[j];
+                    sst[k[j]] = ss[j];
                 }
             }
         }

@@ -147,9 +147,37 @@ __device__ __forceinline__ long long wave_append(bool pred, unsigned int *counte
     int leader = __ffsll((long long)mask) - 1;
     unsigned int base = 0;
     if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
-    base = __shfl(base, leader);
+    base = (unsigned int)__builtin_amdgcn_readlane((int)base, leader);   // no LDS round trip
     if (!pred) return -1;
     return (long long)base + __popcll(mask & lanemask_lt());
+}
+
+// The same for U predicates per lane with one atomic: slot[u] for the lanes
+// with pred[u] (slots of item u follow those of items < u), -1 otherwise.
+template <int U>
+__device__ __forceinline__ void wave_append_n(const bool *pred, unsigned int *counter, long long *slot) {
+    unsigned long long m[U];
+    unsigned int tot = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        m[u] = __ballot(pred[u]);
+        tot += (unsigned int)__popcll(m[u]);
+    }
+    if (tot == 0) {
+#pragma unroll
+        for (int u = 0; u < U; u++) slot[u] = -1;
+        return;
+    }
+    const int leader = __ffsll((long long)__ballot(1)) - 1;
+    unsigned int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, tot);
+    base = (unsigned int)__builtin_amdgcn_readlane((int)base, leader);
+    const unsigned long long lt = lanemask_lt();
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        slot[u] = pred[u] ? (long long)base + __popcll(m[u] & lt) : -1;
+        base += (unsigned int)__popcll(m[u]);
+    }
 }
 
 template <typename T>

@@ -200,16 +200,17 @@ int flsg_batch_set_block(flsg_graph g, int32_t block);
  * of two that fits; binning needs 2^s >= 2048); in the certificate, x32 of
  * the first ids.  -1 = default for the workgroup size (128 KiB at 1024,
  * 48 KiB at 512, 24 KiB at 256), 0 = off, else a multiple of 8 up to the
- * device's LDS per workgroup less the kernel's static LDS (~147,000 bytes on
+ * device's LDS per workgroup less the kernel's static LDS (~145,000 bytes on
  * gfx950).  No counterpart in the reference (tuning knob). */
 int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes);
 
 /* Binned push of the batched kernel: an iteration of at least min_edges
  * edges (0 = never) writes its pushes as (destination, value) pairs into one
- * bin per destination range and sums each bin in LDS before one coalesced
- * add into the residuals; a bin with fewer than dense_pairs pairs is applied
- * with memory-side atomics instead.  Defaults 16384 and 256.  No counterpart
- * in the reference (tuning knob, set by the autotuner). */
+ * bin per destination range and sums each bin in LDS before one add per
+ * destination into its residual (in id order over the range when the bin is
+ * dense); a bin with fewer than dense_pairs pairs is applied with memory-side
+ * atomics instead.  Defaults 16384 and 256.  No counterpart in the reference
+ * (tuning knob, set by the autotuner). */
 int flsg_batch_set_binning(flsg_graph g, int64_t min_edges, int32_t dense_pairs);
 
 /* Degree bins of the batched edge phases: a list entry with more than

@@ -383,13 +383,15 @@ def test_batch_fp64_parity_small(small_graph, dev_small, kind):
 
 
 @pytest.mark.parametrize("hub_bytes,bin_min,dense_min", [(16384, 1, 0), (16384, 1, 1 << 30), (24576, 2000, 64),
-                                                       (-1, 0, 256)])
+                                                       (-1, 1, 0), (-1, 0, 256)])
 def test_batch_binned_push_small(small_graph, hub_bytes, bin_min, dense_min):
     """The binned push (pairs per destination range summed in LDS) keeps the
     FP64 frontier semantics: with a 16 KiB slice the 10,000-node graph has 5
-    ranges (FP64) / 3 (FP32), every range summed in LDS (dense_min 0) or
-    every one applied with memory-side atomics (dense_min 2^30), or a mix;
-    bin_min 0 pushes every iteration directly.  FP64: operation counts equal
+    ranges (FP64) / 3 (FP32), every range summed in LDS and drained by its
+    pairs or swept in id order (dense_min 0), or every one applied with
+    memory-side atomics (dense_min 2^30), or a mix; with the default 128 KiB
+    slice the graph is one range, swept; bin_min 0 pushes every iteration
+    directly.  FP64: operation counts equal
     to the oracle's and x within relative L1 1e-12; FP32: certified, and the
     arena is clean afterwards."""
     g = small_graph
