@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--plain-queries", type=int, default=32)
     ap.add_argument("--cpu-sample", type=int, default=0, help="seeds timed on the CPU (0 = 2 per core)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-l1", action="store_true", help="skip the L1 of every result against the FP64 oracle")
     ap.add_argument("--fp64-batched", type=int, default=1, help="also time the batched FP64 push (1) or not (0)")
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--block", type=int, default=0, help="batched workgroup size (256/512/1024; 0 = default)")
@@ -255,7 +256,9 @@ def main():
     l1 = None
     if not args.no_cpu:
         from oracle import oracle as O
-        cores = min(os.cpu_count() or 1, 16)
+        # every core this process may use (the environment's OpenMP thread
+        # count where it sets one: the host's share for this GPU)
+        cores = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
         sample = args.cpu_sample or 2 * cores
         sample = min(sample, args.nq)
         tc = time.perf_counter()
@@ -264,16 +267,24 @@ def main():
         cpu = {"value": sample / wall_c, "unit": "queries/s", "cores": cores, "kind": "port",
                "sample": f"first {sample} of the {args.nq} seeds, PPR local GD FP64 (C restatement, OpenMP over "
                          f"seeds), median {float(np.median(times)):.3f} s/query single-thread",
-               "host": {"nproc": os.cpu_count(), "model": _cpu_model()}}
+               "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": _cpu_model()}}
         log(f"cpu: {cpu['value']:.2f} q/s on {cores} threads")
-        # L1 error of batched FP32 results against the FP64 oracle (a few seeds)
-        errs = []
-        for q in range(min(4, args.nq)):
-            o = O.ppr_gd(g, int(seeds[q]), args.alpha, args.eps, local=True)
-            xb = last.dense(q, g.n)
-            errs.append((float(np.abs(xb - o["x"]).sum()), float(np.abs(o["x"]).sum())))
-        l1 = {"seeds": len(errs), "max_l1": max(e for e, _ in errs),
-              "max_rel_l1": max(e / t for e, t in errs), "mean_l1": float(np.mean([e for e, _ in errs]))}
+        # L1 error of every batched FP32 result against the FP64 oracle (local
+        # GD of the same seed, OpenMP over queries), and the oracle's op counts
+        if not args.no_l1:
+            tl = time.perf_counter()
+            res_o = last.offsets.astype(np.int64)
+            res_k = res["supp"].astype(np.int64)
+            used = int((res_o + res_k).max())
+            h_ids = last.ids[:used].cpu().numpy()
+            h_vals = last.vals[:used].cpu().numpy()
+            l1s, norms, o_opers = O.ppr_batch_l1(g, seeds, args.alpha, args.eps, res_o, res_k, h_ids, h_vals,
+                                                 nthreads=cores)
+            rel = l1s / np.maximum(norms, 1e-300)
+            l1 = {"queries": int(len(seeds)), "max_l1": float(l1s.max()), "mean_l1": float(l1s.mean()),
+                  "max_rel_l1": float(rel.max()), "mean_rel_l1": float(rel.mean()),
+                  "oper_equal_frac": float(np.mean(o_opers == res["oper"])),
+                  "threads": cores, "seconds": time.perf_counter() - tl}
         log(f"batched FP32 vs FP64 oracle: {l1}")
 
     line = {

@@ -214,8 +214,8 @@ struct BatchArena {
     int64_t pcs_stride = 0;
     int32_t *tl = nullptr;       // slots * n: entries touched by the current query
     int32_t *nl = nullptr;       // slots * n: FP32 push: boundary candidates for an exact recompute
-    int32_t *pv = nullptr;       // slots * pcap: binned push, pair destinations
-    double *pval = nullptr;      // slots * pcap: pair values (FP32 pairs use the first pcap floats)
+    uint4 *pairs = nullptr;      // slots * pcap 16-byte entries: binned push pairs (FP32 pairs
+                                 // are 8 bytes and use the first half of a slot's entries)
     int64_t pcap = 0;            // pairs per slot (0: no binned push)
     uint64_t *ctl = nullptr;     // launch control for the synchronous entry point
     flsg_result *d_res = nullptr;
@@ -249,8 +249,7 @@ struct BatchArgs {
     int64_t pcs_stride;
     int32_t *tl;
     int32_t *nl;
-    int32_t *pv;
-    double *pval;
+    uint4 *pairs;
     int64_t pcap;
     int64_t bin_min;             // binned push for iterations of at least this many edges
     int32_t dense_min;           // bins with fewer pairs use memory-side atomics
@@ -718,6 +717,23 @@ struct BinRes {
     bool lds;
 };
 
+// a binned push pair: key (below) and value, written and read as one access
+template <typename T>
+struct Pair;
+template <>
+struct alignas(8) Pair<float> {
+    unsigned int key;
+    float val;
+};
+template <>
+struct alignas(16) Pair<double> {
+    unsigned int key;
+    unsigned int pad;
+    double val;
+};
+__device__ __forceinline__ Pair<float> make_pair(unsigned int k, float v) { return Pair<float>{k, v}; }
+__device__ __forceinline__ Pair<double> make_pair(unsigned int k, double v) { return Pair<double>{k, 0u, v}; }
+
 // A pair's key: the destination's offset in its range (low `shift` bits) and
 // its degree above them, saturated at deg_sat(shift) (a saturated degree is
 // read from the graph in the drain), so the drain needs no degree load.  The
@@ -966,7 +982,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
 //     cache, whose result decides the crossing at once (-> next[], the nodes
 //     whose node phase the next iteration runs); any other push becomes a
 //     (v, value) pair in the region of its destination range k (ids
-//     [k 2^shift, (k+1) 2^shift)): pv/pval[indptr[k 2^shift] ...].  An
+//     [k 2^shift, (k+1) 2^shift)): pairs[indptr[k 2^shift] ...].  An
 //     iteration pushes each column entry at most once (the frontier is a set)
 //     and, on an undirected graph, the entries pointing into a range number
 //     its row volume, so every range has room with no counting pass.  The
@@ -991,7 +1007,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                                              unsigned int *s_work, WaveLds &W, const int2 *colx,
                                              const int32_t *indptr, const int32_t *degree, Load load, int shift,
                                              int K, int64_t n, T *hub, int hub_n, void *lds, unsigned int *bins,
-                                             int32_t *pv, T *pval, Strided<T> r, Strided<T> x, int dense_min,
+                                             Pair<T> *pairs, Strided<T> r, Strided<T> x, int dense_min,
                                              Post post, Push push,
                                              Ent *ent, uint32_t *pcs_next, unsigned int *s_npc, unsigned int *s_nf,
                                              unsigned long long *s_vol, unsigned long long *s_fvol, int32_t *sup,
@@ -1020,8 +1036,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
             [&](int vtx, int dv, double val, const BinRes<T> &res, bool) -> int {
                 if (res.lds) return post(res.v, res.v + (T)val, dv);
                 if (vtx < n) {   // idle lanes aim past n
-                    pv[res.p] = (int32_t)pair_key(vtx, dv, shift);
-                    pval[res.p] = (T)val;
+                    pairs[res.p] = make_pair(pair_key(vtx, dv, shift), (T)val);   // one store
                 }
                 return 0;
             },
@@ -1120,10 +1135,10 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                     dv[u] = 0;
                     val[u] = -(T)0;
                     if (p < c1) {
-                        const unsigned int key = (unsigned int)pv[p];
-                        v[u] = (int)lo + (int)(key & rmask);
-                        dv[u] = (int)(key >> shift);
-                        val[u] = pval[p];
+                        const Pair<T> q = pairs[p];
+                        v[u] = (int)lo + (int)(q.key & rmask);
+                        dv[u] = (int)(q.key >> shift);
+                        val[u] = q.val;
                     }
                 }
 #pragma unroll
@@ -1149,8 +1164,9 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                 key[u] = ~0u;
                 val[u] = (T)0;
                 if (p < c1) {
-                    key[u] = (unsigned int)pv[p];
-                    val[u] = pval[p];
+                    const Pair<T> q = pairs[p];
+                    key[u] = q.key;
+                    val[u] = q.val;
                 }
             }
 #pragma unroll
@@ -1165,7 +1181,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
 #pragma unroll
                 for (int u = 0; u < kBinUnroll; u++) {
                     const int p = p0 + u * NT + tid;
-                    key[u] = p < c1 ? (unsigned int)pv[p] : ~0u;
+                    key[u] = p < c1 ? pairs[p].key : ~0u;
                 }
             }
             T dl[kBinUnroll], ro[kBinUnroll], xo[kBinUnroll];
@@ -1212,7 +1228,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
 template <int NT, typename T>
 __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, int s, T *rb, int32_t *qa,
                                           int32_t *qb, Ent *ent, int32_t *sup, int32_t *sst, uint32_t *pcs,
-                                          int32_t *tl, int32_t *nl, int32_t *pv, T *pval, WaveLds &W,
+                                          int32_t *tl, int32_t *nl, Pair<T> *pairs, WaveLds &W,
                                           double *s_hub, unsigned int *bins, unsigned int *s_cnt,
                                           unsigned int *s_ncnt, unsigned int *s_sup, unsigned int *s_npc,
                                           unsigned int *s_work, unsigned int *s_tl, unsigned int *s_nl,
@@ -1384,7 +1400,7 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
         if (binned) {
             uint64_t tp = t0;
             binned_edges<NT, T>(cnt, pcs, npc, pw, hw, s_work, W, a.colx, a.indptr, a.degree, load_ent, shift, K,
-                                a.n, hub, hub_n, s_hub, bins, pv, pval, r, x, a.dense_min, post, push, ent, pcs,
+                                a.n, hub, hub_n, s_hub, bins, pairs, r, x, a.dense_min, post, push, ent, pcs,
                                 s_npc, s_nf, s_vol, s_fvol, sup, sst, s_sup, qn, s_ncnt, tl, s_tl, nlp, s_nlp,
                                 (unsigned int)a.n, tp);
             hub_on = false;
@@ -1472,8 +1488,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     uint32_t *pcs = a.pcs + slot * a.pcs_stride;
     int32_t *tl = a.tl + off;
     int32_t *nl = a.nl + off;
-    int32_t *pv = a.pv ? a.pv + slot * a.pcap : nullptr;
-    double *pval = a.pval ? a.pval + slot * a.pcap : nullptr;
+    uint4 *pairs = a.pairs ? a.pairs + slot * a.pcap : nullptr;
     const int32_t *indptr = a.indptr;
     const int32_t *degree = a.degree;
     const int2 *colx = a.colx;
@@ -1530,12 +1545,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         // ---------------- 1. local push (ppr_solver.py:246-288, katz_solver.py:183-224)
         if (P.fp64)
             status = local_push<NT, double>(a, P, s, reinterpret_cast<double *>(S.rx64), qa, qb, ent, sup, sst, pcs,
-                                            tl, nl, pv, pval, W, s_hub, s_bins, &s_cnt, &s_ncnt, &s_sup, &s_npc,
+                                            tl, nl, reinterpret_cast<Pair<double> *>(pairs), W, s_hub, s_bins, &s_cnt, &s_ncnt, &s_sup, &s_npc,
                                             &s_work, &s_tl, &s_nl, &s_nf, &s_vol, &s_ivol, &s_fvol, sink, &iters,
                                             &fnodes, &qc, tick);
         else
             status = local_push<NT, float>(a, P, s, reinterpret_cast<float *>(S.rx32), qa, qb, ent, sup, sst, pcs,
-                                           tl, nl, pv, reinterpret_cast<float *>(pval), W, s_hub, s_bins, &s_cnt,
+                                           tl, nl, reinterpret_cast<Pair<float> *>(pairs), W, s_hub, s_bins, &s_cnt,
                                            &s_ncnt, &s_sup, &s_npc, &s_work, &s_tl, &s_nl, &s_nf, &s_vol, &s_ivol,
                                            &s_fvol, sink, &iters, &fnodes, &qc, tick);
         int32_t *qn = qc == qa ? qb : qa;
@@ -2051,8 +2066,7 @@ void batch_free(flsg_graph g) {
     (void)hipFree(A->pcs);
     (void)hipFree(A->tl);
     (void)hipFree(A->nl);
-    (void)hipFree(A->pv);
-    (void)hipFree(A->pval);
+    (void)hipFree(A->pairs);
     (void)hipFree(A->ctl);
     (void)hipFree(A->d_res);
     (void)hipFree(A->d_q);
@@ -2100,8 +2114,7 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     FLSG_HIP(hipMalloc((void **)&A->nl, sizeof(int32_t) * ns));
     // binned push: one iteration's pairs (at most one per column entry)
     A->pcap = g->nnz_cap > 0 ? g->nnz_cap : 1;
-    FLSG_HIP(hipMalloc((void **)&A->pv, sizeof(int32_t) * (size_t)A->pcap * (size_t)slots));
-    FLSG_HIP(hipMalloc((void **)&A->pval, sizeof(double) * (size_t)A->pcap * (size_t)slots));
+    FLSG_HIP(hipMalloc((void **)&A->pairs, sizeof(uint4) * (size_t)A->pcap * (size_t)slots));
     FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
     FLSG_HIP(hipMemsetAsync(A->rx32, 0, sizeof(float2) * nr, st));
     FLSG_HIP(hipMemsetAsync(A->fl, 0, sizeof(uint32_t) * nr, st));
@@ -2157,8 +2170,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.pcs_stride = A->pcs_stride;
     a.tl = A->tl;
     a.nl = A->nl;
-    a.pv = A->pv;
-    a.pval = A->pval;
+    a.pairs = A->pairs;
     a.pcap = A->pcap;
     a.bin_min = g->batch_bin_min;
     a.dense_min = g->batch_dense_min;

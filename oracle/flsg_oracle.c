@@ -870,6 +870,142 @@ int ora_ppr_batch(int64_t n, const int64_t *indptr, const int32_t *indices, cons
     return status;
 }
 
+/* ------------------------------------------------------------------------
+ * Checkers of sparse results (test and bench infrastructure, no reference
+ * counterpart).  A result set is n_q sparse vectors: query q holds
+ * ids[off[q] .. off[q] + cnt[q]) and vals[...] (caller's node ids).
+ *
+ * ora_ppr_batch_l1: per query, the FP64 local GD of the seed (ora_ppr_gd)
+ * and the L1 distance of result q to it, sum_v |x_v - y_v|, and the oracle's
+ * sum_v |x_v|; OpenMP over queries.
+ * -------------------------------------------------------------------- */
+int ora_ppr_batch_l1(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree,
+                     const int64_t *seeds, int64_t n_q, double alpha, double eps, int nthreads,
+                     const int64_t *off, const int64_t *cnt, const int32_t *ids, const double *vals,
+                     double *l1_out, double *norm_out, int64_t *opers) {
+    int status = ORA_OK;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        double *x = (double *)malloc(sizeof(double) * n);
+        double *r = (double *)malloc(sizeof(double) * n);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t q = 0; q < n_q; q++) {
+            double t = 0.0;
+            int rc = (!x || !r) ? ORA_ENOMEM
+                                : ora_ppr_gd(n, indptr, indices, degree, seeds[q], alpha, eps, 1, x, r, &opers[q], &t, NULL);
+            if (rc != ORA_OK) {
+#pragma omp critical
+                { if (status == ORA_OK) status = rc; }
+                continue;
+            }
+            double norm = 0.0;
+            for (int64_t v = 0; v < n; v++) norm += fabs(x[v]);
+            /* sum over all v of |x_v - y_v|: start from sum |x_v| and correct
+             * the entries the result holds */
+            double l1 = norm;
+            for (int64_t k = off[q]; k < off[q] + cnt[q]; k++) {
+                const int32_t v = ids[k];
+                l1 += fabs(x[v] - vals[k]) - fabs(x[v]);
+            }
+            l1_out[q] = l1;
+            norm_out[q] = norm;
+        }
+        free(x);
+        free(r);
+    }
+    return status;
+}
+
+/* ora_residual_cert: per query, the residual of result x (a sparse vector)
+ * recomputed in FP64 from the graph alone, for PPR (kind 0, a = alpha):
+ *     r = a e_s - x + (1 - a) A D^-1 x,   thr_v = eps a d_v,
+ * or Katz (kind 1, a = beta; x the full solution, e_s included):
+ *     r = e_s - x + a A x,                   thr_v = eps d_v,
+ * scattered over the live adjacency of each support node; cert_out[q] =
+ * max over touched v of |r_v| / thr_v (nodes of degree 0 count only when
+ * r_v != 0, as +inf) and rl1_out[q] = sum |r_v|.  OpenMP over queries.
+ * -------------------------------------------------------------------- */
+int ora_residual_cert(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *degree, int kind,
+                      double a, double eps, const int64_t *seeds, int64_t n_q, int nthreads, const int64_t *off,
+                      const int64_t *cnt, const int32_t *ids, const double *vals, double *cert_out,
+                      double *rl1_out) {
+    int status = ORA_OK;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        double *rv = (double *)calloc(n, sizeof(double));
+        uint8_t *mark = (uint8_t *)calloc(n, 1);
+        int64_t *list = (int64_t *)malloc(sizeof(int64_t) * n);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t q = 0; q < n_q; q++) {
+            if (!rv || !mark || !list) {
+#pragma omp critical
+                { if (status == ORA_OK) status = ORA_ENOMEM; }
+                continue;
+            }
+            int64_t nt = 0;
+#define ORA_TOUCH(v_) do { if (!mark[v_]) { mark[v_] = 1; list[nt++] = (v_); } } while (0)
+            const int64_t s = seeds[q];
+            if (s < 0 || s >= n) {
+#pragma omp critical
+                { if (status == ORA_OK) status = ORA_EARG; }
+                continue;
+            }
+            ORA_TOUCH(s);
+            for (int64_t k = off[q]; k < off[q] + cnt[q]; k++) {
+                const int64_t u = ids[k];
+                const double xu = vals[k];
+                ORA_TOUCH(u);
+                if (kind == 0) {
+                    rv[u] -= xu;
+                    if (degree[u] > 0) {
+                        const double c = (1.0 - a) * xu / (double)degree[u];
+                        for (int64_t e = indptr[u]; e < indptr[u] + degree[u]; e++) {
+                            const int64_t v = indices[e];
+                            ORA_TOUCH(v);
+                            rv[v] += c;
+                        }
+                    }
+                } else {
+                    rv[u] -= xu;
+                    const double c = a * xu;
+                    for (int64_t e = indptr[u]; e < indptr[u] + degree[u]; e++) {
+                        const int64_t v = indices[e];
+                        ORA_TOUCH(v);
+                        rv[v] += c;
+                    }
+                }
+            }
+            rv[s] += kind == 0 ? a : 1.0;
+#undef ORA_TOUCH
+            const double ts = kind == 0 ? eps * a : eps;
+            double cmax = 0.0, l1 = 0.0;
+            for (int64_t i = 0; i < nt; i++) {
+                const int64_t v = list[i];
+                const double rr = fabs(rv[v]);
+                l1 += rr;
+                if (rr > 0.0) {
+                    const double c = degree[v] > 0 ? rr / (ts * (double)degree[v]) : INFINITY;
+                    if (c > cmax) cmax = c;
+                }
+                rv[v] = 0.0;
+                mark[v] = 0;
+            }
+            cert_out[q] = cmax;
+            rl1_out[q] = l1;
+        }
+        free(rv);
+        free(mark);
+        free(list);
+    }
+    return status;
+}
+
 int ora_max_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -38,6 +38,9 @@ reference function:
                                              dense SOR update in colour order,
                                              certificate vs. ppr_sor's solution
   ignn_update     InstantGNN.py:298-329      pinned: invariant on the new graph
+  ppr_batch_l1, residual_cert (checkers of sparse results, no reference
+                  counterpart)            pinned: numpy dense L1 and dense
+                                             residual b - Qx on a small graph
 The reference itself cannot run here (numba / cupy are not importable:
 an ordinary ImportError, not a denial), so the pins are the mathematics and
 hand-worked cases in tests/test_oracle.py and tests/test_oracle_pins.py;
@@ -98,6 +101,8 @@ def lib():
         L.ora_update_graph.argtypes = [i64, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp]
         L.ora_ignn_update_column.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, vp, vp, vp, i64, vp]
         L.ora_ppr_batch.argtypes = [i64, vp, vp, vp, vp, i64, dbl, dbl, i32, i32, vp, vp]
+        L.ora_ppr_batch_l1.argtypes = [i64, vp, vp, vp, vp, i64, dbl, dbl, i32, vp, vp, vp, vp, vp, vp, vp]
+        L.ora_residual_cert.argtypes = [i64, vp, vp, vp, i32, dbl, dbl, vp, i64, i32, vp, vp, vp, vp, vp, vp]
         L.ora_max_threads.restype = i32
         L.ora_ppr_sor.argtypes = [i64, vp, vp, vp, i64, dbl, dbl, dbl, i32, vp, vp, vp, vp, vp]
         L.ora_colour.argtypes = [i64, vp, vp, vp, vp, vp]
@@ -296,6 +301,51 @@ def ppr_batch(g, seeds, alpha, eps, kind=0, nthreads=0):
     if rc:
         raise RuntimeError(f"oracle error {rc}")
     return opers, times
+
+
+def _sparse_set(off, cnt, ids, vals):
+    return (np.ascontiguousarray(off, dtype=np.int64), np.ascontiguousarray(cnt, dtype=np.int64),
+            np.ascontiguousarray(ids, dtype=np.int32), np.ascontiguousarray(vals, dtype=np.float64))
+
+
+def ppr_batch_l1(g, seeds, alpha, eps, off, cnt, ids, vals, nthreads=0):
+    """Checker of batched results: per query q, the FP64 local GD of seeds[q]
+    (ppr_gd) and the L1 distance sum_v |x_v - y_v| of the sparse result
+    y = (ids, vals)[off[q] : off[q] + cnt[q]] to it; OpenMP over queries.
+    Returns (l1, sum |x|, oper) arrays."""
+    indptr, indices, degree = _csr(g)
+    seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+    off, cnt, ids, vals = _sparse_set(off, cnt, ids, vals)
+    l1 = np.zeros(seeds.size)
+    norm = np.zeros(seeds.size)
+    opers = np.zeros(seeds.size, dtype=np.int64)
+    rc = lib().ora_ppr_batch_l1(g.n, _p(indptr), _p(indices), _p(degree), _p(seeds), seeds.size, float(alpha),
+                                float(eps), int(nthreads), _p(off), _p(cnt), _p(ids), _p(vals), _p(l1), _p(norm),
+                                _p(opers))
+    if rc == ORA_EASSERT:
+        raise AssertionError("eps * degree[s_node] <= 1.")
+    if rc:
+        raise RuntimeError(f"oracle error {rc}")
+    return l1, norm, opers
+
+
+def residual_cert(g, kind, a, eps, seeds, off, cnt, ids, vals, nthreads=0):
+    """Independent FP64 certificate of sparse results: per query the residual
+    r = b - Q x recomputed from the graph (PPR kind 0: a = alpha, r = a e_s -
+    x + (1-a) A D^-1 x, thr = eps a d; Katz kind 1: a = beta, x the full
+    solution, r = e_s - x + a A x, thr = eps d), scattered over the support's
+    live adjacency.  Returns (max_v |r_v| / thr_v, sum_v |r_v|) arrays."""
+    indptr, indices, degree = _csr(g)
+    seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+    off, cnt, ids, vals = _sparse_set(off, cnt, ids, vals)
+    cert = np.zeros(seeds.size)
+    rl1 = np.zeros(seeds.size)
+    rc = lib().ora_residual_cert(g.n, _p(indptr), _p(indices), _p(degree), int(kind), float(a), float(eps),
+                                 _p(seeds), seeds.size, int(nthreads), _p(off), _p(cnt), _p(ids), _p(vals),
+                                 _p(cert), _p(rl1))
+    if rc:
+        raise RuntimeError(f"oracle error {rc}")
+    return cert, rl1
 
 
 def ppr_sor(g, s, alpha, eps, omega, local=True, _fn="ora_ppr_sor"):

@@ -12,6 +12,7 @@ for p in (PKG, REPO):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+    config.addinivalue_line("markers", "slow: full-size parity against the oracle (tens of seconds of host work)")
 
 
 def gpu_available() -> bool:
