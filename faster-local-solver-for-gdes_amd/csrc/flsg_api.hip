@@ -149,6 +149,7 @@ int flsg_graph_destroy(flsg_graph g) {
     (void)hipFree(g->work_a);
     (void)hipFree(g->work_b);
     (void)hipFree(g->work_c);
+    (void)hipFree(g->hk_lvl);
     (void)hipFree(g->local_x);
     (void)hipFree(g->local_r);
     (void)hipFree(g->coef);

@@ -84,6 +84,8 @@ struct flsg_graph_s {
     int64_t coef_cap = 0;
     flsg::SolveCounters *ctr = nullptr;
     flsg::SolveCounters *h_ctr = nullptr;   // pinned mirror
+    double *hk_lvl = nullptr;               // heat-kernel per-level sums (2 per level)
+    int64_t hk_lvl_cap = 0;
     int64_t *trace_dev_off = nullptr;       // device per-iteration offsets
     int64_t trace_dev_off_cap = 0;
     // ordered local Chebyshev workspace (flsg_cheb.hip)

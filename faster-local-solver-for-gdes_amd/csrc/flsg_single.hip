@@ -382,12 +382,38 @@ __global__ __launch_bounds__(kNT) void k_gsweep_node(const int32_t *degree, doub
 // Pull: out[v] = sum_{u in row(v)} val[u], sequential in row order; counts the
 // nodes with thr_v <= |out_v| into act[p^1].  mode 0: r_new = sum;
 // mode 1 (PPR Chebyshev): r_new = r + (sum - dx); mode 2 (Katz Chebyshev):
-// r_new = r + sum - dx.
+// r_new = r + sum - dx; mode 3 (last heat-kernel level, hk_solver.py:270-272):
+// r_new is x, the sum starts from x_v and takes v's own residual r_old[v] at
+// its place in the sorted row (before the entries u >= v), the order in which
+// the reference's natural-order scatter adds into x_v; no count.
 template <int MODE>
 __device__ __forceinline__ double sweep_result(double s, const double *r_old, const double *dx, int64_t v) {
-    if constexpr (MODE == 0) return s;
+    if constexpr (MODE == 0 || MODE == 3) return s;
     else if constexpr (MODE == 1) return r_old[v] + (s - dx[v]);
     else return r_old[v] + s - dx[v];
+}
+
+// first position in the sorted row indices[st, st + d) holding an id >= v
+__device__ __forceinline__ int row_lower_bound(const int32_t *indices, int st, int d, int v) {
+    int lo = 0, hi = d;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (indices[st + mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// term j of a row's sum; mode 3 rows have d + 1 terms, v's own residual at k
+template <int MODE>
+__device__ __forceinline__ double row_term(const int32_t *indices, const double *val, const double *r_old, int st,
+                                           int k, int v, int j) {
+    if constexpr (MODE == 3) {
+        if (j == k) return r_old[v];
+        return val[indices[st + (j < k ? j : j - 1)]];
+    } else {
+        return val[indices[st + j]];
+    }
 }
 
 // rows of at most kRowLight entries: one thread each (longer rows are pulled
@@ -405,11 +431,25 @@ __global__ __launch_bounds__(kNT) void k_gsweep_pull(const int32_t *indptr, cons
         if (d > kRowLight) continue;
         const int st = indptr[v];
         double s = 0.0;
+        if constexpr (MODE == 3) {
+            s = r_new[v];
+            bool own = false;
+            for (int j = 0; j < d; j++) {
+                const int u = indices[st + j];
+                if (!own && u >= v) {
+                    s += r_old[v];
+                    own = true;
+                }
+                s += val[u];
+            }
+            if (!own) s += r_old[v];
+        } else {
 #pragma unroll 8
-        for (int j = 0; j < d; j++) s += val[indices[st + j]];
+            for (int j = 0; j < d; j++) s += val[indices[st + j]];
+        }
         const double rn = sweep_result<MODE>(s, r_old, dx, v);
         r_new[v] = rn;
-        if (d > 0 && thr_scale * (double)d <= fabs(rn)) act++;
+        if (MODE != 3 && d > 0 && thr_scale * (double)d <= fabs(rn)) act++;
     }
     act = wave_sum(act);
     if (__lane_id() == 0) s_cnt[threadIdx.x / kWave] = act;
@@ -468,12 +508,14 @@ __global__ __launch_bounds__(kPB) void k_gsweep_pull_long(const int32_t *indptr,
     unsigned int act = 0;
     for (int64_t item = blockIdx.x; item < n_block; item += gridDim.x) {
         const int v = __builtin_amdgcn_readfirstlane(rows_block[item]);
-        const int d = __builtin_amdgcn_readfirstlane(degree[v]);
+        const int d0 = __builtin_amdgcn_readfirstlane(degree[v]);
         const int st = __builtin_amdgcn_readfirstlane(indptr[v]);
-        double s = 0.0;
+        const int k = MODE == 3 ? __builtin_amdgcn_readfirstlane(row_lower_bound(indices, st, d0, v)) : 0;
+        const int d = d0 + (MODE == 3 ? 1 : 0);   // terms of the sum
+        double s = MODE == 3 ? r_new[v] : 0.0;
         if (wid > 0) {
             const int j = t - kWave;
-            s_ad[0][j] = j < d ? val[indices[st + j]] : -0.0;
+            s_ad[0][j] = j < d ? row_term<MODE>(indices, val, r_old, st, k, v, j) : -0.0;
         }
         __syncthreads();
         int b = 0;
@@ -481,7 +523,7 @@ __global__ __launch_bounds__(kPB) void k_gsweep_pull_long(const int32_t *indptr,
             if (wid > 0) {
                 if (j0 + kPRound < d) {
                     const int j = j0 + kPRound + (t - kWave);
-                    s_ad[b ^ 1][t - kWave] = j < d ? val[indices[st + j]] : -0.0;
+                    s_ad[b ^ 1][t - kWave] = j < d ? row_term<MODE>(indices, val, r_old, st, k, v, j) : -0.0;
                 }
             } else {
                 lds_chain(s_ad[b], d - j0 < kPRound ? d - j0 : kPRound, s);
@@ -492,18 +534,20 @@ __global__ __launch_bounds__(kPB) void k_gsweep_pull_long(const int32_t *indptr,
         if (t == 0) {
             const double rn = sweep_result<MODE>(s, r_old, dx, v);
             r_new[v] = rn;
-            if (thr_scale * (double)d <= fabs(rn)) act++;
+            if (MODE != 3 && thr_scale * (double)d0 <= fabs(rn)) act++;
         }
     }
     const int64_t nw = (int64_t)gridDim.x * (kPB / kWave);
     for (int64_t item = (int64_t)blockIdx.x * (kPB / kWave) + wid; item < n_wave; item += nw) {
         const int v = __builtin_amdgcn_readfirstlane(rows_wave[item]);
-        const int d = __builtin_amdgcn_readfirstlane(degree[v]);
+        const int d0 = __builtin_amdgcn_readfirstlane(degree[v]);
         const int st = __builtin_amdgcn_readfirstlane(indptr[v]);
-        double s = 0.0;
+        const int k = MODE == 3 ? __builtin_amdgcn_readfirstlane(row_lower_bound(indices, st, d0, v)) : 0;
+        const int d = d0 + (MODE == 3 ? 1 : 0);
+        double s = MODE == 3 ? r_new[v] : 0.0;
         for (int j0 = 0; j0 < d; j0 += kWave) {
             const int j = j0 + lane;
-            s_w[wid][lane] = j < d ? val[indices[st + j]] : -0.0;
+            s_w[wid][lane] = j < d ? row_term<MODE>(indices, val, r_old, st, k, v, j) : -0.0;
             __builtin_amdgcn_wave_barrier();
             lds_chain(s_w[wid], d - j0 < kWave ? d - j0 : kWave, s);
             __builtin_amdgcn_wave_barrier();
@@ -511,7 +555,7 @@ __global__ __launch_bounds__(kPB) void k_gsweep_pull_long(const int32_t *indptr,
         if (lane == 0) {
             const double rn = sweep_result<MODE>(s, r_old, dx, v);
             r_new[v] = rn;
-            if (thr_scale * (double)d <= fabs(rn)) act++;
+            if (MODE != 3 && thr_scale * (double)d0 <= fabs(rn)) act++;
         }
     }
     act = wave_sum(act);
@@ -764,20 +808,45 @@ int64_t host_estimate_n(double t, double eps) {
     return est_k;
 }
 
+static int solve_hk_global(flsg_graph g, const flsg_params *p, int64_t s, int64_t N, double *d_x, double *d_r,
+                           flsg_result *res, hipStream_t st);
+
+// per-level diagnostics of the local heat-kernel push: 2N doubles on the handle
+static int ensure_hk_levels(flsg_graph g, int64_t N) {
+    if (g->hk_lvl_cap >= 2 * N) return FLSG_OK;
+    if (g->hk_lvl) FLSG_HIP(hipFree(g->hk_lvl));
+    g->hk_lvl = nullptr;
+    g->hk_lvl_cap = 0;
+    FLSG_HIP(hipMalloc((void **)&g->hk_lvl, sizeof(double) * 2 * N));
+    g->hk_lvl_cap = 2 * N;
+    return FLSG_OK;
+}
+
 static int solve_hk(flsg_graph g, const flsg_params *p, int64_t s, double *d_x, double *d_r,
-                    flsg_result *res, flsg_trace *tr, hipStream_t st, double *h_lvl_sums, int64_t *est_n_out) {
+                    flsg_result *res, flsg_trace *tr, hipStream_t st, double *h_lvl_sums, int64_t lvl_cap,
+                    int64_t *est_n_out) {
     const int64_t n = g->n;
     if (p->tau < 1.0) return fail(FLSG_EARG, "heat kernel needs tau >= 1");
     const int64_t N = p->n_terms > 0 ? p->n_terms : host_estimate_n(p->tau, .5 * p->eps);
+    if (est_n_out) *est_n_out = N;
+    if (h_lvl_sums && lvl_cap < 2 * N)
+        return fail(FLSG_ECAP, "level sums need " + std::to_string(2 * N) + " doubles, the buffer holds " +
+                                   std::to_string(lvl_cap));
+    if (!p->local) {
+        if (tr) return fail(FLSG_EARG, "the global heat-kernel sweep has no frontier trace");
+        if (h_lvl_sums) memset(h_lvl_sums, 0, sizeof(double) * 2 * N);   // the local push's diagnostic
+        return solve_hk_global(g, p, s, N, d_x, d_r, res, st);
+    }
     std::vector<double> psi(N + 1);
     psi[N] = 1.;
     for (int64_t i = N - 1; i >= 0; i--) psi[i] = psi[i + 1] * p->tau / ((double)i + 1.) + 1.;
     FLSG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
     FLSG_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * 2 * n, st));
     // per-level diagnostics
-    double *d_lvl = g->work_c;   // needs 2*N doubles; work_c has n+1
-    if (2 * N > n + 1) d_lvl = nullptr;
-    if (d_lvl) FLSG_HIP(hipMemsetAsync(d_lvl, 0, sizeof(double) * 2 * N, st));
+    int rc0 = ensure_hk_levels(g, N);
+    if (rc0) return rc0;
+    double *d_lvl = g->hk_lvl;
+    FLSG_HIP(hipMemsetAsync(d_lvl, 0, sizeof(double) * 2 * N, st));
     PushArgs a = base_args(g, d_x, d_r);
     a.thr_scale = exp(p->tau) * p->eps;
     a.thr_div1 = (double)N;
@@ -816,10 +885,7 @@ static int solve_hk(flsg_graph g, const flsg_params *p, int64_t s, double *d_x, 
     hipLaunchKernelGGL(k_scale_div, dim3(grid), dim3(kNT), 0, st, d_x, n, exp(p->tau));
     res->oper = (int64_t)g->h_ctr->oper;
     res->iters = (int64_t)g->h_ctr->iters;
-    if (h_lvl_sums && d_lvl) {
-        FLSG_HIP(hipMemcpyAsync(h_lvl_sums, d_lvl, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, st));
-    }
-    if (est_n_out) *est_n_out = N;
+    if (h_lvl_sums) FLSG_HIP(hipMemcpyAsync(h_lvl_sums, d_lvl, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, st));
     return finish_trace(g, tr, res->iters, st);
 }
 
@@ -1012,6 +1078,70 @@ static int solve_global(flsg_graph g, const flsg_params *p, int64_t s, double *d
     return FLSG_OK;
 }
 
+// last heat-kernel level: val_u = r_u / d_u (hk_solver.py:271)
+__global__ __launch_bounds__(kNT) void k_hk_last_val(const int32_t *degree, const double *r, double *val, int64_t n) {
+    for (int64_t u = blockIdx.x * (int64_t)kNT + threadIdx.x; u < n; u += (int64_t)gridDim.x * kNT) {
+        const int d = degree[u];
+        val[u] = d > 0 ? r[u] / (double)d : 0.0;
+    }
+}
+
+// hk_sor_queue(local=False), omega = 1 (hk_solver.py:265-279): one sweep per
+// level.  Level j < N-1: x += r_j, r_{j+1}[v] = sum over the sorted row of v
+// of ((tau/(j+1)) r_j[u]) / d_u (the natural-order scatter's order of adds,
+// as in the GD sweeps); level N-1: x_v gets r_j[u] / d_u of its neighbours
+// and its own r_j[v] in the scatter's order (pull mode 3).  With omega = 1 a
+// sweep leaves the level's residual zero, so the reference's repeat test
+// ends each level after one sweep once its isolated nodes are left out of
+// it (their threshold is 0: the reference never ends on such a graph).
+// oper = N (m + n); the returned two level residuals are zero, as the
+// reference's are on return.
+static int solve_hk_global(flsg_graph g, const flsg_params *p, int64_t s, int64_t N, double *d_x, double *d_r,
+                           flsg_result *res, hipStream_t st) {
+    const int64_t n = g->n;
+    const int grid = grid_for(g, 8);
+    const int lgrid = grid_for(g, 2);
+    int rc = ensure_row_lists(g, st);
+    if (rc) return rc;
+    double *r = g->work_a, *val = g->work_b;
+    FLSG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
+    FLSG_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * 2 * n, st));
+    FLSG_HIP(hipMemsetAsync(r, 0, sizeof(double) * n, st));
+    const double one = 1.0;
+    FLSG_HIP(hipMemcpyAsync(r + s, &one, sizeof(double), hipMemcpyHostToDevice, st));   // hk_solver.py:223
+    hipLaunchKernelGGL(k_init_global, dim3(1), dim3(1), 0, st, g->ctr);
+    // the sweeps run while act[parity] > 0: with threshold 0 it counts every
+    // node of degree > 0
+    hipLaunchKernelGGL(k_count_active, dim3(grid), dim3(kNT), 0, st, g->degree, r, n, 0.0, g->ctr);
+    for (int64_t j = 0; j < N; j++) {
+        const int par = (int)(j & 1);
+        if (j + 1 < N) {
+            const double coef = p->tau / ((double)j + 1.);
+            hipLaunchKernelGGL(k_gsweep_node<K_PPR>, dim3(grid), dim3(kNT), 0, st, g->degree, d_x, r, val, n, 0.0,
+                               coef, g->ctr, par);
+            hipLaunchKernelGGL(k_gsweep_pull<0>, dim3(grid), dim3(kNT), 0, st, g->indptr, g->indices, g->degree, val,
+                               r, (const double *)nullptr, r, n, 0.0, g->ctr, par);
+            hipLaunchKernelGGL(k_gsweep_pull_long<0>, dim3(lgrid), dim3(kPB), 0, st, g->indptr, g->indices, g->degree,
+                               val, r, (const double *)nullptr, r, g->rows_wave, g->n_rows_wave, g->rows_block,
+                               g->n_rows_block, 0.0, g->ctr, par);
+        } else {
+            hipLaunchKernelGGL(k_hk_last_val, dim3(grid), dim3(kNT), 0, st, g->degree, r, val, n);
+            hipLaunchKernelGGL(k_gsweep_pull<3>, dim3(grid), dim3(kNT), 0, st, g->indptr, g->indices, g->degree, val,
+                               r, (const double *)nullptr, d_x, n, 0.0, g->ctr, par);
+            hipLaunchKernelGGL(k_gsweep_pull_long<3>, dim3(lgrid), dim3(kPB), 0, st, g->indptr, g->indices, g->degree,
+                               val, r, (const double *)nullptr, d_x, g->rows_wave, g->n_rows_wave, g->rows_block,
+                               g->n_rows_block, 0.0, g->ctr, par);
+        }
+        if ((j & 63) == 63) FLSG_HIP(hipGetLastError());
+    }
+    FLSG_HIP(hipGetLastError());
+    // x /= e^t  (hk_solver.py:291)
+    hipLaunchKernelGGL(k_scale_div, dim3(grid), dim3(kNT), 0, st, d_x, n, exp(p->tau));
+    res->iters = N;
+    res->oper = N * (g->m_live + n);   // hk_solver.py:275 (oper += m + n per sweep)
+    return FLSG_OK;
+}
+
 int solve_local_cheb(flsg_graph g, const flsg_params *p, int64_t s, double *d_x, double *d_r, flsg_result *res,
                      flsg_trace *tr, hipStream_t st);   // flsg_cheb.hip
 
@@ -1044,7 +1174,7 @@ extern "C" int flsg_solve(flsg_graph g, const flsg_params *p, int64_t seed, doub
     FLSG_HIP(hipEventRecord(e0, st));
     if (q.kind == FLSG_HK) {
         int64_t est_n = 0;
-        rc = solve_hk(g, &q, seed, d_x, d_r, h_res, trace, st, nullptr, &est_n);
+        rc = solve_hk(g, &q, seed, d_x, d_r, h_res, trace, st, nullptr, 0, &est_n);
         h_res->n_corr = est_n;   // HK reports the Taylor order here
     } else if (q.kind == FLSG_PPR || q.kind == FLSG_KATZ) {
         if (!q.local) rc = solve_global(g, &q, seed, d_x, d_r, h_res, st);
@@ -1084,7 +1214,9 @@ extern "C" int flsg_hk_solve(flsg_graph g, const flsg_params *p, int64_t seed, d
     FLSG_HIP(hipEventCreate(&e0));
     FLSG_HIP(hipEventCreate(&e1));
     FLSG_HIP(hipEventRecord(e0, st));
-    rc = solve_hk(g, p, seed, d_x, d_r, h_res, trace, st, h_lvl_sums, est_n);
+    if (h_lvl_sums && !est_n) return fail(FLSG_EARG, "level sums need their capacity in *est_n");
+    const int64_t lvl_cap = h_lvl_sums ? *est_n : 0;
+    rc = solve_hk(g, p, seed, d_x, d_r, h_res, trace, st, h_lvl_sums, lvl_cap, est_n);
     hipError_t es = hipEventRecord(e1, st);
     if (es == hipSuccess) es = hipEventSynchronize(e1);   // also reports a fault of the solve's kernels
     float ms = 0.f;
@@ -1250,3 +1382,7 @@ extern "C" int flsg_solve_local_noalloc(flsg_graph g, const flsg_params *p, int6
 }
 
 extern "C" int64_t flsg_solve_local_noalloc_scratch_bytes(void) { return flsg::kNoAllocBytes; }
+
+extern "C" double flsg_hk_estimate_coefficient(double t, int64_t k) { return flsg::host_estimate_coefficient(t, k); }
+
+extern "C" int64_t flsg_hk_estimate_n(double t, double eps) { return flsg::host_estimate_n(t, eps); }

@@ -66,6 +66,8 @@ SIGNATURES = {
     "flsg_hk_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.c_void_p, C.c_void_p,
                                 C.POINTER(Result), C.c_void_p, C.POINTER(C.c_int64), C.POINTER(Trace),
                                 C.c_void_p]),
+    "flsg_hk_estimate_coefficient": (C.c_double, [C.c_double, C.c_int64]),
+    "flsg_hk_estimate_n": (C.c_int64, [C.c_double, C.c_double]),
     "flsg_solve_local_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.c_void_p, C.c_void_p,
                                            C.POINTER(Result), C.c_void_p]),
     "flsg_solve_local_noalloc_scratch_bytes": (C.c_int64, []),

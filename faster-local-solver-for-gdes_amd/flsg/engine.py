@@ -158,8 +158,12 @@ def solve(dg: DeviceGraph, params: _abi.Params, seed: int, trace: bool = False,
     extra = {}
     seed = int(dg.to_internal([seed])[0])
     if params.kind == _abi.FLSG_HK:
-        est_n = C.c_int64(0)
-        lvl = np.zeros(2 * 100000, dtype=np.float64)
+        # Taylor order as the solve computes it (hk_solver.py:225), for the
+        # per-level sums' buffer; its capacity goes in through est_n
+        n_terms = int(params.n_terms) if params.n_terms > 0 else int(
+            dg.lib.flsg_hk_estimate_n(float(params.tau), 0.5 * float(params.eps)))
+        lvl = np.zeros(2 * max(n_terms, 1), dtype=np.float64)
+        est_n = C.c_int64(lvl.size)
         rc = dg.lib.flsg_hk_solve(dg.handle, C.byref(params), int(seed), _ptr(x), _ptr(r), C.byref(res),
                                   C.c_void_p(lvl.ctypes.data), C.byref(est_n), tr_ptr, _stream_ptr(torch))
         _check(rc)

@@ -135,6 +135,7 @@ int flsg_graph_destroy(flsg_graph g);
  *   KATZ GD        : katz_gd / katz_gd_*_gpu    katz_solver.py:24-248
  *   KATZ CHEB      : katz_cheby                 katz_solver.py:340-431
  *   HK             : hk_sor_queue(omega=1)      hk_solver.py:206-295
+ *                    (local = 0: its natural-order level sweep, :265-279)
  * d_x, d_r: dense FP64 outputs of n entries (HK: d_r holds 2n, the two level
  * residuals as the reference returns them).  trace may be NULL. */
 int flsg_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, double *d_r,
@@ -143,11 +144,23 @@ int flsg_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, do
 /* Heat kernel with the reference's extra outputs: the Taylor order N
  * (hk_solver.py:225) in *est_n, and per level j the sums
  * h_lvl_sums[2j] = sum |r_j,u| pushed, h_lvl_sums[2j+1] = residual mass sent to
- * level j+1 (the reference's beta_t diagnostic, hk_solver.py:250,286-290);
- * h_lvl_sums (host, 2N doubles) may be NULL.  Replaces hk_sor_queue with
- * omega=1, local=True (hk_solver.py:206-295). */
+ * level j+1 (the reference's beta_t diagnostic, hk_solver.py:250,286-290).
+ * h_lvl_sums (host) may be NULL; when it is not, *est_n on entry holds its
+ * capacity in doubles, and a capacity below 2N fails with FLSG_ECAP.
+ * Replaces hk_sor_queue with omega=1 (hk_solver.py:206-295): params local = 1
+ * the queue push (:236-264), local = 0 the level sweep over all nodes
+ * (:265-279; one sweep per level, oper = N (m + n), level sums zero: the
+ * reference's beta_t stays [1.0] there). */
 int flsg_hk_solve(flsg_graph g, const flsg_params *p, int64_t seed, double *d_x, double *d_r,
                   flsg_result *h_res, double *h_lvl_sums, int64_t *est_n, flsg_trace *trace, void *stream);
+
+/* The reference's scalar helpers of the heat kernel, host code:
+ * estimate_coefficient(t, k) = t^(k+1) / (k+1)!  (hk_solver.py:15-25) and the
+ * Taylor order estimate_n(t, eps) (hk_solver.py:28-48; its loop factor uses
+ * the initial k).  Arguments are not checked (the drop-in asserts t >= 1,
+ * eps > 0, k >= 0 as the reference does). */
+double flsg_hk_estimate_coefficient(double t, int64_t k);
+int64_t flsg_hk_estimate_n(double t, double eps);
 
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

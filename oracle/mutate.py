@@ -137,6 +137,14 @@ MUTATIONS = {
            "                        xt[vv] += mass;\n                        continue;")]),
     "hk_eps_half": ("heat kernel estimates N from eps instead of eps/2", [
         _r("ora_estimate_n(tau, .5 * eps);   /* :225 */", "ora_estimate_n(tau, eps);")]),
+    # ---- heat kernel, global level sweep (hk_solver.py:265-279)
+    "hk_global_last_level_factor": ("global heat-kernel sweep: last level with the t/(j+1) factor", [
+        _r("if (j + 1 == est_n) xt[vv] += ru_j / (double)degree[uu];", "if (j + 1 == est_n) xt[vv] += mass;")]),
+    "hk_global_coef_shift": ("global heat-kernel sweep: level coefficient t/(j+2)", [
+        _r("double mass = (tau / ((double)j + 1.)) * ru_j / (double)degree[uu];",
+           "double mass = (tau / ((double)j + 2.)) * ru_j / (double)degree[uu];", 1)]),
+    "hk_global_oper_m": ("global heat-kernel sweep counts m instead of m + n per sweep", [
+        _r("oper += (double)(m + n);", "oper += (double)m;")]),
 }
 
 

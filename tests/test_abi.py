@@ -67,3 +67,20 @@ def test_version_without_gpu():
     # per node and sink entry: {r, x} FP32 (8 B), flags (4 B), {r, x} FP64 (16 B),
     # heat-kernel levels (8 B); per node: six 4-byte lists and the 16-byte frontier entry
     assert lib.flsg_batch_slot_bytes(1000) == (1000 + 64) * 36 + 1000 * 40
+
+
+def test_hk_estimates_match_the_oracle():
+    """flsg_hk_estimate_n / _coefficient (the drop-in's estimate_n /
+    estimate_coefficient) against the oracle's restatement of
+    hk_solver.py:15-48, over t and eps spanning the loop's early and late
+    exits."""
+    import hk_solver
+    from oracle import oracle as O
+    lib = _build.load_lib()
+    for t in (1.0, 1.5, 2.0, 3.7, 5.0, 10.0, 20.0):
+        for eps in (1e-2, 1e-4, 1e-6, 1e-7, 5e-8, 1e-9, 1e-12):
+            assert lib.flsg_hk_estimate_n(t, eps) == O.estimate_n(t, eps), (t, eps)
+            assert hk_solver.estimate_n(t, eps) == O.estimate_n(t, eps)
+        for k in (0, 1, 5, 17):
+            assert lib.flsg_hk_estimate_coefficient(t, k) == O.estimate_coefficient(t, k)
+            assert hk_solver.estimate_coefficient(t, k) == O.estimate_coefficient(t, k)

@@ -104,6 +104,14 @@ def test_hk_dropin_matches_oracle(small_graph):
         o = O.hk_push(g, int(s), 5.0, 1e-6, local=True)
         assert est_n == o["est_n"] and oper == o["oper"]
         assert _rel_l1(xt, o["x"]) <= 1e-12
+    # local=False: the level sweep over all nodes, bit-identical; beta_t
+    # stays [1.0] as the reference leaves it there
+    s = int(graph.uniform_seeds(g, 1, seed=34)[0])
+    est_n, xt, rt, oper, beta_t, run_time, algo_time = hk_solver.hk_sor_queue(n, ip, ind, deg, s, 5.0, 1e-6, 1.0,
+                                                                               False)
+    o = O.hk_push(g, s, 5.0, 1e-6, local=False)
+    assert est_n == o["est_n"] and oper == o["oper"] and beta_t == [1.0]
+    assert np.array_equal(xt, o["x"]) and not rt.any()
 
 
 def _certified(ip, ind, deg, x, p, r, eps, beta):

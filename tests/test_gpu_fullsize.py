@@ -9,6 +9,7 @@ configuration), against the oracle run over seeds on every host core.
     Chebyshev (sequential replay): x and r bit-identical on 10 seeds;
   * the demo harness's alpha round trip (0.1 -> 0.10000000000000002) on one
     seed per method, to the same bars;
+  * the heat kernel's global level sweep (t = 5): x bit-identical, 2 seeds;
   * the batched FP32 launch of the bench's 1024 mixed seeds: every result
     re-certified on the host by an FP64 residual recomputed from the graph
     alone (max |r_v| / (eps alpha d_v) < 1, up to the FP64 rounding of x in
@@ -140,3 +141,19 @@ def test_batch_recertified_large(large_graph, dev_large):
     thr_min = EPS * ALPHA
     assert np.all(cert < 1.0 + slack / thr_min), np.flatnonzero(cert >= 1.0 + slack / thr_min)[:10]
     assert np.all(rl1 <= res["r_l1"] + slack * cnt)
+
+
+def test_hk_global_bit_identical_large(large_graph, dev_large):
+    """hk_sor_queue(local=False) at full size (t = 5, eps = 1e-7): x
+    bit-identical to the oracle's level sweep and oper = N (m + n), 2 seeds."""
+    g = large_graph
+    seeds = graph.uniform_seeds(g, 2, seed=35)
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        outs = list(ex.map(lambda s: O.hk_push(g, int(s), 5.0, EPS, local=False), seeds))
+    p = engine.make_params(_abi.FLSG_HK, _abi.FLSG_GD, False, _abi.FLSG_FP64, eps=EPS, tau=5.0)
+    for s, o in zip(seeds, outs):
+        print(f"seed {int(s)} heat kernel global", flush=True)
+        out = engine.solve(dev_large, p, int(s))
+        assert out.extra["est_n"] == o["est_n"]
+        assert out.result["oper"] == int(o["oper"])
+        assert np.array_equal(out.x.cpu().numpy(), o["x"])

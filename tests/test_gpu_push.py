@@ -199,6 +199,26 @@ def test_hk_push_parity_small(small_graph, dev_small, n_terms):
         assert np.abs(rg - o["r"]).sum() <= REL_L1 * max(np.abs(o["r"]).sum(), 1e-300)
 
 
+@pytest.mark.parametrize("which,n_terms", [("small", 0), ("small", 40), ("hub", 0), ("hub", 1)])
+def test_hk_global_sweep_parity(small_graph, dev_small, hub_graph, which, n_terms):
+    """hk_sor_queue(local=False): one sweep per Taylor level, the pull
+    summing each sorted row in the reference's scatter order, so x is
+    bit-identical to the oracle's; the last level's pull inserts the node's
+    own residual at its place in the row (rows of one thread, a wave and a
+    block on the hub graph; n_terms 1 makes the first level the last)."""
+    g = small_graph if which == "small" else hub_graph
+    dg = dev_small if which == "small" else engine.DeviceGraph(g)
+    tau, eps = 5.0, 1e-7
+    p = engine.make_params(_abi.FLSG_HK, _abi.FLSG_GD, False, _abi.FLSG_FP64, eps=eps, tau=tau, n_terms=n_terms)
+    for s in graph.uniform_seeds(g, 3, seed=15):
+        o = O.hk_push(g, s, tau, eps, local=False, n_terms=n_terms)
+        out = engine.solve(dg, p, int(s))
+        assert out.extra["est_n"] == o["est_n"]
+        assert out.result["oper"] == int(o["oper"])
+        assert np.array_equal(out.x.cpu().numpy(), o["x"]), f"seed {s}: rel L1 {_rel_l1(out.x.cpu().numpy(), o['x'])}"
+        assert not out.r.cpu().numpy().any()
+
+
 def _check_batch(g, seeds, bo, eps, alpha=ALPHA, n_check=None):
     n_check = len(seeds) if n_check is None else n_check
     thr = eps * alpha * np.maximum(g.degree, 1)

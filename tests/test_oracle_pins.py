@@ -259,3 +259,36 @@ def test_hk_psi_closed_form_matches_recurrence():
             psi[i] = psi[i + 1] * t / (i + 1.0) + 1.0
         c = _psi_closed(N, t)
         assert max(abs(a - b) / b for a, b in zip(psi, c)) < 1e-14
+
+
+@pytest.mark.parametrize("seed,t,n_terms,connected", [(21, 5.0, 0, True), (22, 3.0, 7, True), (23, 5.0, 12, False)])
+def test_hk_global_sweep_closed_form(seed, t, n_terms, connected):
+    """The global level sweep (hk_solver.py:265-279, omega = 1) is the
+    truncated Taylor series with the reference's last level:
+        x = e^-t ( sum_{j<N} r_j + P r_{N-1} ),  r_j = (t^j / j!) P^j e_s,
+    P = A D^-1 (no t/N factor on the last level's push), one sweep per level
+    (oper = N (m + n)), and both level residuals zero on return.  On a graph
+    with isolated nodes they are left out of the sweep (the reference never
+    ends there)."""
+    g = random_graph(n=60, p=0.06, seed=seed, connected=connected)
+    d = g.degree.astype(float)
+    s = int(np.flatnonzero(d > 0)[0])
+    eps = 1e-4
+    o = O.hk_push(g, s, t, eps, local=False, n_terms=n_terms)
+    N = o["est_n"]
+    assert N == (n_terms if n_terms else O.estimate_n(t, eps / 2))
+    A = O.adjacency(g).toarray()
+    P = A / np.where(d > 0, d, 1.0)[None, :]
+    r = np.zeros(g.n)
+    r[s] = 1.0
+    x = np.zeros(g.n)
+    for j in range(N):
+        x += r
+        if j + 1 == N:
+            x += P @ r
+        else:
+            r = (t / (j + 1.0)) * (P @ r)
+    x /= math.exp(t)
+    assert np.abs(o["x"] - x).sum() <= 1e-13 * np.abs(x).sum()
+    assert o["oper"] == N * (int(g.degree.sum()) + g.n)
+    assert not np.any(o["r"])
