@@ -209,7 +209,6 @@ struct BatchArena {
     int32_t *q1 = nullptr;
     Ent *ent = nullptr;          // slots * n: frontier entries (heat kernel: two touched lists)
     int32_t *sup = nullptr;      // slots * n
-    int32_t *sst = nullptr;      // slots * n: row start of sup[k] (main push)
     uint32_t *pcs = nullptr;     // slots * pcs_stride: hub pieces of the current list
     int64_t pcs_stride = 0;
     int32_t *tl = nullptr;       // slots * n: entries touched by the current query
@@ -244,7 +243,6 @@ struct BatchArgs {
     int32_t *q1;
     Ent *ent;
     int32_t *sup;
-    int32_t *sst;
     uint32_t *pcs;
     int64_t pcs_stride;
     int32_t *tl;
@@ -1011,7 +1009,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                                              Post post, Push push,
                                              Ent *ent, uint32_t *pcs_next, unsigned int *s_npc, unsigned int *s_nf,
                                              unsigned long long *s_vol, unsigned long long *s_fvol, int32_t *sup,
-                                             int32_t *sst, unsigned int *s_sup, int32_t *next, unsigned int *s_next,
+                                             unsigned int *s_sup, int32_t *next, unsigned int *s_next,
                                              int32_t *tl, unsigned int *s_tl, int32_t *nl, unsigned int *s_nl,
                                              unsigned int nl_cap, uint64_t &t_pairs) {
     const int tid = threadIdx.x;
@@ -1070,7 +1068,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
     // one destination's summed add dl (non-zero) into the record of v, with
     // the next iteration's node phase when it crosses; called by every lane
     auto apply = [&](const T *dl, const int *v, const int *dv, const T *ro, const T *xo) {
-        int st[kBinUnroll], bits[kBinUnroll];
+        int bits[kBinUnroll];
         bool fuse[kBinUnroll], first[kBinUnroll];
         double pv_u[kBinUnroll];
 #pragma unroll
@@ -1078,15 +1076,12 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
             bits[u] = 0;
             fuse[u] = false;
             pv_u[u] = 0.0;
-            st[u] = 0;
             if (!pristine(dl[u])) {
                 const T nw = ro[u] + dl[u];
                 bits[u] = post(ro[u], nw, dv[u]);
                 fuse[u] = (bits[u] & 1) != 0;
                 if (fuse[u]) {
-                    // the next iteration's node phase of v (ppr_solver.py:273-282);
-                    // only a crossing node needs its row start
-                    st[u] = indptr[v[u]];
+                    // the next iteration's node phase of v (ppr_solver.py:273-282)
                     r[v[u]] = -(T)0;
                     x[v[u]] = xo[u] + nw;
                     pv_u[u] = push(nw, dv[u]);
@@ -1106,16 +1101,13 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
         for (int u = 0; u < kBinUnroll; u++) {
             if (f[u] >= 0) {
                 Ent e;
-                e.st = st[u];
+                e.st = -v[u] - 1;   // row start read where the entry is used
                 e.d = dv[u];
                 e.v = pv_u[u];
                 ent[f[u]] = e;
                 add_pieces((unsigned int)f[u], dv[u], pw, hw, pcs_next, s_npc);
             }
-            if (sk[u] >= 0) {
-                sup[sk[u]] = v[u];
-                sst[sk[u]] = st[u];
-            }
+            if (sk[u] >= 0) sup[sk[u]] = v[u];
         }
     };
     for (int k = 0; k < K; k++) {
@@ -1227,7 +1219,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
 // *fnodes and s_vol (operation count) are filled.
 template <int NT, typename T>
 __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, int s, T *rb, int32_t *qa,
-                                          int32_t *qb, Ent *ent, int32_t *sup, int32_t *sst, uint32_t *pcs,
+                                          int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs,
                                           int32_t *tl, int32_t *nl, Pair<T> *pairs, WaveLds &W,
                                           double *s_hub, unsigned int *bins, unsigned int *s_cnt,
                                           unsigned int *s_ncnt, unsigned int *s_sup, unsigned int *s_npc,
@@ -1277,9 +1269,10 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
     unsigned long long fnodes = 0;
     int64_t iters = 0;
     int status = FLSG_OK;
+    // entries a binned iteration prepared hold -(v + 1) for the row start
     auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
         const Ent e = ent[i];
-        st = e.st;
+        st = e.st >= 0 ? e.st : a.indptr[-e.st - 1];
         d = e.d;
         v = e.v;
     };
@@ -1377,10 +1370,7 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
             wave_append_n<kUnroll>(fp, s_sup, k);
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) {
-                if (k[j] >= 0) {
-                    sup[k[j]] = uu[j];
-                    sst[k[j]] = ss[j];
-                }
+                if (k[j] >= 0) sup[k[j]] = uu[j];
             }
         }
         if (vol) {
@@ -1401,7 +1391,7 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
             uint64_t tp = t0;
             binned_edges<NT, T>(cnt, pcs, npc, pw, hw, s_work, W, a.colx, a.indptr, a.degree, load_ent, shift, K,
                                 a.n, hub, hub_n, s_hub, bins, pairs, r, x, a.dense_min, post, push, ent, pcs,
-                                s_npc, s_nf, s_vol, s_fvol, sup, sst, s_sup, qn, s_ncnt, tl, s_tl, nlp, s_nlp,
+                                s_npc, s_nf, s_vol, s_fvol, sup, s_sup, qn, s_ncnt, tl, s_tl, nlp, s_nlp,
                                 (unsigned int)a.n, tp);
             hub_on = false;
             const uint64_t t1 = now_ticks();
@@ -1484,7 +1474,6 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     int32_t *qb = a.q1 + off;
     Ent *ent = a.ent + off;
     int32_t *sup = a.sup + off;
-    int32_t *sst = a.sst + off;
     uint32_t *pcs = a.pcs + slot * a.pcs_stride;
     int32_t *tl = a.tl + off;
     int32_t *nl = a.nl + off;
@@ -1544,12 +1533,12 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         uint64_t tick[5] = {0, 0, 0, 0, 0};   // node, pairs, bins, direct (ticks), binned iterations
         // ---------------- 1. local push (ppr_solver.py:246-288, katz_solver.py:183-224)
         if (P.fp64)
-            status = local_push<NT, double>(a, P, s, reinterpret_cast<double *>(S.rx64), qa, qb, ent, sup, sst, pcs,
+            status = local_push<NT, double>(a, P, s, reinterpret_cast<double *>(S.rx64), qa, qb, ent, sup, pcs,
                                             tl, nl, reinterpret_cast<Pair<double> *>(pairs), W, s_hub, s_bins, &s_cnt, &s_ncnt, &s_sup, &s_npc,
                                             &s_work, &s_tl, &s_nl, &s_nf, &s_vol, &s_ivol, &s_fvol, sink, &iters,
                                             &fnodes, &qc, tick);
         else
-            status = local_push<NT, float>(a, P, s, reinterpret_cast<float *>(S.rx32), qa, qb, ent, sup, sst, pcs,
+            status = local_push<NT, float>(a, P, s, reinterpret_cast<float *>(S.rx32), qa, qb, ent, sup, pcs,
                                            tl, nl, reinterpret_cast<Pair<float> *>(pairs), W, s_hub, s_bins, &s_cnt,
                                            &s_ncnt, &s_sup, &s_npc, &s_work, &s_tl, &s_nl, &s_nf, &s_vol, &s_ivol,
                                            &s_fvol, sink, &iters, &fnodes, &qc, tick);
@@ -2062,7 +2051,6 @@ void batch_free(flsg_graph g) {
     (void)hipFree(A->q1);
     (void)hipFree(A->ent);
     (void)hipFree(A->sup);
-    (void)hipFree(A->sst);
     (void)hipFree(A->pcs);
     (void)hipFree(A->tl);
     (void)hipFree(A->nl);
@@ -2079,12 +2067,12 @@ void batch_free(flsg_graph g) {
 // bytes per slot for n nodes (per-node arrays and lists), without the binned
 // push's pair buffer (12 bytes per column entry of the graph)
 int64_t slot_bytes(int64_t n) {
-    // rx32 {r, x} and fl (8 + 4 B), rx64 {r, x} and hk2 (16 + 8 B) over n + kSink entries; q0 q1 sup sst tl nl (4 B) +
+    // rx32 {r, x} and fl (8 + 4 B), rx64 {r, x} and hk2 (16 + 8 B) over n + kSink entries; q0 q1 sup tl nl (4 B) +
     // frontier entry (16 B)
-    return (n + kSink) * (3 * 4 + 3 * 8) + n * (6 * 4 + 16);
+    return (n + kSink) * (3 * 4 + 3 * 8) + n * (5 * 4 + 16);
 }
 
-static int64_t pair_bytes(flsg_graph g) { return g->nnz_cap * 12; }
+static int64_t pair_bytes(flsg_graph g) { return g->nnz_cap * (int64_t)sizeof(uint4); }
 
 static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     BatchArena *A = g->arena;
@@ -2105,7 +2093,6 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     FLSG_HIP(hipMalloc((void **)&A->q1, sizeof(int32_t) * ns));
     FLSG_HIP(hipMalloc((void **)&A->ent, sizeof(Ent) * ns));
     FLSG_HIP(hipMalloc((void **)&A->sup, sizeof(int32_t) * ns));
-    FLSG_HIP(hipMalloc((void **)&A->sst, sizeof(int32_t) * ns));
     // a list's entries are distinct nodes and every split entry has more than
     // 64 edges per piece, so a list never needs more than n + nnz/64 pieces
     A->pcs_stride = g->n + g->nnz_cap / kWave + 1;
@@ -2165,7 +2152,6 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.q1 = A->q1;
     a.ent = A->ent;
     a.sup = A->sup;
-    a.sst = A->sst;
     a.pcs = A->pcs;
     a.pcs_stride = A->pcs_stride;
     a.tl = A->tl;

@@ -37,7 +37,7 @@ def tune_batched(dg: DeviceGraph, params: _abi.Params, seeds, rounds: int = 2, l
     tot = C.c_int64(0)
     _check(dg.lib.flsg_device_info(C.byref(nc), C.byref(free), C.byref(tot)))
     cus = max(1, int(nc.value))
-    slot_b = int(dg.lib.flsg_batch_slot_bytes(dg.n)) + 12 * int(dg.nnz_cap)   # + the binned push's pairs
+    slot_b = int(dg.lib.flsg_batch_slot_bytes(dg.n)) + 16 * int(dg.nnz_cap)   # + the binned push's pairs
     shapes = [(1024, cus)]
     if 2 * cus * slot_b < free.value // 2:
         shapes.append((512, 2 * cus))

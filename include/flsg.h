@@ -195,7 +195,7 @@ int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_t *d_seeds,
 
 /* Size of the per-query device state a batched launch keeps resident, in
  * bytes per workgroup slot (the unit of memory-footprint admission): the
- * per-node arrays and lists of an n-node graph; the binned push adds 12
+ * per-node arrays and lists of an n-node graph; the binned push adds 16
  * bytes per column entry of the graph to every slot. */
 int64_t flsg_batch_slot_bytes(int64_t n);
 

@@ -65,8 +65,8 @@ def test_version_without_gpu():
     lib = _build.load_lib()
     assert lib.flsg_version() >= 1
     # per node and sink entry: {r, x} FP32 (8 B), flags (4 B), {r, x} FP64 (16 B),
-    # heat-kernel levels (8 B); per node: six 4-byte lists and the 16-byte frontier entry
-    assert lib.flsg_batch_slot_bytes(1000) == (1000 + 64) * 36 + 1000 * 40
+    # heat-kernel levels (8 B); per node: five 4-byte lists and the 16-byte frontier entry
+    assert lib.flsg_batch_slot_bytes(1000) == (1000 + 64) * 36 + 1000 * 36
 
 
 def test_hk_estimates_match_the_oracle():
