@@ -150,8 +150,8 @@ def main():
     seeds = graph.mixed_seeds(g, args.nq, seed=0)
     if args.bin_min is not None or args.dense_min is not None:
         from flsg.engine import _check
-        _check(dg.lib.flsg_batch_set_binning(dg.handle, 16384 if args.bin_min is None else args.bin_min,
-                                             256 if args.dense_min is None else args.dense_min))
+        _check(dg.lib.flsg_batch_set_binning(dg.handle, 65536 if args.bin_min is None else args.bin_min,
+                                             64 if args.dense_min is None else args.dense_min))
     p32 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=args.alpha, eps=args.eps)
     p64 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=args.alpha, eps=args.eps)
 
@@ -205,11 +205,27 @@ def main():
     # only for the workload those runs profiled
     traffic = None
     atomics = None
+    # the launch configuration this run timed, in the form the profile records
+    import ctypes as C
+    ncu = C.c_int32(0)
+    engine._check(dg.lib.flsg_device_info(C.byref(ncu), None, None))
+    if tuned:
+        eff = {k: tuned[k] for k in ("block", "slots", "hub_bytes", "bin_min", "dense_min", "split_edges",
+                                     "piece_edges")}
+    else:
+        eff = {"block": args.block or 1024, "slots": args.slots or int(ncu.value),
+               "hub_bytes": -1 if args.hub_bytes is None else args.hub_bytes,
+               "bin_min": 65536 if args.bin_min is None else args.bin_min,
+               "dense_min": 64 if args.dense_min is None else args.dense_min, "split_edges": 256, "piece_edges": 0}
+    if eff["hub_bytes"] == -1:
+        eff["hub_bytes"] = {1024: 136 * 1024, 512: 48 * 1024, 256: 24 * 1024}[eff["block"]]
     tpath = os.path.join(REPO, "profiles", "k_batch_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        if tj.get("n") == g.n and tj.get("seeds") == len(seeds) and tj.get("eps") == args.eps:
+        # only for the workload and launch configuration the counter runs profiled
+        if (tj.get("n") == g.n and tj.get("seeds") == len(seeds) and tj.get("eps") == args.eps
+                and tj.get("config") == eff):
             traffic = tj.get("bytes_per_launch")
             # the limit that binds: memory-side atomics (counted by rocprofv3 on
             # this workload) per second of this run's kernel time, against the
@@ -309,7 +325,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "memory_atomics": atomics,
-            "kernel": "k_batch", "kernel_ms": kern_ms, "algorithmic_bytes": bytes_step,
+            "kernel": "k_batch", "kernel_ms": kern_ms, "algorithmic_bytes": bytes_step, "launch_config": eff,
         },
         "cpu_baseline": cpu,
         "relabelled": not args.no_relabel,

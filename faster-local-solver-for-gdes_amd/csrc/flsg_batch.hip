@@ -2168,7 +2168,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     int pw = g->batch_piece_w > 0 ? g->batch_piece_w : 1024;
     while ((int64_t)pw * 255 < (int64_t)g->d_max_hint) pw <<= 1;
     a.piece_w = pw;
-    a.hub_w = g->batch_hub_w > 0 ? g->batch_hub_w : 512;
+    a.hub_w = g->batch_hub_w > 0 ? g->batch_hub_w : 256;
     a.slot_base = slot_base;
     a.ctl = d_ctl;
     a.out_map = d_out_map;
@@ -2190,7 +2190,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     }
     const int grid = n_slots;
     int hub = g->batch_hub_bytes;
-    if (hub < 0) hub = block == 1024 ? 128 * 1024 : block == 512 ? 48 * 1024 : 24 * 1024;
+    if (hub < 0) hub = block == 1024 ? 136 * 1024 : block == 512 ? 48 * 1024 : 24 * 1024;
     a.hub_bytes = hub;
     const int bi = block == 1024 ? 0 : block == 256 ? 1 : 2;
     const void *fn = bi == 0   ? reinterpret_cast<const void *>(&k_batch<1024>)

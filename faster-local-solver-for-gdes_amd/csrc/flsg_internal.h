@@ -108,10 +108,10 @@ struct flsg_graph_s {
     int32_t batch_slots = 0;
     int32_t batch_block = 1024;
     int32_t batch_hub_bytes = -1;           // LDS hub cache per workgroup (-1 = default for the block)
-    int32_t batch_hub_w = 0;                // degree bin: entries with more edges are split (0 = 512)
+    int32_t batch_hub_w = 0;                // degree bin: entries with more edges are split (0 = 256)
     int32_t batch_piece_w = 0;              // edges per piece (0 = 1024, doubled to <= 255 pieces per entry)
-    int64_t batch_bin_min = 16384;          // binned push for iterations of at least this many edges
-    int32_t batch_dense_min = 256;          // binned push: bins with fewer pairs use memory-side atomics
+    int64_t batch_bin_min = 65536;          // binned push for iterations of at least this many edges
+    int32_t batch_dense_min = 64;           // binned push: bins with fewer pairs use memory-side atomics
     int32_t undirected = -1;                // every edge (u, v) has its (v, u): -1 unknown, 0 no, 1 yes
     const int32_t *out_map = nullptr;       // internal id -> caller id for batched outputs
 };

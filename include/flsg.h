@@ -211,7 +211,7 @@ int flsg_batch_set_block(flsg_graph g, int32_t block);
  * hubs, whose pushes then use LDS atomics; in binned iterations they hold
  * the slice of 2^s residuals of one destination range (2^s the largest power
  * of two that fits; binning needs 2^s >= 2048); in the certificate, x32 of
- * the first ids.  -1 = default for the workgroup size (128 KiB at 1024,
+ * the first ids.  -1 = default for the workgroup size (136 KiB at 1024,
  * 48 KiB at 512, 24 KiB at 256), 0 = off, else a multiple of 8 up to the
  * device's LDS per workgroup less the kernel's static LDS (~145,000 bytes on
  * gfx950).  No counterpart in the reference (tuning knob). */
@@ -222,12 +222,12 @@ int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes);
  * bin per destination range and sums each bin in LDS before one add per
  * destination into its residual (in id order over the range when the bin is
  * dense); a bin with fewer than dense_pairs pairs is applied with memory-side
- * atomics instead.  Defaults 16384 and 256.  No counterpart in the reference
+ * atomics instead.  Defaults 65536 and 64.  No counterpart in the reference
  * (tuning knob, set by the autotuner). */
 int flsg_batch_set_binning(flsg_graph g, int64_t min_edges, int32_t dense_pairs);
 
 /* Degree bins of the batched edge phases: a list entry with more than
- * split_edges edges (0 = 512) is processed as pieces of piece_edges edges
+ * split_edges edges (0 = 256) is processed as pieces of piece_edges edges
  * (0 = 1024; raised to keep at most 255 pieces per entry), each piece by one
  * wavefront; smaller entries are flattened 64 at a time by a wavefront.  No
  * counterpart in the reference (tuning knob, set by the autotuner). */
