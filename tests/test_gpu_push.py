@@ -615,7 +615,11 @@ def test_autotune_small(small_graph):
         best = out["best"]
         assert out["qps"] > 0 and len(out["trials"]) >= 5
         assert best["block"] in (256, 512, 1024) and best["split_edges"] >= 64
-        assert max(t["qps"] for t in out["trials"]) == pytest.approx(out["qps"])
+        # a knob moves only for a gain above 3%: no trial beats the kept
+        # setting by more than that, and the kept rate is a measured one
+        assert max(t["qps"] for t in out["trials"]) <= out["qps"] * 1.03 * (1 + 1e-12)
+        assert any(t["qps"] == out["qps"] and all(t[k] == v for k, v in best.items())
+                   for t in out["trials"])
         bo = engine.batch_solve(dg, p, seeds)
         _check_batch(g, seeds, bo, 1e-6, n_check=8)
     finally:
