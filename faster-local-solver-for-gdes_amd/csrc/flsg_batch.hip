@@ -56,9 +56,10 @@ namespace flsg {
 
 constexpr int kMaxHkTerms = 96;      // heat-kernel Taylor order cap of a batched launch
 constexpr unsigned kInQ = 1u;        // flags: in the correction queue
-constexpr unsigned kSeen = 2u;       // flags: listed in the support
+constexpr unsigned kSeen = 2u;       // flags: listed in the support by the correction push (x32 == 0)
 constexpr unsigned kTouched = 4u;    // flags: in the query's touched list (reset at the end)
-constexpr unsigned kBound = 8u;      // flags: support entry certified by sup_bound, not recomputed
+constexpr unsigned kExact = 8u;      // flags: r64 is the exact FP64 residual of x32 (no FP32 error bound)
+constexpr unsigned kR64 = 16u;       // flags: the entry's FP64 fields may be nonzero (read and reset at the end)
 constexpr int kSink = 64;            // per-slot sink entries (degree 0) for the idle lanes of an edge batch
 constexpr int kBinShiftMin = 11;     // binned push: at least 2^11 ids per destination range
 constexpr int kMaxBins = 512;        // binned push: at most this many destination ranges (static LDS)
@@ -146,11 +147,17 @@ __host__ __device__ __forceinline__ double sup_bound(float x32, float rl, int d,
     return 1.01 * num / (1.0 - g);
 }
 
-// bound of an entry with flags fl and FP32 residual field rl: a kBound entry
-// keeps its support bound, rounded up, as -rl (its pending FP32 residual moved
-// into r64 when it was certified); others the boundary bound
-__host__ __device__ __forceinline__ double rec_bound(uint32_t fl, float rl, int d, int64_t it) {
-    return (fl & kBound) ? fabs((double)rl) : lazy_bound(rl, d, it);
+// Error bound of an entry of the FP32 certificate, from its flags fl, FP32
+// residual field rl (> 0: pending, < 0: moved into x64 by the correction push,
+// the bound staying that of |rl|), FP32 estimate x32 and b = b_v: 0 for an
+// exact entry (r64 recomputed in FP64), sup_bound for a pushed node (x32 > 0),
+// the boundary bound otherwise.  Pure function of the entry: the certificate,
+// the correction push and the output pass compute the same value.
+__host__ __device__ __forceinline__ double rec_bound(uint32_t fl, float rl, float x32, int d, int64_t it,
+                                                     double b) {
+    if (fl & kExact) return 0.0;
+    if (x32 != 0.0f) return sup_bound(x32, rl, d, it, b);
+    return lazy_bound(rl, d, it);
 }
 
 // One slot's arrays (see the file header).
@@ -170,23 +177,22 @@ struct Strided {
     __device__ __forceinline__ T &operator[](int64_t i) const { return p[i * s]; }
 };
 
-// Certifies a pushed node by sup_bound when |r32| + bound < (1 - 2^-8) thr:
-// r64 takes the pending FP32 residual (exactly), the FP32 field the bound as
-// -fl32_up(bound), x64 the FP32 estimate, flags kSeen | kBound.  Returns
-// whether it did (else the node joins the FP64 pull).  Out of line: the
-// bound's FP64 arithmetic would otherwise raise the persistent kernel's
-// register pressure around its hot loops.
-__device__ __attribute__((noinline)) bool certify_support(SlotView S, int v, int d, int64_t iters, double b,
-                                                          double thr_scale) {
-    const float2 rx = S.rx32[v];
-    const double bnd = sup_bound(rx.y, rx.x, d, iters, b);
-    if (!(fabs((double)rx.x) + bnd < kNearFrac * thr_scale * (double)d)) return false;
-    float bf = (float)bnd;
-    if ((double)bf < bnd) bf = __uint_as_float(__float_as_uint(bf) + 1u);   // next float up (bf > 0)
-    S.rx64[v] = make_double2((double)rx.x, (double)rx.y);
-    S.rx32[v] = make_float2(-bf, rx.y);
-    S.fl[v] = kSeen | kBound;
-    return true;
+// Whether sup_bound certifies a pushed node {rl, x32}: |r32| + bound <
+// (1 - 2^-8) thr.  Such a node keeps its FP32 residual and estimate as they
+// are (nothing is written: the correction push and the output pass recompute
+// the same bound from the entry); the others join the FP64 pull.  Out of
+// line: the bound's FP64 arithmetic would otherwise raise the persistent
+// kernel's register pressure around its hot loops.
+__device__ __attribute__((noinline)) bool support_certified(float rl, float x32, int d, int64_t iters, double b,
+                                                            double thr_scale) {
+    const double bnd = sup_bound(x32, rl, d, iters, b);
+    return fabs((double)rl) + bnd < kNearFrac * thr_scale * (double)d;
+}
+
+// rec_bound out of line (the correction push and the output pass)
+__device__ __attribute__((noinline)) double rec_bound_ool(uint32_t fl, float rl, float x32, int d, int64_t it,
+                                                          double b) {
+    return rec_bound(fl, rl, x32, d, it, b);
 }
 
 // A frontier entry as its node phase leaves it for the edge phase: row start,
@@ -345,6 +351,7 @@ __device__ __forceinline__ int32_t load_wg(const int32_t *p) {
 
 constexpr int kUnroll = 4;           // edges in flight per lane
 constexpr int kOutUnroll = 8;        // entries in flight per lane in the output pass
+constexpr int kScanUnroll = 4;       // entries in flight per lane in the certificate's support scan
 constexpr int kBinUnroll = 4;        // pairs / destinations in flight per lane in the binned push
 
 struct WaveLds {
@@ -1612,29 +1619,48 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 }
             }
             __syncthreads();
-            // support rows the local bound certifies (sup_bound) keep their FP32
-            // residual: x64 = x32, flag kBound; the others are listed in nl[]
-            // (the boundary candidates are consumed above) for the pull.  No
-            // block barrier in the loop: every wave runs its rows on its own.
+            // pushed nodes (x32 > 0) the local bound certifies (sup_bound) keep
+            // their FP32 residual and estimate untouched; the others are listed
+            // in nl[] (the boundary candidates are consumed above) for the pull.
+            // Read-only: over the support list, or -- when the query pushed more
+            // than a quarter of the graph -- streamed over the slot's records
+            // in id order (coalesced).  No block barrier in the loop.
             if (tid == 0) s_ex = 0;
             __syncthreads();
-            for (unsigned int i0 = 0; i0 < S32; i0 += NT) {
-                const unsigned int i = i0 + tid;
-                int v = -1;
-                bool exact = false;
-                if (i < S32) {
-                    v = sup[i];
-                    exact = !certify_support(S, v, degree[v], iters, v == s ? P.r0 : 0.0, P.thr_scale);
+            {
+                const bool sstream = (int64_t)S32 * 4 > a.n;
+                const unsigned int slim = sstream ? (unsigned int)a.n : S32;
+                for (unsigned int i0 = 0; i0 < slim; i0 += NT * kScanUnroll) {
+                    int vv[kScanUnroll];
+                    float2 rx[kScanUnroll];
+#pragma unroll
+                    for (int j = 0; j < kScanUnroll; j++) {
+                        const unsigned int i = i0 + j * NT + tid;
+                        vv[j] = i < slim ? (sstream ? (int)i : sup[i]) : -1;
+                    }
+#pragma unroll
+                    for (int j = 0; j < kScanUnroll; j++) rx[j] = vv[j] >= 0 ? S.rx32[vv[j]] : make_float2(0.0f, 0.0f);
+                    int dd[kScanUnroll];
+#pragma unroll
+                    for (int j = 0; j < kScanUnroll; j++) dd[j] = rx[j].y != 0.0f ? degree[vv[j]] : 0;
+#pragma unroll
+                    for (int j = 0; j < kScanUnroll; j++) {
+                        bool exact = false;
+                        if (rx[j].y != 0.0f)
+                            exact = !support_certified(rx[j].x, rx[j].y, dd[j], iters, vv[j] == s ? P.r0 : 0.0,
+                                                       P.thr_scale);
+                        const long long k = wave_append(exact, &s_ex);
+                        if (k >= 0) nl[k] = vv[j];
+                    }
                 }
-                const long long k = wave_append(exact, &s_ex);
-                if (k >= 0) nl[k] = v;
             }
             __syncthreads();
             const unsigned int E32 = s_ex;
             const unsigned int NLIST = E32 + s_ncnt;
-            // a listed entry, finished: r64 = b - x + (pulled pushes) in FP64,
-            // x64 = x32, FP32 residual field -0.0 (exact entry), support flag
-            // for the pushed nodes, queued for the correction push if |r64| >= thr
+            // a listed entry, finished: r64 = b - x32 + (pulled pushes) in FP64,
+            // x64 = 0 (x64 holds the correction push's FP64 additions to x32),
+            // FP32 residual field -0.0, flags kExact | kR64, queued for the
+            // correction push if |r64| >= thr
             auto finish = [&](bool ok, int v, float x32v, int d, double acc) {
                 bool enq = false;
                 if (ok) {
@@ -1642,10 +1668,9 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     double rt = acc - xd;
                     if (v == s) rt = rt + P.r0;
                     enq = fabs(rt) >= P.thr_scale * (double)d;
-                    S.rx64[v].x = rt;
-                    S.rx64[v].y = xd;
+                    S.rx64[v] = make_double2(rt, 0.0);
                     S.rx32[v].x = -0.0f;
-                    S.fl[v] = (pristine(x32v) ? 0u : kSeen) | (enq ? kInQ : 0u);
+                    S.fl[v] = kExact | kR64 | (enq ? kInQ : 0u);
                 }
                 const long long sl = wave_append(enq, &s_cnt);
                 if (sl >= 0) qc[sl] = v;
@@ -1706,19 +1731,22 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     const int u = qc[i];
                     const int d = degree[u];
                     const uint32_t fl = fl_and(&S.fl[u], ~kInQ);
-                    // a pending boundary residual (rl > 0) joins the FP64 one;
-                    // once pushed its error bound stays (rl < 0)
-                    const float rl = S.rx32[u].x;
+                    // a pending FP32 residual (rl > 0) joins the FP64 one; once
+                    // pushed its error bound stays (rl < 0)
+                    const float2 rx = S.rx32[u];
+                    const float rl = rx.x;
                     const double tmp = S.rx64[u].x + (rl > 0.0f ? (double)rl : 0.0);
                     Ent e;
                     e.st = 0;
                     e.d = 0;
                     e.v = 0.0;
-                    if (fabs(tmp) + rec_bound(fl, rl, d, iters) >= P.thr_scale * (double)d) {
+                    if (fabs(tmp) + rec_bound_ool(fl, rl, rx.y, d, iters, u == s ? P.r0 : 0.0) >=
+                        P.thr_scale * (double)d) {
                         S.rx64[u].x = 0.0;
                         if (rl > 0.0f) S.rx32[u].x = -rl;
                         S.rx64[u].y = S.rx64[u].y + tmp;
-                        if ((fl & kSeen) == 0) {
+                        // a node the FP32 push never pushed joins the support
+                        if ((fl & kSeen) == 0 && rx.y == 0.0f) {
                             fl_or(&S.fl[u], kSeen);
                             sup[atomicAdd(&s_sup, 1u)] = u;
                         }
@@ -1743,12 +1771,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                     [&](int vtx, int dv, double val, double old, bool act) -> int {
                         if (!act) return 0;
                         const double nw = old + val;
-                        const float rl = S.rx32[vtx].x;
+                        const float2 rx = S.rx32[vtx];
+                        const float rl = rx.x;
                         int res = 0;
-                        // entries the main push never reached still hold r == +0.0
-                        if (old == 0.0 && pristine(rl) && (fl_or(&S.fl[vtx], kTouched) & kTouched) == 0) res = 2;
+                        // entries the main push never reached still hold r == +0.0;
+                        // every entry the correction push adds to is flagged kR64
+                        const bool fresh = old == 0.0 && pristine(rl);
+                        const uint32_t of = fl_or(&S.fl[vtx], kR64 | (fresh ? kTouched : 0u));
+                        if (fresh && (of & kTouched) == 0) res = 2;
                         const double eff = nw + (rl > 0.0f ? (double)rl : 0.0);
-                        const double bv = rec_bound(S.fl[vtx], rl, dv, iters);
+                        const double bv = rec_bound(of, rl, rx.y, dv, iters, vtx == s ? P.r0 : 0.0);
                         if (fabs(eff) + bv >= P.thr_scale * (double)dv && (fl_or(&S.fl[vtx], kInQ) & kInQ) == 0)
                             res |= 1;
                         return res;
@@ -1793,67 +1825,89 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
                 const unsigned int i = i0 + j * NT + tid;
                 vv[j] = i < lim ? (stream ? (int)i : tl[i]) : -1;
             }
-            // (the FP64 push writes only r64 and x64, so there only those are read
-            // and cleared)
+            // The FP64 push writes only r64 and x64, so there only those are read
+            // and cleared.  The FP32 push's entries hold {r32, x32} and flags;
+            // their FP64 fields are read (and cleared) only where the flags say
+            // the certificate wrote them (kR64).
+            if (P.fp64) {
 #pragma unroll
-            for (int j = 0; j < kOutUnroll; j++) {
-                r32v[j] = 0.0f;
-                x32v[j] = 0.0f;
-                flv[j] = 0u;
-                r64v[j] = 0.0;
-                x64v[j] = 0.0;
-                if (vv[j] >= 0) {
-                    const int v = vv[j];
-                    if (!P.fp64) {
-                        r32v[j] = S.rx32[v].x;
-                        x32v[j] = S.rx32[v].y;
-                        flv[j] = S.fl[v];
+                for (int j = 0; j < kOutUnroll; j++) {
+                    r32v[j] = 0.0f;
+                    x32v[j] = 0.0f;
+                    flv[j] = 0u;
+                    r64v[j] = 0.0;
+                    x64v[j] = 0.0;
+                    if (vv[j] >= 0) {
+                        const double2 w = S.rx64[vv[j]];
+                        r64v[j] = w.x;
+                        x64v[j] = w.y;
                     }
-                    r64v[j] = S.rx64[v].x;
-                    x64v[j] = S.rx64[v].y;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kOutUnroll; j++) {
+                    r32v[j] = 0.0f;
+                    x32v[j] = 0.0f;
+                    flv[j] = 0u;
+                    if (vv[j] >= 0) {
+                        const float2 w = S.rx32[vv[j]];
+                        r32v[j] = w.x;
+                        x32v[j] = w.y;
+                        flv[j] = S.fl[vv[j]];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kOutUnroll; j++) {
+                    r64v[j] = 0.0;
+                    x64v[j] = 0.0;
+                    if (flv[j] & kR64) {
+                        const double2 w = S.rx64[vv[j]];
+                        r64v[j] = w.x;
+                        x64v[j] = w.y;
+                    }
                 }
             }
-            bool is_sup[kOutUnroll];
-            int id[kOutUnroll];
-            double outv[kOutUnroll];
+            // one entry at a time: certificate value, reset, then its output
+            // slot (every lane runs the wave append), so only the entries still
+            // to come stay live across the bound's call
 #pragma unroll
             for (int j = 0; j < kOutUnroll; j++) {
-                is_sup[j] = false;
-                id[j] = vv[j];
-                outv[j] = x64v[j];
-                if (vv[j] < 0) continue;
-                const int v = vv[j];
-                if (stream && (__float_as_uint(r32v[j]) | __float_as_uint(x32v[j]) | flv[j]) == 0u &&
-                    pristine(r64v[j]) && pristine(x64v[j])) {
-                    id[j] = -1;   // untouched entry: nothing to certify, output or reset
-                    continue;
+                bool is_sup = false;
+                int id = vv[j];
+                // FP32 push: x32 plus the correction push's FP64 additions
+                const double outv = P.fp64 ? x64v[j] : (double)x32v[j] + x64v[j];
+                const bool live = vv[j] >= 0 &&
+                                  !(stream && (__float_as_uint(r32v[j]) | __float_as_uint(x32v[j]) | flv[j]) == 0u &&
+                                    pristine(r64v[j]) && pristine(x64v[j]));   // else untouched: nothing to do
+                if (live) {
+                    const int v = vv[j];
+                    const int d = degree[v];
+                    // FP32 push: an entry may hold its residual in FP32 (pending,
+                    // rl > 0) and carries the error bound of its FP32 values
+                    const float rl = r32v[j];
+                    const double res = r64v[j] + (rl > 0.0f ? (double)rl : 0.0);
+                    const double bnd =
+                        P.fp64 ? 0.0 : rec_bound_ool(flv[j], rl, x32v[j], d, iters, v == s ? P.r0 : 0.0);
+                    if (res != 0.0 || bnd != 0.0) {
+                        cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
+                        rl1 += fabs(res) + bnd;
+                    }
+                    // the support: FP32-pushed nodes (x32 > 0) and those the
+                    // correction push pushed first (kSeen)
+                    is_sup = P.fp64 ? x64v[j] != 0.0 : (x32v[j] != 0.0f || (flv[j] & kSeen) != 0u);
+                    if (is_sup && a.out_map) id = a.out_map[v];
+                    if (!P.fp64) {
+                        S.rx32[v] = make_float2(0.0f, 0.0f);
+                        if (flv[j] != 0u) S.fl[v] = 0u;
+                        if (flv[j] & kR64) S.rx64[v] = make_double2(0.0, 0.0);
+                    } else {
+                        S.rx64[v] = make_double2(0.0, 0.0);
+                    }
                 }
-                const int d = degree[v];
-                // FP32 push: a boundary entry may hold its residual in FP32
-                // (pending, rl > 0) and carries the error bound of that sum
-                const float rl = r32v[j];
-                const double res = r64v[j] + (rl > 0.0f ? (double)rl : 0.0);
-                const double bnd = P.fp64 ? 0.0 : rec_bound(flv[j], rl, d, iters);
-                if (res != 0.0 || bnd != 0.0) {
-                    cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
-                    rl1 += fabs(res) + bnd;
-                }
-                is_sup[j] = P.fp64 ? x64v[j] != 0.0 : (flv[j] & kSeen) != 0u;
-                if (is_sup[j] && a.out_map) id[j] = a.out_map[v];
-                if (!P.fp64) {
-                    S.rx32[v].x = 0.0f;
-                    S.rx32[v].y = 0.0f;
-                    S.fl[v] = 0u;
-                }
-                S.rx64[v].x = 0.0;
-                S.rx64[v].y = 0.0;
-            }
-#pragma unroll
-            for (int j = 0; j < kOutUnroll; j++) {
-                const long long k = wave_append(is_sup[j], &s_ncnt);
+                const long long k = wave_append(is_sup, &s_ncnt);
                 if (k >= 0 && fits && (unsigned long long)k < Sn) {
-                    a.out_ids[obase + k] = id[j];
-                    a.out_vals[obase + k] = outv[j];
+                    a.out_ids[obase + k] = id;
+                    a.out_vals[obase + k] = outv;
                 }
             }
         }
