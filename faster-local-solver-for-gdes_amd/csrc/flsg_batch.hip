@@ -1453,7 +1453,7 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(BatchArgs a) {
     __shared__ WaveLds s_w[NT / kWave];
     __shared__ unsigned long long s_red[NT / kWave];
     __shared__ double s_redd[NT / kWave];
