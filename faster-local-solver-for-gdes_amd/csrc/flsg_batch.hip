@@ -43,6 +43,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -270,6 +271,16 @@ struct BatchArgs {
     int64_t out_cap;
     flsg_result *res;
     int64_t max_iters;
+    // serving session (all NULL for a launch over a fixed query list): queries
+    // [0, *pub) are published by the host into host-coherent memory while the
+    // launch runs; *closed != 0 ends it once every published query is claimed;
+    // done[q] is set (system scope) when query q's record and output are
+    // written; a workgroup that sees nothing published for idle_ticks (100 MHz)
+    // exits on its own and counts itself in ctl[2]
+    const uint64_t *pub;
+    const uint32_t *closed;
+    uint32_t *done;
+    uint64_t idle_ticks;
     // heat kernel (shared by the launch's HK queries)
     int32_t hk_n;                // Taylor order N
     double hk_tau;
@@ -1452,6 +1463,63 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
     return status;
 }
 
+// Serving session: the next query index for this workgroup (thread 0).  The
+// index is claimed at once; the workgroup then waits until the host has
+// published it (*pub > c) -- or, once the host has closed the session with c
+// still unpublished, or after idle_ticks with nothing published (then every
+// workgroup of the launch ends: ctl[2] != 0), returns a value >= nq, which
+// ends the workgroup.  Every wait is bounded.
+__device__ __forceinline__ unsigned int session_claim(const BatchArgs &a) {
+    unsigned long long *expired = (unsigned long long *)&a.ctl[2];
+    // once one workgroup has given up waiting, indices it claimed may be
+    // published and never run: every workgroup ends, and the host re-submits
+    // whatever was published but not done in a new launch
+    if (__hip_atomic_load(expired, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) return 0xffffffffu;
+    const unsigned int c = atomicAdd((unsigned int *)&a.ctl[0], 1u);
+    if ((int64_t)c >= a.nq) return c;   // past this session's query capacity
+    const uint64_t t0 = now_ticks();
+    // each poll of host memory crosses PCIe: the count is read every ~4 us
+    // (s_sleep 127 = 8128 clocks), the close flag every 8th round
+    for (unsigned int it = 0;; it++) {
+        if (__hip_atomic_load(expired, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) return 0xffffffffu;
+        if ((uint64_t)c < __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM)) return c;
+        if ((it & 7u) == 7u && __hip_atomic_load(a.closed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+            // the host publishes before it closes: read the count once more
+            if ((uint64_t)c < __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM)) return c;
+            return 0xffffffffu;
+        }
+        if (now_ticks() - t0 > a.idle_ticks) {
+            atomicAdd(expired, 1ull);
+            return 0xffffffffu;
+        }
+        __builtin_amdgcn_s_sleep(127);
+    }
+}
+
+// A published query record, read from host-coherent memory with system-scope
+// loads (the record's line may have been read before the host wrote it).
+__device__ __forceinline__ flsg_query session_query(const BatchArgs &a, unsigned int q) {
+    static_assert(sizeof(flsg_query) == 32, "flsg_query is four 8-byte words");
+    const unsigned long long *w = reinterpret_cast<const unsigned long long *>(a.queries + q);
+    unsigned long long v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        v[k] = __hip_atomic_load(const_cast<unsigned long long *>(w + k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    flsg_query qq;
+    __builtin_memcpy(&qq, v, sizeof(qq));
+    return qq;
+}
+
+// End of a query: every thread's writes done; in a serving session thread 0
+// then makes them visible system-wide and sets the query's done flag.
+__device__ __forceinline__ void query_done(const BatchArgs &a, unsigned int q) {
+    __syncthreads();
+    if (a.done && threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(a.done + q, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(BatchArgs a) {
     __shared__ WaveLds s_w[NT / kWave];
@@ -1492,14 +1560,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int hw = a.hub_w;
 
     for (;;) {
-        if (tid == 0) s_q = atomicAdd((unsigned int *)&a.ctl[0], 1u);
+        if (tid == 0) s_q = a.pub ? session_claim(a) : atomicAdd((unsigned int *)&a.ctl[0], 1u);
         __syncthreads();
         const unsigned int c = s_q;
         __syncthreads();
         if ((int64_t)c >= a.nq) break;
         const unsigned int q = a.order ? (unsigned int)a.order[c] : c;
         const uint64_t t_start = now_ticks();
-        const flsg_query qq = a.queries[q];
+        const flsg_query qq = a.pub ? session_query(a, q) : a.queries[q];
         // a query the launch cannot run fails on its own record, the other
         // queries of the launch go on: seed outside the graph, unknown kind,
         // a heat-kernel query in a launch without heat-kernel settings, PPR /
@@ -1519,7 +1587,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 a.res[q] = rr;
                 a.out_off[q] = -1;
             }
-            __syncthreads();
+            query_done(a, q);
             continue;
         }
         if (qq.kind == FLSG_HK) {
@@ -1529,6 +1597,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             else
                 hk_query<NT, false>(a, qq, q, S, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
                                     &s_work, &s_tn, &s_vol, &s_base, t_start, sink);
+            query_done(a, q);
             continue;
         }
         const QParams P = qparams(qq);
@@ -1939,7 +2008,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             a.res[q] = rr;
             a.out_off[q] = fits ? (int64_t)obase : -1;
         }
-        __syncthreads();
+        query_done(a, q);
     }
 }
 
@@ -2175,10 +2244,19 @@ static int32_t default_slots(flsg_graph g) {
     return (int32_t)want;
 }
 
+// the host-coherent words of a serving session a launch polls (see BatchArgs)
+struct SessionWords {
+    const uint64_t *pub;
+    const uint32_t *closed;
+    uint32_t *done;
+    uint64_t idle_ticks;
+};
+
 static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_base, int32_t n_slots,
                   int32_t block, const int32_t *d_out_map, int32_t *d_out_ids, double *d_out_vals,
                   int64_t *d_out_off, int64_t out_cap, uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters,
-                  const flsg_hk_config *hk, hipStream_t st, const int32_t *d_order = nullptr) {
+                  const flsg_hk_config *hk, hipStream_t st, const int32_t *d_order = nullptr,
+                  const SessionWords *sw = nullptr) {
     BatchArena *A = g->arena;
     if (!A) return fail(FLSG_EARG, "no batch arena: call flsg_batch_reserve first");
     if (slot_base < 0 || n_slots <= 0 || slot_base + n_slots > A->slots)
@@ -2187,8 +2265,14 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     if (rc) return rc;
     rc = undirected_ensure(g, st);
     if (rc) return rc;
-    FLSG_HIP(hipMemsetAsync(d_ctl, 0, sizeof(uint64_t) * 2, st));
+    FLSG_HIP(hipMemsetAsync(d_ctl, 0, sizeof(uint64_t) * (sw ? 3 : 2), st));
     BatchArgs a{};
+    if (sw) {
+        a.pub = sw->pub;
+        a.closed = sw->closed;
+        a.done = sw->done;
+        a.idle_ticks = sw->idle_ticks;
+    }
     a.indptr = g->indptr;
     a.indices = g->indices;
     a.degree = g->degree;
@@ -2480,5 +2564,231 @@ extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_
     FLSG_HIP(hipStreamSynchronize(st));
     for (int64_t i = 0; i < n_queries; i++)
         if (h_res[i].status != FLSG_OK) return fail(h_res[i].status, "a batched query failed (see its status)");
+    return FLSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Serving sessions: continuous admission into one persistent batched launch.
+// The host writes query records into a host-coherent ring and publishes them
+// by raising a count the launch's idle workgroups poll; each workgroup takes
+// the next published query as soon as its slot is free, writes the result
+// record and the done flag into host-coherent memory, and the host reads
+// them without synchronising the launch's stream.
+
+struct flsg_session_s {
+    flsg_graph g = nullptr;
+    int device = 0;
+    hipStream_t st = nullptr;        // non-blocking: the open launch never holds up other streams
+    int64_t qcap = 0;
+    int64_t out_cap = 0;
+    flsg_query *h_q = nullptr;       // host-coherent, mapped (ring of qcap records)
+    flsg_result *h_res = nullptr;
+    int64_t *h_off = nullptr;
+    uint32_t *h_done = nullptr;
+    uint64_t *h_pub = nullptr;
+    uint32_t *h_closed = nullptr;
+    flsg_query *d_q = nullptr;       // device addresses of the same memory
+    flsg_result *d_res = nullptr;
+    int64_t *d_off = nullptr;
+    uint32_t *d_done = nullptr;
+    uint64_t *d_pub = nullptr;
+    uint32_t *d_closed = nullptr;
+    uint64_t *d_ctl = nullptr;       // device: [0] claims, [1] output entries used, [2] idle exits
+    int32_t *d_ids = nullptr;        // output arena (device)
+    double *d_vals = nullptr;
+    int running = 0;                 // a launch was enqueued and has not been seen to end
+    int closed = 0;
+    int64_t published = 0;
+};
+
+namespace {
+
+template <typename T>
+int host_coherent(T **h, T **d, int64_t count) {
+    FLSG_HIP(hipHostMalloc((void **)h, sizeof(T) * (size_t)std::max<int64_t>(count, 1),
+                           hipHostMallocCoherent | hipHostMallocMapped));
+    FLSG_HIP(hipHostGetDevicePointer((void **)d, *h, 0));
+    return FLSG_OK;
+}
+
+void session_free(flsg_session s) {
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    (void)hipHostFree(s->h_q);
+    (void)hipHostFree(s->h_res);
+    (void)hipHostFree(s->h_off);
+    (void)hipHostFree(s->h_done);
+    (void)hipHostFree(s->h_pub);
+    (void)hipHostFree(s->h_closed);
+    (void)hipFree(s->d_ctl);
+    (void)hipFree(s->d_ids);
+    (void)hipFree(s->d_vals);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+}
+
+// whether the session's launch has ended (the stream is idle); a fault of
+// the launch is returned as FLSG_EHIP
+int session_ended(flsg_session s, bool *ended) {
+    const hipError_t e = hipStreamQuery(s->st);
+    if (e == hipSuccess) {
+        *ended = true;
+        s->running = 0;
+        return FLSG_OK;
+    }
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();
+        *ended = false;
+        return FLSG_OK;
+    }
+    *ended = false;
+    return fail(FLSG_EHIP, std::string("serving session: ") + hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" int flsg_session_create(flsg_graph g, int64_t query_cap, int64_t out_cap, flsg_session *out) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (!out) return fail(FLSG_EARG, "NULL argument");
+    if (query_cap <= 0 || query_cap >= (int64_t)UINT32_MAX / 2) return fail(FLSG_EARG, "query_cap out of range");
+    if (out_cap < 0) return fail(FLSG_EARG, "out_cap < 0");
+    *out = nullptr;
+    flsg_session s = new flsg_session_s();
+    s->g = g;
+    s->qcap = query_cap;
+    s->out_cap = out_cap;
+    FLSG_HIP(hipGetDevice(&s->device));
+    rc = FLSG_OK;
+    auto chk = [&](int r) {
+        if (r && !rc) rc = r;
+    };
+    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) chk(fail(FLSG_EHIP, "stream create"));
+    if (!rc) chk(host_coherent(&s->h_q, &s->d_q, query_cap));
+    if (!rc) chk(host_coherent(&s->h_res, &s->d_res, query_cap));
+    if (!rc) chk(host_coherent(&s->h_off, &s->d_off, query_cap));
+    if (!rc) chk(host_coherent(&s->h_done, &s->d_done, query_cap));
+    if (!rc) chk(host_coherent(&s->h_pub, &s->d_pub, 1));
+    if (!rc) chk(host_coherent(&s->h_closed, &s->d_closed, 1));
+    if (!rc && hipMalloc((void **)&s->d_ctl, sizeof(uint64_t) * 3) != hipSuccess) chk(fail(FLSG_ENOMEM, "ctl"));
+    if (!rc && out_cap > 0) {
+        if (hipMalloc((void **)&s->d_ids, sizeof(int32_t) * out_cap) != hipSuccess ||
+            hipMalloc((void **)&s->d_vals, sizeof(double) * out_cap) != hipSuccess)
+            chk(fail(FLSG_ENOMEM, "serving session: output arena"));
+    }
+    if (rc) {
+        session_free(s);
+        return rc;
+    }
+    *s->h_pub = 0;
+    *s->h_closed = 1;   // no launch yet
+    s->closed = 1;
+    *out = s;
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_open(flsg_session s, int32_t slot_base, int32_t n_slots, const flsg_hk_config *hk,
+                                 int64_t max_iters, double idle_seconds) {
+    if (!s) return fail(FLSG_EARG, "NULL session");
+    if (s->running) return fail(FLSG_EARG, "serving session: the previous launch has not ended");
+    if (!(idle_seconds > 0.0) || idle_seconds > 3600.0) return fail(FLSG_EARG, "idle_seconds outside (0, 3600]");
+    flsg_graph g = s->g;
+    int rc = check_graph(g);
+    if (rc) return rc;
+    std::memset(s->h_done, 0, sizeof(uint32_t) * (size_t)s->qcap);
+    std::memset(s->h_res, 0, sizeof(flsg_result) * (size_t)s->qcap);
+    __atomic_store_n(s->h_pub, (uint64_t)0, __ATOMIC_RELEASE);
+    __atomic_store_n(s->h_closed, 0u, __ATOMIC_RELEASE);
+    s->published = 0;
+    SessionWords sw{s->d_pub, s->d_closed, s->d_done, (uint64_t)(idle_seconds * 1e8)};
+    rc = launch(g, s->d_q, s->qcap, slot_base, n_slots, g->batch_block, g->out_map, s->d_ids, s->d_vals, s->d_off,
+                s->out_cap, s->d_ctl, s->d_res, max_iters, hk, s->st, nullptr, &sw);
+    if (rc) {
+        __atomic_store_n(s->h_closed, 1u, __ATOMIC_RELEASE);
+        s->closed = 1;
+        (void)hipStreamSynchronize(s->st);
+        return rc;
+    }
+    s->running = 1;
+    s->closed = 0;
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_submit(flsg_session s, const flsg_query *h_queries, int64_t n, int64_t *h_first) {
+    if (!s || (n > 0 && !h_queries)) return fail(FLSG_EARG, "NULL argument");
+    if (s->closed) return fail(FLSG_EARG, "serving session: not open");
+    if (n < 0) return fail(FLSG_EARG, "n < 0");
+    if (s->published + n > s->qcap) return fail(FLSG_ECAP, "serving session: query ring full (close and reopen)");
+    if (h_first) *h_first = s->published;
+    std::memcpy(s->h_q + s->published, h_queries, sizeof(flsg_query) * (size_t)n);
+    s->published += n;
+    // the records before the count (the launch reads the count with acquire)
+    __atomic_store_n(s->h_pub, (uint64_t)s->published, __ATOMIC_RELEASE);
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_done(flsg_session s, int64_t first, int64_t count, int32_t *h_done) {
+    if (!s || (count > 0 && !h_done)) return fail(FLSG_EARG, "NULL argument");
+    if (first < 0 || count < 0 || first + count > s->qcap) return fail(FLSG_EARG, "query range outside the ring");
+    for (int64_t i = 0; i < count; i++)
+        h_done[i] = (int32_t)__atomic_load_n(s->h_done + first + i, __ATOMIC_ACQUIRE);
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_result(flsg_session s, int64_t i, flsg_result *h_res, int64_t *h_out_off) {
+    if (!s || !h_res) return fail(FLSG_EARG, "NULL argument");
+    if (i < 0 || i >= s->published) return fail(FLSG_EARG, "query index not published");
+    if (__atomic_load_n(s->h_done + i, __ATOMIC_ACQUIRE) == 0u) return fail(FLSG_EARG, "query not done");
+    std::memcpy(h_res, s->h_res + i, sizeof(flsg_result));
+    if (h_out_off) *h_out_off = s->h_off[i];
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_close(flsg_session s) {
+    if (!s) return fail(FLSG_EARG, "NULL session");
+    __atomic_store_n(s->h_closed, 1u, __ATOMIC_RELEASE);
+    s->closed = 1;
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_poll(flsg_session s, int32_t *h_ended, int64_t *h_idle_exits) {
+    if (!s || !h_ended) return fail(FLSG_EARG, "NULL argument");
+    bool ended = true;
+    if (s->running) {
+        int rc = session_ended(s, &ended);
+        if (rc) return rc;
+    }
+    *h_ended = ended ? 1 : 0;
+    if (h_idle_exits) {
+        *h_idle_exits = 0;
+        if (ended) {
+            uint64_t c[3] = {0, 0, 0};
+            FLSG_HIP(hipMemcpyAsync(c, s->d_ctl, sizeof(c), hipMemcpyDeviceToHost, s->st));
+            FLSG_HIP(hipStreamSynchronize(s->st));
+            *h_idle_exits = (int64_t)c[2];
+        }
+    }
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_output(flsg_session s, int64_t off, int64_t count, int32_t *h_ids, double *h_vals) {
+    if (!s || (count > 0 && (!h_ids || !h_vals))) return fail(FLSG_EARG, "NULL argument");
+    if (off < 0 || count < 0 || off + count > s->out_cap) return fail(FLSG_EARG, "range outside the output arena");
+    bool ended = true;
+    if (s->running) {
+        int rc = session_ended(s, &ended);
+        if (rc) return rc;
+    }
+    if (!ended) return fail(FLSG_EARG, "serving session: outputs are read after the launch has ended");
+    if (count == 0) return FLSG_OK;
+    FLSG_HIP(hipMemcpyAsync(h_ids, s->d_ids + off, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s->st));
+    FLSG_HIP(hipMemcpyAsync(h_vals, s->d_vals + off, sizeof(double) * count, hipMemcpyDeviceToHost, s->st));
+    FLSG_HIP(hipStreamSynchronize(s->st));
+    return FLSG_OK;
+}
+
+extern "C" int flsg_session_destroy(flsg_session s) {
+    if (!s) return FLSG_OK;
+    if (!s->closed) flsg_session_close(s);
+    session_free(s);
     return FLSG_OK;
 }

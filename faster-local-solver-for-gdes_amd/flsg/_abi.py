@@ -85,6 +85,16 @@ SIGNATURES = {
                                     C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
                                     C.c_void_p, C.c_void_p]),
     "flsg_stream_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "flsg_session_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
+    "flsg_session_open": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(HkConfig), C.c_int64,
+                                    C.c_double]),
+    "flsg_session_submit": (C.c_int, [C.c_void_p, C.POINTER(Query), C.c_int64, C.POINTER(C.c_int64)]),
+    "flsg_session_done": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+    "flsg_session_result": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result), C.POINTER(C.c_int64)]),
+    "flsg_session_close": (C.c_int, [C.c_void_p]),
+    "flsg_session_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "flsg_session_output": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]),
+    "flsg_session_destroy": (C.c_int, [C.c_void_p]),
     "flsg_batch_refresh_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "flsg_batch_arena_dirty": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "flsg_boundary_bound": (C.c_double, [C.c_float, C.c_int32, C.c_int64]),

@@ -6,27 +6,30 @@ graph to the device on every call (ppr_solver.py:101-107).  Here:
 
 * the graph (relabelled by degree, stored as a slack CSR with per-row headroom
   for inserts) and a slot arena stay resident;
-* the slot arena is split into lanes -- each lane has its own HIP stream, slot
-  range, query / result / output buffers -- so launches on different lanes run
-  concurrently on the device;
-* requests that arrive while lanes are busy wait in a queue and are admitted
-  into the next free lane's launch as long as the lane's output arena can hold
-  their expected footprint (a per-kind support estimate, raised whenever a
-  query overflows; an overflowing query is re-queued);
-* edge-insert snapshots are applied between launches: the loop stops admitting,
-  lets in-flight launches finish, inserts on the device and refreshes the slots'
-  degree copies;
-* latency = completion (the host observes, by flsg_stream_poll, that the
-  lane's stream has drained) minus
-  arrival; p50 / p99 and queries/s are reported for the FP32-batched path and
-  for the plain FP64 single-source path serving the same arrival stream.
+* one persistent batched launch (a serving session, flsg_session_*) runs over
+  every slot: each slot takes the next published query as soon as its
+  previous one is done, so requests that arrive while others run are
+  admitted as slots free up, and PPR, Katz and heat-kernel queries of any
+  length share the slots with no launch boundary to wait for;
+* admission by memory footprint: a request is published while the session's
+  output arena can hold its expected support (a per-kind estimate, raised
+  whenever a query overflows; an overflowing query is re-submitted in the
+  next session); when the arena or the query ring is used up the session is
+  closed and a new one opened;
+* edge-insert snapshots are applied between sessions: the loop closes the
+  session, lets the published queries finish, inserts on the device and
+  opens a new session;
+* latency = completion (the host sees the query's done flag, written by the
+  kernel into host-coherent memory) minus arrival; p50 / p99 and queries/s
+  are reported for the batched path and for the plain FP64 single-source
+  path serving the same arrival stream.
 """
 from __future__ import annotations
 
 import ctypes as C
 import time
 from collections import deque
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -86,6 +89,7 @@ def random_insert_batches(g: CSRGraph, every: float, size: int, count: int, seed
     for b in range(count):
         us = rng.integers(0, g.n, size=size).astype(np.int32)
         vs = rng.integers(0, g.n, size=size).astype(np.int32)
+        vs = np.where(vs == us, (vs + 1) % g.n, vs).astype(np.int32)   # no self-loops (the insert rejects them)
         out.append(InsertBatch(at=every * (b + 1), us=us, vs=vs))
     return out
 
@@ -104,40 +108,16 @@ def slack_with_headroom(g: CSRGraph, headroom: int) -> SlackGraph:
                       us=np.zeros((0, 0), np.int32), vs=np.zeros((0, 0), np.int32), name=g.name + "-slack")
 
 
-class _Lane:
-    def __init__(self, srv: "Server", idx: int, slot_base: int, n_slots: int, hk: bool = False):
-        torch = srv.torch
-        dev = srv.dg.device
-        self.idx = idx
-        self.hk = hk                 # heat-kernel lane (takes short queries only when no HK query waits)
-        self.slot_base = slot_base
-        self.n_slots = n_slots
-        self.stream = torch.cuda.Stream(device=dev)
-        self.qcap = srv.max_batch
-        # queries are staged in pinned memory and uploaded asynchronously on the lane's stream
-        self.h_qbuf = torch.empty(self.qcap * C.sizeof(_abi.Query), dtype=torch.uint8).pin_memory()
-        self.h_q = (_abi.Query * self.qcap).from_address(self.h_qbuf.data_ptr())
-        self.d_q = torch.empty(self.qcap * C.sizeof(_abi.Query), dtype=torch.uint8, device=dev)
-        self.d_res = torch.empty(self.qcap * C.sizeof(_abi.Result), dtype=torch.uint8, device=dev)
-        self.h_res = torch.empty(self.qcap * C.sizeof(_abi.Result), dtype=torch.uint8).pin_memory()
-        self.d_off = torch.empty(self.qcap, dtype=torch.int64, device=dev)
-        self.h_off = torch.empty(self.qcap, dtype=torch.int64).pin_memory()
-        self.d_ctl = torch.zeros(2, dtype=torch.int64, device=dev)
-        # output arena in (id, value) entries: sized to the lane's slots and the
-        # kind it serves (heat-kernel supports cover most of the graph)
-        self.out_cap = srv.out_per_lane or max(1 << 20, int(n_slots * srv.g.n * (0.9 if hk else 0.5)))
-        self.ids = torch.empty(self.out_cap, dtype=torch.int32, device=dev)
-        self.vals = torch.empty(self.out_cap, dtype=torch.float64, device=dev)
-        self.busy = False
-        self.event = None
-        self.reqs: list = []
-
-
 class Server:
-    def __init__(self, g: CSRGraph, lanes: int = 4, slots: int = 0, block: int = 1024, headroom: int = 8,
-                 max_batch: int = 256, out_per_lane: int | None = None, hk_tau: float = 5.0,
+    """Continuous-batching server over one resident graph (see the module
+    docstring).  `slots` workgroups serve queries (0: one per compute unit);
+    `query_cap` queries at most per session; `out_cap` output entries per
+    session (default: 3 n per slot)."""
+
+    def __init__(self, g: CSRGraph, slots: int = 0, block: int = 1024, headroom: int = 8,
+                 query_cap: int = 1 << 16, out_cap: int | None = None, hk_tau: float = 5.0,
                  hk_eps: float = 1e-7, hk_terms: int = 40, precision: int = _abi.FLSG_FP32,
-                 hk_lanes: int | None = None, hk_slot_share: float = 0.5, device: str = "cuda:0"):
+                 idle_seconds: float = 30.0, device: str = "cuda:0"):
         h, perm, iperm = relabel_by_degree(g)
         self.g = g
         self.precision = precision
@@ -161,71 +141,58 @@ class Server:
         self.slots = slots
         _check(lib.flsg_batch_reserve(self.dg.handle, int(slots), None))
         torch.cuda.synchronize()
-        self.max_batch = max_batch
-        self.out_per_lane = out_per_lane
-        # A launch ends with its slowest query, and a heat-kernel query does ~7x
-        # the work of a PPR query: heat-kernel queries get lanes (and a slot
-        # share) of their own so they never hold PPR / Katz slots idle.
-        if hk_lanes is None:
-            hk_lanes = 1 if lanes >= 2 else 0
-        hk_lanes = max(0, min(hk_lanes, lanes - 1))
-        short = lanes - hk_lanes
-        hk_slots = int(round(slots * hk_slot_share)) if hk_lanes else 0
-        hk_slots = min(max(hk_slots, hk_lanes), slots - short)
-        sizes = [((slots - hk_slots) // short) + (1 if i < (slots - hk_slots) % short else 0) for i in range(short)]
-        if hk_lanes:
-            sizes += [(hk_slots // hk_lanes) + (1 if i < hk_slots % hk_lanes else 0) for i in range(hk_lanes)]
-        self.lanes = []
-        base = 0
-        for i, sz in enumerate(sizes):
-            self.lanes.append(_Lane(self, i, base, sz, hk=i >= short))
-            base += sz
+        self.query_cap = int(query_cap)
+        self.out_cap = int(out_cap) if out_cap else max(1 << 20, int(3 * g.n * slots))
+        self.idle_seconds = float(idle_seconds)
+        sess = C.c_void_p()
+        _check(lib.flsg_session_create(self.dg.handle, self.query_cap, self.out_cap, C.byref(sess)))
+        self.sess = sess
         self.hk = _abi.HkConfig(tau=hk_tau, eps=hk_eps, n_terms=hk_terms)
         # per-kind support estimate used for admission (raised on overflow)
         self.est = {_abi.FLSG_PPR: max(1, g.n // 4), _abi.FLSG_KATZ: min(g.n, 50_000),
                     _abi.FLSG_HK: max(1, (g.n * 4) // 5)}
         self.update_stream = torch.cuda.Stream(device=self.dg.device)
+        self._done = np.zeros(self.query_cap, dtype=np.int32)
+        self._qbuf = (_abi.Query * self.query_cap)()
+        self.is_open = False
 
-    # ------------------------------------------------------------------ plumbing
-    def _launch(self, lane: _Lane, reqs: list, t0: float):
-        torch = self.torch
+    # ------------------------------------------------------------------ session plumbing
+    def _open(self):
+        _check(self.lib.flsg_session_open(self.sess, 0, self.slots, C.byref(self.hk), 0, self.idle_seconds))
+        self.is_open = True
+        self.ring = []             # ring index -> Request
+        self.first_open = 0        # ring indices below are all done
+        self.budget = self.out_cap
+        self.collected = []        # (Request, out_off, supp) whose output stays in the arena
+
+    def _submit(self, reqs: list) -> None:
         for j, r in enumerate(reqs):
-            q = lane.h_q[j]
+            q = self._qbuf[j]
             q.seed = int(self.iperm[r.seed])
             q.kind = r.kind
             q.precision = self.precision
             q.alpha = r.alpha
             q.eps = r.eps
-        nbytes = len(reqs) * C.sizeof(_abi.Query)
-        with torch.cuda.stream(lane.stream):
-            lane.d_q[:nbytes].copy_(lane.h_qbuf[:nbytes], non_blocking=True)
-            _check(self.lib.flsg_batch_launch(self.dg.handle, _ptr(lane.d_q), len(reqs), lane.slot_base,
-                                              min(lane.n_slots, len(reqs)), _ptr(lane.ids), _ptr(lane.vals),
-                                              _ptr(lane.d_off), lane.out_cap, _ptr(lane.d_ctl), _ptr(lane.d_res),
-                                              0, C.byref(self.hk), C.c_void_p(lane.stream.cuda_stream)))
-            rb = len(reqs) * C.sizeof(_abi.Result)
-            lane.h_res[:rb].copy_(lane.d_res[:rb], non_blocking=True)
-            lane.h_off[:len(reqs)].copy_(lane.d_off[:len(reqs)], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(lane.stream)
-        lane.event = ev
-        lane.busy = True
-        lane.reqs = reqs
+        first = C.c_int64(0)
+        _check(self.lib.flsg_session_submit(self.sess, self._qbuf, len(reqs), C.byref(first)))
+        assert first.value == len(self.ring)
+        self.ring.extend(reqs)
 
-    def _poll(self, lane: _Lane) -> bool:
-        """Whether the lane's launch and result copies have completed
-        (flsg_stream_poll on the lane's stream; never blocks)."""
-        done = C.c_int32(0)
-        _check(self.lib.flsg_stream_poll(C.c_void_p(lane.stream.cuda_stream), C.byref(done)))
-        return bool(done.value)
-
-    def _complete(self, lane: _Lane, now: float, collect: bool, requeue: deque) -> int:
-        res = (_abi.Result * len(lane.reqs)).from_buffer_copy(
-            lane.h_res[:len(lane.reqs) * C.sizeof(_abi.Result)].numpy().tobytes())
-        offs = lane.h_off[:len(lane.reqs)].numpy()
+    def _harvest(self, now: float, requeue: deque) -> int:
+        """Record every query of the open session whose done flag is set."""
+        lo, hi = self.first_open, len(self.ring)
+        if hi <= lo:
+            return 0
+        _check(self.lib.flsg_session_done(self.sess, lo, hi - lo, self._done.ctypes.data))
         finished = 0
-        for j, r in enumerate(lane.reqs):
-            rr = res[j]
+        rr = _abi.Result()
+        off = C.c_int64(0)
+        for i in range(lo, hi):
+            r = self.ring[i]
+            if r is None or not self._done[i - lo]:
+                continue
+            _check(self.lib.flsg_session_result(self.sess, i, C.byref(rr), C.byref(off)))
+            self.ring[i] = None
             if rr.status == _abi.FLSG_ECAP:
                 self.est[r.kind] = max(self.est[r.kind], int(rr.supp * 1.25) + 1)
                 requeue.append(r)
@@ -238,11 +205,41 @@ class Server:
             r.cert = rr.cert
             r.device_ms = rr.ms
             self.est[r.kind] = max(self.est[r.kind], int(rr.supp * 1.1) + 1)
-            if collect and rr.status == _abi.FLSG_OK:
-                o, k = int(offs[j]), int(rr.supp)
-                r.result = (lane.ids[o:o + k].cpu().numpy(), lane.vals[o:o + k].cpu().numpy())
-        lane.busy = False
-        lane.reqs = []
+            if rr.status == _abi.FLSG_OK and off.value >= 0:
+                self.collected.append((r, int(off.value), int(rr.supp)))
+        while self.first_open < hi and self.ring[self.first_open] is None:
+            self.first_open += 1
+        return finished
+
+    def _end(self, t0: float, time_scale: float, collect: bool, requeue: deque) -> int:
+        """Close the open session, wait for its launch to end (recording the
+        queries as they finish), then copy the outputs out if asked and
+        re-queue whatever was published but not done (an idle exit)."""
+        _check(self.lib.flsg_session_close(self.sess))
+        finished = 0
+        ended = C.c_int32(0)
+        idle = C.c_int64(0)
+        while True:
+            _check(self.lib.flsg_session_poll(self.sess, C.byref(ended), None))
+            finished += self._harvest((time.perf_counter() - t0) / time_scale, requeue)
+            if ended.value:
+                break
+            time.sleep(0)
+        _check(self.lib.flsg_session_poll(self.sess, C.byref(ended), C.byref(idle)))
+        finished += self._harvest((time.perf_counter() - t0) / time_scale, requeue)
+        for i in range(self.first_open, len(self.ring)):
+            if self.ring[i] is not None:     # published, never run (idle exit)
+                requeue.append(self.ring[i])
+                self.ring[i] = None
+        self.idle_exits = int(idle.value)
+        if collect:
+            for r, o, k in self.collected:
+                ids = np.empty(k, dtype=np.int32)
+                vals = np.empty(k, dtype=np.float64)
+                _check(self.lib.flsg_session_output(self.sess, o, k, ids.ctypes.data, vals.ctypes.data))
+                r.result = (ids, vals)
+        self.collected = []
+        self.is_open = False
         return finished
 
     def _apply_insert(self, ins: InsertBatch):
@@ -269,60 +266,76 @@ class Server:
         scaled by time_scale) and apply `inserts` at their times."""
         inserts = sorted(inserts or [], key=lambda b: b.at)
         order = sorted(requests, key=lambda r: r.arrival)
-        pend_short: deque = deque()
-        pend_hk: deque = deque()
+        pending: deque = deque()
         requeue: deque = deque()
-        has_hk_lane = any(l.hk for l in self.lanes)
         nxt = 0
         ins_i = 0
-        self.torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        done = 0
         total = len(order)
-        launches = 0
-        while done < total or ins_i < len(inserts):
-            now = (time.perf_counter() - t0) / time_scale
-            while nxt < total and order[nxt].arrival <= now:
-                r = order[nxt]
-                (pend_hk if has_hk_lane and r.kind == _abi.FLSG_HK else pend_short).append(r)
-                nxt += 1
-            for lane in self.lanes:
-                if lane.busy and self._poll(lane):
-                    done += self._complete(lane, now, collect, requeue)
-            while requeue:
-                r = requeue.pop()
-                (pend_hk if has_hk_lane and r.kind == _abi.FLSG_HK else pend_short).appendleft(r)
-            due = ins_i < len(inserts) and inserts[ins_i].at <= now
-            if due:
-                if any(l.busy for l in self.lanes):
-                    continue          # drain before mutating the graph
-                self._apply_insert(inserts[ins_i])
-                inserts[ins_i].applied = (time.perf_counter() - t0) / time_scale
-                ins_i += 1
-                continue
-            if not pend_short and not pend_hk:
-                if nxt >= total and not any(l.busy for l in self.lanes) and ins_i >= len(inserts):
-                    break
-                time.sleep(0)
-                continue
-            for lane in self.lanes:
-                if lane.busy:
-                    continue
-                # heat-kernel lanes fall back to short queries when no HK query waits
-                pending = pend_hk if (lane.hk and pend_hk) else pend_short
-                if not pending:
-                    continue
-                batch, budget = [], lane.out_cap
-                while pending and len(batch) < lane.qcap:
-                    need = self.est[pending[0].kind]
-                    if batch and need > budget:
+        done = 0
+        sessions = 0
+        submits = 0
+        idle_exits = 0
+        t0 = time.perf_counter()
+        self._open()
+        sessions += 1
+        try:
+            while True:
+                now = (time.perf_counter() - t0) / time_scale
+                while nxt < total and order[nxt].arrival <= now:
+                    pending.append(order[nxt])
+                    nxt += 1
+                done += self._harvest(now, requeue)
+                while requeue:
+                    pending.appendleft(requeue.pop())
+                due = ins_i < len(inserts) and inserts[ins_i].at <= now
+                finished_all = done >= total and nxt >= total and not pending
+                if due or (finished_all and ins_i >= len(inserts)):
+                    # drain the session before mutating the graph (or at the end)
+                    done += self._end(t0, time_scale, collect, requeue)
+                    idle_exits += self.idle_exits
+                    if due:
+                        self._apply_insert(inserts[ins_i])
+                        inserts[ins_i].applied = (time.perf_counter() - t0) / time_scale
+                        ins_i += 1
+                    while requeue:
+                        pending.appendleft(requeue.pop())
+                    if done >= total and nxt >= total and not pending and ins_i >= len(inserts):
                         break
-                    budget -= need
-                    batch.append(pending.popleft())
-                self._launch(lane, batch, t0)
-                launches += 1
+                    self._open()
+                    sessions += 1
+                    continue
+                if pending:
+                    # admission: publish while the output arena can hold the
+                    # expected supports; a full arena or ring starts a new session
+                    batch = []
+                    room = self.query_cap - len(self.ring)
+                    while pending and len(batch) < room:
+                        need = self.est[pending[0].kind]
+                        if need > self.budget:
+                            break
+                        self.budget -= need
+                        batch.append(pending.popleft())
+                    if batch:
+                        self._submit(batch)
+                        submits += 1
+                    elif self.first_open >= len(self.ring):
+                        # nothing in flight and the next request does not fit
+                        done += self._end(t0, time_scale, collect, requeue)
+                        idle_exits += self.idle_exits
+                        while requeue:
+                            pending.appendleft(requeue.pop())
+                        self._open()
+                        sessions += 1
+                        continue
+                time.sleep(0)
+        finally:
+            if self.is_open:
+                self._end(t0, time_scale, collect, requeue)
         wall = time.perf_counter() - t0
-        return self._stats(order, wall, launches, inserts)
+        st = self._stats(order, wall, sessions, inserts)
+        st["submits"] = submits
+        st["idle_exits"] = idle_exits
+        return st
 
     @staticmethod
     def _stats(order: list, wall: float, launches: int, inserts: list) -> dict:
@@ -342,6 +355,11 @@ class Server:
         }
 
     def close(self):
+        if self.sess:
+            if self.is_open:
+                self._end(time.perf_counter(), 1.0, False, deque())
+            self.lib.flsg_session_destroy(self.sess)
+            self.sess = None
         self.dg.close()
 
 

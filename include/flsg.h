@@ -281,6 +281,52 @@ int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queri
  * reference (it has no serving loop). */
 int flsg_stream_poll(void *stream, int32_t *h_done);
 
+/* Serving session: continuous admission into ONE persistent batched launch.
+ * No counterpart in the reference (it has no serving loop; its per-query GPU
+ * functions, ppr_solver.py:82-146, copy the graph and launch per query).
+ *
+ * flsg_session_create allocates a ring of query_cap query / result records
+ * and done flags in host-coherent memory, and an output arena of out_cap
+ * (id, value) entries on the device.  flsg_session_open launches k_batch over
+ * slots [slot_base, slot_base + n_slots) of the reserved arena on the
+ * session's own non-blocking stream: every workgroup claims the next ring
+ * index and waits (bounded) until the host publishes it; a slot takes the
+ * next published query as soon as its previous query is done, so queries
+ * that arrive while others run start as slots free up.  The launch serves at
+ * most query_cap queries; it ends when the host closes the session and every
+ * published query is done, or -- a safety net -- when a workgroup has seen
+ * nothing published for idle_seconds (then every workgroup ends and
+ * flsg_session_poll reports the idle exit; queries published but not done
+ * must be submitted again in a new launch).  Heat-kernel settings and the
+ * iteration cap are per launch.
+ *
+ * flsg_session_submit copies n records into the ring and publishes them
+ * (*h_first = the ring index of the first); FLSG_ECAP when the ring is full.
+ * flsg_session_done copies the done flags of [first, first + count) (1 = the
+ * query's result record and output are written); flsg_session_result copies
+ * a done query's record and output offset (-1: no output).  Neither waits.
+ * flsg_session_close publishes nothing more: the launch ends after the
+ * published queries.  flsg_session_poll: *h_ended = 1 once the launch has
+ * ended (never blocks), *h_idle_exits the idle exits of the ended launch.
+ * flsg_session_output copies output entries to the host, only once the
+ * launch has ended; the arena is kept until the next open.
+ *
+ * While a session is open the graph must not be modified, and nothing may
+ * wait for the whole device (hipDeviceSynchronize, torch.cuda.synchronize):
+ * the launch runs until the session is closed.  Destroy sessions before
+ * their graph. */
+typedef struct flsg_session_s *flsg_session;
+int flsg_session_create(flsg_graph g, int64_t query_cap, int64_t out_cap, flsg_session *out);
+int flsg_session_open(flsg_session s, int32_t slot_base, int32_t n_slots, const flsg_hk_config *hk,
+                      int64_t max_iters, double idle_seconds);
+int flsg_session_submit(flsg_session s, const flsg_query *h_queries, int64_t n, int64_t *h_first);
+int flsg_session_done(flsg_session s, int64_t first, int64_t count, int32_t *h_done);
+int flsg_session_result(flsg_session s, int64_t i, flsg_result *h_res, int64_t *h_out_off);
+int flsg_session_close(flsg_session s);
+int flsg_session_poll(flsg_session s, int32_t *h_ended, int64_t *h_idle_exits);
+int flsg_session_output(flsg_session s, int64_t off, int64_t count, int32_t *h_ids, double *h_vals);
+int flsg_session_destroy(flsg_session s);
+
 /* Rounding-error bound the batched FP32 certificate applies to a boundary
  * node (reached by the push, never pushed) of degree d whose FP32 residual is
  * r32 after `iters` push iterations: |r_FP64 - r32| <= bound, r_FP64 being the

@@ -36,7 +36,7 @@ def _katz_residual(g, s, beta, x):
 
 @pytest.fixture(scope="module")
 def srv_small(small_graph):
-    srv = serving.Server(small_graph, lanes=3, slots=24, block=256, max_batch=32)
+    srv = serving.Server(small_graph, slots=24, block=256)
     yield srv
     srv.close()
 
@@ -47,7 +47,9 @@ def test_serving_mix_certified_and_same_alone(small_graph, srv_small):
     reqs = serving.mixed_requests(g, rate=2000.0, count=120, seed=3, eps=eps)
     st = srv_small.run(reqs, collect=True)
     assert st["ok"] == len(reqs) and st["certified"]
-    assert st["launches"] >= 2
+    # continuous admission: one persistent launch took requests published
+    # while it ran (no launch boundary between them)
+    assert st["launches"] == 1 and st["submits"] >= 2 and st["idle_exits"] == 0
     # every launch (PPR, Katz and heat-kernel queries mixed in the same slots)
     # left every slot array in its between-queries state
     srv_small.torch.cuda.synchronize()
@@ -108,7 +110,7 @@ def test_serving_hk_matches_oracle(small_graph, srv_small):
 
 def test_serving_inserts_in_place(small_graph):
     g = small_graph
-    srv = serving.Server(g, lanes=2, slots=16, block=256, max_batch=16, headroom=6)
+    srv = serving.Server(g, slots=16, block=256, headroom=6)
     try:
         eps = 1e-6
         ins = serving.random_insert_batches(g, every=0.01, size=300, count=2, seed=5)
@@ -138,7 +140,7 @@ def test_serving_hk_and_ppr_with_and_without_hub_cache(small_graph, hub_bytes):
     residual in the node records) and on (the small graph fits in it whole):
     heat kernel equal to the oracle up to FP32 rounding, PPR certified."""
     g = small_graph
-    srv = serving.Server(g, lanes=2, slots=8, block=256, max_batch=16)
+    srv = serving.Server(g, slots=8, block=256)
     try:
         assert srv.lib.flsg_batch_set_hub_cache(srv.dg.handle, hub_bytes) == 0
         seeds = graph.uniform_seeds(g, 4, seed=17)
@@ -173,7 +175,7 @@ def test_serving_hk_large(large_graph, precision):
     partial sum, so its relative error is bounded by ~10^3 * 6e-8 = 6e-5: the
     bar is relative L1 1e-4 (observed ~1e-5), op counts within 2%."""
     g = large_graph
-    srv = serving.Server(g, lanes=1, slots=4, max_batch=4, precision=precision)
+    srv = serving.Server(g, slots=4, precision=precision)
     fp64 = precision == _abi.FLSG_FP64
     try:
         seeds = graph.uniform_seeds(g, 3, seed=23)
@@ -194,7 +196,7 @@ def test_serving_hk_fp64_small(small_graph):
     """Batched FP64 heat kernel on the small graph: equal to the oracle's level
     push up to summation order (relative L1 1e-12), the same operation count."""
     g = small_graph
-    srv = serving.Server(g, lanes=1, slots=8, block=256, max_batch=8, precision=_abi.FLSG_FP64)
+    srv = serving.Server(g, slots=8, block=256, precision=_abi.FLSG_FP64)
     try:
         seeds = graph.uniform_seeds(g, 6, seed=29)
         reqs = [serving.Request(rid=i, kind=_abi.FLSG_HK, seed=int(s), arrival=0.0) for i, s in enumerate(seeds)]
@@ -205,5 +207,79 @@ def test_serving_hk_fp64_small(small_graph):
             x = _dense(r, g.n)
             assert np.abs(x - o["x"]).sum() <= 1e-12 * np.abs(o["x"]).sum()
             assert r.oper == int(o["oper"])
+    finally:
+        srv.close()
+
+
+@pytest.mark.slow
+def test_serving_mix_large_with_insert(large_graph):
+    """The serving mix on the 2.39M-node graph (eps = 1e-7): PPR, Katz and heat
+    kernel queries arriving over time, a 10,000-edge insert batch in the
+    middle.  Every PPR / Katz result is certified on the graph its session
+    ran on (the one before the insert for queries done before it, the one
+    after for the rest), heat-kernel results equal the oracle's level push up
+    to FP32 rounding, and one query of each kind served alone afterwards
+    equals its result in the mix (PPR / Katz within the two certified
+    residuals, heat kernel within relative L1 1e-4)."""
+    g = large_graph
+    eps = 1e-7
+    srv = serving.Server(g)
+    try:
+        reqs = serving.mixed_requests(g, rate=300.0, count=40, seed=11, eps=eps)
+        mid = reqs[len(reqs) // 2].arrival
+        ins = serving.random_insert_batches(g, every=mid, size=10_000, count=1, seed=13)
+        st = srv.run(reqs, inserts=ins, collect=True)
+        print(f"\n  large mix: {st}")
+        assert st["ok"] == len(reqs) and st["certified"] and st["idle_exits"] == 0
+        assert st["inserts_applied"] == 1 and st["edges_inserted"] > 9_000
+        b = ins[0]
+        u = np.concatenate([g.indices.astype(np.int64), b.us, b.vs])
+        rows = np.concatenate([np.repeat(np.arange(g.n), g.degree), b.vs, b.us])
+        g2 = graph.CSRGraph.from_edges(g.n, rows, u)
+        assert int(srv.dg.degree.cpu().numpy().sum()) == g2.m
+        ops = {}
+
+        def residual(gg, r, x):
+            key = (id(gg), r.kind, r.alpha)
+            if key not in ops:
+                ops[key] = O.ppr_operator(gg, r.alpha) if r.kind == _abi.FLSG_PPR else None
+            if r.kind == _abi.FLSG_PPR:
+                bb = np.zeros(gg.n)
+                bb[r.seed] = r.alpha
+                return bb - ops[key] @ x
+            return _katz_residual(gg, r.seed, r.alpha, x)
+
+        before = [r for r in reqs if r.done < b.applied]
+        after = [r for r in reqs if r.done >= b.applied]
+        assert before and after
+        n_hk = 0
+        for r in reqs:
+            gg = g if r.done < b.applied else g2
+            x = _dense(r, g.n)
+            if r.kind == _abi.FLSG_HK:
+                if n_hk < 2:
+                    o = O.hk_push(gg, r.seed, 5.0, 1e-7, local=True, n_terms=40)
+                    assert np.abs(x - o["x"]).sum() <= 1e-4 * np.abs(o["x"]).sum()
+                    n_hk += 1
+                continue
+            rr = residual(gg, r, x)
+            thr = eps * (r.alpha if r.kind == _abi.FLSG_PPR else 1.0) * np.maximum(gg.degree, 1)
+            assert (np.abs(rr) / thr).max() < 1 + 1e-9, (r.kind, r.seed)
+        # alone (on the final graph) vs in the mix
+        for k in (_abi.FLSG_PPR, _abi.FLSG_KATZ, _abi.FLSG_HK):
+            r = next((q for q in after if q.kind == k), None)
+            if r is None:
+                continue
+            alone = serving.Request(rid=0, kind=r.kind, seed=r.seed, arrival=0.0, alpha=r.alpha, eps=r.eps)
+            srv.run([alone], collect=True)
+            a, bx = _dense(alone, g.n), _dense(r, g.n)
+            if k == _abi.FLSG_HK:
+                assert np.abs(a - bx).sum() <= 1e-4 * np.abs(a).sum()
+            else:
+                ra = np.abs(residual(g2, r, a)).sum()
+                rb = np.abs(residual(g2, r, bx)).sum()
+                scale = r.alpha if k == _abi.FLSG_PPR else 1.0 - r.alpha * g2.d_max
+                assert np.abs(a - bx).sum() <= (ra + rb) / scale * (1 + 1e-9) + 1e-15
+        assert engine.arena_dirty(srv.dg) == 0
     finally:
         srv.close()
