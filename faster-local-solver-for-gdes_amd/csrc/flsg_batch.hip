@@ -276,7 +276,8 @@ struct BatchArgs {
     // launch runs; *closed != 0 ends it once every published query is claimed;
     // done[q] is set (system scope) when query q's record and output are
     // written; a workgroup that sees nothing published for idle_ticks (100 MHz)
-    // exits on its own and counts itself in ctl[2]
+    // exits on its own and counts itself in ctl[2] (ctl[3], ctl[4]: see
+    // session_claim)
     const uint64_t *pub;
     const uint32_t *closed;
     uint32_t *done;
@@ -1465,34 +1466,64 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
 
 // Serving session: the next query index for this workgroup (thread 0).  The
 // index is claimed at once; the workgroup then waits until the host has
-// published it (*pub > c) -- or, once the host has closed the session with c
+// published it -- or, once the host has closed the session with the index
 // still unpublished, or after idle_ticks with nothing published (then every
 // workgroup of the launch ends: ctl[2] != 0), returns a value >= nq, which
 // ends the workgroup.  Every wait is bounded.
+// Only the workgroup waiting for the first unpublished index reads the host's
+// count and close flag (each read crosses PCIe); it mirrors them into device
+// memory (ctl[3]: published count, ctl[4]: closed), where the other waiting
+// workgroups read them.
 __device__ __forceinline__ unsigned int session_claim(const BatchArgs &a) {
     unsigned long long *expired = (unsigned long long *)&a.ctl[2];
+    unsigned long long *mpub = (unsigned long long *)&a.ctl[3];
+    unsigned long long *mclosed = (unsigned long long *)&a.ctl[4];
+    auto dload = [](unsigned long long *p) {
+        return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    };
     // once one workgroup has given up waiting, indices it claimed may be
     // published and never run: every workgroup ends, and the host re-submits
     // whatever was published but not done in a new launch
-    if (__hip_atomic_load(expired, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) return 0xffffffffu;
+    if (dload(expired) != 0ull) return 0xffffffffu;
     const unsigned int c = atomicAdd((unsigned int *)&a.ctl[0], 1u);
     if ((int64_t)c >= a.nq) return c;   // past this session's query capacity
     const uint64_t t0 = now_ticks();
-    // each poll of host memory crosses PCIe: the count is read every ~4 us
-    // (s_sleep 127 = 8128 clocks), the close flag every 8th round
     for (unsigned int it = 0;; it++) {
-        if (__hip_atomic_load(expired, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) return 0xffffffffu;
-        if ((uint64_t)c < __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM)) return c;
-        if ((it & 7u) == 7u && __hip_atomic_load(a.closed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
-            // the host publishes before it closes: read the count once more
-            if ((uint64_t)c < __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM)) return c;
+        if (dload(expired) != 0ull) return 0xffffffffu;
+        unsigned long long p = dload(mpub);
+        if ((uint64_t)c < p) return c;
+        if ((uint64_t)c == p) {
+            // the poller: host count (and, every 8th round, the close flag)
+            const unsigned long long hp = __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (hp > p) {
+                __hip_atomic_fetch_max(mpub, hp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                continue;
+            }
+            if ((it & 7u) == 7u && __hip_atomic_load(a.closed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+                // the host publishes before it closes: read the count once more
+                const unsigned long long hp2 =
+                    __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_fetch_max(mpub, hp2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(mclosed, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint64_t)c < hp2) return c;
+                return 0xffffffffu;
+            }
+        } else if (dload(mclosed) != 0ull) {
+            // closed: the mirror was raised to the final count before the flag
+            if ((uint64_t)c < dload(mpub)) return c;
             return 0xffffffffu;
         }
         if (now_ticks() - t0 > a.idle_ticks) {
             atomicAdd(expired, 1ull);
             return 0xffffffffu;
         }
+        // the poller reads every ~3.4 us (s_sleep 127 = 8128 clocks), the
+        // others the device mirror every ~10 us
         __builtin_amdgcn_s_sleep(127);
+        if ((uint64_t)c != p) {
+            __builtin_amdgcn_s_sleep(127);
+            __builtin_amdgcn_s_sleep(127);
+        }
     }
 }
 
@@ -2265,7 +2296,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     if (rc) return rc;
     rc = undirected_ensure(g, st);
     if (rc) return rc;
-    FLSG_HIP(hipMemsetAsync(d_ctl, 0, sizeof(uint64_t) * (sw ? 3 : 2), st));
+    FLSG_HIP(hipMemsetAsync(d_ctl, 0, sizeof(uint64_t) * (sw ? 5 : 2), st));
     BatchArgs a{};
     if (sw) {
         a.pub = sw->pub;
@@ -2593,7 +2624,8 @@ struct flsg_session_s {
     uint32_t *d_done = nullptr;
     uint64_t *d_pub = nullptr;
     uint32_t *d_closed = nullptr;
-    uint64_t *d_ctl = nullptr;       // device: [0] claims, [1] output entries used, [2] idle exits
+    uint64_t *d_ctl = nullptr;       // device: [0] claims, [1] output entries used, [2] idle exits,
+                                     // [3] / [4] device mirrors of the published count / close flag
     int32_t *d_ids = nullptr;        // output arena (device)
     double *d_vals = nullptr;
     int running = 0;                 // a launch was enqueued and has not been seen to end
@@ -2669,7 +2701,7 @@ extern "C" int flsg_session_create(flsg_graph g, int64_t query_cap, int64_t out_
     if (!rc) chk(host_coherent(&s->h_done, &s->d_done, query_cap));
     if (!rc) chk(host_coherent(&s->h_pub, &s->d_pub, 1));
     if (!rc) chk(host_coherent(&s->h_closed, &s->d_closed, 1));
-    if (!rc && hipMalloc((void **)&s->d_ctl, sizeof(uint64_t) * 3) != hipSuccess) chk(fail(FLSG_ENOMEM, "ctl"));
+    if (!rc && hipMalloc((void **)&s->d_ctl, sizeof(uint64_t) * 5) != hipSuccess) chk(fail(FLSG_ENOMEM, "ctl"));
     if (!rc && out_cap > 0) {
         if (hipMalloc((void **)&s->d_ids, sizeof(int32_t) * out_cap) != hipSuccess ||
             hipMalloc((void **)&s->d_vals, sizeof(double) * out_cap) != hipSuccess)

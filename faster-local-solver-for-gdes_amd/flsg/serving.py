@@ -216,6 +216,7 @@ class Server:
         queries as they finish), then copy the outputs out if asked and
         re-queue whatever was published but not done (an idle exit)."""
         _check(self.lib.flsg_session_close(self.sess))
+        t_close = time.perf_counter()
         finished = 0
         ended = C.c_int32(0)
         idle = C.c_int64(0)
@@ -225,6 +226,7 @@ class Server:
             if ended.value:
                 break
             time.sleep(0)
+        self.drain_s += time.perf_counter() - t_close
         _check(self.lib.flsg_session_poll(self.sess, C.byref(ended), C.byref(idle)))
         finished += self._harvest((time.perf_counter() - t0) / time_scale, requeue)
         for i in range(self.first_open, len(self.ring)):
@@ -276,6 +278,7 @@ class Server:
         submits = 0
         idle_exits = 0
         t0 = time.perf_counter()
+        self.drain_s = 0.0
         self._open()
         sessions += 1
         try:
@@ -335,6 +338,7 @@ class Server:
         st = self._stats(order, wall, sessions, inserts)
         st["submits"] = submits
         st["idle_exits"] = idle_exits
+        st["drain_s"] = self.drain_s
         return st
 
     @staticmethod
@@ -352,7 +356,18 @@ class Server:
             "certified": certified,
             "inserts_applied": sum(1 for b in inserts if b.applied >= 0),
             "edges_inserted": sum(b.inserted for b in inserts),
+            "by_kind": {name: Server._kind_stats([r for r in order if r.kind == k])
+                        for k, name in ((_abi.FLSG_PPR, "ppr"), (_abi.FLSG_KATZ, "katz"), (_abi.FLSG_HK, "hk"))},
         }
+
+    @staticmethod
+    def _kind_stats(rs: list) -> dict:
+        lat = np.array([r.done - r.arrival for r in rs if r.done >= 0])
+        dev = np.array([r.device_ms for r in rs if r.done >= 0])
+        if not lat.size:
+            return {"n": 0}
+        return {"n": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "device_ms_p50": float(np.percentile(dev, 50))}
 
     def close(self):
         if self.sess:
@@ -452,6 +467,7 @@ if __name__ == "__main__":
     ap.add_argument("--max-queries", type=int, default=4000)
     ap.add_argument("--plain-queries", type=int, default=600)
     ap.add_argument("--out", default=None, help="JSON-lines file for the results")
+    ap.add_argument("--precisions", default="fp32,fp64", help="batched paths to run: fp32, fp64 or both")
     args = ap.parse_args()
     fh = open(args.out, "w") if args.out else None
 
@@ -462,8 +478,9 @@ if __name__ == "__main__":
             fh.write(line + "\n")
             fh.flush()
 
+    precs = tuple({"fp32": _abi.FLSG_FP32, "fp64": _abi.FLSG_FP64}[p] for p in args.precisions.split(",") if p)
     serving_bench(rates=tuple(int(r) for r in args.rates.split(",")), seconds=args.seconds,
-                  max_queries=args.max_queries, plain_queries=args.plain_queries, log=_log)
+                  max_queries=args.max_queries, plain_queries=args.plain_queries, precisions=precs, log=_log)
     if fh:
         fh.close()
     sys.exit(0)
