@@ -27,17 +27,18 @@ for _p in (PKG, REPO):
         sys.path.insert(0, _p)
 
 # algorithmic bytes of the batched FP32 path, per unit (the model the
-# roofline's "achieved" uses):
-#   push, per frontier entry: 4 queue id + 16 record read (r, x, d) + 8 record
-#     write (r, x) + 4 row start + 4 next-frontier id                     = 36
-#   push, per edge: 8 column entry {v, d_v} + 4 r_v read + 4 r_v write    = 16
-#   certificate, per pulled column entry: 8 column entry + 4 x32_u read  = 12
-#     (the few correction-push edges, 24 B each, are counted at 12)
-#   certificate, per support record: 8 list entry (id, row start) + 16
-#     record read + 32 record write (r64, x64, flags)                     = 56
-#   per touched record: 16 read (r64, x) + 16 reset write                 = 32
-#   per output entry: 4 id + 8 value                                       = 12
-B_NODE_FP32, B_EDGE_FP32, B_CERT_EDGE, B_CERT_ROW, B_TOUCHED, B_OUT = 36, 16, 12, 56, 32, 12
+# roofline's "achieved" uses -- the contract's per-unit figures for the FP32
+# push and its FP64 certificate):
+#   push, per frontier entry (node phase): 8 row bounds + 4 r_u read + 4 r_u
+#     write + 8 x_u read-modify-write + 4 d_u + 4 frontier id read + 4
+#     next-frontier id write                                               = 36
+#   push, per edge: 4 column index + 8 r_v read-modify-write + 4 d_v       = 16
+#   certificate, per edge of the support's rows (FP64 residual recompute
+#     of every support node): 4 column index + 4 x_v gather                 =  8
+#   certificate, per support node: 8 row bounds + 4 x_u + 4 d_u + 8 r_u
+#     write                                                                 = 24
+#   correction push (FP64): 52 per pushed node, 24 per edge
+B_NODE_FP32, B_EDGE_FP32, B_CERT_EDGE, B_CERT_ROW, B_CORR_NODE, B_CORR_EDGE = 36, 16, 8, 24, 52, 24
 HBM_PEAK_GBS = 8000.0
 
 
@@ -193,11 +194,18 @@ def main():
 
     oper = int(res["oper"].sum())
     nodes = int(res["nodes"].sum())
-    cert_edges = int(res["oper_cert"].sum())
     supp = int(res["supp"].sum())
-    touched = int(res["touched"].astype(np.int64).sum())
-    bytes_step = (B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * cert_edges + B_CERT_ROW * supp
-                  + B_TOUCHED * touched + B_OUT * supp)
+    n_corr = int(res["n_corr"].sum())
+    corr_edges = int(res["corr_edges"].sum())
+    # volume of every query's support (its ids are caller labels), summed on the device
+    deg_dev = torch.from_numpy(np.ascontiguousarray(g.degree, dtype=np.int64)).to(last.ids.device)
+    used = int((last.offsets.astype(np.int64) + res["supp"].astype(np.int64)).max())
+    supp_vol = int(deg_dev[last.ids[:used].long()].sum().item())
+    del deg_dev
+    bytes_step = (B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * supp_vol + B_CERT_ROW * supp
+                  + B_CORR_NODE * n_corr + B_CORR_EDGE * corr_edges)
+    log(f"algorithmic bytes per launch: {bytes_step / 1e9:.1f} GB (frontier entries {nodes}, push edges {oper}, "
+        f"support nodes {supp}, support volume {supp_vol}, correction nodes {n_corr} / edges {corr_edges})")
     kern_ms = float(np.mean(kern))
     achieved = bytes_step / (kern_ms * 1e-3) / 1e9
 

@@ -1652,7 +1652,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int32_t *qn = qc == qa ? qb : qa;
         const unsigned long long oper = s_vol;
         const uint64_t t_push = now_ticks();
-        unsigned long long cert_ops = 0;
+        unsigned long long cert_ops = 0, corr_ops = 0;
         unsigned long long n_corr = 0;
         uint64_t t_rec = t_push, t_corr = t_push;
         auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
@@ -1861,6 +1861,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 if (pushed) atomicAdd(&s_vol, pushed);
                 cert_ops += cvol;
+                corr_ops += cvol;
                 __syncthreads();
                 n_corr += s_vol;
                 wave_edges<double>(
@@ -2014,6 +2015,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         cmax = block_max<NT>(cmax, s_redd);
         rl1 = block_sum_f64<NT>(rl1, s_redd);
         const unsigned long long cops = block_sum_u64<NT>(cert_ops, s_red);
+        const unsigned long long kops = block_sum_u64<NT>(corr_ops, s_red);
         if (tid == 0) {
             const uint64_t t_end = now_ticks();
             flsg_result rr{};
@@ -2036,6 +2038,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             rr.ms_bins = (double)tick[2] * 1e-5;
             rr.ms_direct = (double)tick[3] * 1e-5;
             rr.bin_iters = (int64_t)tick[4];
+            rr.corr_edges = (int64_t)kops;
             a.res[q] = rr;
             a.out_off[q] = fits ? (int64_t)obase : -1;
         }

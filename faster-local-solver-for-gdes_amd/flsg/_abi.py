@@ -30,7 +30,7 @@ class Result(C.Structure):
         ("ms_push", C.c_double), ("status", C.c_int32), ("touched", C.c_int32),
         ("ms_recompute", C.c_double), ("ms_correct", C.c_double), ("r_l1", C.c_double),
         ("ms_node", C.c_double), ("ms_pairs", C.c_double), ("ms_bins", C.c_double), ("ms_direct", C.c_double),
-        ("bin_iters", C.c_int64),
+        ("bin_iters", C.c_int64), ("corr_edges", C.c_int64),
     ]
 
     def as_dict(self) -> dict:

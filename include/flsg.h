@@ -97,6 +97,7 @@ typedef struct {
     double ms_bins;      /* binned iterations: the ranges' sums and adds            */
     double ms_direct;    /* directly pushed iterations: the edge phases            */
     int64_t bin_iters;   /* iterations pushed binned                               */
+    int64_t corr_edges;  /* batched FP32: edges of the FP64 correction push (part of oper_cert) */
 } flsg_result;
 
 /* Optional per-iteration frontier trace (device buffers): iteration t

@@ -28,7 +28,7 @@ def test_library_exports_every_symbol():
 
 def test_struct_layouts():
     assert ctypes.sizeof(_abi.Params) == 4 * 4 + 5 * 8 + 2 * 8
-    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 3 * 8 + 2 * 4 + 3 * 8 + 5 * 8
+    assert ctypes.sizeof(_abi.Result) == 6 * 8 + 3 * 8 + 2 * 4 + 3 * 8 + 5 * 8 + 8
     assert ctypes.sizeof(_abi.Query) == 8 + 2 * 4 + 2 * 8
     assert ctypes.sizeof(_abi.Trace) == 5 * 8 + 2 * 4
 
