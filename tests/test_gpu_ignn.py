@@ -174,9 +174,13 @@ def test_ignn_update_matches_oracle_update():
         assert np.abs(P1[:, c] - p - K @ (r - R1[:, c])).max() < 1e-10
 
 
+@pytest.mark.slow
 def test_ignn_128_columns_full_size():
-    """169,343 nodes, 128 columns, eps = 1e-7: initial propagation plus two
-    snapshots, each certified in FP64 and checked against the host recompute."""
+    """169,343 nodes, 128 columns, eps = 1e-7: the initial propagation and all
+    16 insert snapshots of the workload (59,375 pairs each).  After every
+    snapshot: indices, degree and the insert_neighbor key order equal the
+    oracle's updetaGraph byte for byte, and P / R are certified in FP64 and
+    equal to the host recompute."""
     sg = graph.citation_dynamic_graph()
     X = _features(sg.n, 128, seed=3)
     eps = 1e-7
@@ -184,7 +188,13 @@ def test_ignn_128_columns_full_size():
     st = prop.initial()
     assert st.cert < 1.0
     _certify(prop, eps, X)
-    for k in range(2):
+    ind_o = sg.indices.copy()
+    deg_o = sg.degree.copy()
+    for k in range(sg.us.shape[0]):
+        keys, _, _ = O.update_graph(sg.indptr, ind_o, deg_o, sg.us[k], sg.vs[k])
         st = prop.insert(sg.us[k], sg.vs[k])
-        assert st.cert < 1.0 and st.inserted == sg.us.shape[1]
-    _certify(prop, eps, X)
+        assert st.cert < 1.0 and st.inserted == sg.us.shape[1], k
+        ind, deg = prop.graph_arrays()
+        assert np.array_equal(deg, deg_o) and np.array_equal(ind, ind_o), k
+        assert np.array_equal(prop.keys[:keys.size].cpu().numpy(), keys), k
+        _certify(prop, eps, X)
