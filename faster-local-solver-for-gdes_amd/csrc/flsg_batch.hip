@@ -45,6 +45,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -1556,7 +1557,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ WaveLds s_w[NT / kWave];
     __shared__ unsigned long long s_red[NT / kWave];
     __shared__ double s_redd[NT / kWave];
-    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl, s_ex, s_nf;
+    __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl, s_ex, s_nf, s_dirty;
     __shared__ unsigned long long s_base, s_vol, s_ivol, s_fvol;
     __shared__ unsigned int s_bins[kMaxBins];   // binned push: write cursors, then ends of the pair regions
     // dynamic LDS: the hub cache (residuals of nodes [0, H) for the whole
@@ -1757,6 +1758,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             __syncthreads();
             const unsigned int E32 = s_ex;
             const unsigned int NLIST = E32 + s_ncnt;
+            // every entry whose flags the certificate sets is listed in
+            // nl[0, s_dirty): the exact support rows nl[0, E32), then the
+            // other exact rows and the entries the correction push reaches
+            // first (at most the touched entries, so at most n)
+            if (tid == 0) s_dirty = E32;
+            __syncthreads();
             // a listed entry, finished: r64 = b - x32 + (pulled pushes) in FP64,
             // x64 = 0 (x64 holds the correction push's FP64 additions to x32),
             // FP32 residual field -0.0, flags kExact | kR64, queued for the
@@ -1771,6 +1778,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     S.rx64[v] = make_double2(rt, 0.0);
                     S.rx32[v].x = -0.0f;
                     S.fl[v] = kExact | kR64 | (enq ? kInQ : 0u);
+                    // a boundary row (never pushed) is not in nl[0, E32)
+                    if (pristine(x32v)) nl[atomicAdd(&s_dirty, 1u)] = v;
                 }
                 const long long sl = wave_append(enq, &s_cnt);
                 if (sl >= 0) qc[sl] = v;
@@ -1880,6 +1889,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         const bool fresh = old == 0.0 && pristine(rl);
                         const uint32_t of = fl_or(&S.fl[vtx], kR64 | (fresh ? kTouched : 0u));
                         if (fresh && (of & kTouched) == 0) res = 2;
+                        if (of == 0u) nl[atomicAdd(&s_dirty, 1u)] = vtx;   // first flags of this entry
                         const double eff = nw + (rl > 0.0f ? (double)rl : 0.0);
                         const double bv = rec_bound(of, rl, rx.y, dv, iters, vtx == s ? P.r0 : 0.0);
                         if (fabs(eff) + bv >= P.thr_scale * (double)dv && (fl_or(&S.fl[vtx], kInQ) & kInQ) == 0)
@@ -1916,101 +1926,115 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         double cmax = 0.0, rl1 = 0.0;
         const bool stream = (int64_t)T * 4 > a.n;
         const unsigned int lim = stream ? (unsigned int)a.n : T;
-        for (unsigned int i0 = 0; i0 < lim; i0 += NT * kOutUnroll) {
-            int vv[kOutUnroll];
-            float r32v[kOutUnroll], x32v[kOutUnroll];
-            uint32_t flv[kOutUnroll];
-            double r64v[kOutUnroll], x64v[kOutUnroll];
-#pragma unroll
-            for (int j = 0; j < kOutUnroll; j++) {
-                const unsigned int i = i0 + j * NT + tid;
-                vv[j] = i < lim ? (stream ? (int)i : tl[i]) : -1;
-            }
-            // The FP64 push writes only r64 and x64, so there only those are read
-            // and cleared.  The FP32 push's entries hold {r32, x32} and flags;
-            // their FP64 fields are read (and cleared) only where the flags say
-            // the certificate wrote them (kR64).
-            if (P.fp64) {
+        // mode 0 (FP32): the entries the certificate flagged, nl[0, s_dirty):
+        //   flags read, FP64 fields read where kR64; every one reset here;
+        // mode 1 (FP32): the rest, streamed or from the touched list, with no
+        //   flags to read (every flagged entry was listed); entries whose
+        //   {r32, x32} is all zero bits are untouched or already done;
+        // mode 2 (FP64 push): only {r64, x64} are read and cleared.
+        auto out_pass = [&](auto mode_c, const unsigned int cnt) {
+            constexpr int mode = decltype(mode_c)::value;
+            for (unsigned int i0 = 0; i0 < cnt; i0 += NT * kOutUnroll) {
+                int vv[kOutUnroll];
+                float r32v[kOutUnroll], x32v[kOutUnroll];
+                uint32_t flv[kOutUnroll];
+                double r64v[kOutUnroll], x64v[kOutUnroll];
 #pragma unroll
                 for (int j = 0; j < kOutUnroll; j++) {
-                    r32v[j] = 0.0f;
-                    x32v[j] = 0.0f;
-                    flv[j] = 0u;
-                    r64v[j] = 0.0;
-                    x64v[j] = 0.0;
-                    if (vv[j] >= 0) {
-                        const double2 w = S.rx64[vv[j]];
-                        r64v[j] = w.x;
-                        x64v[j] = w.y;
+                    const unsigned int i = i0 + j * NT + tid;
+                    vv[j] = i < cnt ? (mode == 0 ? nl[i] : stream ? (int)i : tl[i]) : -1;
+                }
+                if (mode == 2) {
+#pragma unroll
+                    for (int j = 0; j < kOutUnroll; j++) {
+                        r32v[j] = 0.0f;
+                        x32v[j] = 0.0f;
+                        flv[j] = 0u;
+                        r64v[j] = 0.0;
+                        x64v[j] = 0.0;
+                        if (vv[j] >= 0) {
+                            const double2 w = S.rx64[vv[j]];
+                            r64v[j] = w.x;
+                            x64v[j] = w.y;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kOutUnroll; j++) {
+                        r32v[j] = 0.0f;
+                        x32v[j] = 0.0f;
+                        flv[j] = 0u;
+                        if (vv[j] >= 0) {
+                            const float2 w = S.rx32[vv[j]];
+                            r32v[j] = w.x;
+                            x32v[j] = w.y;
+                            if (mode == 0) flv[j] = S.fl[vv[j]];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < kOutUnroll; j++) {
+                        r64v[j] = 0.0;
+                        x64v[j] = 0.0;
+                        if (flv[j] & kR64) {
+                            const double2 w = S.rx64[vv[j]];
+                            r64v[j] = w.x;
+                            x64v[j] = w.y;
+                        }
                     }
                 }
-            } else {
+                // one entry at a time: certificate value, reset, then its output
+                // slot (every lane runs the wave append), so only the entries still
+                // to come stay live across the bound's call
 #pragma unroll
                 for (int j = 0; j < kOutUnroll; j++) {
-                    r32v[j] = 0.0f;
-                    x32v[j] = 0.0f;
-                    flv[j] = 0u;
-                    if (vv[j] >= 0) {
-                        const float2 w = S.rx32[vv[j]];
-                        r32v[j] = w.x;
-                        x32v[j] = w.y;
-                        flv[j] = S.fl[vv[j]];
+                    bool is_sup = false;
+                    int id = vv[j];
+                    // FP32 push: x32 plus the correction push's FP64 additions
+                    const double outv = mode == 2 ? x64v[j] : (double)x32v[j] + x64v[j];
+                    bool live = vv[j] >= 0;
+                    if (live && mode == 1)
+                        live = (__float_as_uint(r32v[j]) | __float_as_uint(x32v[j])) != 0u;
+                    else if (live && mode == 2 && stream)
+                        live = !(pristine(r64v[j]) && pristine(x64v[j]));
+                    if (live) {
+                        const int v = vv[j];
+                        const int d = degree[v];
+                        // FP32 push: an entry may hold its residual in FP32 (pending,
+                        // rl > 0) and carries the error bound of its FP32 values
+                        const float rl = r32v[j];
+                        const double res = r64v[j] + (rl > 0.0f ? (double)rl : 0.0);
+                        const double bnd =
+                            mode == 2 ? 0.0 : rec_bound_ool(flv[j], rl, x32v[j], d, iters, v == s ? P.r0 : 0.0);
+                        if (res != 0.0 || bnd != 0.0) {
+                            cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
+                            rl1 += fabs(res) + bnd;
+                        }
+                        // the support: FP32-pushed nodes (x32 > 0) and those the
+                        // correction push pushed first (kSeen)
+                        is_sup = mode == 2 ? x64v[j] != 0.0 : (x32v[j] != 0.0f || (flv[j] & kSeen) != 0u);
+                        if (is_sup && a.out_map) id = a.out_map[v];
+                        if (mode != 2) {
+                            S.rx32[v] = make_float2(0.0f, 0.0f);
+                            if (flv[j] != 0u) S.fl[v] = 0u;
+                            if (flv[j] & kR64) S.rx64[v] = make_double2(0.0, 0.0);
+                        } else {
+                            S.rx64[v] = make_double2(0.0, 0.0);
+                        }
                     }
-                }
-#pragma unroll
-                for (int j = 0; j < kOutUnroll; j++) {
-                    r64v[j] = 0.0;
-                    x64v[j] = 0.0;
-                    if (flv[j] & kR64) {
-                        const double2 w = S.rx64[vv[j]];
-                        r64v[j] = w.x;
-                        x64v[j] = w.y;
+                    const long long k = wave_append(is_sup, &s_ncnt);
+                    if (k >= 0 && fits && (unsigned long long)k < Sn) {
+                        a.out_ids[obase + k] = id;
+                        a.out_vals[obase + k] = outv;
                     }
                 }
             }
-            // one entry at a time: certificate value, reset, then its output
-            // slot (every lane runs the wave append), so only the entries still
-            // to come stay live across the bound's call
-#pragma unroll
-            for (int j = 0; j < kOutUnroll; j++) {
-                bool is_sup = false;
-                int id = vv[j];
-                // FP32 push: x32 plus the correction push's FP64 additions
-                const double outv = P.fp64 ? x64v[j] : (double)x32v[j] + x64v[j];
-                const bool live = vv[j] >= 0 &&
-                                  !(stream && (__float_as_uint(r32v[j]) | __float_as_uint(x32v[j]) | flv[j]) == 0u &&
-                                    pristine(r64v[j]) && pristine(x64v[j]));   // else untouched: nothing to do
-                if (live) {
-                    const int v = vv[j];
-                    const int d = degree[v];
-                    // FP32 push: an entry may hold its residual in FP32 (pending,
-                    // rl > 0) and carries the error bound of its FP32 values
-                    const float rl = r32v[j];
-                    const double res = r64v[j] + (rl > 0.0f ? (double)rl : 0.0);
-                    const double bnd =
-                        P.fp64 ? 0.0 : rec_bound_ool(flv[j], rl, x32v[j], d, iters, v == s ? P.r0 : 0.0);
-                    if (res != 0.0 || bnd != 0.0) {
-                        cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
-                        rl1 += fabs(res) + bnd;
-                    }
-                    // the support: FP32-pushed nodes (x32 > 0) and those the
-                    // correction push pushed first (kSeen)
-                    is_sup = P.fp64 ? x64v[j] != 0.0 : (x32v[j] != 0.0f || (flv[j] & kSeen) != 0u);
-                    if (is_sup && a.out_map) id = a.out_map[v];
-                    if (!P.fp64) {
-                        S.rx32[v] = make_float2(0.0f, 0.0f);
-                        if (flv[j] != 0u) S.fl[v] = 0u;
-                        if (flv[j] & kR64) S.rx64[v] = make_double2(0.0, 0.0);
-                    } else {
-                        S.rx64[v] = make_double2(0.0, 0.0);
-                    }
-                }
-                const long long k = wave_append(is_sup, &s_ncnt);
-                if (k >= 0 && fits && (unsigned long long)k < Sn) {
-                    a.out_ids[obase + k] = id;
-                    a.out_vals[obase + k] = outv;
-                }
-            }
+        };
+        if (P.fp64) {
+            out_pass(std::integral_constant<int, 2>{}, lim);
+        } else {
+            out_pass(std::integral_constant<int, 0>{}, s_dirty);
+            __syncthreads();   // the listed entries are reset before the main pass skips them
+            out_pass(std::integral_constant<int, 1>{}, lim);
         }
         cmax = block_max<NT>(cmax, s_redd);
         rl1 = block_sum_f64<NT>(rl1, s_redd);
