@@ -204,8 +204,10 @@ def main():
     del deg_dev
     bytes_step = (B_NODE_FP32 * nodes + B_EDGE_FP32 * oper + B_CERT_EDGE * supp_vol + B_CERT_ROW * supp
                   + B_CORR_NODE * n_corr + B_CORR_EDGE * corr_edges)
+    pulled = int(res["oper_cert"].sum()) - corr_edges
     log(f"algorithmic bytes per launch: {bytes_step / 1e9:.1f} GB (frontier entries {nodes}, push edges {oper}, "
-        f"support nodes {supp}, support volume {supp_vol}, correction nodes {n_corr} / edges {corr_edges})")
+        f"support nodes {supp}, support volume {supp_vol}, correction nodes {n_corr} / edges {corr_edges}); "
+        f"the certificate pulled {pulled} row entries ({pulled / max(supp_vol, 1):.4f} of the support volume)")
     kern_ms = float(np.mean(kern))
     achieved = bytes_step / (kern_ms * 1e-3) / 1e9
 
