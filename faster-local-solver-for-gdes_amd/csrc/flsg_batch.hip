@@ -67,11 +67,15 @@ constexpr int kBinShiftMin = 11;     // binned push: at least 2^11 ids per desti
 constexpr int kMaxBins = 512;        // binned push: at most this many destination ranges (static LDS)
 
 // Boundary certificate of the FP32 push.  A node the push reached but never
-// pushed (a boundary node) holds r32, the FP32 sum of the m <= d * iters push
-// values it received (undirected graph: a node's in-degree is its degree d;
-// each neighbour pushes at most once per iteration).  Against the FP64 residual
-// of the returned x -- the pushed nodes' x32 -- r32 is off by at most
-//     g(d, iters) * r32 + (d * iters + 1) * 2^-125,
+// pushed (a boundary node) holds r32, the FP32 sum of the m push values it
+// received.  m <= d * iters (undirected graph: a node's in-degree is its
+// degree d; each neighbour pushes at most once per iteration), and on a graph
+// with no repeated entry in a row also m <= F, the query's pushes (frontier
+// entries over all iterations): a push of u adds one value to each entry of
+// u's row, so at most one to v.  adds_max(d, iters, F) is that minimum (F =
+// +inf where rows may repeat an entry).  Against the FP64 residual of the
+// returned x -- the pushed nodes' x32 -- r32 is off by at most
+//     g(m, iters) * r32 + (m + 1) * 2^-125,
 // with u = 2^-24, u' = u + 2^-51, gm(k) = 2ku / (1 - 2ku) and
 //     g = (gm(iters) + u' + gm(m) + u' gm(m)) / (1 - u' - gm(m) - u' gm(m)),
 // because, writing a_j = c_u rho for the exact push terms (rho the FP32
@@ -98,25 +102,29 @@ __host__ __device__ __forceinline__ double lazy_gm(double k) {
     return t / (1.0 - t);
 }
 
-__host__ __device__ __forceinline__ double lazy_g(int d, int64_t it) {
+// the most push values a node of degree d can have received (see above)
+__host__ __device__ __forceinline__ double adds_max(int d, int64_t it, double F) {
+    return fmin((double)d * (double)it, F);
+}
+
+__host__ __device__ __forceinline__ double lazy_g(double m, int64_t it) {
     const double up = 0x1p-24 + 0x1p-51;
-    const double gm = lazy_gm((double)d * (double)it);
+    const double gm = lazy_gm(m);
     const double eta = up + gm + up * gm;
     return (lazy_gm((double)it) + eta) / (1.0 - eta);
 }
 
-// where lazy_g is used: 2 m u <= 1/4 (m = d iters)
-__host__ __device__ __forceinline__ bool lazy_valid(int d, int64_t it) {
-    return 2.0 * (double)d * (double)it * 0x1p-24 <= 0.25;
-}
+// where lazy_g is used: 2 m u <= 1/4
+__host__ __device__ __forceinline__ bool lazy_valid(double m) { return 2.0 * m * 0x1p-24 <= 0.25; }
 
 // error bound of an entry whose FP32 residual field holds rl: > 0 a boundary
 // residual still pending, < 0 one the correction push has moved into x (its
 // error stays), +-0 an exact (FP64) entry
-__host__ __device__ __forceinline__ double lazy_bound(float rl, int d, int64_t it) {
+// (m: adds_max of the entry)
+__host__ __device__ __forceinline__ double lazy_bound(float rl, double m, int64_t it) {
     if (rl == 0.0f) return 0.0;
-    if (!lazy_valid(d, it)) return __builtin_inf();   // never certifies
-    return lazy_g(d, it) * fabs((double)rl) + ((double)d * (double)it + 1.0) * 0x1p-125;
+    if (!lazy_valid(m)) return __builtin_inf();   // never certifies
+    return lazy_g(m, it) * fabs((double)rl) + (m + 1.0) * 0x1p-125;
 }
 
 // Error bound of a pushed node v certified without the FP64 pull.  The true
@@ -128,24 +136,26 @@ __host__ __device__ __forceinline__ double lazy_bound(float rl, int d, int64_t i
 //   db_v  = fl32(b_v) - b_v                       |db| <= 2u b
 //   dx_v  = x32_v - (sum of the r moved out of v)   |dx_v| <= 2uK x32_v (1 + 4uK)
 //   E_v   = received push values - sum_u c_u (moved out of u)
-//   H_v   = rounding of every add into r32_v    |H_v| <= 2u d K max(x32_v(1+4uK), r32_v) + d K 2^-126
-// (at most d K adds: one per received push value, a bin's summed add
-// included), and |E_v| + sum_u c_u |dx_u| <= (2K + 1.01) u (1 + 4uK) S_v,
+//   H_v   = rounding of every add into r32_v    |H_v| <= 2u m max(x32_v(1+4uK), r32_v) + m 2^-126
+// (at most m = adds_max(d, K, F) adds: one per received push value, a bin's
+// summed add included; every partial sum of r32_v is at most the residual the
+// node next moves into x32_v, or its final r32_v), and |E_v| + sum_u c_u |dx_u| <= (2K + 1.01) u (1 + 4uK) S_v,
 // while S_v = r_v - b_v + x32_v <= |r32_v| + B + x32_v -- the residual
 // identity itself bounds the neighbour sum, so no gather is needed.  Solving
 // for B:
-//   B (1 - g) <= 2u b + 2uK x (1+4uK) + g (|r32| + x) + 2u d K max(x(1+4uK), |r32|) + d K 2^-126,
+//   B (1 - g) <= 2u b + 2uK x (1+4uK) + g (|r32| + x) + 2u m max(x(1+4uK), |r32|) + m 2^-126,
 //   g = (2K + 1.01) u (1 + 4uK),
 // returned with a 1.01 margin.  It is small next to eps*alpha*d_v unless
-// x32_v is large (about x32 > 2e-3 at eps 1e-7 and K = 45): the seed and
-// its near neighbourhood stay in the FP64 pull.
-__host__ __device__ __forceinline__ double sup_bound(float x32, float rl, int d, int64_t K, double b) {
+// x32_v is large (about x32 > 2e-3 at eps 1e-7 and m = d K): the seed and
+// its near neighbourhood stay in the FP64 pull; the cap m <= F lets hubs, whose
+// d K far exceeds the query's pushes, keep their FP32 values.
+__host__ __device__ __forceinline__ double sup_bound(float x32, float rl, double m, int64_t K, double b) {
     const double u = 0x1p-24;
-    const double x = fabs((double)x32), r = fabs((double)rl), k = (double)K, dd = (double)d;
+    const double x = fabs((double)x32), r = fabs((double)rl), k = (double)K;
     const double c = 1.0 + 4.0 * u * k;
     const double g = (2.0 * k + 1.01) * u * c;
-    const double num = 2.0 * u * b + 2.0 * u * k * x * c + g * (r + x) + 2.0 * u * dd * k * fmax(x * c, r) +
-                       dd * k * 0x1p-126;
+    const double num = 2.0 * u * b + 2.0 * u * k * x * c + g * (r + x) + 2.0 * u * m * fmax(x * c, r) +
+                       m * 0x1p-126;
     return 1.01 * num / (1.0 - g);
 }
 
@@ -153,13 +163,15 @@ __host__ __device__ __forceinline__ double sup_bound(float x32, float rl, int d,
 // residual field rl (> 0: pending, < 0: moved into x64 by the correction push,
 // the bound staying that of |rl|), FP32 estimate x32 and b = b_v: 0 for an
 // exact entry (r64 recomputed in FP64), sup_bound for a pushed node (x32 > 0),
-// the boundary bound otherwise.  Pure function of the entry: the certificate,
-// the correction push and the output pass compute the same value.
+// the boundary bound otherwise; F caps the adds (adds_max).  Pure function of
+// the entry and the query's push: the certificate, the correction push and the
+// output pass compute the same value.
 __host__ __device__ __forceinline__ double rec_bound(uint32_t fl, float rl, float x32, int d, int64_t it,
-                                                     double b) {
+                                                     double b, double F) {
     if (fl & kExact) return 0.0;
-    if (x32 != 0.0f) return sup_bound(x32, rl, d, it, b);
-    return lazy_bound(rl, d, it);
+    const double m = adds_max(d, it, F);
+    if (x32 != 0.0f) return sup_bound(x32, rl, m, it, b);
+    return lazy_bound(rl, m, it);
 }
 
 // One slot's arrays (see the file header).
@@ -179,22 +191,26 @@ struct Strided {
     __device__ __forceinline__ T &operator[](int64_t i) const { return p[i * s]; }
 };
 
+// LDS add that stops at 2^32 - 1
+__device__ __forceinline__ void atomic_add_sat(unsigned int *p, unsigned int v) {
+    const unsigned int old = atomicAdd(p, v);
+    if (old + v < old) atomicMax(p, ~0u);
+}
+
 // Whether sup_bound certifies a pushed node {rl, x32}: |r32| + bound <
 // (1 - 2^-8) thr.  Such a node keeps its FP32 residual and estimate as they
 // are (nothing is written: the correction push and the output pass recompute
-// the same bound from the entry); the others join the FP64 pull.  Out of
-// line: the bound's FP64 arithmetic would otherwise raise the persistent
-// kernel's register pressure around its hot loops.
-__device__ __attribute__((noinline)) bool support_certified(float rl, float x32, int d, int64_t iters, double b,
-                                                            double thr_scale) {
-    const double bnd = sup_bound(x32, rl, d, iters, b);
+// the same bound from the entry); the others join the FP64 pull.
+__device__ __forceinline__ bool support_certified(float rl, float x32, int d, int64_t iters, double b,
+                                                            double thr_scale, double F) {
+    const double bnd = sup_bound(x32, rl, adds_max(d, iters, F), iters, b);
     return fabs((double)rl) + bnd < kNearFrac * thr_scale * (double)d;
 }
 
 // rec_bound out of line (the correction push and the output pass)
 __device__ __attribute__((noinline)) double rec_bound_ool(uint32_t fl, float rl, float x32, int d, int64_t it,
-                                                          double b) {
-    return rec_bound(fl, rl, x32, d, it, b);
+                                                          double b, double F) {
+    return rec_bound(fl, rl, x32, d, it, b, F);
 }
 
 // A frontier entry as its node phase leaves it for the edge phase: row start,
@@ -264,6 +280,7 @@ struct BatchArgs {
     int32_t hub_bytes;           // dynamic LDS: hub cache / bin slice (0 = none)
     int32_t slot_base;
     int32_t undirected;          // 1 if every edge (u, v) has its (v, u): FP32 certificate precondition
+    int32_t simple;              // 1 if no row repeats an entry: the FP32 bounds cap adds by the pushes
     uint64_t *ctl;               // [0] next query (low 32 bits used), [1] output entries used
     const int32_t *out_map;      // internal id -> caller id (NULL = identity)
     int32_t *out_ids;
@@ -1558,6 +1575,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ unsigned long long s_red[NT / kWave];
     __shared__ double s_redd[NT / kWave];
     __shared__ unsigned int s_q, s_cnt, s_ncnt, s_sup, s_tn, s_nlong, s_work, s_npc, s_tl, s_nl, s_ex, s_nf, s_dirty;
+    __shared__ unsigned int s_pvol;   // FP32 certificate: volume of the pulled rows (saturating)
     __shared__ unsigned long long s_base, s_vol, s_ivol, s_fvol;
     __shared__ unsigned int s_bins[kMaxBins];   // binned push: write cursors, then ends of the pair regions
     // dynamic LDS: the hub cache (residuals of nodes [0, H) for the whole
@@ -1655,6 +1673,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const uint64_t t_push = now_ticks();
         unsigned long long cert_ops = 0, corr_ops = 0;
         unsigned long long n_corr = 0;
+        // the adds cap of the FP32 error bounds: the query's pushes, on a
+        // graph whose rows repeat no entry
+        const double Fq = a.simple ? (double)fnodes : __builtin_inf();
         uint64_t t_rec = t_push, t_corr = t_push;
         auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
             const Ent e = ent[i];
@@ -1675,42 +1696,61 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const unsigned int S32 = s_sup;
             const unsigned int NL = s_nl;
             const unsigned int T32 = s_tl;
-            const bool lazy = (int64_t)NL <= a.n && lazy_g(kLazyDeg, iters) <= kLazyGmax;
-            // the dynamic LDS now holds x32 of the first ids for the pull
-            const int xn = (int)min((int64_t)(a.hub_bytes / 4), a.n);
+            const bool lazy = (int64_t)NL <= a.n && lazy_g(adds_max(kLazyDeg, iters, Fq), iters) <= kLazyGmax;
+            // the dynamic LDS may hold x32 of the first ids for the pull (filled
+            // below when the pulled rows are long enough to pay for it)
+            const int xn_max = (int)min((int64_t)(a.hub_bytes / 4), a.n);
             float *xc = reinterpret_cast<float *>(s_hub);
-            for (int v = tid; v < xn; v += NT) xc[v] = S.rx32[v].y;
             if (tid == 0) {
                 s_cnt = 0;
                 s_ncnt = 0;
                 s_npc = 0;
                 s_work = 0;
                 s_nlong = 0;
+                s_pvol = lazy ? 0u : ~0u;   // every touched boundary row pulled: fill
             }
             __syncthreads();
             {
                 const unsigned int lim = lazy ? NL : T32;
                 const int32_t *src = lazy ? nl : tl;
-                for (unsigned int i0 = 0; i0 < lim; i0 += NT) {
-                    const unsigned int i = i0 + tid;
-                    int v = -1;
-                    if (i < lim) {
-                        v = src[i];
-                        const float xv = S.rx32[v].y;
-                        if (!pristine(xv)) {
-                            v = -1;   // pushed: already in the support list
-                        } else if (lazy) {
-                            // a candidate listed for its degree (> kLazyDeg) whose
-                            // final FP32 residual the bound still certifies keeps it
-                            const int d = degree[v];
-                            const float rv = S.rx32[v].x;
-                            if (d > kLazyDeg && lazy_valid(d, iters) &&
-                                fabs((double)rv) + lazy_bound(rv, d, iters) < kNearFrac * P.thr_scale * (double)d)
-                                v = -1;
-                        }
+                // kScanUnroll entries per lane, their record and degree loads
+                // issued together (one memory round trip per batch)
+                for (unsigned int i0 = 0; i0 < lim; i0 += NT * kScanUnroll) {
+                    int vv[kScanUnroll], dd[kScanUnroll];
+                    float2 rx[kScanUnroll];
+#pragma unroll
+                    for (int j = 0; j < kScanUnroll; j++) {
+                        const unsigned int i = i0 + j * NT + tid;
+                        vv[j] = i < lim ? src[i] : -1;
                     }
-                    const long long k = wave_append(v >= 0, &s_ncnt);
-                    if (k >= 0) qn[k] = v;
+#pragma unroll
+                    for (int j = 0; j < kScanUnroll; j++) {
+                        rx[j] = vv[j] >= 0 ? S.rx32[vv[j]] : make_float2(0.0f, 0.0f);
+                        dd[j] = (lazy && vv[j] >= 0) ? degree[vv[j]] : 0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < kScanUnroll; j++) {
+                        int v = vv[j];
+                        if (v >= 0) {
+                            if (!pristine(rx[j].y)) {
+                                v = -1;   // pushed: already in the support list
+                            } else if (lazy) {
+                                // a candidate listed for its degree (> kLazyDeg) whose
+                                // final FP32 residual the bound still certifies keeps it
+                                const int d = dd[j];
+                                const float rv = rx[j].x;
+                                const double m = adds_max(d, iters, Fq);
+                                if (d > kLazyDeg && lazy_valid(m) &&
+                                    fabs((double)rv) + lazy_bound(rv, m, iters) <
+                                        kNearFrac * P.thr_scale * (double)d)
+                                    v = -1;
+                                else
+                                    atomic_add_sat(&s_pvol, (unsigned int)d);
+                            }
+                        }
+                        const long long k = wave_append(v >= 0, &s_ncnt);
+                        if (k >= 0) qn[k] = v;
+                    }
                 }
                 // the seed's residual was set, not added: never listed by the push
                 if (lazy && tid == 0) {
@@ -1739,17 +1779,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         const unsigned int i = i0 + j * NT + tid;
                         vv[j] = i < slim ? (sstream ? (int)i : sup[i]) : -1;
                     }
+                    // the support list's entries are all pushed: degrees loaded
+                    // with the records; streamed, only for the pushed entries
+                    int dd[kScanUnroll];
+                    if (!sstream) {
+#pragma unroll
+                        for (int j = 0; j < kScanUnroll; j++) dd[j] = vv[j] >= 0 ? degree[vv[j]] : 0;
+                    }
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) rx[j] = vv[j] >= 0 ? S.rx32[vv[j]] : make_float2(0.0f, 0.0f);
-                    int dd[kScanUnroll];
+                    if (sstream) {
 #pragma unroll
-                    for (int j = 0; j < kScanUnroll; j++) dd[j] = rx[j].y != 0.0f ? degree[vv[j]] : 0;
+                        for (int j = 0; j < kScanUnroll; j++) dd[j] = rx[j].y != 0.0f ? degree[vv[j]] : 0;
+                    }
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) {
                         bool exact = false;
                         if (rx[j].y != 0.0f)
                             exact = !support_certified(rx[j].x, rx[j].y, dd[j], iters, vv[j] == s ? P.r0 : 0.0,
-                                                       P.thr_scale);
+                                                       P.thr_scale, Fq);
+                        if (exact) atomic_add_sat(&s_pvol, (unsigned int)dd[j]);
                         const long long k = wave_append(exact, &s_ex);
                         if (k >= 0) nl[k] = vv[j];
                     }
@@ -1758,6 +1807,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             __syncthreads();
             const unsigned int E32 = s_ex;
             const unsigned int NLIST = E32 + s_ncnt;
+            // x32 of the first xn ids into LDS when the pulled rows' volume is
+            // at least an eighth of xn (a pulled entry into those ids then reads
+            // LDS instead of a memory line; the copy reads xn / 16 lines)
+            const int xn = s_pvol >= (unsigned int)xn_max / 8u ? xn_max : 0;
+            for (int v0 = tid; v0 < xn; v0 += NT * 4) {
+                float t[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int v = v0 + k * NT;
+                    t[k] = v < xn ? S.rx32[v].y : 0.0f;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (v0 + k * NT < xn) xc[v0 + k * NT] = t[k];
+            }
             // every entry whose flags the certificate sets is listed in
             // nl[0, s_dirty): the exact support rows nl[0, E32), then the
             // other exact rows and the entries the correction push reaches
@@ -1849,7 +1913,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     e.st = 0;
                     e.d = 0;
                     e.v = 0.0;
-                    if (fabs(tmp) + rec_bound_ool(fl, rl, rx.y, d, iters, u == s ? P.r0 : 0.0) >=
+                    if (fabs(tmp) + rec_bound_ool(fl, rl, rx.y, d, iters, u == s ? P.r0 : 0.0, Fq) >=
                         P.thr_scale * (double)d) {
                         S.rx64[u].x = 0.0;
                         if (rl > 0.0f) S.rx32[u].x = -rl;
@@ -1891,7 +1955,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         if (fresh && (of & kTouched) == 0) res = 2;
                         if (of == 0u) nl[atomicAdd(&s_dirty, 1u)] = vtx;   // first flags of this entry
                         const double eff = nw + (rl > 0.0f ? (double)rl : 0.0);
-                        const double bv = rec_bound(of, rl, rx.y, dv, iters, vtx == s ? P.r0 : 0.0);
+                        const double bv = rec_bound(of, rl, rx.y, dv, iters, vtx == s ? P.r0 : 0.0, Fq);
                         if (fabs(eff) + bv >= P.thr_scale * (double)dv && (fl_or(&S.fl[vtx], kInQ) & kInQ) == 0)
                             res |= 1;
                         return res;
@@ -1934,8 +1998,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // mode 2 (FP64 push): only {r64, x64} are read and cleared.
         auto out_pass = [&](auto mode_c, const unsigned int cnt) {
             constexpr int mode = decltype(mode_c)::value;
+            const bool pre_deg = mode == 0 || !stream;
             for (unsigned int i0 = 0; i0 < cnt; i0 += NT * kOutUnroll) {
-                int vv[kOutUnroll];
+                int vv[kOutUnroll], dg[kOutUnroll];
                 float r32v[kOutUnroll], x32v[kOutUnroll];
                 uint32_t flv[kOutUnroll];
                 double r64v[kOutUnroll], x64v[kOutUnroll];
@@ -1944,6 +2009,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const unsigned int i = i0 + j * NT + tid;
                     vv[j] = i < cnt ? (mode == 0 ? nl[i] : stream ? (int)i : tl[i]) : -1;
                 }
+                // listed entries are nearly all live: their degrees are loaded
+                // with the records (one memory round trip, not two); a streamed
+                // pass loads them only for the live entries
+#pragma unroll
+                for (int j = 0; j < kOutUnroll; j++) dg[j] = (pre_deg && vv[j] >= 0) ? degree[vv[j]] : 0;
                 if (mode == 2) {
 #pragma unroll
                     for (int j = 0; j < kOutUnroll; j++) {
@@ -1998,13 +2068,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         live = !(pristine(r64v[j]) && pristine(x64v[j]));
                     if (live) {
                         const int v = vv[j];
-                        const int d = degree[v];
+                        const int d = pre_deg ? dg[j] : degree[v];
                         // FP32 push: an entry may hold its residual in FP32 (pending,
                         // rl > 0) and carries the error bound of its FP32 values
                         const float rl = r32v[j];
                         const double res = r64v[j] + (rl > 0.0f ? (double)rl : 0.0);
                         const double bnd =
-                            mode == 2 ? 0.0 : rec_bound_ool(flv[j], rl, x32v[j], d, iters, v == s ? P.r0 : 0.0);
+                            mode == 2 ? 0.0 : rec_bound_ool(flv[j], rl, x32v[j], d, iters, v == s ? P.r0 : 0.0, Fq);
                         if (res != 0.0 || bnd != 0.0) {
                             cmax = fmax(cmax, (fabs(res) + bnd) / (P.thr_scale * (double)d));
                             rl1 += fabs(res) + bnd;
@@ -2160,11 +2230,17 @@ __global__ void k_edge_keys(const int32_t *indptr, const int32_t *indices, const
     }
 }
 
+// cnt[0]: positions where the sorted forward and reverse keys differ (0: every
+// (u, v) has its (v, u)); cnt[1]: live keys equal to the one before (0: no
+// row repeats an entry).  Unused slack entries sort last as ~0.
 __global__ void k_count_diff(const uint64_t *a, const uint64_t *b, int64_t m, unsigned long long *cnt) {
-    unsigned long long c = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    unsigned long long c = 0, r = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         c += a[i] != b[i] ? 1ull : 0ull;
-    if (c) atomicAdd(cnt, c);
+        r += (i > 0 && a[i] != ~0ull && a[i] == a[i - 1]) ? 1ull : 0ull;
+    }
+    if (c) atomicAdd(&cnt[0], c);
+    if (r) atomicAdd(&cnt[1], r);
 }
 
 int colx_refresh(flsg_graph g, hipStream_t st) {
@@ -2200,14 +2276,16 @@ static int vol2_ensure(flsg_graph g, hipStream_t st) {
 }
 
 // Whether every live entry (u, v) has its (v, u) -- the FP32 certificate's
-// error bounds count a node's incoming pushes by its degree.  Decided once per
-// handle by sorting the entries both ways (edge insertions add both
-// directions, so it stays valid).
+// error bounds count a node's incoming pushes by its degree -- and whether no
+// row repeats an entry (then they also cap them by the query's pushes).
+// Decided once per handle by sorting the entries both ways (edge insertions
+// add both directions of pairs not yet present, so both stay valid).
 static int undirected_ensure(flsg_graph g, hipStream_t st) {
     if (g->undirected >= 0) return FLSG_OK;
     const int64_t m = g->nnz_cap;
     if (m == 0) {
         g->undirected = 1;
+        g->simple = 1;
         return FLSG_OK;
     }
     if (m >= INT32_MAX) return fail(FLSG_EARG, "graph too large for the symmetry check");
@@ -2217,13 +2295,13 @@ static int undirected_ensure(flsg_graph g, hipStream_t st) {
     size_t tb = 0;
     hipError_t e = hipSuccess;
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipMalloc((void **)&k[i], sizeof(uint64_t) * (size_t)m);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_cnt, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_cnt, 2 * sizeof(unsigned long long));
     if (e == hipSuccess)
         e = hipcub::DeviceRadixSort::SortKeys(nullptr, tb, (const uint64_t *)nullptr, (uint64_t *)nullptr, (int)m, 0,
                                               64, st);
     if (e == hipSuccess) e = hipMalloc(&tmp, tb);
-    unsigned long long h = 0;
-    if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st);
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, 2 * sizeof(unsigned long long), st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_edge_keys, dim3(g->num_cus * 4), dim3(256), 0, st, g->indptr,
                            (const int32_t *)g->indices, (const int32_t *)g->degree, g->n, m, k[0], k[1]);
@@ -2236,13 +2314,14 @@ static int undirected_ensure(flsg_graph g, hipStream_t st) {
                            (const uint64_t *)k[3], m, d_cnt);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     for (int i = 0; i < 4; i++) (void)hipFree(k[i]);
     (void)hipFree(d_cnt);
     (void)hipFree(tmp);
     FLSG_HIP(e);
-    g->undirected = h == 0 ? 1 : 0;
+    g->undirected = h[0] == 0 ? 1 : 0;
+    g->simple = h[1] == 0 ? 1 : 0;
     return FLSG_OK;
 }
 
@@ -2387,6 +2466,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.bin_min = g->batch_bin_min;
     a.dense_min = g->batch_dense_min;
     a.undirected = g->undirected;
+    a.simple = g->simple;
     // hub pieces are encoded (entry << 8) | piece: entries < 2^24, < 256 pieces per node
     if (g->n >= (1 << 24)) return fail(FLSG_EARG, "batched solves need n < 2^24");
     // degree bins: entries with more than hub_w edges are split into pieces of
@@ -2457,10 +2537,17 @@ using namespace flsg;
 
 extern "C" int64_t flsg_batch_slot_bytes(int64_t n) { return slot_bytes(n); }
 
-extern "C" double flsg_boundary_bound(float r32, int32_t d, int64_t iters) { return lazy_bound(r32, d, iters); }
+// pushes <= 0: no cap (m = d iters)
+static double host_adds(int32_t d, int64_t iters, int64_t pushes) {
+    return adds_max(d, iters, pushes > 0 ? (double)pushes : __builtin_inf());
+}
 
-extern "C" double flsg_support_bound(float x32, float r32, int32_t d, int64_t iters, double b) {
-    return sup_bound(x32, r32, d, iters, b);
+extern "C" double flsg_boundary_bound(float r32, int32_t d, int64_t iters, int64_t pushes) {
+    return lazy_bound(r32, host_adds(d, iters, pushes), iters);
+}
+
+extern "C" double flsg_support_bound(float x32, float r32, int32_t d, int64_t iters, double b, int64_t pushes) {
+    return sup_bound(x32, r32, host_adds(d, iters, pushes), iters, b);
 }
 
 extern "C" int flsg_batch_set_slots(flsg_graph g, int32_t slots) {

@@ -97,8 +97,8 @@ SIGNATURES = {
     "flsg_session_destroy": (C.c_int, [C.c_void_p]),
     "flsg_batch_refresh_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "flsg_batch_arena_dirty": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
-    "flsg_boundary_bound": (C.c_double, [C.c_float, C.c_int32, C.c_int64]),
-    "flsg_support_bound": (C.c_double, [C.c_float, C.c_float, C.c_int32, C.c_int64, C.c_double]),
+    "flsg_boundary_bound": (C.c_double, [C.c_float, C.c_int32, C.c_int64, C.c_int64]),
+    "flsg_support_bound": (C.c_double, [C.c_float, C.c_float, C.c_int32, C.c_int64, C.c_double, C.c_int64]),
     "flsg_sor_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_double, C.c_int64, C.c_void_p, C.c_void_p,
                                  C.POINTER(Result), C.c_void_p]),
     "flsg_sor_ignn_column": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -332,18 +332,24 @@ int flsg_session_destroy(flsg_session s);
  * node (reached by the push, never pushed) of degree d whose FP32 residual is
  * r32 after `iters` push iterations: |r_FP64 - r32| <= bound, r_FP64 being the
  * FP64 residual of the returned x (0 for r32 == 0; |r32| is used, the sign
- * marks a residual the correction push has moved into x).  Valid while
- * 2 d iters 2^-24 <= 1/4 (+inf beyond); the kernels recompute every node the
- * bound does not certify in FP64.  Host function, no GPU needed.  No counterpart in the
- * reference (the certificate's error bound, exposed for tests). */
-double flsg_boundary_bound(float r32, int32_t d, int64_t iters);
+ * marks a residual the correction push has moved into x).  The bound counts
+ * the push values the node can have received, m = min(d iters, pushes):
+ * `pushes` is the query's number of pushes (frontier entries over all
+ * iterations), a valid cap when no row of the graph repeats an entry (a push
+ * sends one value per row entry); pushes <= 0 means no cap (m = d iters).
+ * Valid while 2 m 2^-24 <= 1/4 (+inf beyond); the kernels recompute every
+ * node the bound does not certify in FP64.  Host function, no GPU needed.  No
+ * counterpart in the reference (the certificate's error bound, exposed for
+ * tests). */
+double flsg_boundary_bound(float r32, int32_t d, int64_t iters, int64_t pushes);
 
 /* The error bound a pushed node of the batched FP32 path is certified with
  * when it is not recomputed: |r_v - r32_v| <= flsg_support_bound(x32_v,
- * r32_v, d_v, iters, b_v), r_v the exact residual of the returned x32 and b_v
- * the right-hand side at v (alpha or 1 at the seed, else 0).  Host-callable,
- * for tests.  No counterpart in the reference. */
-double flsg_support_bound(float x32, float r32, int32_t d, int64_t iters, double b);
+ * r32_v, d_v, iters, b_v, pushes), r_v the exact residual of the returned x32
+ * and b_v the right-hand side at v (alpha or 1 at the seed, else 0); pushes
+ * caps the adds as in flsg_boundary_bound.  Host-callable, for tests.  No
+ * counterpart in the reference. */
+double flsg_support_bound(float x32, float r32, int32_t d, int64_t iters, double b, int64_t pushes);
 
 /* Number of slot-arena records not back in the between-queries state (every
  * field zero except the degree copy) -- 0 after any completed launch.

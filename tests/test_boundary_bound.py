@@ -26,21 +26,27 @@ ALPHA = 0.1
 OM = 1.0 - ALPHA   # the kernels' FP64 1 - alpha
 
 
-def bound(r32, d, iters):
-    return _build.load_lib().flsg_boundary_bound(float(r32), int(d), int(iters))
+def bound(r32, d, iters, pushes=0):
+    return _build.load_lib().flsg_boundary_bound(float(r32), int(d), int(iters), int(pushes))
 
 
 def push_value(rho, du):
     return np.float32((OM * float(rho)) / du)
 
 
-def run(d_v, iters, rng, adversarial):
-    """(r32, exact residual of x32) at a boundary node of degree d_v."""
+def run(d_v, iters, rng, adversarial, active=1.0, with_adds=False):
+    """(r32, exact residual of x32) at a boundary node of degree d_v; each
+    iteration a neighbour pushes with probability `active` (with_adds: also
+    the number of push values the node received)."""
     deg = rng.integers(1, 200, size=d_v)
     x32 = [np.float32(0.0)] * d_v
     r32 = np.float32(0.0)
+    adds = 0
     for it in range(iters):
         for k in rng.permutation(d_v):
+            if active < 1.0 and rng.random() >= active:
+                continue
+            adds += 1
             du = int(deg[k])
             if adversarial and not (it == 0 and k == 0 and r32 == 0):
                 # a push value just past half an ulp of the running sum: the
@@ -56,7 +62,7 @@ def run(d_v, iters, rng, adversarial):
             x32[k] = np.float32(x32[k] + rho)
             r32 = np.float32(r32 + p)
     exact = sum(Fraction(OM) * Fraction(float(x32[k])) / int(deg[k]) for k in range(d_v))
-    return r32, exact
+    return (r32, exact, adds) if with_adds else (r32, exact)
 
 
 @pytest.mark.parametrize("d_v,iters", [(1, 1), (1, 60), (8, 25), (64, 10), (64, 60)])
@@ -92,3 +98,28 @@ def test_bound_special_values():
     assert bound(2e-6, 5, 10) > 1.99 * bound(1e-6, 5, 10)
     assert bound(1e-6, 64, 10) > bound(1e-6, 5, 10)
     assert bound(1e-6, 5, 100) > bound(1e-6, 5, 10)
+
+
+@pytest.mark.parametrize("d_v,iters,active", [(400, 20, 0.02), (2000, 12, 0.005), (300, 40, 0.05)])
+@pytest.mark.parametrize("adversarial", [False, True])
+def test_bound_capped_by_received_values(d_v, iters, active, adversarial):
+    """A high-degree node reached by few pushes: the bound taken with the cap
+    m = the push values the node actually received (the kernels use the
+    query's push count, which is at least that on a graph with no repeated
+    row entry) still covers the FP32 error, and is far below the degree-based
+    bound."""
+    rng = np.random.default_rng(d_v + 17 * iters + int(adversarial))
+    worst = 0.0
+    for _ in range(3):
+        r32, exact, adds = run(d_v, iters, rng, adversarial, active=active, with_adds=True)
+        if adds == 0:
+            continue
+        err = abs(Fraction(float(r32)) - exact)
+        b = bound(r32, d_v, iters, pushes=adds)
+        assert err <= Fraction(b), (d_v, iters, adds, float(err), b)
+        # a cap above d iters changes nothing; the cap is far below it here
+        assert bound(r32, d_v, iters, pushes=d_v * iters + 5) == bound(r32, d_v, iters)
+        assert b < 0.2 * bound(r32, d_v, iters)
+        worst = max(worst, float(err) / b)
+    if adversarial:
+        assert worst > 0.05, worst

@@ -316,6 +316,51 @@ def test_batch_fp32_boundary_regimes(small_graph, dev_small, alpha, eps):
     assert engine.arena_dirty(dev_small) == 0
 
 
+def test_batch_fp32_graph_preconditions(small_graph):
+    """The FP32 certificate's preconditions are decided per graph.  On a graph
+    whose rows repeat entries (every edge listed twice: the same PPR vector)
+    the push-count cap of its error bounds is off and every query is still
+    certified, by the device and by a host FP64 recompute.  A directed graph
+    (leaves pointing at a centre that points back at one of them) is refused
+    for the FP32 push with FLSG_EARG; the FP64 push runs on it."""
+    g = small_graph
+    ip = np.asarray(g.indptr, dtype=np.int64)
+    rows = [np.asarray(g.indices[ip[u]:ip[u + 1]]) for u in range(g.n)]
+    ind2 = np.concatenate([np.concatenate([r, r]) for r in rows]).astype(np.int32)
+    deg2 = (2 * np.asarray(g.degree)).astype(np.int32)
+    ip2 = np.zeros(g.n + 1, dtype=np.int64)
+    ip2[1:] = np.cumsum(deg2)
+    g2 = graph.CSRGraph(n=g.n, indptr=ip2, indices=ind2, degree=deg2, name="doubled")
+    dg2 = engine.DeviceGraph(g2)
+    try:
+        eps = 1e-6
+        seeds = graph.mixed_seeds(g, 24, seed=23)
+        p = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=eps)
+        bo = engine.batch_solve(dg2, p, seeds)
+        _check_batch(g2, seeds, bo, eps)
+        assert engine.arena_dirty(dg2) == 0
+    finally:
+        dg2.close()
+    k = 50
+    # row 0 = [1] (the centre points at leaf 1), rows 1..k = [0]
+    ipd = np.arange(k + 2, dtype=np.int64)
+    ind = np.concatenate([[1], np.zeros(k, dtype=np.int64)]).astype(np.int32)
+    degd = np.diff(ipd).astype(np.int32)
+    gd = graph.CSRGraph(n=k + 1, indptr=ipd, indices=ind, degree=degd, name="directed star")
+    dgd = engine.DeviceGraph(gd)
+    try:
+        s = np.array([1, 2], dtype=np.int64)
+        p32 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP32, alpha=ALPHA, eps=1e-4)
+        with pytest.raises(engine.FlsgError) as ei:
+            engine.batch_solve(dgd, p32, s)
+        assert ei.value.code == _abi.FLSG_EARG
+        p64 = engine.make_params(_abi.FLSG_PPR, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=ALPHA, eps=1e-4)
+        bo = engine.batch_solve(dgd, p64, s)
+        assert all(r["status"] == 0 for r in bo.results)
+    finally:
+        dgd.close()
+
+
 def test_batch_bad_seed_fails_alone(small_graph, dev_small):
     """A query whose seed lies outside the graph fails with FLSG_EARG on its
     own record (the kernel checks before touching a slot); the rest of the

@@ -12,9 +12,12 @@ allows 2u per add for atomics of any rounding mode, and rounding one way
 makes the errors add up instead of cancelling.  The exact residual of the
 returned x32, alpha e_s - x32 + (1 - alpha) A D^-1 x32, is taken in FP64 (its
 own rounding, ~1e-17 of x, is far below the bound) and every pushed node's
-|r - r32| must stay below flsg_support_bound(x32, r32, d, iterations, b).
-Measured: the largest error reaches 2-6% of the bound (eps 1e-5 and 1e-6,
-25-63 iterations), so the bound is neither violated nor loose by orders of
+|r - r32| must stay below flsg_support_bound(x32, r32, d, iterations, b), and
+below the same bound with its count of adds capped by the push values the
+node actually received and by the run's pushes (frontier entries over all
+iterations), as the kernels cap it on graphs with no repeated row entry.
+Measured: the largest error reaches 2-6% of the bound (eps 1e-4 to 1e-6,
+7-63 iterations), so the bound is neither violated nor loose by orders of
 magnitude; the boundary bound, which has no x term, does not cover these
 nodes (shown below).
 """
@@ -28,8 +31,8 @@ ALPHA = 0.1
 OM = 1.0 - ALPHA
 
 
-def sup_bound(x32, r32, d, iters, b):
-    return _build.load_lib().flsg_support_bound(float(x32), float(r32), int(d), int(iters), float(b))
+def sup_bound(x32, r32, d, iters, b, pushes=0):
+    return _build.load_lib().flsg_support_bound(float(x32), float(r32), int(d), int(iters), float(b), int(pushes))
 
 
 def round_up32(s: float) -> np.float32:
@@ -40,7 +43,9 @@ def round_up32(s: float) -> np.float32:
 
 
 def fp32_push(g, s, eps, max_iters=100000):
-    """(x32, r32, iterations, pushed mask) of the FP32 push, additions rounded up."""
+    """(x32, r32, iterations, pushed mask, pushes, adds) of the FP32 push,
+    additions rounded up; pushes = frontier entries over all iterations, adds
+    = push values each node received."""
     n = g.n
     deg = np.asarray(g.degree, dtype=np.int64)
     ip = np.asarray(g.indptr, dtype=np.int64)
@@ -50,10 +55,13 @@ def fp32_push(g, s, eps, max_iters=100000):
     r32 = np.zeros(n, dtype=np.float32)
     r32[s] = np.float32(ALPHA)
     pushed = np.zeros(n, dtype=bool)
+    adds = np.zeros(n, dtype=np.int64)
     front = [s]
     iters = 0
+    pushes = 0
     while front and iters < max_iters:
         iters += 1
+        pushes += len(front)
         vals = []
         for u in front:
             t = r32[u]
@@ -65,9 +73,10 @@ def fp32_push(g, s, eps, max_iters=100000):
         for u, val in zip(front, vals):
             for v in ix[ip[u]:ip[u + 1]]:
                 r32[v] = round_up32(float(r32[v]) + float(val))
+                adds[v] += 1
                 touched.add(int(v))
         front = [v for v in sorted(touched) if thr[v] <= float(r32[v])]
-    return x32, r32, iters, pushed
+    return x32, r32, iters, pushed, pushes, adds
 
 
 @pytest.fixture(scope="module")
@@ -76,17 +85,19 @@ def runs(small_graph):
     g = small_graph
     out = []
     seeds = [int(v) for v in graph.uniform_seeds(g, 3, seed=5)]
-    for s, eps in [(seeds[0], 1e-5), (seeds[1], 1e-5), (seeds[2], 1e-6)]:
-        x32, r32, iters, pushed = fp32_push(g, s, eps)
+    hub = int(np.argmax(g.degree))
+    # eps 1e-4: few pushes, so the push-count cap binds at the high-degree nodes
+    for s, eps in [(seeds[0], 1e-5), (seeds[1], 1e-5), (seeds[2], 1e-6), (seeds[0], 1e-4), (hub, 1e-4)]:
+        x32, r32, iters, pushed, pushes, adds = fp32_push(g, s, eps)
         r = O.ppr_residual(g, s, ALPHA, x32.astype(np.float64))
-        out.append((s, x32, r32, iters, pushed, r))
+        out.append((s, x32, r32, iters, pushed, r, pushes, adds))
     return out
 
 
 def test_support_bound_covers_rounded_up_push(small_graph, runs):
     g = small_graph
     worst = 0.0
-    for s, x32, r32, iters, pushed, r in runs:
+    for s, x32, r32, iters, pushed, r, pushes, adds in runs:
         for v in np.flatnonzero(pushed):
             b = ALPHA if v == s else 0.0
             bnd = sup_bound(x32[v], r32[v], g.degree[v], iters, b)
@@ -96,16 +107,42 @@ def test_support_bound_covers_rounded_up_push(small_graph, runs):
     assert worst > 1e-2, worst   # within two orders of magnitude of the errors it covers
 
 
+def test_support_bound_capped_by_pushes(small_graph, runs):
+    """The count of adds into a node is at most the run's pushes (one value
+    per row entry per push, no repeated entries) and at most d * iterations;
+    the bound taken with the received values themselves, and with the
+    pushes, covers every pushed node's error -- and the cap binds at the
+    high-degree nodes, where the bound is tighter than with d * iterations."""
+    g = small_graph
+    assert all(len(set(g.indices[g.indptr[u]:g.indptr[u + 1]].tolist())) == g.degree[u] for u in range(g.n))
+    tighter = 0
+    worst = 0.0
+    for s, x32, r32, iters, pushed, r, pushes, adds in runs:
+        assert adds.max() <= pushes
+        assert np.all(adds <= np.asarray(g.degree, dtype=np.int64) * iters)
+        for v in np.flatnonzero(pushed):
+            b = ALPHA if v == s else 0.0
+            err = abs(r[v] - float(r32[v]))
+            own = sup_bound(x32[v], r32[v], g.degree[v], iters, b, pushes=max(int(adds[v]), 1))
+            cap = sup_bound(x32[v], r32[v], g.degree[v], iters, b, pushes=pushes)
+            assert err <= own <= cap, (s, v, err, own, cap)
+            worst = max(worst, err / own)
+            if cap < 0.5 * sup_bound(x32[v], r32[v], g.degree[v], iters, b):
+                tighter += 1
+    assert tighter > 0
+    assert worst > 1e-2, worst
+
+
 def test_boundary_bound_alone_would_not_cover_pushed_nodes(small_graph, runs):
     """The x-dependent terms are needed: the boundary bound (r32 only) is
     exceeded at some pushed node."""
     g = small_graph
     lib = _build.load_lib()
     exceeded = 0
-    for s, x32, r32, iters, pushed, r in runs:
+    for s, x32, r32, iters, pushed, r, pushes, adds in runs:
         for v in np.flatnonzero(pushed):
             err = abs(r[v] - float(r32[v]))
-            if err > lib.flsg_boundary_bound(float(r32[v]), int(g.degree[v]), int(iters)):
+            if err > lib.flsg_boundary_bound(float(r32[v]), int(g.degree[v]), int(iters), 0):
                 exceeded += 1
     assert exceeded > 0
 
