@@ -2155,9 +2155,9 @@ __global__ void k_nonzero_u32(const uint32_t *p, int64_t total, unsigned long lo
     if (c) atomicAdd(cnt, c);
 }
 
-// cost estimate of each query for the claim order: the seed's two-hop volume,
-// sum of its neighbours' degrees (-1 for a seed outside the graph); one
-// wavefront per seed
+// cost estimate of each query for the claim order: the sum of vol[] over the
+// seed's neighbours (vol = the three-hop volumes: the seed's four-hop
+// volume; -1 for a seed outside the graph); one wavefront per seed
 __global__ void k_seed_cost(const int64_t *seeds, int64_t nq, const int32_t *indptr, const int32_t *degree,
                             const int2 *colx, const int64_t *vol2, int64_t n, int64_t *out) {
     const int lane = __lane_id();
@@ -2176,17 +2176,23 @@ __global__ void k_seed_cost(const int64_t *seeds, int64_t nq, const int32_t *ind
     }
 }
 
-// vol2[u] = sum of u's neighbours' degrees, one wavefront per row
-__global__ void k_vol2(const int32_t *indptr, const int32_t *degree, const int2 *colx, int64_t n, int64_t *vol2) {
+// vout[u] = sum over u's neighbours w of vin[w] (vin = NULL: of their degrees),
+// one wavefront per row
+__global__ void k_vol_hop(const int32_t *indptr, const int32_t *degree, const int2 *colx, int64_t n,
+                          const int64_t *vin, int64_t *vout) {
     const int lane = __lane_id();
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t u = blockIdx.x * (int64_t)(blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += waves) {
         const int st = indptr[u];
         const int d = degree[u];
         long long c = 0;
-        for (int e = lane; e < d; e += kWave) c += colx[st + e].y;
+        if (vin) {
+            for (int e = lane; e < d; e += kWave) c += vin[colx[st + e].x];
+        } else {
+            for (int e = lane; e < d; e += kWave) c += colx[st + e].y;
+        }
         c = wave_sum(c);
-        if (lane == 0) vol2[u] = c;
+        if (lane == 0) vout[u] = c;
     }
 }
 
@@ -2263,13 +2269,20 @@ static int colx_ensure(flsg_graph g, hipStream_t st) {
     return FLSG_OK;
 }
 
-// Two-hop volume of every node (for the claim order of flsg_batch_solve),
-// rebuilt after edge insertions.
+// Three-hop volume of every node, vol2[u] = sum over u's neighbours w of
+// their two-hop volumes (the sums of w's neighbours' degrees), for the claim
+// order of flsg_batch_solve; rebuilt after edge insertions.  The buffer holds
+// 2n entries: the two-hop volumes go to its second half.
 static int vol2_ensure(flsg_graph g, hipStream_t st) {
     if (g->vol2 && g->vol2_valid) return FLSG_OK;
-    if (!g->vol2) FLSG_HIP(hipMalloc((void **)&g->vol2, sizeof(int64_t) * (size_t)std::max<int64_t>(g->n, 1)));
-    hipLaunchKernelGGL(k_vol2, dim3(g->num_cus * 8), dim3(256), 0, st, (const int32_t *)g->indptr,
-                       (const int32_t *)g->degree, (const int2 *)g->colx, g->n, g->vol2);
+    const size_t nn = (size_t)std::max<int64_t>(g->n, 1);
+    if (!g->vol2) FLSG_HIP(hipMalloc((void **)&g->vol2, sizeof(int64_t) * 2 * nn));
+    hipLaunchKernelGGL(k_vol_hop, dim3(g->num_cus * 8), dim3(256), 0, st, (const int32_t *)g->indptr,
+                       (const int32_t *)g->degree, (const int2 *)g->colx, g->n, (const int64_t *)nullptr,
+                       g->vol2 + nn);
+    hipLaunchKernelGGL(k_vol_hop, dim3(g->num_cus * 8), dim3(256), 0, st, (const int32_t *)g->indptr,
+                       (const int32_t *)g->degree, (const int2 *)g->colx, g->n, (const int64_t *)(g->vol2 + nn),
+                       g->vol2);
     FLSG_HIP(hipGetLastError());
     g->vol2_valid = 1;
     return FLSG_OK;
@@ -2714,12 +2727,12 @@ extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_
                        p->eps, p->precision);
     // claim order: the costliest queries first (started first, the long ones
     // leave the short ones to fill the launch's tail); the cost estimate is the
-    // seed's three-hop volume (the sum over its neighbours of their
-    // neighbours' degrees), which ranks the push work of eps = 1e-7 PPR
-    // queries on the 2.39M-node power-law graph with Spearman 0.87 (two-hop
-    // volume 0.84, seed degree 0.40); scheduled greedily by it onto 256 slots
-    // with time proportional to work, the makespan is 1.08x the perfectly
-    // balanced one (two-hop volume: 1.19x)
+    // seed's four-hop volume (the sum over its neighbours of their three-hop
+    // volumes), which ranks the push work of eps = 1e-7 PPR queries on the
+    // 2.39M-node power-law graph with Spearman 0.87 (three-hop 0.87, two-hop
+    // 0.84, seed degree 0.40); scheduled greedily by it onto 256 slots with
+    // time proportional to work, the makespan is 1.064x the perfectly balanced
+    // one (three-hop 1.078x, two-hop 1.19x, the true work 1.062x)
     std::vector<int64_t> cost((size_t)n_queries);
     std::vector<int32_t> order((size_t)n_queries);
     rc = colx_ensure(g, st);

@@ -104,7 +104,7 @@ struct flsg_graph_s {
     double *local_r = nullptr;
     // batched workspace
     int2 *colx = nullptr;                   // nnz_cap column entries {v, degree[v]} (batched edge phases)
-    int64_t *vol2 = nullptr;                // n: sum of the neighbours' degrees (batched claim order)
+    int64_t *vol2 = nullptr;                // 2n: three-hop volume per node, then two-hop (batched claim order)
     int32_t vol2_valid = 0;                 // vol2 follows the current degrees
     flsg::BatchArena *arena = nullptr;
     int32_t batch_slots = 0;
