@@ -191,6 +191,13 @@ struct Strided {
     __device__ __forceinline__ T &operator[](int64_t i) const { return p[i * s]; }
 };
 
+// A support-list entry of the PPR / Katz push: the node id (< 2^24) with its
+// degree in the top 8 bits (255: 255 or more, read from the graph), so the
+// certificate's scan of the list needs no degree load for most entries.
+__device__ __forceinline__ int32_t sup_entry(int v, int d) {
+    return (int32_t)((uint32_t)v | ((uint32_t)min(d, 255) << 24));
+}
+
 // LDS add that stops at 2^32 - 1
 __device__ __forceinline__ void atomic_add_sat(unsigned int *p, unsigned int v) {
     const unsigned int old = atomicAdd(p, v);
@@ -1144,7 +1151,7 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                 ent[f[u]] = e;
                 add_pieces((unsigned int)f[u], dv[u], pw, hw, pcs_next, s_npc);
             }
-            if (sk[u] >= 0) sup[sk[u]] = v[u];
+            if (sk[u] >= 0) sup[sk[u]] = sup_entry(v[u], dv[u]);
         }
     };
     for (int k = 0; k < K; k++) {
@@ -1407,7 +1414,7 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
             wave_append_n<kUnroll>(fp, s_sup, k);
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) {
-                if (k[j] >= 0) sup[k[j]] = uu[j];
+                if (k[j] >= 0) sup[k[j]] = sup_entry(uu[j], dd[j]);
             }
         }
         if (vol) {
@@ -1774,17 +1781,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 for (unsigned int i0 = 0; i0 < slim; i0 += NT * kScanUnroll) {
                     int vv[kScanUnroll];
                     float2 rx[kScanUnroll];
+                    // a support-list entry carries its degree (below 255);
+                    // streamed, degrees are loaded for the pushed entries only
+                    int dd[kScanUnroll];
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) {
                         const unsigned int i = i0 + j * NT + tid;
-                        vv[j] = i < slim ? (sstream ? (int)i : sup[i]) : -1;
+                        vv[j] = -1;
+                        dd[j] = 0;
+                        if (i < slim) {
+                            if (sstream) {
+                                vv[j] = (int)i;
+                            } else {
+                                const uint32_t e = (uint32_t)sup[i];
+                                vv[j] = (int)(e & 0xffffffu);
+                                dd[j] = (int)(e >> 24);
+                            }
+                        }
                     }
-                    // the support list's entries are all pushed: degrees loaded
-                    // with the records; streamed, only for the pushed entries
-                    int dd[kScanUnroll];
                     if (!sstream) {
 #pragma unroll
-                        for (int j = 0; j < kScanUnroll; j++) dd[j] = vv[j] >= 0 ? degree[vv[j]] : 0;
+                        for (int j = 0; j < kScanUnroll; j++)
+                            if (dd[j] == 255) dd[j] = degree[vv[j]];
                     }
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) rx[j] = vv[j] >= 0 ? S.rx32[vv[j]] : make_float2(0.0f, 0.0f);
@@ -1921,7 +1939,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         // a node the FP32 push never pushed joins the support
                         if ((fl & kSeen) == 0 && rx.y == 0.0f) {
                             fl_or(&S.fl[u], kSeen);
-                            sup[atomicAdd(&s_sup, 1u)] = u;
+                            sup[atomicAdd(&s_sup, 1u)] = sup_entry(u, d);
                         }
                         e.st = indptr[u];
                         e.d = d;
