@@ -289,6 +289,12 @@ struct BatchArgs {
     int32_t undirected;          // 1 if every edge (u, v) has its (v, u): FP32 certificate precondition
     int32_t simple;              // 1 if no row repeats an entry: the FP32 bounds cap adds by the pushes
     uint64_t *ctl;               // [0] next query (low 32 bits used), [1] output entries used
+    // workgroups of the launch running a query (NULL: not counted); an
+    // iteration is binned only while at least bin_busy_min of them run: the
+    // binned push pays off when many slots push at once (memory-side atomics
+    // are the chip-wide limit), the direct push is quicker when few do
+    unsigned int *busy;
+    int32_t bin_busy_min;
     const int32_t *out_map;      // internal id -> caller id (NULL = identity)
     int32_t *out_ids;
     double *out_vals;
@@ -1356,6 +1362,8 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
         iters++;
         fnodes += cnt;
         const uint64_t t_it = now_ticks();
+        unsigned int busy_now = 0;
+        if (tid == 0 && a.busy) busy_now = __hip_atomic_load(a.busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (tid == 0) {
             *s_ncnt = 0;
@@ -1421,10 +1429,13 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
             atomicAdd(s_vol, vol);
             atomicAdd(s_ivol, vol);
         }
+        __shared__ unsigned int s_busy;
+        if (tid == 0) s_busy = busy_now;
         __syncthreads();
         const uint64_t t0 = now_ticks();
         tick[0] += t0 - t_it;
-        const bool binned = can_bin && *s_ivol >= (unsigned long long)a.bin_min;
+        const bool binned = can_bin && *s_ivol >= (unsigned long long)a.bin_min &&
+                            (!a.busy || s_busy >= (unsigned int)a.bin_busy_min);
         const unsigned int npc = *s_npc;
         if (!hub_on) {
             for (int v = tid; v < hub_n; v += NT) hub[v] = r[v];
@@ -1570,6 +1581,7 @@ __device__ __forceinline__ flsg_query session_query(const BatchArgs &a, unsigned
 // then makes them visible system-wide and sets the query's done flag.
 __device__ __forceinline__ void query_done(const BatchArgs &a, unsigned int q) {
     __syncthreads();
+    if (a.busy && threadIdx.x == 0) atomicSub(a.busy, 1u);
     if (a.done && threadIdx.x == 0) {
         __threadfence_system();
         __hip_atomic_store(a.done + q, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1622,6 +1634,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const unsigned int c = s_q;
         __syncthreads();
         if ((int64_t)c >= a.nq) break;
+        if (tid == 0 && a.busy) atomicAdd(a.busy, 1u);   // back down in query_done
         const unsigned int q = a.order ? (unsigned int)a.order[c] : c;
         const uint64_t t_start = now_ticks();
         const flsg_query qq = a.pub ? session_query(a, q) : a.queries[q];
@@ -2423,7 +2436,7 @@ static int ensure_arena(flsg_graph g, int32_t slots, hipStream_t st) {
     // binned push: one iteration's pairs (at most one per column entry)
     A->pcap = g->nnz_cap > 0 ? g->nnz_cap : 1;
     FLSG_HIP(hipMalloc((void **)&A->pairs, sizeof(uint4) * (size_t)A->pcap * (size_t)slots));
-    FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 2));
+    FLSG_HIP(hipMalloc((void **)&A->ctl, sizeof(uint64_t) * 3));   // [2]: busy workgroups
     FLSG_HIP(hipMemsetAsync(A->rx32, 0, sizeof(float2) * nr, st));
     FLSG_HIP(hipMemsetAsync(A->fl, 0, sizeof(uint32_t) * nr, st));
     FLSG_HIP(hipMemsetAsync(A->rx64, 0, sizeof(double2) * nr, st));
@@ -2454,7 +2467,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
                   int32_t block, const int32_t *d_out_map, int32_t *d_out_ids, double *d_out_vals,
                   int64_t *d_out_off, int64_t out_cap, uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters,
                   const flsg_hk_config *hk, hipStream_t st, const int32_t *d_order = nullptr,
-                  const SessionWords *sw = nullptr) {
+                  const SessionWords *sw = nullptr, unsigned int *d_busy = nullptr) {
     BatchArena *A = g->arena;
     if (!A) return fail(FLSG_EARG, "no batch arena: call flsg_batch_reserve first");
     if (slot_base < 0 || n_slots <= 0 || slot_base + n_slots > A->slots)
@@ -2464,6 +2477,7 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     rc = undirected_ensure(g, st);
     if (rc) return rc;
     FLSG_HIP(hipMemsetAsync(d_ctl, 0, sizeof(uint64_t) * (sw ? 5 : 2), st));
+    if (d_busy) FLSG_HIP(hipMemsetAsync(d_busy, 0, sizeof(unsigned int), st));
     BatchArgs a{};
     if (sw) {
         a.pub = sw->pub;
@@ -2508,6 +2522,11 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.hub_w = g->batch_hub_w > 0 ? g->batch_hub_w : 256;
     a.slot_base = slot_base;
     a.ctl = d_ctl;
+    a.busy = d_busy;
+    // binned iterations while a quarter of the launch's workgroups push (PPR
+    // at eps 1e-7 on the 2.39M-node graph: the binned push is 1.5x faster
+    // than the direct one with 256 slots busy, even with 64, 14% slower with 16)
+    a.bin_busy_min = std::max<int32_t>(1, n_slots / 4);
     a.out_map = d_out_map;
     a.out_ids = d_out_ids;
     a.out_vals = d_out_vals;
@@ -2768,7 +2787,8 @@ extern "C" int flsg_batch_solve(flsg_graph g, const flsg_params *p, const int64_
     FLSG_HIP(hipMemcpyAsync(A->d_order, order.data(), sizeof(int32_t) * n_queries, hipMemcpyHostToDevice, st));
     const int32_t used = (int32_t)(n_queries < slots ? n_queries : slots);
     rc = launch(g, A->d_q, n_queries, 0, used, g->batch_block, g->out_map, d_out_ids, d_out_vals, d_out_off, out_cap,
-                A->ctl, A->d_res, p->max_iters, nullptr, st, A->d_order);
+                A->ctl, A->d_res, p->max_iters, nullptr, st, A->d_order, nullptr,
+                reinterpret_cast<unsigned int *>(A->ctl + 2));
     if (rc) {
         (void)hipStreamSynchronize(st);   // the order copy reads a host vector of this frame
         return rc;
@@ -2807,7 +2827,8 @@ struct flsg_session_s {
     uint64_t *d_pub = nullptr;
     uint32_t *d_closed = nullptr;
     uint64_t *d_ctl = nullptr;       // device: [0] claims, [1] output entries used, [2] idle exits,
-                                     // [3] / [4] device mirrors of the published count / close flag
+                                     // [3] / [4] device mirrors of the published count / close flag,
+                                     // [5] workgroups running a query (low 32 bits)
     int32_t *d_ids = nullptr;        // output arena (device)
     double *d_vals = nullptr;
     int running = 0;                 // a launch was enqueued and has not been seen to end
@@ -2883,7 +2904,7 @@ extern "C" int flsg_session_create(flsg_graph g, int64_t query_cap, int64_t out_
     if (!rc) chk(host_coherent(&s->h_done, &s->d_done, query_cap));
     if (!rc) chk(host_coherent(&s->h_pub, &s->d_pub, 1));
     if (!rc) chk(host_coherent(&s->h_closed, &s->d_closed, 1));
-    if (!rc && hipMalloc((void **)&s->d_ctl, sizeof(uint64_t) * 5) != hipSuccess) chk(fail(FLSG_ENOMEM, "ctl"));
+    if (!rc && hipMalloc((void **)&s->d_ctl, sizeof(uint64_t) * 6) != hipSuccess) chk(fail(FLSG_ENOMEM, "ctl"));
     if (!rc && out_cap > 0) {
         if (hipMalloc((void **)&s->d_ids, sizeof(int32_t) * out_cap) != hipSuccess ||
             hipMalloc((void **)&s->d_vals, sizeof(double) * out_cap) != hipSuccess)
@@ -2915,7 +2936,8 @@ extern "C" int flsg_session_open(flsg_session s, int32_t slot_base, int32_t n_sl
     s->published = 0;
     SessionWords sw{s->d_pub, s->d_closed, s->d_done, (uint64_t)(idle_seconds * 1e8)};
     rc = launch(g, s->d_q, s->qcap, slot_base, n_slots, g->batch_block, g->out_map, s->d_ids, s->d_vals, s->d_off,
-                s->out_cap, s->d_ctl, s->d_res, max_iters, hk, s->st, nullptr, &sw);
+                s->out_cap, s->d_ctl, s->d_res, max_iters, hk, s->st, nullptr, &sw,
+                reinterpret_cast<unsigned int *>(s->d_ctl + 5));
     if (rc) {
         __atomic_store_n(s->h_closed, 1u, __ATOMIC_RELEASE);
         s->closed = 1;
