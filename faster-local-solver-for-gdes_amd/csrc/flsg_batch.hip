@@ -458,8 +458,9 @@ __device__ __forceinline__ void append_bits(const int *bits, const int *v, int32
 // unpredicated (a predicated atomic whose result is used makes the compiler
 // wait inside the predicated block, one round trip per edge): an idle lane
 // aims at its own sink entry (id sink + lane, degree 0) with value -0.0,
-// which leaves any value unchanged under IEEE addition.
-template <typename S, typename Load, typename Pre, typename Post>
+// which leaves any value unchanged under IEEE addition.  kEnc: the listed ids
+// carry the node's degree (sup_entry).
+template <typename S, bool kEnc = false, typename Load, typename Pre, typename Post>
 __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w, int hub_w,
                                            unsigned int *s_work, WaveLds &W, const int2 *colx, Load load,
                                            Pre pre, Post post, int32_t *next, unsigned int *s_next, int32_t *tl,
@@ -536,7 +537,14 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
             int cr[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 7 : 0);
-            append_bits<kUnroll>(cr, vtx, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+            if constexpr (kEnc) {
+                int ev[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) ev[u] = sup_entry(vtx[u], dv[u]);
+                append_bits<kUnroll>(cr, ev, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+            } else {
+                append_bits<kUnroll>(cr, vtx, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+            }
         }
         if (group) wave_sync();
     }
@@ -848,7 +856,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
     if (tid == 0) {
         if (s < hub_n) hub[s] = (T)1;   // hk_solver.py:223
         else F::lvl(S, a.rn, false)[s] = (T)1;
-        qa[0] = s;
+        qa[0] = sup_entry(s, a.degree[s]);   // level lists carry the degree
         *s_cnt = 1;
         *s_sup = 0;
         *s_vol = 0;
@@ -875,60 +883,77 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
         const Strided<T> lv = F::lvl(S, a.rn, odd);
         const Strided<T> lnx = F::lvl(S, a.rn, !odd);
         const int nb = odd ? 0 : hub_n;   // LDS base of level j + 1
+        __shared__ unsigned int s_np;   // this level's pushers, listed compactly in ent[]
         __syncthreads();
         if (tid == 0) {
             *s_ncnt = 0;
             *s_npc = 0;
             *s_work = 0;
+            s_np = 0;
         }
         __syncthreads();
         unsigned long long vol = 0;
         unsigned int pushers = 0;
+        // every node the level touched: its level residual (and degree, carried
+        // by the list entry below 255) decides whether it pushes; only pushers
+        // load their estimate and row start and get a frontier entry
         for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
             int uu[kUnroll], ss[kUnroll], dd[kUnroll];
             T ro[kUnroll], xo[kUnroll];
+            bool pu[kUnroll];
 #pragma unroll
             for (int k = 0; k < kUnroll; k++) {
                 const unsigned int i = i0 + k * NT;
-                uu[k] = i < cnt ? lc[i] : -1;
-            }
-#pragma unroll
-            for (int k = 0; k < kUnroll; k++) {
-                ro[k] = (T)0;
-                xo[k] = (T)0;
-                ss[k] = 0;
+                uu[k] = -1;
                 dd[k] = 0;
-                if (uu[k] >= 0) {
-                    dd[k] = a.degree[uu[k]];
-                    xo[k] = est[uu[k]];
-                    ro[k] = uu[k] < hub_n ? hc[uu[k]] : lv[uu[k]];
-                    ss[k] = indptr[uu[k]];
+                if (i < cnt) {
+                    const uint32_t e = (uint32_t)lc[i];
+                    uu[k] = (int)(e & 0xffffffu);
+                    dd[k] = (int)(e >> 24);
                 }
             }
 #pragma unroll
             for (int k = 0; k < kUnroll; k++) {
+                ro[k] = (T)0;
+                if (uu[k] >= 0) {
+                    if (dd[k] == 255) dd[k] = a.degree[uu[k]];
+                    ro[k] = uu[k] < hub_n ? hc[uu[k]] : lv[uu[k]];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++)   // hk_solver.py:244
+                pu[k] = uu[k] >= 0 && !(fabs((double)ro[k]) < ((a.hk_ete * (double)dd[k]) / (double)N) / psic);
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                xo[k] = (T)0;
+                ss[k] = 0;
+                if (pu[k]) {
+                    xo[k] = est[uu[k]];
+                    ss[k] = indptr[uu[k]];
+                }
+            }
+            long long slot[kUnroll];
+            wave_append_n<kUnroll>(pu, &s_np, slot);
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
                 if (uu[k] < 0) continue;
-                const unsigned int i = i0 + k * NT;
                 const int u = uu[k];
                 const int d = dd[k];
                 const T tmp = ro[k];
                 if (u < hub_n) hc[u] = (T)0;            // hk_solver.py:282-285
                 else lv[u] = (T)0;
-                Ent e;
-                e.st = 0;
-                e.d = 0;
-                e.v = 0.0;
-                if (!(fabs((double)tmp) < ((a.hk_ete * (double)d) / (double)N) / psic)) {   // hk_solver.py:244
+                if (pu[k]) {
                     est[u] = xo[k] + tmp;
                     if (pristine(xo[k])) sup[atomicAdd(s_sup, 1u)] = u;
+                    Ent e;
                     e.st = ss[k];
                     e.d = d;
                     e.v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
+                    ent[slot[k]] = e;
                     vol += (unsigned long long)d;
                     pushers++;
-                    add_pieces(i, d, pw, hw, pcs, s_npc);
+                    add_pieces((unsigned int)slot[k], d, pw, hw, pcs, s_npc);
                 }
-                ent[i] = e;
             }
         }
         if (vol) atomicAdd(s_vol, vol);
@@ -936,8 +961,8 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
         __syncthreads();
         if (!last) {
             // next level's residual; first touch lists the node for level j + 1
-            wave_edges<ResT<T>>(
-                cnt, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
+            wave_edges<ResT<T>, true>(
+                s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
                 [&](int vtx, double val, ResT<T> &r, int stage) {
                     const int li = vtx < hub_n ? nb + vtx : vtx >= sink ? vtx + hsink : -1;
                     if (stage == 0) {
@@ -954,7 +979,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
         } else {
             // last level pushes straight into x (hk_solver.py:254-256)
             wave_edges<T>(
-                cnt, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
+                s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
                 [&](int vtx, double val, T &old, int stage) {
                     if (stage == 1) old = atomic_add_wg(&est[vtx], (T)val);
                 },
