@@ -2913,11 +2913,13 @@ extern "C" int flsg_session_create(flsg_graph g, int64_t query_cap, int64_t out_
     if (query_cap <= 0 || query_cap >= (int64_t)UINT32_MAX / 2) return fail(FLSG_EARG, "query_cap out of range");
     if (out_cap < 0) return fail(FLSG_EARG, "out_cap < 0");
     *out = nullptr;
+    int device = 0;
+    FLSG_HIP(hipGetDevice(&device));
     flsg_session s = new flsg_session_s();
     s->g = g;
     s->qcap = query_cap;
     s->out_cap = out_cap;
-    FLSG_HIP(hipGetDevice(&s->device));
+    s->device = device;
     rc = FLSG_OK;
     auto chk = [&](int r) {
         if (r && !rc) rc = r;
