@@ -847,10 +847,12 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
     const int32_t *indptr = a.indptr;
     const Strided<T> est = F::est(S);
     T *hub = reinterpret_cast<T *>(hub_raw);
-    // hub cache: two level arrays of hub_n entries, then kWave sink slots
+    // hub cache: one level array of hub_n entries, then kWave sink slots.  A
+    // level's node phase reads and clears its hub entries before its edge
+    // phase adds the next level's into the same array (a block barrier apart)
     const int cap = a.hub_bytes / (int)sizeof(T);
-    const int hub_n = cap > 3 * kWave ? (int)min((int64_t)((cap - kWave) / 2), a.n) : 0;
-    const int hsink = hub_n > 0 ? 2 * hub_n - sink : INT32_MIN;   // LDS index of an idle lane = vtx + hsink
+    const int hub_n = cap > 2 * kWave ? (int)min((int64_t)(cap - kWave), a.n) : 0;
+    const int hsink = hub_n > 0 ? hub_n - sink : INT32_MIN;   // LDS index of an idle lane = vtx + hsink
     for (int i = tid; i < a.hub_bytes / 8; i += NT) hub_raw[i] = 0.0;
     __syncthreads();
     if (tid == 0) {
@@ -879,10 +881,10 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
         const bool last = (j + 1 == N);
         const double coef = a.hk_tau / ((double)j + 1.);
         const double psic = a.hk_psi[j];
-        T *hc = hub + (odd ? hub_n : 0);
+        T *hc = hub;   // level j's hub entries, then level j + 1's
         const Strided<T> lv = F::lvl(S, a.rn, odd);
         const Strided<T> lnx = F::lvl(S, a.rn, !odd);
-        const int nb = odd ? 0 : hub_n;   // LDS base of level j + 1
+        const int nb = 0;   // LDS base of level j + 1
         __shared__ unsigned int s_np;   // this level's pushers, listed compactly in ent[]
         __syncthreads();
         if (tid == 0) {
