@@ -410,11 +410,12 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // appends of U touched destinations per lane: bit 0 of bits[u] -> next[]
-// (counter *s_next), bit 1 -> tl[] (*s_tl), bit 2 -> nl[] (*s_nl, entries
-// past nl_cap dropped but counted); one LDS atomic per list and wave; every
-// lane of the wave calls it
+// (counter *s_next), bit 1 -> tl[] (*s_tl; the entry vt[u], the id with its
+// degree, sup_entry), bit 2 -> nl[] (*s_nl, entries past nl_cap dropped but
+// counted); one LDS atomic per list and wave; every lane of the wave calls it
 template <int U>
-__device__ __forceinline__ void append_bits(const int *bits, const int *v, int32_t *next, unsigned int *s_next,
+__device__ __forceinline__ void append_bits(const int *bits, const int *v, const int *vt, int32_t *next,
+                                            unsigned int *s_next,
                                             int32_t *tl, unsigned int *s_tl, int32_t *nl, unsigned int *s_nl,
                                             unsigned int nl_cap) {
     bool pr[U];
@@ -430,7 +431,7 @@ __device__ __forceinline__ void append_bits(const int *bits, const int *v, int32
     wave_append_n<U>(pr, s_tl, sl);
 #pragma unroll
     for (int u = 0; u < U; u++)
-        if (sl[u] >= 0) tl[sl[u]] = v[u];
+        if (sl[u] >= 0) tl[sl[u]] = vt[u];
     if (s_nl) {
 #pragma unroll
         for (int u = 0; u < U; u++) pr[u] = (bits[u] & 4) != 0;
@@ -458,8 +459,9 @@ __device__ __forceinline__ void append_bits(const int *bits, const int *v, int32
 // unpredicated (a predicated atomic whose result is used makes the compiler
 // wait inside the predicated block, one round trip per edge): an idle lane
 // aims at its own sink entry (id sink + lane, degree 0) with value -0.0,
-// which leaves any value unchanged under IEEE addition.  kEnc: the listed ids
-// carry the node's degree (sup_entry).
+// which leaves any value unchanged under IEEE addition.  Touched-list (tl)
+// entries carry the node's degree (sup_entry); with kEnc the next[] and nl[]
+// entries do too.
 template <typename S, bool kEnc = false, typename Load, typename Pre, typename Post>
 __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int piece_w, int hub_w,
                                            unsigned int *s_work, WaveLds &W, const int2 *colx, Load load,
@@ -537,14 +539,10 @@ __device__ __forceinline__ void wave_edges(unsigned int cnt, const uint32_t *pcs
             int cr[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) cr[u] = post(vtx[u], dv[u], vv[u], res[u], act[u]) & (act[u] ? 7 : 0);
-            if constexpr (kEnc) {
-                int ev[kUnroll];
+            int ev[kUnroll];
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++) ev[u] = sup_entry(vtx[u], dv[u]);
-                append_bits<kUnroll>(cr, ev, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
-            } else {
-                append_bits<kUnroll>(cr, vtx, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
-            }
+            for (int u = 0; u < kUnroll; u++) ev[u] = sup_entry(vtx[u], dv[u]);
+            append_bits<kUnroll>(cr, kEnc ? ev : vtx, ev, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
         }
         if (group) wave_sync();
     }
@@ -1009,7 +1007,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
 #pragma unroll
         for (int k = 0; k < kUnroll; k++) {
             const unsigned int i = i0 + k * NT;
-            uu[k] = i < Sn ? sup[i] : -1;
+            uu[k] = i < Sn ? (int)((uint32_t)sup[i] & 0xffffffu) : -1;   // the last level's entries carry a degree
         }
 #pragma unroll
         for (int k = 0; k < kUnroll; k++) {
@@ -1170,7 +1168,10 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
             bits[u] &= 6;
             first[u] = fuse[u] && xo[u] == (T)0;   // first push of a node: list it in the support
         }
-        append_bits<kBinUnroll>(bits, v, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+        int vt[kBinUnroll];
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; u++) vt[u] = sup_entry(v[u], dv[u]);
+        append_bits<kBinUnroll>(bits, v, vt, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
         long long f[kBinUnroll], sk[kBinUnroll];
         wave_append_n<kBinUnroll>(fuse, s_nf, f);
         wave_append_n<kBinUnroll>(first, s_sup, sk);
@@ -1218,7 +1219,10 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                 int bits[kBinUnroll];
 #pragma unroll
                 for (int u = 0; u < kBinUnroll; u++) bits[u] = v[u] < n ? post(old[u], old[u] + val[u], dv[u]) : 0;
-                append_bits<kBinUnroll>(bits, v, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
+                int vt[kBinUnroll];
+#pragma unroll
+                for (int u = 0; u < kBinUnroll; u++) vt[u] = sup_entry(v[u], dv[u]);
+                append_bits<kBinUnroll>(bits, v, vt, next, s_next, tl, s_tl, nl, s_nl, nl_cap);
             }
             continue;
         }
@@ -1329,7 +1333,7 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
     if (tid == 0) {
         if (s < hub_n) hub[s] = (T)P.r0;
         else r[s] = (T)P.r0;
-        tl[0] = s;
+        tl[0] = sup_entry(s, a.degree[s]);
         qa[0] = s;
         *s_cnt = 1;
         *s_nf = 0;
@@ -1768,7 +1772,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) {
                         const unsigned int i = i0 + j * NT + tid;
-                        vv[j] = i < lim ? src[i] : -1;
+                        vv[j] = i < lim ? (int)((uint32_t)src[i] & 0xffffffu) : -1;   // tl entries carry a degree
                     }
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) {
@@ -2062,16 +2066,29 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 float r32v[kOutUnroll], x32v[kOutUnroll];
                 uint32_t flv[kOutUnroll];
                 double r64v[kOutUnroll], x64v[kOutUnroll];
+                // touched-list entries carry the degree (255: read it); the
+                // flagged entries nl[] are plain ids, their degrees loaded with
+                // the records; a streamed pass loads them for the live entries
 #pragma unroll
                 for (int j = 0; j < kOutUnroll; j++) {
                     const unsigned int i = i0 + j * NT + tid;
-                    vv[j] = i < cnt ? (mode == 0 ? nl[i] : stream ? (int)i : tl[i]) : -1;
+                    vv[j] = -1;
+                    dg[j] = 255;
+                    if (i < cnt) {
+                        if (mode == 0) {
+                            vv[j] = nl[i];
+                        } else if (stream) {
+                            vv[j] = (int)i;
+                        } else {
+                            const uint32_t e = (uint32_t)tl[i];
+                            vv[j] = (int)(e & 0xffffffu);
+                            dg[j] = (int)(e >> 24);
+                        }
+                    }
                 }
-                // listed entries are nearly all live: their degrees are loaded
-                // with the records (one memory round trip, not two); a streamed
-                // pass loads them only for the live entries
 #pragma unroll
-                for (int j = 0; j < kOutUnroll; j++) dg[j] = (pre_deg && vv[j] >= 0) ? degree[vv[j]] : 0;
+                for (int j = 0; j < kOutUnroll; j++)
+                    if (pre_deg && vv[j] >= 0 && dg[j] == 255) dg[j] = degree[vv[j]];
                 if (mode == 2) {
 #pragma unroll
                     for (int j = 0; j < kOutUnroll; j++) {
