@@ -819,235 +819,6 @@ struct HkF<true> {
     static __device__ __forceinline__ Strided<double> est(const SlotView &S) { return Strided<double>{&S.rx64[0].y, 2}; }
 };
 
-// Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), with
-// FP32 or FP64 level residuals (the query's precision) and FP64 threshold
-// tests.  Level j's residual lives in the level array of parity j & 1 -- for
-// the hub cache's nodes in one of two LDS level arrays.  Each level keeps
-// the list of nodes it touched; the node phase walks that list, pushes every
-// node at or above the level threshold (level residuals only grow, so this is
-// the set the reference's crossing queue holds, hk_solver.py:258) and clears
-// every entry (the reference zeroes the whole level, hk_solver.py:282-285),
-// so no separate clearing pass is needed.
-template <int NT, bool F64>
-__device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, const SlotView &S,
-                                         int32_t *qa, int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs, WaveLds &W,
-                                         double *hub_raw, unsigned int *s_cnt, unsigned int *s_ncnt,
-                                         unsigned int *s_sup, unsigned int *s_npc, unsigned int *s_work,
-                                         unsigned int *s_tn, unsigned long long *s_vol, unsigned long long *s_base,
-                                         uint64_t t_start, int sink) {
-    using F = HkF<F64>;
-    using T = typename F::T;
-    const int tid = threadIdx.x;
-    const int N = a.hk_n;
-    const int s = (int)qq.seed;
-    const int pw = a.piece_w;
-    const int hw = a.hub_w;
-    const int32_t *indptr = a.indptr;
-    const Strided<T> est = F::est(S);
-    T *hub = reinterpret_cast<T *>(hub_raw);
-    // hub cache: one level array of hub_n entries, then kWave sink slots.  A
-    // level's node phase reads and clears its hub entries before its edge
-    // phase adds the next level's into the same array (a block barrier apart)
-    const int cap = a.hub_bytes / (int)sizeof(T);
-    const int hub_n = cap > 2 * kWave ? (int)min((int64_t)(cap - kWave), a.n) : 0;
-    const int hsink = hub_n > 0 ? hub_n - sink : INT32_MIN;   // LDS index of an idle lane = vtx + hsink
-    for (int i = tid; i < a.hub_bytes / 8; i += NT) hub_raw[i] = 0.0;
-    __syncthreads();
-    if (tid == 0) {
-        if (s < hub_n) hub[s] = (T)1;   // hk_solver.py:223
-        else F::lvl(S, a.rn, false)[s] = (T)1;
-        qa[0] = sup_entry(s, a.degree[s]);   // level lists carry the degree
-        *s_cnt = 1;
-        *s_sup = 0;
-        *s_vol = 0;
-        *s_tn = 0;
-    }
-    __syncthreads();
-    int32_t *lc = qa, *ln = qb;
-    int64_t levels = 0;
-    auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
-        const Ent e = ent[i];
-        st = e.st;
-        d = e.d;
-        v = e.v;
-    };
-    for (int j = 0; j < N; j++) {
-        const unsigned int cnt = *s_cnt;
-        if (cnt == 0) break;   // hk_solver.py:237
-        levels++;
-        const bool odd = (j & 1) != 0;
-        const bool last = (j + 1 == N);
-        const double coef = a.hk_tau / ((double)j + 1.);
-        const double psic = a.hk_psi[j];
-        T *hc = hub;   // level j's hub entries, then level j + 1's
-        const Strided<T> lv = F::lvl(S, a.rn, odd);
-        const Strided<T> lnx = F::lvl(S, a.rn, !odd);
-        const int nb = 0;   // LDS base of level j + 1
-        __shared__ unsigned int s_np;   // this level's pushers, listed compactly in ent[]
-        __syncthreads();
-        if (tid == 0) {
-            *s_ncnt = 0;
-            *s_npc = 0;
-            *s_work = 0;
-            s_np = 0;
-        }
-        __syncthreads();
-        unsigned long long vol = 0;
-        unsigned int pushers = 0;
-        // every node the level touched: its level residual (and degree, carried
-        // by the list entry below 255) decides whether it pushes; only pushers
-        // load their estimate and row start and get a frontier entry
-        for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
-            int uu[kUnroll], ss[kUnroll], dd[kUnroll];
-            T ro[kUnroll], xo[kUnroll];
-            bool pu[kUnroll];
-#pragma unroll
-            for (int k = 0; k < kUnroll; k++) {
-                const unsigned int i = i0 + k * NT;
-                uu[k] = -1;
-                dd[k] = 0;
-                if (i < cnt) {
-                    const uint32_t e = (uint32_t)lc[i];
-                    uu[k] = (int)(e & 0xffffffu);
-                    dd[k] = (int)(e >> 24);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kUnroll; k++) {
-                ro[k] = (T)0;
-                if (uu[k] >= 0) {
-                    if (dd[k] == 255) dd[k] = a.degree[uu[k]];
-                    ro[k] = uu[k] < hub_n ? hc[uu[k]] : lv[uu[k]];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kUnroll; k++)   // hk_solver.py:244
-                pu[k] = uu[k] >= 0 && !(fabs((double)ro[k]) < ((a.hk_ete * (double)dd[k]) / (double)N) / psic);
-#pragma unroll
-            for (int k = 0; k < kUnroll; k++) {
-                xo[k] = (T)0;
-                ss[k] = 0;
-                if (pu[k]) {
-                    xo[k] = est[uu[k]];
-                    ss[k] = indptr[uu[k]];
-                }
-            }
-            long long slot[kUnroll];
-            wave_append_n<kUnroll>(pu, &s_np, slot);
-#pragma unroll
-            for (int k = 0; k < kUnroll; k++) {
-                if (uu[k] < 0) continue;
-                const int u = uu[k];
-                const int d = dd[k];
-                const T tmp = ro[k];
-                if (u < hub_n) hc[u] = (T)0;            // hk_solver.py:282-285
-                else lv[u] = (T)0;
-                if (pu[k]) {
-                    est[u] = xo[k] + tmp;
-                    if (pristine(xo[k])) sup[atomicAdd(s_sup, 1u)] = u;
-                    Ent e;
-                    e.st = ss[k];
-                    e.d = d;
-                    e.v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
-                    ent[slot[k]] = e;
-                    vol += (unsigned long long)d;
-                    pushers++;
-                    add_pieces((unsigned int)slot[k], d, pw, hw, pcs, s_npc);
-                }
-            }
-        }
-        if (vol) atomicAdd(s_vol, vol);
-        if (pushers) atomicAdd(s_tn, pushers);
-        __syncthreads();
-        if (!last) {
-            // next level's residual; first touch lists the node for level j + 1
-            wave_edges<ResT<T>, true>(
-                s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
-                [&](int vtx, double val, ResT<T> &r, int stage) {
-                    const int li = vtx < hub_n ? nb + vtx : vtx >= sink ? vtx + hsink : -1;
-                    if (stage == 0) {
-                        r.lds = li >= 0;
-                        if (li >= 0) r.v = atomicAdd(&hub[li], (T)val);
-                    } else if (li < 0) {
-                        r.v = atomic_add_wg(&lnx[vtx], (T)val);
-                    }
-                },
-                [&](int, int, double val, const ResT<T> &r, bool) -> int {
-                    return (pristine(r.v) && r.v + (T)val != (T)0) ? 2 : 0;
-                },
-                ln, s_ncnt, ln, s_ncnt, sink);
-        } else {
-            // last level pushes straight into x (hk_solver.py:254-256)
-            wave_edges<T>(
-                s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
-                [&](int vtx, double val, T &old, int stage) {
-                    if (stage == 1) old = atomic_add_wg(&est[vtx], (T)val);
-                },
-                [&](int, int, double val, T old, bool) -> int {
-                    return (pristine(old) && old + (T)val != (T)0) ? 2 : 0;
-                },
-                ln, s_ncnt, sup, s_sup, sink);
-        }
-        __syncthreads();
-        if (tid == 0) *s_cnt = *s_ncnt;
-        int32_t *t = lc;
-        lc = ln;
-        ln = t;
-        __syncthreads();
-    }
-    const unsigned int Sn = *s_sup;
-    if (tid == 0) *s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)Sn);
-    __syncthreads();
-    const unsigned long long obase = *s_base;
-    const bool fits = obase + Sn <= (unsigned long long)a.out_cap;
-    const double et = exp(a.hk_tau);
-    for (unsigned int i0 = tid; i0 < Sn; i0 += NT * kUnroll) {
-        int uu[kUnroll], id[kUnroll];
-        T xv[kUnroll];
-#pragma unroll
-        for (int k = 0; k < kUnroll; k++) {
-            const unsigned int i = i0 + k * NT;
-            uu[k] = i < Sn ? (int)((uint32_t)sup[i] & 0xffffffu) : -1;   // the last level's entries carry a degree
-        }
-#pragma unroll
-        for (int k = 0; k < kUnroll; k++) {
-            id[k] = 0;
-            xv[k] = (T)0;
-            if (uu[k] >= 0) {
-                xv[k] = est[uu[k]];
-                id[k] = a.out_map ? a.out_map[uu[k]] : uu[k];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kUnroll; k++) {
-            if (uu[k] < 0) continue;
-            const unsigned int i = i0 + k * NT;
-            if (fits) {
-                a.out_ids[obase + i] = id[k];
-                a.out_vals[obase + i] = (double)xv[k] / et;   // hk_solver.py:291
-            }
-            est[uu[k]] = (T)0;
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const uint64_t t_end = now_ticks();
-        flsg_result rr{};
-        rr.oper = (int64_t)*s_vol;
-        rr.iters = levels;
-        rr.supp = (int64_t)Sn;
-        rr.nodes = (int64_t)*s_tn;
-        rr.ms = (double)(t_end - t_start) * 1e-5;
-        rr.ms_push = rr.ms;
-        rr.ms_recompute = rr.ms;
-        rr.ms_correct = rr.ms;
-        rr.status = fits ? FLSG_OK : FLSG_ECAP;
-        a.res[q] = rr;
-        a.out_off[q] = fits ? (int64_t)obase : -1;
-    }
-    __syncthreads();
-}
-
 // Binned edge phase of one push iteration (see the file header), with the
 // next iteration's node phase folded into it for the nodes it sends across
 // their threshold.
@@ -1289,6 +1060,272 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
         atomicAdd(s_vol, fvol);
         atomicAdd(s_fvol, fvol);
     }
+}
+
+// Heat kernel query (hk_sor_queue with omega = 1, hk_solver.py:206-295), with
+// FP32 or FP64 level residuals (the query's precision) and FP64 threshold
+// tests.  Level j's residual lives in the level array of parity j & 1 -- for
+// the hub cache's nodes in one of two LDS level arrays.  Each level keeps
+// the list of nodes it touched; the node phase walks that list, pushes every
+// node at or above the level threshold (level residuals only grow, so this is
+// the set the reference's crossing queue holds, hk_solver.py:258) and clears
+// every entry (the reference zeroes the whole level, hk_solver.py:282-285),
+// so no separate clearing pass is needed.
+template <int NT, bool F64>
+__device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &qq, unsigned int q, const SlotView &S,
+                                         int32_t *qa, int32_t *qb, Ent *ent, int32_t *sup, uint32_t *pcs, WaveLds &W,
+                                         double *hub_raw, unsigned int *s_cnt, unsigned int *s_ncnt,
+                                         unsigned int *s_sup, unsigned int *s_npc, unsigned int *s_work,
+                                         unsigned int *s_tn, unsigned long long *s_vol, unsigned long long *s_base,
+                                         uint64_t t_start, int sink, uint4 *pairs_raw, unsigned int *bins) {
+    using F = HkF<F64>;
+    using T = typename F::T;
+    const int tid = threadIdx.x;
+    const int N = a.hk_n;
+    const int s = (int)qq.seed;
+    const int pw = a.piece_w;
+    const int hw = a.hub_w;
+    const int32_t *indptr = a.indptr;
+    const Strided<T> est = F::est(S);
+    T *hub = reinterpret_cast<T *>(hub_raw);
+    // hub cache: one level array of hub_n entries, then kWave sink slots.  A
+    // level's node phase reads and clears its hub entries before its edge
+    // phase adds the next level's into the same array (a block barrier apart)
+    const int cap = a.hub_bytes / (int)sizeof(T);
+    const int hub_n = cap > 2 * kWave ? (int)min((int64_t)(cap - kWave), a.n) : 0;
+    const int hsink = hub_n > 0 ? hub_n - sink : INT32_MIN;   // LDS index of an idle lane = vtx + hsink
+    for (int i = tid; i < a.hub_bytes / 8; i += NT) hub_raw[i] = 0.0;
+    // large levels under load take the binned push (as the PPR push does):
+    // destination ranges of 2^shift ids summed in the LDS
+    int shift = 0;
+    while (((int64_t)sizeof(T) << (shift + 1)) <= (int64_t)a.hub_bytes) shift++;
+    const int K = (int)((a.n + ((int64_t)1 << shift) - 1) >> shift);
+    const bool can_bin = a.undirected && pairs_raw && a.pcap >= (int64_t)a.indptr[a.n] && shift >= kBinShiftMin &&
+                         K <= kMaxBins && hub_n > 0;
+    __shared__ unsigned long long s_lvol, s_dvol, s_dfvol;   // this level's push volume; unused sums
+    __shared__ unsigned int s_dnf, s_hbusy;
+    bool hub_lds = true;   // the hub cache holds the current level's hub residuals
+    __syncthreads();
+    if (tid == 0) {
+        if (s < hub_n) hub[s] = (T)1;   // hk_solver.py:223
+        else F::lvl(S, a.rn, false)[s] = (T)1;
+        qa[0] = sup_entry(s, a.degree[s]);   // level lists carry the degree
+        *s_cnt = 1;
+        *s_sup = 0;
+        *s_vol = 0;
+        *s_tn = 0;
+    }
+    __syncthreads();
+    int32_t *lc = qa, *ln = qb;
+    int64_t levels = 0;
+    auto load_ent = [&](unsigned int i, int &st, int &d, double &v) {
+        const Ent e = ent[i];
+        st = e.st;
+        d = e.d;
+        v = e.v;
+    };
+    for (int j = 0; j < N; j++) {
+        const unsigned int cnt = *s_cnt;
+        if (cnt == 0) break;   // hk_solver.py:237
+        levels++;
+        const bool odd = (j & 1) != 0;
+        const bool last = (j + 1 == N);
+        const double coef = a.hk_tau / ((double)j + 1.);
+        const double psic = a.hk_psi[j];
+        T *hc = hub;   // level j's hub entries, then level j + 1's
+        const Strided<T> lv = F::lvl(S, a.rn, odd);
+        const Strided<T> lnx = F::lvl(S, a.rn, !odd);
+        const int nb = 0;   // LDS base of level j + 1
+        __shared__ unsigned int s_np;   // this level's pushers, listed compactly in ent[]
+        unsigned int busy_now = 0;
+        if (tid == 0 && a.busy) busy_now = __hip_atomic_load(a.busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (tid == 0) {
+            *s_ncnt = 0;
+            *s_npc = 0;
+            *s_work = 0;
+            s_np = 0;
+            s_lvol = 0;
+        }
+        __syncthreads();
+        unsigned long long vol = 0;
+        unsigned int pushers = 0;
+        // every node the level touched: its level residual (and degree, carried
+        // by the list entry below 255) decides whether it pushes; only pushers
+        // load their estimate and row start and get a frontier entry
+        for (unsigned int i0 = tid; i0 < cnt; i0 += NT * kUnroll) {
+            int uu[kUnroll], ss[kUnroll], dd[kUnroll];
+            T ro[kUnroll], xo[kUnroll];
+            bool pu[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                const unsigned int i = i0 + k * NT;
+                uu[k] = -1;
+                dd[k] = 0;
+                if (i < cnt) {
+                    const uint32_t e = (uint32_t)lc[i];
+                    uu[k] = (int)(e & 0xffffffu);
+                    dd[k] = (int)(e >> 24);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                ro[k] = (T)0;
+                if (uu[k] >= 0) {
+                    if (dd[k] == 255) dd[k] = a.degree[uu[k]];
+                    ro[k] = hub_lds && uu[k] < hub_n ? hc[uu[k]] : lv[uu[k]];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++)   // hk_solver.py:244
+                pu[k] = uu[k] >= 0 && !(fabs((double)ro[k]) < ((a.hk_ete * (double)dd[k]) / (double)N) / psic);
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                xo[k] = (T)0;
+                ss[k] = 0;
+                if (pu[k]) {
+                    xo[k] = est[uu[k]];
+                    ss[k] = indptr[uu[k]];
+                }
+            }
+            long long slot[kUnroll];
+            wave_append_n<kUnroll>(pu, &s_np, slot);
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                if (uu[k] < 0) continue;
+                const int u = uu[k];
+                const int d = dd[k];
+                const T tmp = ro[k];
+                if (hub_lds && u < hub_n) hc[u] = (T)0;            // hk_solver.py:282-285
+                else lv[u] = (T)0;
+                if (pu[k]) {
+                    est[u] = xo[k] + tmp;
+                    if (pristine(xo[k])) sup[atomicAdd(s_sup, 1u)] = u;
+                    Ent e;
+                    e.st = ss[k];
+                    e.d = d;
+                    e.v = last ? (double)tmp / (double)d : coef * (double)tmp / (double)d;   // hk_solver.py:249,255
+                    ent[slot[k]] = e;
+                    vol += (unsigned long long)d;
+                    pushers++;
+                    add_pieces((unsigned int)slot[k], d, pw, hw, pcs, s_npc);
+                }
+            }
+        }
+        if (vol) {
+            atomicAdd(s_vol, vol);
+            atomicAdd(&s_lvol, vol);
+        }
+        if (pushers) atomicAdd(s_tn, pushers);
+        if (tid == 0) s_hbusy = busy_now;
+        __syncthreads();
+        if (!hub_lds) {
+            // a binned level left its hub residuals in memory (read above) and
+            // stale copies in the LDS: the next level's hub cache starts at zero
+            for (int i = tid; i < a.hub_bytes / 8; i += NT) hub_raw[i] = 0.0;
+            hub_lds = true;
+            __syncthreads();
+        }
+        const bool binned = !last && can_bin && s_lvol >= (unsigned long long)a.bin_min &&
+                            (!a.busy || s_hbusy >= (unsigned int)a.bin_busy_min);
+        if (binned) {
+            // the next level's residual through pairs per destination range; first
+            // touch lists the node for level j + 1 (the hub residuals end in memory)
+            uint64_t tp = 0;
+            binned_edges<NT, T>(
+                s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, a.indptr, a.degree, load_ent, shift, K, a.n, hub,
+                hub_n, hub_raw, bins, reinterpret_cast<Pair<T> *>(pairs_raw), lnx, lnx, a.dense_min,
+                [&](T old, T nw, int) -> int { return (pristine(old) && nw != (T)0) ? 2 : 0; },
+                [&](T, int) -> double { return 0.0; }, ent, pcs, s_npc, &s_dnf, &s_dvol, &s_dfvol, sup, s_sup, ln,
+                s_ncnt, ln, s_ncnt, (int32_t *)nullptr, (unsigned int *)nullptr, 0u, tp);
+            hub_lds = false;
+        } else if (!last) {
+            // next level's residual; first touch lists the node for level j + 1
+            wave_edges<ResT<T>, true>(
+                s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
+                [&](int vtx, double val, ResT<T> &r, int stage) {
+                    const int li = vtx < hub_n ? nb + vtx : vtx >= sink ? vtx + hsink : -1;
+                    if (stage == 0) {
+                        r.lds = li >= 0;
+                        if (li >= 0) r.v = atomicAdd(&hub[li], (T)val);
+                    } else if (li < 0) {
+                        r.v = atomic_add_wg(&lnx[vtx], (T)val);
+                    }
+                },
+                [&](int, int, double val, const ResT<T> &r, bool) -> int {
+                    return (pristine(r.v) && r.v + (T)val != (T)0) ? 2 : 0;
+                },
+                ln, s_ncnt, ln, s_ncnt, sink);
+        } else {
+            // last level pushes straight into x (hk_solver.py:254-256)
+            wave_edges<T>(
+                s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, load_ent,
+                [&](int vtx, double val, T &old, int stage) {
+                    if (stage == 1) old = atomic_add_wg(&est[vtx], (T)val);
+                },
+                [&](int, int, double val, T old, bool) -> int {
+                    return (pristine(old) && old + (T)val != (T)0) ? 2 : 0;
+                },
+                ln, s_ncnt, sup, s_sup, sink);
+        }
+        __syncthreads();
+        if (tid == 0) *s_cnt = *s_ncnt;
+        int32_t *t = lc;
+        lc = ln;
+        ln = t;
+        __syncthreads();
+    }
+    const unsigned int Sn = *s_sup;
+    if (tid == 0) *s_base = atomicAdd((unsigned long long *)&a.ctl[1], (unsigned long long)Sn);
+    __syncthreads();
+    const unsigned long long obase = *s_base;
+    const bool fits = obase + Sn <= (unsigned long long)a.out_cap;
+    const double et = exp(a.hk_tau);
+    for (unsigned int i0 = tid; i0 < Sn; i0 += NT * kUnroll) {
+        int uu[kUnroll], id[kUnroll];
+        T xv[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            const unsigned int i = i0 + k * NT;
+            uu[k] = i < Sn ? (int)((uint32_t)sup[i] & 0xffffffu) : -1;   // the last level's entries carry a degree
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            id[k] = 0;
+            xv[k] = (T)0;
+            if (uu[k] >= 0) {
+                xv[k] = est[uu[k]];
+                id[k] = a.out_map ? a.out_map[uu[k]] : uu[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            if (uu[k] < 0) continue;
+            const unsigned int i = i0 + k * NT;
+            if (fits) {
+                a.out_ids[obase + i] = id[k];
+                a.out_vals[obase + i] = (double)xv[k] / et;   // hk_solver.py:291
+            }
+            est[uu[k]] = (T)0;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint64_t t_end = now_ticks();
+        flsg_result rr{};
+        rr.oper = (int64_t)*s_vol;
+        rr.iters = levels;
+        rr.supp = (int64_t)Sn;
+        rr.nodes = (int64_t)*s_tn;
+        rr.ms = (double)(t_end - t_start) * 1e-5;
+        rr.ms_push = rr.ms;
+        rr.ms_recompute = rr.ms;
+        rr.ms_correct = rr.ms;
+        rr.status = fits ? FLSG_OK : FLSG_ECAP;
+        a.res[q] = rr;
+        a.out_off[q] = fits ? (int64_t)obase : -1;
+    }
+    __syncthreads();
 }
 
 // Local push of one PPR / Katz query in residual type T (float: the FP32
@@ -1694,10 +1731,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (qq.kind == FLSG_HK) {
             if (qq.precision == FLSG_FP64)
                 hk_query<NT, true>(a, qq, q, S, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
-                                   &s_work, &s_tn, &s_vol, &s_base, t_start, sink);
+                                   &s_work, &s_tn, &s_vol, &s_base, t_start, sink, pairs, s_bins);
             else
                 hk_query<NT, false>(a, qq, q, S, qa, qb, ent, sup, pcs, W, s_hub, &s_cnt, &s_ncnt, &s_sup, &s_npc,
-                                    &s_work, &s_tn, &s_vol, &s_base, t_start, sink);
+                                    &s_work, &s_tn, &s_vol, &s_base, t_start, sink, pairs, s_bins);
             query_done(a, q);
             continue;
         }
