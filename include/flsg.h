@@ -263,23 +263,24 @@ typedef struct {
 } flsg_hk_config;
 
 /* Asynchronous batched launch over slots [slot_base, slot_base + n_slots) of
- * the reserved arena: queries may mix PPR and Katz and their parameters.
- * d_ctl: 2 uint64 of device scratch owned by this launch; d_res (device,
- * n_queries records) receives the results; nothing is synchronised.  Launches
- * whose slot ranges are disjoint may run concurrently on different streams
- * (the serving loop's continuous batching).  The graph must not be modified
- * while a launch runs.  This is the enqueue half of a serving session;
- * flsg_stream_poll is the other. */
+ * the reserved arena, over a query list in device memory: queries may mix
+ * PPR, Katz and (with hk non-NULL) heat-kernel queries, precisions and
+ * parameters.  d_ctl: 2 uint64 of device scratch owned by this launch; d_res
+ * (device, n_queries records) receives the results; nothing is
+ * synchronised.  Launches whose slot ranges are disjoint may run
+ * concurrently on different streams.  The graph must not be modified while
+ * a launch runs.  flsg_stream_poll tells when it has ended; the serving
+ * session (flsg_session_*) admits queries into a running launch instead. */
 int flsg_batch_launch(flsg_graph g, const flsg_query *d_queries, int64_t n_queries, int32_t slot_base,
                       int32_t n_slots, int32_t *d_out_ids, double *d_out_vals, int64_t *d_out_off, int64_t out_cap,
                       uint64_t *d_ctl, flsg_result *d_res, int64_t max_iters, const flsg_hk_config *hk,
                       void *stream);
 
-/* Poll half of a serving session: *h_done = 1 when every operation queued
- * on `stream` so far (a flsg_batch_launch and the caller's copies of its
- * results) has completed, 0 while any is still running; never blocks.  A
- * fault of that work is returned as FLSG_EHIP.  No counterpart in the
- * reference (it has no serving loop). */
+/* *h_done = 1 when every operation queued on `stream` so far (a
+ * flsg_batch_launch and the caller's copies of its results) has completed,
+ * 0 while any is still running; never blocks.  A fault of that work is
+ * returned as FLSG_EHIP.  No counterpart in the reference (it has no
+ * asynchronous launches). */
 int flsg_stream_poll(void *stream, int32_t *h_done);
 
 /* Serving session: continuous admission into ONE persistent batched launch.
