@@ -2604,10 +2604,11 @@ static int launch(flsg_graph g, const flsg_query *d_q, int64_t nq, int32_t slot_
     a.slot_base = slot_base;
     a.ctl = d_ctl;
     a.busy = d_busy;
-    // binned iterations while a quarter of the launch's workgroups push (PPR
-    // at eps 1e-7 on the 2.39M-node graph: the binned push is 1.5x faster
-    // than the direct one with 256 slots busy, even with 64, 14% slower with 16)
-    a.bin_busy_min = std::max<int32_t>(1, n_slots / 4);
+    // binned iterations while at least a quarter of the device's compute
+    // units run a query (PPR at eps 1e-7 on the 2.39M-node graph: the binned
+    // push is 1.5x faster than the direct one with 256 slots busy, even with
+    // 64, 14% slower with 16, 25% slower for a lone query)
+    a.bin_busy_min = g->batch_bin_busy > 0 ? g->batch_bin_busy : std::max<int32_t>(1, g->num_cus / 4);
     a.out_map = d_out_map;
     a.out_ids = d_out_ids;
     a.out_vals = d_out_vals;
@@ -2698,6 +2699,14 @@ extern "C" int flsg_batch_set_degree_bins(flsg_graph g, int32_t split_edges, int
         return fail(FLSG_EARG, "piece_edges: 0 or 64..2^24");
     g->batch_hub_w = split_edges;
     g->batch_piece_w = piece_edges;
+    return FLSG_OK;
+}
+
+extern "C" int flsg_batch_set_bin_load(flsg_graph g, int32_t min_busy) {
+    int rc = check_graph(g);
+    if (rc) return rc;
+    if (min_busy < 0) return fail(FLSG_EARG, "bin load: min_busy >= 0");
+    g->batch_bin_busy = min_busy;
     return FLSG_OK;
 }
 

@@ -114,6 +114,7 @@ struct flsg_graph_s {
     int32_t batch_piece_w = 0;              // edges per piece (0 = 1024, doubled to <= 255 pieces per entry)
     int64_t batch_bin_min = 65536;          // binned push for iterations of at least this many edges
     int32_t batch_dense_min = 64;           // binned push: bins with fewer pairs use memory-side atomics
+    int32_t batch_bin_busy = 0;             // binned push only with this many busy workgroups (0 = CUs / 4)
     int32_t undirected = -1;                // every edge (u, v) has its (v, u): -1 unknown, 0 no, 1 yes
     int32_t simple = -1;                    // no row repeats an entry (set with undirected)
     const int32_t *out_map = nullptr;       // internal id -> caller id for batched outputs

@@ -79,6 +79,7 @@ SIGNATURES = {
     "flsg_batch_set_hub_cache": (C.c_int, [C.c_void_p, C.c_int32]),
     "flsg_batch_set_degree_bins": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "flsg_batch_set_binning": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
+    "flsg_batch_set_bin_load": (C.c_int, [C.c_void_p, C.c_int32]),
     "flsg_batch_reserve": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "flsg_graph_set_out_map": (C.c_int, [C.c_void_p, C.c_void_p]),
     "flsg_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,

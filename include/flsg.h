@@ -227,6 +227,16 @@ int flsg_batch_set_hub_cache(flsg_graph g, int32_t bytes);
  * (tuning knob, set by the autotuner). */
 int flsg_batch_set_binning(flsg_graph g, int64_t min_edges, int32_t dense_pairs);
 
+/* Load rule of the binned push: in flsg_batch_solve and serving sessions an
+ * iteration is binned only while at least min_busy workgroups of the launch
+ * run a query (the binned push pays off when many slots push at once --
+ * memory-side atomics are then the chip-wide limit -- and the direct push is
+ * quicker when few do).  0 = the default, a quarter of the device's compute
+ * units (64 on MI355X); 1 bins whenever an iteration is large enough.
+ * flsg_batch_launch does not count its workgroups and bins by size alone.
+ * No counterpart in the reference (tuning knob). */
+int flsg_batch_set_bin_load(flsg_graph g, int32_t min_busy);
+
 /* Degree bins of the batched edge phases: a list entry with more than
  * split_edges edges (0 = 256) is processed as pieces of piece_edges edges
  * (0 = 1024; raised to keep at most 255 pieces per entry), each piece by one

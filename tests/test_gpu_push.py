@@ -464,6 +464,7 @@ def test_batch_binned_push_small(small_graph, hub_bytes, bin_min, dense_min):
     try:
         engine._check(dg.lib.flsg_batch_set_hub_cache(dg.handle, hub_bytes))
         engine._check(dg.lib.flsg_batch_set_binning(dg.handle, bin_min, dense_min))
+        engine._check(dg.lib.flsg_batch_set_bin_load(dg.handle, 1))   # bin with any number of busy slots
         seeds = graph.mixed_seeds(g, 40, seed=77)
         for kind, a, eps in ((_abi.FLSG_PPR, ALPHA, 1e-6), (_abi.FLSG_KATZ, 1.0 / (g.d_max + 1), 1e-7)):
             p = engine.make_params(kind, _abi.FLSG_GD, True, _abi.FLSG_FP64, alpha=a, eps=eps)
