@@ -1577,13 +1577,19 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
 // Only the workgroup waiting for the first unpublished index reads the host's
 // count and close flag (each read crosses PCIe); it mirrors them into device
 // memory (ctl[3]: published count, ctl[4]: closed), where the other waiting
-// workgroups read them.
+// workgroups read them.  The device words are polled with relaxed loads (an
+// acquire at device scope invalidates caches the other slots of the XCD are
+// using); one acquire fence orders a claimed index before its query is read.
 __device__ __forceinline__ unsigned int session_claim(const BatchArgs &a) {
     unsigned long long *expired = (unsigned long long *)&a.ctl[2];
     unsigned long long *mpub = (unsigned long long *)&a.ctl[3];
     unsigned long long *mclosed = (unsigned long long *)&a.ctl[4];
     auto dload = [](unsigned long long *p) {
-        return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto claimed = [](unsigned int c) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return c;
     };
     // once one workgroup has given up waiting, indices it claimed may be
     // published and never run: every workgroup ends, and the host re-submits
@@ -1595,7 +1601,7 @@ __device__ __forceinline__ unsigned int session_claim(const BatchArgs &a) {
     for (unsigned int it = 0;; it++) {
         if (dload(expired) != 0ull) return 0xffffffffu;
         unsigned long long p = dload(mpub);
-        if ((uint64_t)c < p) return c;
+        if ((uint64_t)c < p) return claimed(c);
         if ((uint64_t)c == p) {
             // the poller: host count (and, every 8th round, the close flag)
             const unsigned long long hp = __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1609,12 +1615,12 @@ __device__ __forceinline__ unsigned int session_claim(const BatchArgs &a) {
                     __hip_atomic_load(a.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
                 __hip_atomic_fetch_max(mpub, hp2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(mclosed, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint64_t)c < hp2) return c;
+                if ((uint64_t)c < hp2) return claimed(c);
                 return 0xffffffffu;
             }
         } else if (dload(mclosed) != 0ull) {
             // closed: the mirror was raised to the final count before the flag
-            if ((uint64_t)c < dload(mpub)) return c;
+            if ((uint64_t)c < dload(mpub)) return claimed(c);
             return 0xffffffffu;
         }
         if (now_ticks() - t0 > a.idle_ticks) {
