@@ -191,6 +191,23 @@ struct Strided {
     __device__ __forceinline__ T &operator[](int64_t i) const { return p[i * s]; }
 };
 
+// {residual, estimate} of one node as one vector access (one load or store
+// instruction instead of two to the same record)
+template <typename T>
+struct Rec2;
+template <>
+struct Rec2<float> {
+    using type = float2;
+};
+template <>
+struct Rec2<double> {
+    using type = double2;
+};
+template <typename T>
+__device__ __forceinline__ typename Rec2<T>::type rec2(T r, T x) {
+    return typename Rec2<T>::type{r, x};
+}
+
 // A support-list entry of the PPR / Katz push: the node id (< 2^24) with its
 // degree in the top 8 bits (255: 255 or more, read from the graph), so the
 // certificate's scan of the list needs no degree load for most entries.
@@ -843,10 +860,12 @@ struct HkF<true> {
 //     iteration (counter *s_nf; its hub pieces to pcs[], counter *s_npc; its
 //     volume to *s_vol and *s_fvol).  Ranges with fewer than dense_min pairs
 //     take one memory-side atomic per pair instead (crossings -> next[]).
-// r and x are the residual and estimate halves of the slot's {r, x} records;
+// r and x are the residual and estimate halves of the slot's {r, x} records
+// (kRX: one record array, r.p[2 v] and r.p[2 v + 1], read and written as one
+// vector; otherwise two strided arrays, accessed apart);
 // post(old, new, d_v) -> append bits of one destination's add;
 // push(r, d) -> the push value of a node moving r with degree d.
-template <int NT, typename T, typename Load, typename Post, typename Push>
+template <int NT, typename T, bool kRX, typename Load, typename Post, typename Push>
 __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *pcs, unsigned int npc, int pw, int hw,
                                              unsigned int *s_work, WaveLds &W, const int2 *colx,
                                              const int32_t *indptr, const int32_t *degree, Load load, int shift,
@@ -928,8 +947,12 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                 fuse[u] = (bits[u] & 1) != 0;
                 if (fuse[u]) {
                     // the next iteration's node phase of v (ppr_solver.py:273-282)
-                    r[v[u]] = -(T)0;
-                    x[v[u]] = xo[u] + nw;
+                    if constexpr (kRX) {
+                        reinterpret_cast<typename Rec2<T>::type *>(r.p)[v[u]] = rec2<T>(-(T)0, xo[u] + nw);
+                    } else {
+                        r[v[u]] = -(T)0;
+                        x[v[u]] = xo[u] + nw;
+                    }
                     pv_u[u] = push(nw, dv[u]);
                     fvol += (unsigned long long)dv[u];
                 } else {
@@ -1047,8 +1070,14 @@ __device__ __forceinline__ void binned_edges(unsigned int cnt, const uint32_t *p
                 ro[u] = (T)0;
                 xo[u] = (T)0;
                 if (!pristine(dl[u])) {
-                    ro[u] = r[v[u]];
-                    xo[u] = x[v[u]];
+                    if constexpr (kRX) {
+                        const auto w = reinterpret_cast<const typename Rec2<T>::type *>(r.p)[v[u]];
+                        ro[u] = w.x;
+                        xo[u] = w.y;
+                    } else {
+                        ro[u] = r[v[u]];
+                        xo[u] = x[v[u]];
+                    }
                     if ((unsigned int)dv[u] == dsat) dv[u] = degree[v[u]];
                 }
             }
@@ -1232,7 +1261,7 @@ __device__ __forceinline__ void hk_query(const BatchArgs &a, const flsg_query &q
             // the next level's residual through pairs per destination range; first
             // touch lists the node for level j + 1 (the hub residuals end in memory)
             uint64_t tp = 0;
-            binned_edges<NT, T>(
+            binned_edges<NT, T, false>(
                 s_np, pcs, *s_npc, pw, hw, s_work, W, a.colx, a.indptr, a.degree, load_ent, shift, K, a.n, hub,
                 hub_n, hub_raw, bins, reinterpret_cast<Pair<T> *>(pairs_raw), lnx, lnx, a.dense_min,
                 [&](T old, T nw, int) -> int { return (pristine(old) && nw != (T)0) ? 2 : 0; },
@@ -1460,8 +1489,14 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
                 ss[j] = 0;
                 if (uu[j] >= 0) {
                     const int u = uu[j];
-                    xx[j] = x[u];
-                    rr[j] = (hub_on && u < hub_n) ? hub[u] : r[u];
+                    if (hub_on && u < hub_n) {
+                        xx[j] = x[u];
+                        rr[j] = hub[u];
+                    } else {
+                        const auto w = reinterpret_cast<const typename Rec2<T>::type *>(rb)[u];
+                        rr[j] = w.x;
+                        xx[j] = w.y;
+                    }
                     dd[j] = a.degree[u];
                     ss[j] = a.indptr[u];
                 }
@@ -1474,9 +1509,12 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
                 Ent e;
                 e.st = ss[j];
                 e.d = dd[j];
-                if (hub_on && u < hub_n) hub[u] = -(T)0;
-                else r[u] = -(T)0;
-                x[u] = xx[j] + rr[j];
+                if (hub_on && u < hub_n) {
+                    hub[u] = -(T)0;
+                    x[u] = xx[j] + rr[j];
+                } else {
+                    reinterpret_cast<typename Rec2<T>::type *>(rb)[u] = rec2<T>(-(T)0, xx[j] + rr[j]);
+                }
                 e.v = push(rr[j], dd[j]);
                 ent[i] = e;
                 vol += (unsigned long long)dd[j];
@@ -1512,7 +1550,7 @@ __device__ __forceinline__ int local_push(const BatchArgs &a, const QParams &P, 
         }
         if (binned) {
             uint64_t tp = t0;
-            binned_edges<NT, T>(cnt, pcs, npc, pw, hw, s_work, W, a.colx, a.indptr, a.degree, load_ent, shift, K,
+            binned_edges<NT, T, true>(cnt, pcs, npc, pw, hw, s_work, W, a.colx, a.indptr, a.degree, load_ent, shift, K,
                                 a.n, hub, hub_n, s_hub, bins, pairs, r, x, a.dense_min, post, push, ent, pcs,
                                 s_npc, s_nf, s_vol, s_fvol, sup, s_sup, qn, s_ncnt, tl, s_tl, nlp, s_nlp,
                                 (unsigned int)a.n, tp);
