@@ -2141,15 +2141,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // mode 2 (FP64 push): only {r64, x64} are read and cleared.
         auto out_pass = [&](auto mode_c, const unsigned int cnt) {
             constexpr int mode = decltype(mode_c)::value;
-            const bool pre_deg = mode == 0 || !stream;
             for (unsigned int i0 = 0; i0 < cnt; i0 += NT * kOutUnroll) {
                 int vv[kOutUnroll], dg[kOutUnroll];
                 float r32v[kOutUnroll], x32v[kOutUnroll];
                 uint32_t flv[kOutUnroll];
                 double r64v[kOutUnroll], x64v[kOutUnroll];
                 // touched-list entries carry the degree (255: read it); the
-                // flagged entries nl[] are plain ids, their degrees loaded with
-                // the records; a streamed pass loads them for the live entries
+                // flagged entries nl[] are plain ids and a streamed pass reads
+                // every entry's degree (coalesced): all issued with the records,
+                // before any output store (an int store may alias degree[])
 #pragma unroll
                 for (int j = 0; j < kOutUnroll; j++) {
                     const unsigned int i = i0 + j * NT + tid;
@@ -2169,7 +2169,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
 #pragma unroll
                 for (int j = 0; j < kOutUnroll; j++)
-                    if (pre_deg && vv[j] >= 0 && dg[j] == 255) dg[j] = degree[vv[j]];
+                    if (vv[j] >= 0 && dg[j] == 255) dg[j] = degree[vv[j]];
                 if (mode == 2) {
 #pragma unroll
                     for (int j = 0; j < kOutUnroll; j++) {
@@ -2224,7 +2224,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         live = !(pristine(r64v[j]) && pristine(x64v[j]));
                     if (live) {
                         const int v = vv[j];
-                        const int d = pre_deg ? dg[j] : degree[v];
+                        const int d = dg[j];
                         // FP32 push: an entry may hold its residual in FP32 (pending,
                         // rl > 0) and carries the error bound of its FP32 values
                         const float rl = r32v[j];
