@@ -1907,7 +1907,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     int vv[kScanUnroll];
                     float2 rx[kScanUnroll];
                     // a support-list entry carries its degree (below 255);
-                    // streamed, degrees are loaded for the pushed entries only
+                    // streamed, every entry's degree loads with its record
+                    // (coalesced, one memory round trip per batch)
                     int dd[kScanUnroll];
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) {
@@ -1924,17 +1925,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                             }
                         }
                     }
-                    if (!sstream) {
 #pragma unroll
-                        for (int j = 0; j < kScanUnroll; j++)
-                            if (dd[j] == 255) dd[j] = degree[vv[j]];
-                    }
+                    for (int j = 0; j < kScanUnroll; j++)
+                        if (vv[j] >= 0 && (sstream || dd[j] == 255)) dd[j] = degree[vv[j]];
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) rx[j] = vv[j] >= 0 ? S.rx32[vv[j]] : make_float2(0.0f, 0.0f);
-                    if (sstream) {
-#pragma unroll
-                        for (int j = 0; j < kScanUnroll; j++) dd[j] = rx[j].y != 0.0f ? degree[vv[j]] : 0;
-                    }
 #pragma unroll
                     for (int j = 0; j < kScanUnroll; j++) {
                         bool exact = false;
